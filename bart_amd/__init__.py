@@ -1,0 +1,341 @@
+"""bart_amd -- gfx950 NUFFT / CG-SENSE behind BART's linop interface.
+
+Python mirror of the C ABI in ``include/bart_amd.h`` (``libbart_amd.so``):
+``nufft_create2`` / ``nufft_create`` / ``sense_nc_init`` build operators,
+``linop_forward`` / ``linop_adjoint`` / ``linop_normal`` apply them,
+``lsqr_conjgrad`` runs CG on the normal equations, ``fft`` / ``ifft`` are the
+batched FFTs.  Arrays use BART's layout: a list of (normally 16) dims,
+column-major, complex64.  Arguments may be numpy arrays (host) or torch CUDA
+tensors (device); the library selects the device path by pointer location as
+BART does.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is present
+the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libbart_amd.so")
+_LIB = None
+
+DIMS = 16
+
+
+class BartError(RuntimeError):
+    pass
+
+
+class NufftConf(ctypes.Structure):
+    """struct nufft_conf_s (noncart/nufft.h:7-29)."""
+    _fields_ = [
+        ("toeplitz", ctypes.c_bool),
+        ("pcycle", ctypes.c_bool),
+        ("periodic", ctypes.c_bool),
+        ("lowmem", ctypes.c_bool),
+        ("flags", ctypes.c_ulong),
+        ("cfft", ctypes.c_ulong),
+        ("decomp", ctypes.c_bool),
+        ("nopsf", ctypes.c_bool),
+        ("upper_triag", ctypes.c_bool),
+        ("real", ctypes.c_bool),
+        ("compress_psf", ctypes.c_bool),
+        ("decomposed_psf", ctypes.c_bool),
+        ("precomp", ctypes.c_bool),
+        ("precomp_linphase", ctypes.c_bool),
+        ("precomp_fftmod", ctypes.c_bool),
+        ("precomp_roll", ctypes.c_bool),
+        ("zero_overhead", ctypes.c_bool),
+        ("width", ctypes.c_float),
+        ("os", ctypes.c_float),
+    ]
+
+    @classmethod
+    def defaults(cls, **kw):
+        """nufft_conf_defaults (noncart/nufft.c:53-75) with overrides."""
+        c = cls(toeplitz=True, pcycle=False, periodic=False, lowmem=False, flags=7, cfft=0,
+                decomp=True, nopsf=False, upper_triag=False, real=False, compress_psf=False,
+                decomposed_psf=False, precomp=True, precomp_linphase=True, precomp_fftmod=True,
+                precomp_roll=True, zero_overhead=False, width=6.0, os=2.0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+class CgStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("resid", ctypes.c_float),
+        ("bnorm", ctypes.c_float),
+        ("time_ms", ctypes.c_float),
+        ("fell_back", ctypes.c_bool),
+    ]
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def build(verbose=False):
+    """Compile libbart_amd.so for gfx950 (hipcc, in-tree)."""
+    import subprocess
+    out = None if verbose else subprocess.DEVNULL
+    jobs = str(min(8, os.cpu_count() or 1))
+    subprocess.run(["make", "-j", jobs, "-C", os.path.join(_HERE, "csrc")], check=True, stdout=out)
+
+
+def _init_torch_first():
+    """PyTorch-ROCm wheels bundle their own HIP runtime.  When torch is used in
+    the same process (as device-memory plumbing), its runtime must initialise
+    the GPU before this library's does, or torch's device init fails.  Only
+    done when torch is already imported; a device count query does not
+    initialise the GPU."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is None:
+        return
+    try:
+        if torch.cuda.device_count() > 0:
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
+def lib():
+    """Load libbart_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(_LIB_PATH):
+        raise BartError("libbart_amd.so not built: run bart_amd.build() (hipcc, gfx950)")
+    _init_torch_first()
+    L = ctypes.CDLL(_LIB_PATH)
+    vp = ctypes.c_void_p
+    lp = ctypes.POINTER(ctypes.c_long)
+    L.nufft_create2.restype = vp
+    L.nufft_create2.argtypes = [ctypes.c_int, lp, lp, lp, vp, lp, vp, lp, vp, NufftConf]
+    L.nufft_create.restype = vp
+    L.nufft_create.argtypes = [ctypes.c_int, lp, lp, lp, vp, vp, NufftConf]
+    L.sense_nc_init.restype = vp
+    L.sense_nc_init.argtypes = [lp, lp, vp, lp, lp, vp, ctypes.POINTER(NufftConf), lp, vp, lp, vp, vp, ctypes.c_ulong]
+    for f in (L.linop_forward, L.linop_adjoint):
+        f.restype = None
+        f.argtypes = [vp, ctypes.c_int, lp, vp, ctypes.c_int, lp, vp]
+    L.linop_normal.restype = None
+    L.linop_normal.argtypes = [vp, ctypes.c_int, lp, vp, vp]
+    L.linop_free.restype = None
+    L.linop_free.argtypes = [vp]
+    L.nufft_get_psf_dims.restype = ctypes.c_int
+    L.nufft_get_psf_dims.argtypes = [vp, ctypes.c_int, lp]
+    L.nufft_get_psf.restype = None
+    L.nufft_get_psf.argtypes = [vp, ctypes.c_int, lp, vp]
+    L.nufft_update_psf.restype = None
+    L.nufft_update_psf.argtypes = [vp, ctypes.c_int, lp, vp]
+    for f in (L.fft, L.ifft):
+        f.restype = None
+        f.argtypes = [ctypes.c_int, lp, ctypes.c_ulong, vp, vp]
+    L.lsqr_conjgrad.restype = ctypes.c_int
+    L.lsqr_conjgrad.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, lp, vp, vp,
+                                ctypes.c_bool, ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
+    L.bart_amd_set_error_mode.argtypes = [ctypes.c_int]
+    L.bart_amd_last_error.restype = ctypes.c_char_p
+    L.bart_amd_clear_error.restype = None
+    L.bart_amd_version.restype = ctypes.c_char_p
+    L.bart_amd_set_error_mode(1)   # raise in Python instead of exit(1)
+    _LIB = L
+    return L
+
+
+def _check():
+    e = lib().bart_amd_last_error()
+    if e:
+        lib().bart_amd_clear_error()
+        raise BartError(e.decode())
+
+
+def _dims(d, n=DIMS):
+    d = [int(v) for v in d] + [1] * (n - len(d))
+    return (ctypes.c_long * n)(*d[:n])
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a):
+    """Pointer of a contiguous complex64 numpy array or torch tensor."""
+    if a is None:
+        return None
+    if _is_torch(a):
+        assert a.is_contiguous() and str(a.dtype) == "torch.complex64", "need contiguous complex64 tensor"
+        return ctypes.c_void_p(a.data_ptr())
+    assert a.dtype == np.complex64 and (a.flags.f_contiguous or a.flags.c_contiguous)
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _like(src, shape):
+    """Output array on the same side (host/device) as src, BART dims shape."""
+    n = int(np.prod(shape))
+    if _is_torch(src):
+        import torch
+        return torch.empty(n, dtype=torch.complex64, device=src.device)
+    return np.empty(n, np.complex64)
+
+
+def _host_c64(a):
+    """complex64 array whose memory is taken as BART's column-major layout:
+    a C-ordered array of shape dims[::-1] or an F-ordered array of shape dims
+    (outputs are flat)."""
+    if a is None or _is_torch(a):
+        return a
+    a = np.asarray(a, np.complex64)
+    if not (a.flags.c_contiguous or a.flags.f_contiguous):
+        a = np.ascontiguousarray(a)
+    return a
+
+
+class Linop:
+    """A BART linop_s handle with its output (odims) and input (idims) dims."""
+
+    def __init__(self, handle, odims, idims):
+        if not handle:
+            _check()
+            raise BartError("operator creation failed")
+        self.h = handle
+        self.odims = list(odims) + [1] * (DIMS - len(odims))
+        self.idims = list(idims) + [1] * (DIMS - len(idims))
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().linop_free(self.h)
+            self.h = None
+            _check()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def forward(self, x, out=None):
+        x = _host_c64(x)
+        out = _like(x, self.odims) if out is None else out
+        lib().linop_forward(self.h, DIMS, _dims(self.odims), _ptr(out), DIMS, _dims(self.idims), _ptr(x))
+        _check()
+        return out
+
+    def adjoint(self, y, out=None):
+        y = _host_c64(y)
+        out = _like(y, self.idims) if out is None else out
+        lib().linop_adjoint(self.h, DIMS, _dims(self.idims), _ptr(out), DIMS, _dims(self.odims), _ptr(y))
+        _check()
+        return out
+
+    def normal(self, x, out=None):
+        x = _host_c64(x)
+        out = _like(x, self.idims) if out is None else out
+        lib().linop_normal(self.h, DIMS, _dims(self.idims), _ptr(out), _ptr(x))
+        _check()
+        return out
+
+    def psf_dims(self):
+        n = lib().nufft_get_psf_dims(self.h, 0, None)
+        _check()
+        d = (ctypes.c_long * n)()
+        lib().nufft_get_psf_dims(self.h, n, d)
+        _check()
+        return list(d)
+
+    def get_psf(self):
+        d = self.psf_dims()
+        out = np.empty(int(np.prod(d)), np.complex64)
+        lib().nufft_get_psf(self.h, len(d), _dims(d, len(d)), _ptr(out))
+        _check()
+        return out
+
+    def update_psf(self, psf):
+        d = self.psf_dims()
+        psf = _host_c64(psf)
+        lib().nufft_update_psf(self.h, len(d), _dims(d, len(d)), _ptr(psf))
+        _check()
+
+
+def traj_array(traj_m3):
+    """[M, 3] coordinates -> BART trajectory memory [3, M] complex64."""
+    t = np.asarray(traj_m3)
+    return np.ascontiguousarray(t.reshape(-1, 3).astype(np.complex64))
+
+
+def nufft_create2(ksp_dims, cim_dims, traj_dims, traj, wgh_dims=None, weights=None, conf=None):
+    """noncart/nufft.h:48-57."""
+    conf = NufftConf.defaults() if conf is None else conf
+    traj = _host_c64(traj) if not _is_torch(traj) else traj
+    w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
+    if weights is not None and wgh_dims is None:
+        wgh_dims = [1] + list(traj_dims[1:])
+    h = lib().nufft_create2(DIMS, _dims(ksp_dims), _dims(cim_dims), _dims(traj_dims), _ptr(traj),
+                            _dims(wgh_dims) if wgh_dims is not None else None, _ptr(w), None, None, conf)
+    return Linop(h, ksp_dims, cim_dims)
+
+
+def nufft_create(ksp_dims, cim_dims, traj_dims, traj, weights=None, conf=None):
+    """noncart/nufft.h:40-46."""
+    conf = NufftConf.defaults() if conf is None else conf
+    traj = _host_c64(traj) if not _is_torch(traj) else traj
+    w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
+    h = lib().nufft_create(DIMS, _dims(ksp_dims), _dims(cim_dims), _dims(traj_dims), _ptr(traj), _ptr(w), conf)
+    return Linop(h, ksp_dims, cim_dims)
+
+
+def sense_nc_init(max_dims, map_dims, maps, ksp_dims, traj_dims, traj, conf=None, wgs_dims=None, weights=None):
+    """sense/modelnc.c:28-67: A = NUFFT S, image -> k-space."""
+    conf = NufftConf.defaults() if conf is None else conf
+    maps = _host_c64(maps) if not _is_torch(maps) else maps
+    traj = _host_c64(traj) if not _is_torch(traj) else traj
+    w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
+    if weights is not None and wgs_dims is None:
+        wgs_dims = [1] + list(traj_dims[1:3]) + [1] * 13
+    h = lib().sense_nc_init(_dims(max_dims), _dims(map_dims), _ptr(maps), _dims(ksp_dims), _dims(traj_dims),
+                            _ptr(traj), ctypes.byref(conf), _dims(wgs_dims) if wgs_dims is not None else None,
+                            _ptr(w), None, None, None, 0)
+    img_dims = list(max_dims) + [1] * (DIMS - len(max_dims))
+    img_dims[3] = 1
+    img_dims[4] = 1
+    return Linop(h, ksp_dims, img_dims)
+
+
+def lsqr_conjgrad(op, b, maxiter, lam=0.0, tol=0.0, x0=None, fallback=False):
+    """CG on (A^H A + lam I) x = b.  Returns (x, stats, residual trace)."""
+    b = _host_c64(b)
+    warm = x0 is not None
+    if warm:
+        x = x0.clone() if _is_torch(x0) else np.array(_host_c64(x0), copy=True)
+    else:
+        x = _like(b, op.idims)
+    trace = np.zeros(max(1, maxiter), np.float32)
+    st = CgStats()
+    it = lib().lsqr_conjgrad(op.h, maxiter, lam, tol, DIMS, _dims(op.idims), _ptr(x), _ptr(b), warm, fallback,
+                             trace.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
+    _check()
+    return x, st, trace[:max(0, it)]
+
+
+def fft(dims, flags, x, inverse=False):
+    """num/fft.c:387-417 (unnormalised, -1 forward / +1 inverse)."""
+    x = _host_c64(x)
+    out = _like(x, dims)
+    f = lib().ifft if inverse else lib().fft
+    f(len(dims), _dims(dims, len(dims)), flags, _ptr(out), _ptr(x))
+    _check()
+    return out
+
+
+def ifft(dims, flags, x):
+    return fft(dims, flags, x, inverse=True)
+
+
+def version():
+    return lib().bart_amd_version().decode()

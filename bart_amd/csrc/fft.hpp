@@ -1,0 +1,425 @@
+// fft.hpp -- LDS Stockham FFT along one axis of a stack of volumes (gfx950).
+//
+// This replaces the FFTW / cuFFT calls of BART (num/fft_plan.c:188-281,
+// num/fft-cuda.c) with a hand-written kernel whose load, middle and store
+// stages are functors, so that element-wise work before, between and after
+// the transforms (deapodisation, linear phases, fftmod, coil maps, the
+// Toeplitz PSF, the sum over sub-grids) is fused into the FFT passes.
+//
+// Convention (FFTW's, as used by BART): unnormalised, sign -1 forward,
+// +1 backward.
+//
+// One work-group (256 threads = 4 wave64) holds P transforms of length n in
+// LDS (P * n <= 8192 complex).  Each Stockham pass reads R strided values per
+// butterfly into registers, applies twiddles from an FP64-built table, does
+// a radix-R DFT and writes back in auto-sorted order.
+#pragma once
+
+#include "common.hpp"
+
+namespace bam {
+
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_MAX_ELEMS = 8192;
+constexpr int FFT_MAX_PASSES = 16;
+
+struct AxisPlan {
+
+	int n = 1;
+	int nr = 0;
+	int radix[FFT_MAX_PASSES] = {};
+	const float2* tw = nullptr;  // exp(-2 pi i k / n), k < n
+};
+
+// host: radix plan + shared device twiddle table for length n
+AxisPlan make_axis_plan(int n);
+bool fft_size_supported(int n);
+
+// Geometry of the transforms of one launch.
+//   contig: transforms are contiguous rows (element stride 1); the P
+//           transforms of a block are consecutive rows.
+//   otherwise: element stride > 1; the P transforms of a block are adjacent
+//           columns, so loads of the same k are coalesced.
+struct FftLaunch {
+
+	AxisPlan plan;
+	int P = 1;
+	int logP = 0;
+	bool contig = true;
+	long total = 0;      // number of transforms
+	float sign1 = -1.f;  // first transform
+	float sign2 = +1.f;  // second transform (only when a middle stage is present)
+	const int* skip = nullptr;  // device flag: when set (non-zero) the launch does nothing
+};
+
+// ---------------------------------------------------------------------------
+// radix-R DFT in registers, w = exp(sgn 2 pi i / R)
+// ---------------------------------------------------------------------------
+
+__device__ inline float2 mul_i(float2 a, float sgn)  // a * (sgn i)
+{
+	return make_float2(-sgn * a.y, sgn * a.x);
+}
+
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+	__device__ static void run(float2* v, float) {
+		float2 a = v[0], b = v[1];
+		v[0] = cadd(a, b);
+		v[1] = csub(a, b);
+	}
+};
+
+template <> struct Dft<3> {
+	__device__ static void run(float2* v, float sgn) {
+		const float c = -0.5f;
+		const float s = 0.86602540378443864676f;
+		float2 t1 = cadd(v[1], v[2]);
+		float2 t2 = make_float2(v[0].x + c * t1.x, v[0].y + c * t1.y);
+		float2 t3 = mul_i(cscale(csub(v[1], v[2]), s), sgn);
+		v[0] = cadd(v[0], t1);
+		v[1] = cadd(t2, t3);
+		v[2] = csub(t2, t3);
+	}
+};
+
+template <> struct Dft<4> {
+	__device__ static void run(float2* v, float sgn) {
+		float2 a0 = cadd(v[0], v[2]);
+		float2 a1 = csub(v[0], v[2]);
+		float2 a2 = cadd(v[1], v[3]);
+		float2 a3 = mul_i(csub(v[1], v[3]), sgn);
+		v[0] = cadd(a0, a2);
+		v[1] = cadd(a1, a3);
+		v[2] = csub(a0, a2);
+		v[3] = csub(a1, a3);
+	}
+};
+
+template <> struct Dft<5> {
+	__device__ static void run(float2* v, float sgn) {
+		const float c1 = 0.30901699437494742410f;   // cos(2 pi / 5)
+		const float c2 = -0.80901699437494742410f;  // cos(4 pi / 5)
+		const float s1 = 0.95105651629515357212f;   // sin(2 pi / 5)
+		const float s2 = 0.58778525229247312917f;   // sin(4 pi / 5)
+		float2 a = v[0];
+		float2 t1 = cadd(v[1], v[4]);
+		float2 t2 = cadd(v[2], v[3]);
+		float2 t3 = csub(v[1], v[4]);
+		float2 t4 = csub(v[2], v[3]);
+		float2 r1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+		float2 r2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+		float2 i1 = mul_i(make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), sgn);
+		float2 i2 = mul_i(make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), sgn);
+		v[0] = cadd(a, cadd(t1, t2));
+		v[1] = cadd(r1, i1);
+		v[4] = csub(r1, i1);
+		v[2] = cadd(r2, i2);
+		v[3] = csub(r2, i2);
+	}
+};
+
+template <> struct Dft<8> {
+	__device__ static void run(float2* v, float sgn) {
+		float2 e[4] = { v[0], v[2], v[4], v[6] };
+		float2 o[4] = { v[1], v[3], v[5], v[7] };
+		Dft<4>::run(e, sgn);
+		Dft<4>::run(o, sgn);
+		const float r = 0.70710678118654752440f;
+		// w8^1 = r (1 + sgn i), w8^2 = sgn i, w8^3 = r (-1 + sgn i)
+		float2 o1 = make_float2(r * (o[1].x - sgn * o[1].y), r * (o[1].y + sgn * o[1].x));
+		float2 o2 = mul_i(o[2], sgn);
+		float2 o3 = make_float2(r * (-o[3].x - sgn * o[3].y), r * (-o[3].y + sgn * o[3].x));
+		v[0] = cadd(e[0], o[0]);
+		v[4] = csub(e[0], o[0]);
+		v[1] = cadd(e[1], o1);
+		v[5] = csub(e[1], o1);
+		v[2] = cadd(e[2], o2);
+		v[6] = csub(e[2], o2);
+		v[3] = cadd(e[3], o3);
+		v[7] = csub(e[3], o3);
+	}
+};
+
+// Generic odd prime radix (7, 11, 13): direct DFT with table twiddles.
+template <int R> struct DftGeneric {
+	__device__ static void run(float2* v, float sgn, const float2* tw, int n) {
+		float2 o[R];
+		const int step = n / R;
+#pragma unroll
+		for (int q = 0; q < R; q++) {
+			float2 s = v[0];
+#pragma unroll
+			for (int r = 1; r < R; r++) {
+				float2 w = tw[((r * q) % R) * step];
+				if (sgn > 0.f)
+					w.y = -w.y;
+				s = cadd(s, cmul(v[r], w));
+			}
+			o[q] = s;
+		}
+#pragma unroll
+		for (int q = 0; q < R; q++)
+			v[q] = o[q];
+	}
+};
+
+template <int R> __device__ inline void dft_any(float2* v, float sgn, const float2* tw, int n)
+{
+	if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8)
+		Dft<R>::run(v, sgn);
+	else
+		DftGeneric<R>::run(v, sgn, tw, n);
+}
+
+// One Stockham pass of radix R over P transforms of length n in LDS.
+// EPT = upper bound of elements per thread (P * n / FFT_THREADS).
+template <int R, int EPT>
+__device__ inline void stockham_pass(float2* lds, int n, int P, int Ns, const float2* tw, float sgn)
+{
+	constexpr int NB = (EPT + R - 1) / R;
+	const int nR = n / R;
+	const int nb = P * nR;
+	const int tstep = n / (Ns * R);
+	float2 v[NB][R];
+	int dst[NB];
+
+#pragma unroll
+	for (int i = 0; i < NB; i++) {
+
+		int b = threadIdx.x + i * FFT_THREADS;
+		dst[i] = -1;
+
+		if (b < nb) {
+
+			int p = b / nR;
+			int j = b - p * nR;
+			int k = j % Ns;
+			const float2* src = lds + p * n + j;
+
+#pragma unroll
+			for (int r = 0; r < R; r++) {
+
+				float2 x = src[r * nR];
+
+				if (r > 0 && k > 0) {
+
+					float2 w = tw[r * k * tstep];
+					if (sgn > 0.f)
+						w.y = -w.y;
+					x = cmul(x, w);
+				}
+
+				v[i][r] = x;
+			}
+
+			dft_any<R>(v[i], sgn, tw, n);
+			dst[i] = p * n + (j / Ns) * Ns * R + k;
+		}
+	}
+
+	__syncthreads();
+
+#pragma unroll
+	for (int i = 0; i < NB; i++)
+		if (dst[i] >= 0)
+#pragma unroll
+			for (int q = 0; q < R; q++)
+				lds[dst[i] + q * Ns] = v[i][q];
+
+	__syncthreads();
+}
+
+// GEN: also instantiate the generic odd-prime radices (7, 11, 13), which
+// cost registers; lengths of the form 2^a 3^b 5^c use GEN = false.
+template <int EPT, bool GEN>
+__device__ inline void run_passes(float2* lds, const AxisPlan& plan, int P, float sgn)
+{
+	int Ns = 1;
+
+	for (int ip = 0; ip < plan.nr; ip++) {
+
+		int R = plan.radix[ip];
+
+		switch (R) {
+		case 2: stockham_pass<2, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+		case 3: stockham_pass<3, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+		case 4: stockham_pass<4, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+		case 5: stockham_pass<5, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+		case 8: stockham_pass<8, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+		default:
+			if constexpr (GEN) {
+				switch (R) {
+				case 7: stockham_pass<7, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+				case 11: stockham_pass<11, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+				case 13: stockham_pass<13, EPT>(lds, plan.n, P, Ns, plan.tw, sgn); break;
+				default: break;
+				}
+			}
+			break;
+		}
+
+		Ns *= R;
+	}
+}
+
+// Identity middle stage.
+struct NoMid {
+	static constexpr bool active = false;
+	__device__ float2 operator()(long, int, float2 v) const { return v; }
+};
+
+// ---------------------------------------------------------------------------
+// The axis kernel.  Ld: float2 operator()(long t, int k) const.
+// Md: float2 operator()(long t, int k, float2 v) const (when active).
+// St: void store(const float2* lds, const FftLaunch&, long t0) const -- runs
+//     the block's whole store loop so that it may reduce over transforms.
+// ---------------------------------------------------------------------------
+
+template <int EPT, bool GEN, class Ld, class Md, class St>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftLaunch L, Ld ld, Md md, St st)
+{
+	extern __shared__ float2 lds[];
+
+	if (nullptr != L.skip && 0 != *L.skip)
+		return;
+
+	const int n = L.plan.n;
+	const int P = L.P;
+	const long t0 = (long)blockIdx.x * P;
+	const int tot = P * n;
+
+	if (L.contig) {
+
+		int p = threadIdx.x / n;
+		int k = threadIdx.x - p * n;
+		const int dp = FFT_THREADS / n;       // 0 when n > FFT_THREADS
+		const int dk = FFT_THREADS - dp * n;
+
+		for (int i = threadIdx.x; i < tot; i += FFT_THREADS) {
+
+			long t = t0 + p;
+			lds[p * n + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+			p += dp;
+			k += dk;
+			while (k >= n) {
+				k -= n;
+				p++;
+			}
+		}
+
+	} else {
+
+		for (int i = threadIdx.x; i < tot; i += FFT_THREADS) {
+
+			int p = i & (P - 1);
+			int k = i >> L.logP;
+			long t = t0 + p;
+			lds[p * n + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+		}
+	}
+
+	__syncthreads();
+
+	run_passes<EPT, GEN>(lds, L.plan, P, L.sign1);
+
+	if constexpr (Md::active) {
+
+		for (int i = threadIdx.x; i < tot; i += FFT_THREADS) {
+
+			int p, k;
+
+			if (L.contig) {
+				p = i / n;
+				k = i - p * n;
+			} else {
+				p = i & (P - 1);
+				k = i >> L.logP;
+			}
+
+			long t = t0 + p;
+
+			if (t < L.total)
+				lds[p * n + k] = md(t, k, lds[p * n + k]);
+		}
+
+		__syncthreads();
+
+		run_passes<EPT, GEN>(lds, L.plan, P, L.sign2);
+	}
+
+	st.store(lds, L, t0);
+}
+
+// Helper for element-wise stores: calls f(t, k, value) for every element of
+// the block in an order that is coalesced for the launch's geometry.
+template <class F>
+__device__ inline void store_each(const float2* lds, const FftLaunch& L, long t0, const F& f)
+{
+	const int n = L.plan.n;
+	const int P = L.P;
+	const int tot = P * n;
+
+	for (int i = threadIdx.x; i < tot; i += FFT_THREADS) {
+
+		int p, k;
+
+		if (L.contig) {
+			p = i / n;
+			k = i - p * n;
+		} else {
+			p = i & (P - 1);
+			k = i >> L.logP;
+		}
+
+		long t = t0 + p;
+
+		if (t < L.total)
+			f(t, k, lds[p * n + k]);
+	}
+}
+
+// Launch helper: picks P and EPT for the transform length and launches.
+// P_min: minimum transforms per block (e.g. the group size of a reducing
+// store); `contig` selects the geometry.
+struct FftLaunchCfg {
+	FftLaunch L;
+	unsigned blocks;
+	size_t lds_bytes;
+	int ept;
+};
+
+FftLaunchCfg fft_config(const AxisPlan& plan, long total, bool contig, int P_min, float sign1, float sign2);
+
+template <class Ld, class Md, class St>
+void launch_fft_axis(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, const Md& md, const St& st)
+{
+	if (0 == cfg.blocks)
+		return;
+
+	bool gen = false;
+
+	for (int i = 0; i < cfg.L.plan.nr; i++)
+		gen = gen || (cfg.L.plan.radix[i] > 8 || 7 == cfg.L.plan.radix[i]);
+
+	const dim3 gb(cfg.blocks), tb(FFT_THREADS);
+
+	if (gen) {
+		switch (cfg.ept) {
+		case 8: hipLaunchKernelGGL((k_fft_axis<8, true, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break;
+		case 16: hipLaunchKernelGGL((k_fft_axis<16, true, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break;
+		default: hipLaunchKernelGGL((k_fft_axis<32, true, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break;
+		}
+	} else {
+		switch (cfg.ept) {
+		case 8: hipLaunchKernelGGL((k_fft_axis<8, false, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break;
+		case 16: hipLaunchKernelGGL((k_fft_axis<16, false, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break;
+		default: hipLaunchKernelGGL((k_fft_axis<32, false, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break;
+		}
+	}
+
+	BAM_HIP(hipGetLastError());
+}
+
+}  // namespace bam

@@ -1,0 +1,631 @@
+// linop.hip -- the C ABI (include/bart_amd.h): BART's linop surface over the
+// gfx950 NUFFT plan (linops/linop.c, noncart/nufft.c:1174-1327,
+// sense/modelnc.c:28-67), FFT entry points and the CG solver.
+#include "../../include/bart_amd.h"
+
+#include "cg.hpp"
+#include "nufft_impl.hpp"
+
+#include <atomic>
+#include <cstdarg>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+namespace bam {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static std::atomic<int> error_mode{ BART_AMD_ERRORS_EXIT };
+static thread_local std::string last_error;
+static thread_local bool has_error = false;
+
+void fail(const char* fmt, ...)
+{
+	char buf[1024];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+
+	if (BART_AMD_ERRORS_EXIT == error_mode.load()) {
+		fprintf(stderr, "Error: %s\n", buf);
+		exit(1);
+	}
+
+	throw Error{ buf };
+}
+
+template <class F> static auto guarded(F&& f, decltype(f()) on_error) -> decltype(f())
+{
+	try {
+		return f();
+	} catch (const Error& e) {
+		last_error = e.msg;
+		has_error = true;
+	} catch (const std::exception& e) {
+		last_error = e.what();
+		has_error = true;
+		if (BART_AMD_ERRORS_EXIT == error_mode.load()) {
+			fprintf(stderr, "Error: %s\n", e.what());
+			exit(1);
+		}
+	}
+	return on_error;
+}
+
+template <class F> static void guarded_void(F&& f)
+{
+	guarded([&]() { f(); return 0; }, 0);
+}
+
+hipStream_t default_stream() { return 0; }
+
+// ---------------------------------------------------------------------------
+// pointer location (BART: cuda_ondevice, num/gpuops.c:445)
+// ---------------------------------------------------------------------------
+
+static bool on_device(const void* p)
+{
+	if (nullptr == p)
+		return false;
+
+	hipPointerAttribute_t attr;
+	hipError_t e = hipPointerGetAttributes(&attr, p);
+
+	if (hipSuccess != e) {
+		(void)hipGetLastError();
+		return false;
+	}
+
+	return (hipMemoryTypeDevice == attr.type) || (hipMemoryTypeManaged == attr.type);
+}
+
+// ---------------------------------------------------------------------------
+// operator handle
+// ---------------------------------------------------------------------------
+
+enum OpKind { OP_NUFFT = 1, OP_SENSE = 2 };
+
+struct Op {
+
+	int kind = 0;
+	int N = 0;
+	long odims[32];
+	long idims[32];
+	long osize = 0, isize = 0;
+	std::shared_ptr<Nufft> nufft;
+	std::shared_ptr<DevBuf<float2>> sens;
+	hipStream_t stream = 0;
+	std::mutex mtx;
+	DevBuf<float2> stage_in, stage_out, kbuf;
+	std::unique_ptr<CgSolver> cg;
+
+	~Op()
+	{
+		cg.reset();
+		if (0 != stream)
+			(void)hipStreamDestroy(stream);
+	}
+};
+
+static long md_size(int N, const long* d)
+{
+	long s = 1;
+	for (int i = 0; i < N; i++)
+		s *= d[i];
+	return s;
+}
+
+static void check_dims(const char* what, int N, const long* a, int M, const long* b)
+{
+	BAM_CHECK(N == M, "%s: dimension count %d != %d", what, N, M);
+
+	for (int i = 0; i < N; i++)
+		BAM_CHECK(a[i] == b[i], "%s: dims differ in dim %d (%ld != %ld)", what, i, a[i], b[i]);
+}
+
+// copy of a host or device array into host memory
+template <class T> static std::vector<T> to_host(const T* p, long n)
+{
+	std::vector<T> h(n);
+
+	if (n > 0) {
+		if (on_device(p))
+			BAM_HIP(hipMemcpy(h.data(), p, sizeof(T) * n, hipMemcpyDeviceToHost));
+		else
+			memcpy(h.data(), p, sizeof(T) * n);
+	}
+
+	return h;
+}
+
+// Applies f(dst_dev, src_dev) with staging of host pointers and a final sync.
+template <class F>
+static void apply_staged(Op* op, float2* dst, long dn, const float2* src, long sn, F&& f)
+{
+	std::lock_guard<std::mutex> lock(op->mtx);
+
+	bool dd = on_device(dst);
+	bool sd = on_device(src);
+
+	const float2* s = src;
+	float2* d = dst;
+
+	if (!sd) {
+		if ((long)op->stage_in.n < sn)
+			op->stage_in.alloc(sn);
+		BAM_HIP(hipMemcpyAsync(op->stage_in.ptr, src, sizeof(float2) * sn, hipMemcpyHostToDevice, op->stream));
+		s = op->stage_in.ptr;
+	}
+
+	if (!dd) {
+		if ((long)op->stage_out.n < dn)
+			op->stage_out.alloc(dn);
+		d = op->stage_out.ptr;
+	}
+
+	f(d, s);
+
+	if (!dd)
+		BAM_HIP(hipMemcpyAsync(dst, d, sizeof(float2) * dn, hipMemcpyDeviceToHost, op->stream));
+
+	BAM_HIP(hipStreamSynchronize(op->stream));
+}
+
+// Parse BART dims for a NUFFT: image dims 0-2, coils dim 3, samples dims 1-2
+// of ksp/traj (noncart/nufft.c:1208-1238 checks).
+struct NufftDims {
+	int img[3];
+	int C;
+	long M;
+};
+
+static NufftDims parse_nufft_dims(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims)
+{
+	BAM_CHECK(N >= 4 && N <= 32, "nufft: unsupported dimension count %d", N);
+	BAM_CHECK(3 == traj_dims[0], "nufft: trajectory dim 0 must be 3 (got %ld)", traj_dims[0]);
+	BAM_CHECK(1 == ksp_dims[0], "nufft: k-space dim 0 must be 1");
+	BAM_CHECK(ksp_dims[1] == traj_dims[1] && ksp_dims[2] == traj_dims[2],
+		  "Incompatible dimensions of k-space and trajectory!");
+	BAM_CHECK(1 == traj_dims[3], "nufft: trajectory must not have a coil dim");
+	BAM_CHECK(ksp_dims[3] == cim_dims[3], "nufft: coil dims of k-space (%ld) and image (%ld) differ",
+		  ksp_dims[3], cim_dims[3]);
+
+	for (int i = 4; i < N; i++)
+		BAM_CHECK(1 == ksp_dims[i] && 1 == cim_dims[i] && 1 == traj_dims[i],
+			  "nufft: dim %d must be 1 (batching over dims >= 4 is not supported)", i);
+
+	NufftDims r;
+	for (int i = 0; i < 3; i++) {
+		BAM_CHECK(cim_dims[i] >= 1 && cim_dims[i] < (1l << 20), "nufft: bad image dim");
+		r.img[i] = (int)cim_dims[i];
+	}
+	r.C = (int)cim_dims[3];
+	r.M = ksp_dims[1] * ksp_dims[2];
+	BAM_CHECK(r.C >= 1, "nufft: no coils");
+	return r;
+}
+
+static void check_conf(const nufft_conf_s& conf, bool has_basis)
+{
+	BAM_CHECK(!has_basis, "nufft: subspace basis is not supported by this library");
+	BAM_CHECK(2.f == conf.os, "nufft: only oversampling 2 is supported (chained NUFFT not implemented)");
+	BAM_CHECK(conf.decomp, "nufft: the non-decomposed oversampled grid (-1) is not supported");
+	BAM_CHECK(!conf.pcycle, "nufft: phase cycling is not supported");
+	BAM_CHECK(!conf.real && !conf.compress_psf && !conf.upper_triag,
+		  "nufft: real / compressed / upper-triangular PSF storage is not supported");
+	BAM_CHECK(0 == conf.cfft, "nufft: cfft flags are not supported");
+	BAM_CHECK(7 == (conf.flags & 7), "nufft: flags must include dims 0-2");
+}
+
+// weights of shape wgh_dims (broadcast over coils) -> per-sample vector
+static std::vector<float2> expand_weights(int N, const long* wgh_dims, const float2* w, const long* ksp_dims)
+{
+	std::vector<float2> out;
+
+	if (nullptr == w)
+		return out;
+
+	long R = ksp_dims[1], Sp = ksp_dims[2];
+	long wr = wgh_dims[1], ws = wgh_dims[2];
+
+	BAM_CHECK(1 == wgh_dims[0], "nufft: weights dim 0 must be 1");
+	BAM_CHECK((wr == R || wr == 1) && (ws == Sp || ws == 1), "Incompatible dimensions of k-space and weights!");
+
+	for (int i = 3; i < N; i++)
+		BAM_CHECK(1 == wgh_dims[i], "nufft: weights must not vary over dims >= 3");
+
+	std::vector<float2> h = to_host(w, wr * ws);
+	out.resize(R * Sp);
+
+	for (long s = 0; s < Sp; s++)
+		for (long r = 0; r < R; r++)
+			out[r + R * s] = h[(wr > 1 ? r : 0) + wr * (ws > 1 ? s : 0)];
+
+	return out;
+}
+
+static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims, const float2* traj,
+			 const long* wgh_dims, const float2* weights, const nufft_conf_s& conf)
+{
+	NufftDims nd = parse_nufft_dims(N, ksp_dims, cim_dims, traj_dims);
+
+	std::vector<float2> th = to_host(traj, 3 * nd.M);
+	std::vector<float> tr(3 * nd.M);
+
+	for (long i = 0; i < 3 * nd.M; i++)
+		tr[i] = th[i].x;
+
+	std::vector<float2> w = expand_weights(N, wgh_dims, weights, ksp_dims);
+
+	auto op = new Op();
+	op->kind = OP_NUFFT;
+	op->N = N;
+	memcpy(op->odims, ksp_dims, sizeof(long) * N);
+	memcpy(op->idims, cim_dims, sizeof(long) * N);
+	op->osize = md_size(N, ksp_dims);
+	op->isize = md_size(N, cim_dims);
+	BAM_HIP(hipStreamCreate(&op->stream));
+
+	try {
+		op->nufft = std::make_shared<Nufft>();
+		op->nufft->stream = op->stream;
+		op->nufft->create(nd.img, nd.C, nd.M, tr.data(), w.empty() ? nullptr : w.data(), conf.toeplitz,
+				  conf.periodic, conf.width, conf.os, conf.toeplitz && !conf.nopsf);
+	} catch (...) {
+		delete op;
+		throw;
+	}
+
+	return op;
+}
+
+}  // namespace bam
+
+using namespace bam;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const struct nufft_conf_s nufft_conf_defaults = {
+	.toeplitz = true,
+	.pcycle = false,
+	.periodic = false,
+	.lowmem = false,
+	.flags = 7,
+	.cfft = 0u,
+	.decomp = true,
+	.nopsf = false,
+	.upper_triag = false,
+	.real = false,
+	.compress_psf = false,
+	.decomposed_psf = false,
+	.precomp = true,
+	.precomp_linphase = true,
+	.precomp_fftmod = true,
+	.precomp_roll = true,
+	.zero_overhead = false,
+	.width = 6,
+	.os = 2.,
+};
+
+void bart_amd_set_error_mode(int mode) { error_mode.store(mode); }
+
+const char* bart_amd_last_error(void) { return has_error ? last_error.c_str() : nullptr; }
+
+void bart_amd_clear_error(void)
+{
+	has_error = false;
+	last_error.clear();
+}
+
+const char* bart_amd_version(void) { return "bart_amd 0.1 (gfx950)"; }
+
+struct linop_s* nufft_create2(int N, const long ksp_dims[], const long cim_dims[], const long traj_dims[],
+			      const float _Complex* traj, const long wgh_dims[], const float _Complex* weights,
+			      const long bas_dims[], const float _Complex* basis, struct nufft_conf_s conf)
+{
+	(void)bas_dims;
+	return guarded([&]() -> struct linop_s* {
+		check_conf(conf, nullptr != basis);
+		BAM_CHECK(nullptr != traj, "nufft: trajectory is NULL");
+		Op* op = make_nufft_op(N, ksp_dims, cim_dims, traj_dims, (const float2*)traj, wgh_dims,
+				       (const float2*)weights, conf);
+		return (struct linop_s*)op;
+	}, nullptr);
+}
+
+struct linop_s* nufft_create(int N, const long ksp_dims[], const long cim_dims[], const long traj_dims[],
+			     const float _Complex* traj, const float _Complex* weights, struct nufft_conf_s conf)
+{
+	long wgh_dims[32];
+
+	for (int i = 0; i < N && i < 32; i++)
+		wgh_dims[i] = (0 == i) ? 1 : traj_dims[i];
+
+	return nufft_create2(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, nullptr, nullptr, conf);
+}
+
+const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims[16], const float _Complex* maps,
+				    const long ksp_dims[16], const long traj_dims[16], const float _Complex* traj,
+				    const struct nufft_conf_s* _conf, const long wgs_dims[16],
+				    const float _Complex* weights, const long basis_dims[16],
+				    const float _Complex* basis, const struct linop_s** fft_opp,
+				    unsigned long shared_img_dims)
+{
+	(void)basis_dims;
+	return guarded([&]() -> const struct linop_s* {
+		const int N = 16;
+		nufft_conf_s conf = *_conf;
+		check_conf(conf, nullptr != basis);
+		BAM_CHECK(0 == shared_img_dims, "sense_nc_init: shared image dims are not supported");
+		BAM_CHECK(nullptr == fft_opp, "sense_nc_init: fft_opp is not supported");
+		BAM_CHECK(1 == max_dims[4] && 1 == map_dims[4], "sense_nc_init: only one set of maps is supported");
+
+		long coilim_dims[16], img_dims[16];
+
+		for (int i = 0; i < N; i++) {
+			coilim_dims[i] = (4 == i) ? 1 : max_dims[i];
+			img_dims[i] = (3 == i || 4 == i) ? 1 : max_dims[i];
+		}
+
+		for (int i = 0; i < 4; i++)
+			BAM_CHECK(map_dims[i] == coilim_dims[i], "sense_nc_init: map dims do not match (dim %d)", i);
+
+		long ksp2[16];
+		memcpy(ksp2, ksp_dims, sizeof(ksp2));
+
+		Op* op = make_nufft_op(N, ksp2, coilim_dims, traj_dims, (const float2*)traj, wgs_dims,
+				       (const float2*)weights, conf);
+		op->kind = OP_SENSE;
+		memcpy(op->idims, img_dims, sizeof(long) * N);
+		op->isize = md_size(N, img_dims);
+
+		long nmap = md_size(4, map_dims);
+
+		try {
+			op->sens = std::make_shared<DevBuf<float2>>(nmap);
+
+			if (on_device(maps))
+				BAM_HIP(hipMemcpy(op->sens->ptr, maps, sizeof(float2) * nmap, hipMemcpyDeviceToDevice));
+			else
+				BAM_HIP(hipMemcpy(op->sens->ptr, maps, sizeof(float2) * nmap, hipMemcpyHostToDevice));
+		} catch (...) {
+			delete op;
+			throw;
+		}
+
+		return (const struct linop_s*)op;
+	}, nullptr);
+}
+
+void linop_free(const struct linop_s* _op)
+{
+	guarded_void([&]() {
+		Op* op = (Op*)_op;
+		if (nullptr != op) {
+			BAM_HIP(hipStreamSynchronize(op->stream));
+			delete op;
+		}
+	});
+}
+
+void linop_forward(const struct linop_s* _op, int DN, const long ddims[], float _Complex* dst, int SN,
+		   const long sdims[], const float _Complex* src)
+{
+	guarded_void([&]() {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "linop_forward: NULL operator");
+		check_dims("linop_forward (output)", DN, ddims, op->N, op->odims);
+		check_dims("linop_forward (input)", SN, sdims, op->N, op->idims);
+
+		apply_staged(op, (float2*)dst, op->osize, (const float2*)src, op->isize, [&](float2* d, const float2* s) {
+			op->nufft->forward(d, s, (OP_SENSE == op->kind) ? op->sens->ptr : nullptr);
+		});
+	});
+}
+
+void linop_adjoint(const struct linop_s* _op, int DN, const long ddims[], float _Complex* dst, int SN,
+		   const long sdims[], const float _Complex* src)
+{
+	guarded_void([&]() {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "linop_adjoint: NULL operator");
+		check_dims("linop_adjoint (output)", DN, ddims, op->N, op->idims);
+		check_dims("linop_adjoint (input)", SN, sdims, op->N, op->odims);
+
+		apply_staged(op, (float2*)dst, op->isize, (const float2*)src, op->osize, [&](float2* d, const float2* s) {
+			op->nufft->adjoint(d, s, (OP_SENSE == op->kind) ? op->sens->ptr : nullptr);
+		});
+	});
+}
+
+void linop_normal(const struct linop_s* _op, int N, const long dims[], float _Complex* dst, const float _Complex* src)
+{
+	guarded_void([&]() {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "linop_normal: NULL operator");
+		check_dims("linop_normal", N, dims, op->N, op->idims);
+
+		apply_staged(op, (float2*)dst, op->isize, (const float2*)src, op->isize, [&](float2* d, const float2* s) {
+			const float2* sens = (OP_SENSE == op->kind) ? op->sens->ptr : nullptr;
+
+			if (op->nufft->toeplitz) {
+				op->nufft->normal(d, s, sens);
+			} else {
+				// no normal operator: adjoint(forward(x)) (linops/linop.c:489-525)
+				if ((long)op->kbuf.n < op->osize)
+					op->kbuf.alloc(op->osize);
+				op->nufft->forward(op->kbuf.ptr, s, sens);
+				op->nufft->adjoint(d, op->kbuf.ptr, sens);
+			}
+		});
+	});
+}
+
+int nufft_get_psf_dims(const struct linop_s* _op, int N, long psf_dims[])
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op && op->nufft->has_psf, "nufft_get_psf_dims: operator has no PSF");
+
+		if (N > 0) {
+			BAM_CHECK(N == op->N + 1, "nufft_get_psf_dims: need %d dims", op->N + 1);
+			for (int i = 0; i < op->N; i++)
+				psf_dims[i] = (i < 3) ? op->nufft->n[i] : 1;
+			psf_dims[op->N] = op->nufft->S;
+		}
+
+		return op->N + 1;
+	}, -1);
+}
+
+void nufft_get_psf(const struct linop_s* _op, int N, const long psf_dims[], float _Complex* psf)
+{
+	guarded_void([&]() {
+		Op* op = (Op*)_op;
+		long want[33];
+		BAM_CHECK(N == nufft_get_psf_dims(_op, 0, nullptr), "nufft_get_psf: dims");
+		nufft_get_psf_dims(_op, N, want);
+		check_dims("nufft_get_psf", N, psf_dims, N, want);
+		long n = (long)op->nufft->S * op->nufft->N;
+		BAM_HIP(hipMemcpy(psf, op->nufft->psf.ptr, sizeof(float2) * n,
+				  on_device(psf) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+	});
+}
+
+void nufft_update_psf(const struct linop_s* _op, int ND, const long psf_dims[], const float _Complex* psf)
+{
+	guarded_void([&]() {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "nufft_update_psf: NULL operator");
+		BAM_CHECK(op->nufft->toeplitz, "nufft_update_psf: operator is not in Toeplitz mode");
+		long want[33];
+		BAM_CHECK(ND == op->N + 1, "nufft_update_psf: dims");
+		for (int i = 0; i < op->N; i++)
+			want[i] = (i < 3) ? op->nufft->n[i] : 1;
+		want[op->N] = op->nufft->S;
+		check_dims("nufft_update_psf", ND, psf_dims, ND, want);
+		long n = (long)op->nufft->S * op->nufft->N;
+		if ((long)op->nufft->psf.n < n)
+			op->nufft->psf.alloc(n);
+		BAM_HIP(hipMemcpy(op->nufft->psf.ptr, psf, sizeof(float2) * n,
+				  on_device(psf) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+		op->nufft->has_psf = true;
+		if (op->cg)
+			op->cg->drop_graph();
+	});
+}
+
+static void fft_common(int D, const long dims[], unsigned long flags, float _Complex* dst, const float _Complex* src,
+		       float sign)
+{
+	guarded_void([&]() {
+		BAM_CHECK(D >= 1, "fft: bad dims");
+		BAM_CHECK(0 == (flags & ~7ul), "fft: only dims 0-2 can be transformed");
+		int d3[3] = { 1, 1, 1 };
+		for (int i = 0; i < D && i < 3; i++)
+			d3[i] = (int)dims[i];
+		long batch = 1;
+		for (int i = 3; i < D; i++)
+			batch *= dims[i];
+		long n = (long)d3[0] * d3[1] * d3[2] * batch;
+
+		for (int a = 0; a < 3; a++)
+			if ((flags & (1ul << a)) && d3[a] > 1)
+				BAM_CHECK(fft_size_supported(d3[a]), "fft: length %d not supported", d3[a]);
+
+		bool dd = on_device(dst), sd = on_device(src);
+		hipStream_t st = 0;
+
+		if (dd && sd) {
+			fft_nd(d3, batch, (unsigned)flags, sign, (float2*)dst, (const float2*)src, 1.f, st);
+			BAM_HIP(hipStreamSynchronize(st));
+			return;
+		}
+
+		DevBuf<float2> buf(n);
+		BAM_HIP(hipMemcpy(buf.ptr, src, sizeof(float2) * n, sd ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+		fft_nd(d3, batch, (unsigned)flags, sign, buf.ptr, buf.ptr, 1.f, st);
+		BAM_HIP(hipMemcpy(dst, buf.ptr, sizeof(float2) * n, dd ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+	});
+}
+
+void fft(int D, const long dimensions[], unsigned long flags, float _Complex* dst, const float _Complex* src)
+{
+	fft_common(D, dimensions, flags, dst, src, -1.f);
+}
+
+void ifft(int D, const long dimensions[], unsigned long flags, float _Complex* dst, const float _Complex* src)
+{
+	fft_common(D, dimensions, flags, dst, src, +1.f);
+}
+
+int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float tol, int N, const long img_dims[],
+		  float _Complex* x, const float _Complex* b, bool warmstart, bool warm_fallback, float* resid_trace,
+		  struct cg_stats_s* stats)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "lsqr_conjgrad: NULL operator");
+		check_dims("lsqr_conjgrad", N, img_dims, op->N, op->idims);
+		BAM_CHECK(op->nufft->toeplitz && op->nufft->has_psf, "lsqr_conjgrad: needs a Toeplitz normal operator");
+
+		std::lock_guard<std::mutex> lock(op->mtx);
+
+		long ne = op->isize;
+
+		if (!op->cg)
+			op->cg.reset(new CgSolver(2 * ne, op->stream));
+
+		const float2* sens = (OP_SENSE == op->kind) ? op->sens->ptr : nullptr;
+		Nufft* nf = op->nufft.get();
+
+		NormalFn normal = [nf, sens](float2* d, const float2* s, const int* skip) { nf->normal(d, s, sens, skip); };
+
+		bool xd = on_device(x), bd = on_device(b);
+		float2* xdev = (float2*)x;
+		const float2* bdev = (const float2*)b;
+		DevBuf<float2> xs, bs;
+
+		if (!xd) {
+			xs.alloc(ne);
+			BAM_HIP(hipMemcpy(xs.ptr, x, sizeof(float2) * ne, hipMemcpyHostToDevice));
+			xdev = xs.ptr;
+		}
+
+		if (!bd) {
+			bs.alloc(ne);
+			BAM_HIP(hipMemcpy(bs.ptr, b, sizeof(float2) * ne, hipMemcpyHostToDevice));
+			bdev = bs.ptr;
+		}
+
+		CgResult r = op->cg->solve(normal, l2lambda, maxiter, tol, xdev, bdev, warmstart, warm_fallback, true);
+
+		if (!xd)
+			BAM_HIP(hipMemcpy(x, xdev, sizeof(float2) * ne, hipMemcpyDeviceToHost));
+
+		if (nullptr != resid_trace && r.iterations > 0)
+			memcpy(resid_trace, r.trace.data(), sizeof(float) * r.iterations);
+
+		if (nullptr != stats) {
+			stats->iterations = r.iterations;
+			stats->status = r.status;
+			stats->resid = r.resid;
+			stats->bnorm = r.bnorm;
+			stats->time_ms = r.ms;
+			stats->fell_back = r.fell_back;
+		}
+
+		return r.iterations;
+	}, -1);
+}
+
+}  // extern "C"

@@ -1,0 +1,875 @@
+// nufft.hip -- the decomposed 2x-oversampled NUFFT, its Toeplitz normal
+// operator and PSF on gfx950.
+//
+// Algorithm of noncart/nufft.c (decomp = true, os = 2):
+//   forward  (:1388-1458): roll -> x linphase_s -> FFT -> x fftmod -> sum_s gridH_s -> x weights
+//   adjoint  (:1462-1540): x conj(weights) -> grid_s -> x conj(fftmod) -> IFFT -> sum_s conj(linphase_s) -> roll
+//   normal   (:1546-1625): sum_s conj(linphase_s) IFFT(psf_s FFT(linphase_s x))
+//   psf      (:727-776):   decompose(fftuc_2n(NUFFT^H_2n(|w|^2; 2 traj), periodic))
+// Every element-wise factor is fused into the first or last FFT pass; the
+// product with the coil maps (sense/modelnc.c, linops/fmac.c) and the sum
+// over coils of S^H are fused there as well.
+#include "nufft_impl.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+
+namespace bam {
+
+// ---------------------------------------------------------------------------
+// Separable tables (host, FP64 maths)
+// ---------------------------------------------------------------------------
+
+// num/fft.c:64-90
+static double fftmod_phase(long length, long j)
+{
+	long center1 = length / 2;
+	double shift = (double)center1 / (double)length;
+	return ((double)j - (double)center1 / 2.) * shift;
+}
+
+static std::complex<double> fftmod_value(double phase)
+{
+	double rem = phase - floor(phase);
+
+	if (rem == 0.)
+		return 1.;
+	if (rem == 0.5)
+		return -1.;
+	if (rem == 0.25)
+		return std::complex<double>(0., 1.);
+	if (rem == 0.75)
+		return std::complex<double>(0., -1.);
+
+	return std::polar(1., 2. * M_PI * rem);
+}
+
+static double bessel_i0_d(double x)
+{
+	double q = 0.25 * x * x, term = 1., sum = 1.;
+
+	for (int k = 1; k < 500; k++) {
+		term *= q / ((double)k * (double)k);
+		sum += term;
+		if (term < 1.e-17 * sum)
+			break;
+	}
+
+	return sum;
+}
+
+// noncart/grid.c:100-117, 522-531
+static double rolloff_1d(int d, int i, float os, double beta, double width)
+{
+	if (1 == d)
+		return 1.;
+
+	int od = os * d;
+	int oi = i + (od / 2 - d / 2);
+	float x = ((float)oi - (float)(od / 2)) / (float)od;
+
+	double i0b = bessel_i0_d(beta);
+	double a = beta * beta - pow(M_PI * x * width, 2.);
+	double ft;
+
+	if (0. == a)
+		ft = 1. / i0b;
+	else if (a > 0)
+		ft = (sinh(sqrt(a)) / sqrt(a)) / i0b;
+	else
+		ft = (sin(sqrt(-a)) / sqrt(-a)) / i0b;
+
+	return 1. / ft / width;
+}
+
+static void build_tables(Nufft& p)
+{
+	long tot = 0;
+	long totr = 0;
+
+	for (int d = 0; d < 3; d++) {
+		tot += 3 * p.n[d];
+		totr += p.n[d];
+	}
+
+	std::vector<float2> h2(tot);
+	std::vector<float> hr(totr);
+	long o2[3], orr[3];
+	long a2 = 0, ar = 0;
+
+	int bit = 0;
+
+	for (int d = 0; d < 3; d++) {
+
+		int n = p.n[d];
+		o2[d] = a2;
+		orr[d] = ar;
+		a2 += 3 * n;
+		ar += n;
+
+		p.tabs.n[d] = n;
+		p.tabs.bitpos[d] = (n > 1) ? bit++ : -1;
+
+		for (int j = 0; j < n; j++) {
+
+			hr[orr[d] + j] = (float)rolloff_1d(n, j, p.os, p.beta, p.width);
+
+			std::complex<double> fm = (n > 1) ? fftmod_value(fftmod_phase(n, j)) : 1.;
+			h2[o2[d] + 2 * n + j] = make_float2((float)fm.real(), (float)fm.imag());
+
+			for (int b = 0; b < 2; b++) {
+
+				std::complex<double> v = 1.;
+
+				if (n > 1) {
+
+					// compute_linphases (nufft.c:196-248) with linear_phase
+					// (num/filter.c:241-265), one dim at a time
+					double shift = -0.5 * b + (n / 2. - n / 2);
+					double grad = 2. * M_PI * shift / (double)n;
+					double ph = grad * (double)j - grad * (double)n / 2.;
+					v = std::polar(1., ph) * fm / sqrt((double)n);
+				}
+
+				h2[o2[d] + b * n + j] = make_float2((float)v.real(), (float)v.imag());
+			}
+		}
+	}
+
+	p.tab2.alloc(tot);
+	p.tabr.alloc(totr);
+	BAM_HIP(hipMemcpy(p.tab2.ptr, h2.data(), sizeof(float2) * tot, hipMemcpyHostToDevice));
+	BAM_HIP(hipMemcpy(p.tabr.ptr, hr.data(), sizeof(float) * totr, hipMemcpyHostToDevice));
+
+	for (int d = 0; d < 3; d++) {
+		p.tabs.lin[d][0] = p.tab2.ptr + o2[d];
+		p.tabs.lin[d][1] = p.tab2.ptr + o2[d] + p.n[d];
+		p.tabs.fm[d] = p.tab2.ptr + o2[d] + 2 * p.n[d];
+		p.tabs.roll[d] = p.tabr.ptr + orr[d];
+	}
+}
+
+__device__ inline float2 lin_val(const SepTables& T, int s, const int c[3])
+{
+	float2 v = make_float2(1.f, 0.f);
+
+#pragma unroll
+	for (int d = 0; d < 3; d++)
+		if (T.bitpos[d] >= 0)
+			v = cmul(v, T.lin[d][(s >> T.bitpos[d]) & 1][c[d]]);
+
+	return v;
+}
+
+__device__ inline float2 fm_val(const SepTables& T, const int c[3])
+{
+	float2 v = make_float2(1.f, 0.f);
+
+#pragma unroll
+	for (int d = 0; d < 3; d++)
+		if (T.bitpos[d] >= 0)
+			v = cmul(v, T.fm[d][c[d]]);
+
+	return v;
+}
+
+__device__ inline float roll_val(const SepTables& T, const int c[3])
+{
+	return T.roll[0][c[0]] * T.roll[1][c[1]] * T.roll[2][c[2]];
+}
+
+__device__ inline long pix_index(const SepTables& T, const int c[3])
+{
+	return c[0] + (long)T.n[0] * (c[1] + (long)T.n[1] * c[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Geometry of the first axis with the sub-grid index fastest, then the coils
+// of a coil group, so that the S x cc transforms of one image row and coil
+// group share a work-group (for the sums over sub-grids and coils).
+// t = s + S (cl + cc (g + G o)), coil c = cl + cc g;
+// grid offset = (c + C s) N + o n + k.
+// ---------------------------------------------------------------------------
+
+struct GeomS {
+
+	VolAxis va;  // first active axis, nvol = S C (grid layout [s][c])
+	int S, C, cc, G;
+	FastDiv fS, fcc, fG;
+
+	static GeomS make(const int n[3], int axis, int S, int C, int cc)
+	{
+		GeomS g;
+		g.va = VolAxis::make(n, axis, (long)S * C);
+		g.S = S;
+		g.C = C;
+		g.cc = cc;
+		g.G = C / cc;
+		g.fS = FastDiv((uint32_t)S);
+		g.fcc = FastDiv((uint32_t)cc);
+		g.fG = FastDiv((uint32_t)g.G);
+		return g;
+	}
+
+	__device__ inline void split(long t, uint32_t& s, uint32_t& o, uint32_t& c) const
+	{
+		uint32_t q, q2, cl, gi;
+		fS.divmod((uint32_t)t, q, s);
+		fcc.divmod(q, q2, cl);
+		fG.divmod(q2, o, gi);
+		c = cl + (uint32_t)cc * gi;
+	}
+
+	__device__ inline long grid_offset(uint32_t s, uint32_t o, uint32_t c, int k) const
+	{
+		return ((long)c + (long)C * s) * va.N + (long)o * va.n[va.axis] + k;
+	}
+};
+
+// coils per reduction group: a power of two dividing C with S cc n <= 4096
+static int coil_group(int S, int C, int n, bool reduce_coils)
+{
+	if (!reduce_coils)
+		return 1;
+
+	int cc = 1;
+
+	while (0 == C % (2 * cc) && (long)S * 2 * cc * n <= 4096)
+		cc *= 2;
+
+	return cc;
+}
+
+// ---------------------------------------------------------------------------
+// Pass functors
+// ---------------------------------------------------------------------------
+
+// First forward pass: image side -> grid, x roll x linphase_s
+struct LdImgLin {
+	GeomS g;
+	SepTables T;
+	const float2* src;   // coil images [C][N] or single image [N] (with sens)
+	const float2* sens;  // [C][N] or null
+	bool roll;
+
+	__device__ float2 operator()(long t, int k) const
+	{
+		uint32_t s, o, c;
+		g.split(t, s, o, c);
+		int px[3];
+		g.va.coords(0, o, k, px);
+		long pi = pix_index(T, px);
+		float2 v;
+
+		if (nullptr != sens)
+			v = cmul(src[pi], sens[(long)c * g.va.N + pi]);
+		else
+			v = src[(long)c * g.va.N + pi];
+
+		if (roll)
+			v = cscale(v, roll_val(T, px));
+
+		return cmul(v, lin_val(T, (int)s, px));
+	}
+};
+
+// Plain grid loads in S-grouped order (optionally x conj(fftmod))
+struct LdGridS {
+	GeomS g;
+	SepTables T;
+	const float2* grid;
+	bool conj_fm;
+
+	__device__ float2 operator()(long t, int k) const
+	{
+		uint32_t s, o, c;
+		g.split(t, s, o, c);
+		float2 v = grid[g.grid_offset(s, o, c, k)];
+
+		if (conj_fm) {
+			int px[3];
+			g.va.coords(0, o, k, px);
+			v = cmulc(v, fm_val(T, px));
+		}
+
+		return v;
+	}
+};
+
+// Grid stores in S-grouped order (optionally x fftmod)
+struct StGridS {
+	GeomS g;
+	SepTables T;
+	float2* grid;
+	bool fm;
+
+	__device__ void store(const float2* lds, const FftLaunch& L, long t0) const
+	{
+		store_each(lds, L, t0, [&](long t, int k, float2 v) {
+			uint32_t s, o, c;
+			g.split(t, s, o, c);
+			if (fm) {
+				int px[3];
+				g.va.coords(0, o, k, px);
+				v = cmul(v, fm_val(T, px));
+			}
+			grid[g.grid_offset(s, o, c, k)] = v;
+		});
+	}
+};
+
+// Standard-order grid loads / stores for the other axes
+struct LdGrid {
+	VolAxis g;
+	SepTables T;
+	const float2* grid;
+	bool conj_fm;
+
+	__device__ float2 operator()(long t, int k) const
+	{
+		uint32_t i, o, v;
+		g.split(t, i, o, v);
+		float2 x = grid[g.offset_iov(i, o, v, k)];
+
+		if (conj_fm) {
+			int px[3];
+			g.coords(i, o, k, px);
+			x = cmulc(x, fm_val(T, px));
+		}
+
+		return x;
+	}
+};
+
+struct StGrid {
+	VolAxis g;
+	SepTables T;
+	float2* grid;
+	bool fm;
+
+	__device__ void store(const float2* lds, const FftLaunch& L, long t0) const
+	{
+		store_each(lds, L, t0, [&](long t, int k, float2 x) {
+			uint32_t i, o, v;
+			g.split(t, i, o, v);
+			if (fm) {
+				int px[3];
+				g.coords(i, o, k, px);
+				x = cmul(x, fm_val(T, px));
+			}
+			grid[g.offset_iov(i, o, v, k)] = x;
+		});
+	}
+};
+
+// PSF multiply between forward and inverse transform (standard order,
+// volumes v = c + C s)
+struct MdPsf {
+	static constexpr bool active = true;
+	VolAxis g;
+	int C;
+	const float2* psf;
+
+	__device__ float2 operator()(long t, int k, float2 x) const
+	{
+		uint32_t i, o, v;
+		g.split(t, i, o, v);
+		uint32_t s = v / (uint32_t)C;
+		long off = g.offset_iov(i, o, 0, k);
+		return cmul(x, psf[(long)s * g.N + off]);
+	}
+};
+
+// PSF multiply in the S-grouped geometry (single active axis)
+struct MdPsfS {
+	static constexpr bool active = true;
+	GeomS g;
+	const float2* psf;
+
+	__device__ float2 operator()(long t, int k, float2 x) const
+	{
+		uint32_t s, o, c;
+		g.split(t, s, o, c);
+		return cmul(x, psf[(long)s * g.va.N + (long)o * g.va.n[g.va.axis] + k]);
+	}
+};
+
+// Last pass: sum_s conj(linphase_s) x (roll) -> image side.  With coil maps
+// the block also sums conj(sens_c) x over the cc coils of its group, in a
+// fixed order, and writes the group's partial image dst[g N + pixel] (the
+// groups are summed afterwards, also in a fixed order); without maps it
+// stores coil image c (cc = 1).  No atomics: the result is deterministic.
+struct StReduceS {
+	GeomS g;
+	SepTables T;
+	float2* dst;
+	const float2* sens;
+	bool roll;
+
+	__device__ void store(const float2* lds, const FftLaunch& L, long t0) const
+	{
+		const int n = L.plan.n;
+		const int S = g.S;
+		const int SC = S * g.cc;
+		const int groups = L.P / SC;
+
+		for (int i = threadIdx.x; i < groups * n; i += FFT_THREADS) {
+
+			int gi = i / n;
+			int k = i - gi * n;
+			long t = t0 + (long)gi * SC;
+
+			if (t >= L.total)
+				continue;
+
+			uint32_t s, o, c;
+			g.split(t, s, o, c);
+			int px[3];
+			g.va.coords(0, o, k, px);
+			long pi = pix_index(T, px);
+			const float r = roll ? roll_val(T, px) : 1.f;
+
+			float2 out = make_float2(0.f, 0.f);
+
+			for (int cl = 0; cl < g.cc; cl++) {
+
+				float2 acc = make_float2(0.f, 0.f);
+
+				for (int j = 0; j < S; j++)
+					acc = cadd(acc, cmulc(lds[(gi * SC + cl * S + j) * n + k], lin_val(T, j, px)));
+
+				if (roll)
+					acc = cscale(acc, r);
+
+				if (nullptr != sens)
+					out = cadd(out, cmulc(acc, sens[(long)(c + cl) * g.va.N + pi]));
+				else
+					out = acc;
+			}
+
+			if (nullptr != sens)
+				dst[(long)(c / g.cc) * g.va.N + pi] = out;
+			else
+				dst[(long)c * g.va.N + pi] = out;
+		}
+	}
+};
+
+// ---------------------------------------------------------------------------
+// Small element-wise kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill(float2* p, long n, float2 v, const int* skip)
+{
+	if (nullptr != skip && 0 != *skip)
+		return;
+
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+		p[i] = v;
+}
+
+// dst[i] = sum_g part[g N + i], g = 0 .. G-1 in order
+__global__ void k_sum_groups(float2* __restrict__ dst, const float2* __restrict__ part, long N, int G,
+			     const int* skip)
+{
+	if (nullptr != skip && 0 != *skip)
+		return;
+
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+
+		float2 a = part[i];
+
+		for (int g = 1; g < G; g++)
+			a = cadd(a, part[(long)g * N + i]);
+
+		dst[i] = a;
+	}
+}
+
+static void sum_groups(float2* dst, const float2* part, long N, int G, hipStream_t st, const int* skip)
+{
+	unsigned blocks = (unsigned)std::min<long>((N + 255) / 256, 4096);
+	hipLaunchKernelGGL(k_sum_groups, dim3(blocks), dim3(256), 0, st, dst, part, N, G, skip);
+	BAM_HIP(hipGetLastError());
+}
+
+void fill(float2* p, long n, float2 v, hipStream_t st, const int* skip)
+{
+	if (n <= 0)
+		return;
+	unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
+	hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, v, skip);
+	BAM_HIP(hipGetLastError());
+}
+
+static FftLaunchCfg fft_config_skip(const int* skip, const AxisPlan& plan, long total, bool contig, int P_min,
+				   float sign1, float sign2)
+{
+	FftLaunchCfg c = fft_config(plan, total, contig, P_min, sign1, sign2);
+	c.L.skip = skip;
+	return c;
+}
+
+// ---------------------------------------------------------------------------
+// Plan creation
+// ---------------------------------------------------------------------------
+
+void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const float2* weights_host,
+		   bool toeplitz_, bool periodic_, float width_, float os_, bool compute_psf)
+{
+	BAM_CHECK(2.f == os_, "only oversampling 2 is supported (got %f)", os_);
+	BAM_CHECK(width_ > 0.f && width_ / 2.f <= (float)(MAX_TAPS - 1), "kernel width %f not supported", width_);
+
+	N = 1;
+	nact = 0;
+	S = 1;
+
+	for (int d = 0; d < 3; d++) {
+
+		n[d] = dims[d];
+		BAM_CHECK(n[d] >= 1, "bad image dims");
+		N *= n[d];
+
+		if (n[d] > 1) {
+			BAM_CHECK(fft_size_supported(n[d]), "image dim %d not supported by the FFT", n[d]);
+			act[nact++] = d;
+			S *= 2;
+		}
+	}
+
+	BAM_CHECK(nact >= 1, "NUFFT needs at least one non-trivial image dim");
+
+	C = C_;
+	M = M_;
+	width = width_;
+	os = os_;
+	beta = calc_beta(os, width);
+	periodic = periodic_;
+	toeplitz = toeplitz_;
+
+	std::vector<float> kbh(KB_SIZE + 1);
+	kb_table_host(beta, kbh.data());
+	kb.alloc(KB_SIZE + 1);
+	BAM_HIP(hipMemcpy(kb.ptr, kbh.data(), sizeof(float) * (KB_SIZE + 1), hipMemcpyHostToDevice));
+
+	build_tables(*this);
+
+	for (int j = 0; j < nact; j++)
+		axis[j] = make_axis_plan(n[act[j]]);
+
+	has_weights = (nullptr != weights_host);
+
+	if (has_weights) {
+		weights.alloc(std::max<long>(M, 1));
+		if (M > 0)
+			BAM_HIP(hipMemcpy(weights.ptr, weights_host, sizeof(float2) * M, hipMemcpyHostToDevice));
+	}
+
+	// gridding geometry of the decomposition (nufft.c:147-188)
+	GridGeom g = {};
+
+	for (int d = 0; d < 3; d++)
+		g.n[d] = n[d];
+
+	g.N = N;
+	g.C = C;
+	g.M = M;
+	g.S = S;
+	g.os = 1.f;
+	g.width = width / 2.f;
+	g.periodic = periodic;
+
+	for (int s = 0; s < S; s++) {
+
+		int idx = s;
+
+		for (int d = 0; d < 3; d++) {
+
+			int f = (n[d] > 1) ? 2 : 1;
+			float sh = -(float)(idx % f) / f;
+			idx /= f;
+
+			if (f > 1)
+				sh += (n[d] / 2.0 - n[d] / 2) / g.os;
+
+			g.shift[s][d] = sh;
+		}
+	}
+
+	gp.build(g, traj, 0);
+
+	grid.alloc((size_t)S * C * N);
+
+	// partial images of the coil groups of S^H (allocated here so that no
+	// allocation happens inside a captured CG iteration)
+	const int G = C / coil_group(S, C, n[act[0]], true);
+
+	if (G > 1)
+		part.alloc((size_t)G * N);
+
+	if (toeplitz && compute_psf)
+		compute_psf_from_traj(traj, weights_host);
+}
+
+// ---------------------------------------------------------------------------
+// Forward / adjoint / normal
+// ---------------------------------------------------------------------------
+
+void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
+{
+	const int a0 = act[0];
+
+	// first axis: image side -> grid (x roll x linphase), FFT
+	{
+		int cc = coil_group(S, C, n[a0], nullptr != sens);
+		GeomS gs = GeomS::make(n, a0, S, C, cc);
+		FftLaunchCfg cfg = fft_config(axis[0], gs.va.transforms(), true, S * cc, -1.f, -1.f);
+		LdImgLin ld{ gs, tabs, src, sens, true };
+		StGridS st{ gs, tabs, grid.ptr, 1 == nact };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	for (int j = 1; j < nact; j++) {
+
+		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
+		FftLaunchCfg cfg = fft_config(axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
+		LdGrid ld{ va, tabs, grid.ptr, false };
+		StGrid st{ va, tabs, grid.ptr, j == nact - 1 };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	grid_forward(gp, kb.ptr, ksp, grid.ptr, has_weights ? weights.ptr : nullptr, stream);
+}
+
+void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
+{
+	fill(grid.ptr, (long)S * C * N, make_float2(0.f, 0.f), stream);
+	grid_adjoint(gp, kb.ptr, grid.ptr, ksp, has_weights ? weights.ptr : nullptr, stream);
+
+	for (int j = nact - 1; j >= 1; j--) {
+
+		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
+		FftLaunchCfg cfg = fft_config(axis[j], va.transforms(), va.contig(), 1, +1.f, +1.f);
+		LdGrid ld{ va, tabs, grid.ptr, j == nact - 1 };
+		StGrid st{ va, tabs, grid.ptr, false };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	const int cc = coil_group(S, C, n[act[0]], nullptr != sens);
+	const int G = C / cc;
+	GeomS gs = GeomS::make(n, act[0], S, C, cc);
+	FftLaunchCfg cfg = fft_config(axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
+	LdGridS ld{ gs, tabs, grid.ptr, 1 == nact };
+	float2* out = dst;
+
+	if (nullptr != sens && G > 1) {
+		if ((long)part.n < (long)G * N)
+			part.alloc((size_t)G * N);
+		out = part.ptr;
+	}
+
+	StReduceS st{ gs, tabs, out, sens, true };
+	launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+
+	if (out != dst)
+		sum_groups(dst, part.ptr, N, G, stream, nullptr);
+}
+
+void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int* skip)
+{
+	BAM_CHECK(has_psf, "Toeplitz normal operator without PSF");
+
+	const int cc = coil_group(S, C, n[act[0]], nullptr != sens);
+	const int G = C / cc;
+	GeomS gs = GeomS::make(n, act[0], S, C, cc);
+	float2* out = dst;
+
+	if (nullptr != sens && G > 1) {
+		if ((long)part.n < (long)G * N)
+			part.alloc((size_t)G * N);
+		out = part.ptr;
+	}
+
+	if (1 == nact) {
+
+		FftLaunchCfg cfg = fft_config_skip(skip, axis[0], gs.va.transforms(), true, S * cc, -1.f, +1.f);
+		LdImgLin ld{ gs, tabs, src, sens, false };
+		MdPsfS md{ gs, psf.ptr };
+		StReduceS st{ gs, tabs, out, sens, false };
+		launch_fft_axis(cfg, stream, ld, md, st);
+
+		if (out != dst)
+			sum_groups(dst, part.ptr, N, G, stream, skip);
+		return;
+	}
+
+	{
+		FftLaunchCfg cfg = fft_config_skip(skip, axis[0], gs.va.transforms(), true, S * cc, -1.f, -1.f);
+		LdImgLin ld{ gs, tabs, src, sens, false };
+		StGridS st{ gs, tabs, grid.ptr, false };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	for (int j = 1; j < nact; j++) {
+
+		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
+		LdGrid ld{ va, tabs, grid.ptr, false };
+		StGrid st{ va, tabs, grid.ptr, false };
+
+		if (j == nact - 1) {
+			FftLaunchCfg cfg = fft_config_skip(skip, axis[j], va.transforms(), va.contig(), 1, -1.f, +1.f);
+			MdPsf md{ va, C, psf.ptr };
+			launch_fft_axis(cfg, stream, ld, md, st);
+		} else {
+			FftLaunchCfg cfg = fft_config_skip(skip, axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
+			launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+		}
+	}
+
+	for (int j = nact - 2; j >= 1; j--) {
+
+		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
+		FftLaunchCfg cfg = fft_config_skip(skip, axis[j], va.transforms(), va.contig(), 1, +1.f, +1.f);
+		LdGrid ld{ va, tabs, grid.ptr, false };
+		StGrid st{ va, tabs, grid.ptr, false };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	FftLaunchCfg cfg = fft_config_skip(skip, axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
+	LdGridS ld{ gs, tabs, grid.ptr, false };
+	StReduceS st{ gs, tabs, out, sens, false };
+	launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+
+	if (out != dst)
+		sum_groups(dst, part.ptr, N, G, stream, skip);
+}
+
+// ---------------------------------------------------------------------------
+// PSF (noncart/nufft.c:727-776 compute_psf2, :535-586 compute_psf_int)
+// ---------------------------------------------------------------------------
+
+// fftuc over the 2x grid: first pass loads x fftmod; the last pass stores
+// x fftmod x 1/sqrt(N2) into the decomposed layout psf[s][n] (md_decompose,
+// num/shuffle.c:84-130)
+struct LdFm2 {
+	VolAxis g;
+	SepTables T2;
+	const float2* src;
+	bool fm;
+
+	__device__ float2 operator()(long t, int k) const
+	{
+		uint32_t i, o, v;
+		g.split(t, i, o, v);
+		float2 x = src[g.offset_iov(i, o, v, k)];
+
+		if (fm) {
+			int px[3];
+			g.coords(i, o, k, px);
+			x = cmul(x, fm_val(T2, px));
+		}
+
+		return x;
+	}
+};
+
+struct StDecomp {
+	VolAxis g;
+	SepTables T2;
+	float2* dst;      // 2x grid (intermediate passes) or decomposed psf (last)
+	bool last;
+	bool fm_first;    // single-axis case: the load already applied the first fftmod
+	float scale;
+	int n1[3];        // sub-grid dims
+	long N1;
+
+	__device__ void store(const float2* lds, const FftLaunch& L, long t0) const
+	{
+		store_each(lds, L, t0, [&](long t, int k, float2 x) {
+			uint32_t i, o, v;
+			g.split(t, i, o, v);
+
+			if (!last) {
+				dst[g.offset_iov(i, o, v, k)] = x;
+				return;
+			}
+
+			int px[3];
+			g.coords(i, o, k, px);
+			x = cmul(x, fm_val(T2, px));
+			x = cscale(x, scale);
+
+			int s = 0;
+			int q[3];
+
+			for (int d = 0; d < 3; d++) {
+				if (T2.bitpos[d] >= 0) {
+					s |= (px[d] & 1) << T2.bitpos[d];
+					q[d] = px[d] >> 1;
+				} else {
+					q[d] = px[d];
+				}
+			}
+
+			dst[(long)s * N1 + q[0] + (long)n1[0] * (q[1] + (long)n1[1] * q[2])] = x;
+		});
+	}
+};
+
+void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host)
+{
+	int n2[3];
+
+	for (int d = 0; d < 3; d++)
+		n2[d] = (1 == n[d]) ? 1 : 2 * n[d];
+
+	std::vector<float> traj2(3 * M);
+
+	for (long i = 0; i < 3 * M; i++)
+		traj2[i] = traj[i] * 2.f;
+
+	std::vector<float2> sqw;
+
+	if (nullptr != weights_host) {
+
+		sqw.resize(M);
+
+		for (long m = 0; m < M; m++) {
+			float2 w = weights_host[m];
+			sqw[m] = make_float2(w.x * w.x + w.y * w.y, w.y * w.x - w.x * w.y);
+		}
+	}
+
+	Nufft p2;
+	p2.stream = stream;
+	p2.create(n2, 1, M, traj2.data(), sqw.empty() ? nullptr : sqw.data(), false, true, width, os, false);
+
+	long N2 = p2.N;
+	DevBuf<float2> ones(std::max<long>(M, 1));
+	DevBuf<float2> psft(N2);
+
+	fill(ones.ptr, M, make_float2(1.f, 0.f), stream);
+	p2.adjoint(psft.ptr, ones.ptr, nullptr);
+
+	psf.alloc((size_t)S * N);
+
+	const float scale = 1.f / sqrtf((float)N2);
+	int last = p2.act[p2.nact - 1];
+
+	for (int j = 0; j < p2.nact; j++) {
+
+		int a = p2.act[j];
+		VolAxis va = VolAxis::make(n2, a, 1);
+		FftLaunchCfg cfg = fft_config(p2.axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
+		LdFm2 ld{ va, p2.tabs, psft.ptr, 0 == j };
+		StDecomp st{ va, p2.tabs, (a == last) ? psf.ptr : psft.ptr, a == last, false, scale,
+			     { n[0], n[1], n[2] }, N };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	BAM_HIP(hipStreamSynchronize(stream));
+	has_psf = true;
+}
+
+}  // namespace bam

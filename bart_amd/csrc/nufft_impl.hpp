@@ -1,0 +1,118 @@
+// nufft_impl.hpp -- internal interface of the gfx950 NUFFT plan.
+#pragma once
+
+#include "common.hpp"
+#include "fft.hpp"
+#include "volume.hpp"
+
+namespace bam {
+
+constexpr int KB_SIZE = 1000;
+constexpr int MAX_TAPS = 8;  // taps per dimension (kernel width <= 7)
+
+// Gridding configuration of one grid2/grid2H call or of one decomposition
+// sub-grid (noncart/grid.h:8-16 struct grid_conf_s, noncart/nufft.c:147-188).
+struct GridGeom {
+
+	int n[3];         // grid dims
+	long N;           // grid points per coil
+	int C;            // coils
+	long M;           // samples per coil
+	int S;            // sub-grids (shift sets) handled by one launch
+	float os;
+	float width;
+	bool periodic;
+	float shift[8][3];  // per sub-grid shift (already including odd-size correction)
+
+	// binning
+	int T[3];         // tile size
+	int nt[3];        // tiles per dim
+	int lo[3], hi[3]; // window margins
+	int W[3];         // window dims
+	int Wtot;
+};
+
+struct GridPlan {
+
+	GridGeom g;
+	DevBuf<int> sm;      // sorted sample -> original sample index
+	DevBuf<float4> sk;   // sorted sample coordinates (kx, ky, kz, 0)
+	DevBuf<int4> work;   // (tile, begin, end, 0)
+	int nwork = 0;
+	int cc_adj = 1;      // coils per adjoint work-group
+	int cc_fwd = 1;      // coils per forward thread
+
+	// traj: host, M x 3 floats (real parts); shift0: sub-grid 0 shift used
+	// for binning
+	void build(const GridGeom& geom, const float* traj, int tile_override);
+};
+
+// Separable per-dimension tables for the element-wise factors of the NUFFT:
+// linear phase (with fftmod and 1/sqrt(n) folded in, one per shift bit),
+// fftmod and rolloff.  Every factor of BART's tables is a product over the
+// three dims (noncart/nufft.c:196-248, num/fft.c:64-157, grid.c:537-556).
+struct SepTables {
+
+	const float2* lin[3][2];
+	const float2* fm[3];
+	const float* roll[3];
+	int bitpos[3];  // decomposition bit of each dim, -1 if the dim is trivial
+	int n[3];
+};
+
+struct Nufft {
+
+	// geometry
+	int n[3];
+	long N;
+	int S;
+	int C;
+	long M;
+	int nact;
+	int act[3];
+	float width, os;
+	double beta;
+	bool periodic, toeplitz;
+
+	// device data
+	DevBuf<float> kb;
+	DevBuf<float2> tab2;   // separable complex tables
+	DevBuf<float> tabr;    // separable rolloff
+	SepTables tabs;
+	DevBuf<float2> weights;
+	bool has_weights = false;
+	GridPlan gp;
+	DevBuf<float2> psf;    // [S][N]
+	bool has_psf = false;
+	DevBuf<float2> grid;   // work [S][C][N]
+	DevBuf<float2> part;   // per-coil-group partial images [G][N]
+	AxisPlan axis[3];
+
+	hipStream_t stream = 0;
+
+	// dims: image dims; traj: host M x 3 (real parts); weights: host M or null
+	void create(const int dims[3], int C, long M, const float* traj, const float2* weights_host,
+		    bool toeplitz, bool periodic, float width, float os, bool compute_psf);
+
+	// device pointers; sens == nullptr: coil images in/out, else single image
+	// combined with coil maps
+	void forward(float2* ksp, const float2* cim, const float2* sens);
+	void adjoint(float2* cim, const float2* ksp, const float2* sens);
+	// Toeplitz; skip: optional device flag that turns the launches into no-ops
+	void normal(float2* dst, const float2* src, const float2* sens, const int* skip = nullptr);
+	void compute_psf_from_traj(const float* traj, const float2* weights_host);
+};
+
+void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float2* ksp, const float2* weights,
+		  hipStream_t st);
+void grid_forward(const GridPlan& gp, const float* kb, float2* ksp, const float2* grid, const float2* weights,
+		  hipStream_t st);
+
+void fft_nd(const int dims[3], long batch, unsigned flags, float sign, float2* dst, const float2* src,
+	    float scale, hipStream_t stream);
+
+void kb_table_host(double beta, float* table);
+void fill(float2* p, long n, float2 v, hipStream_t st, const int* skip = nullptr);
+double calc_beta(float os, float width);
+
+}  // namespace bam

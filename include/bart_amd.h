@@ -1,0 +1,144 @@
+/*
+ * bart_amd.h -- C ABI of the gfx950 NUFFT / CG-SENSE library.
+ *
+ * The entry points keep the names, argument order and meaning of BART's
+ * interfaces for this path so that a BART build (or any C/FFI caller) can
+ * link them in place of the CPU/CUDA implementation.  Array arguments use
+ * BART's layout: N dims (normally 16, misc/mri.h:9-27), column-major,
+ * complex float (re, im interleaved).  Pointers may be host or device
+ * (hipMalloc) memory; the library checks where each pointer lives, as BART
+ * selects its GPU path by cuda_ondevice() (num/gpuops.c:445).  Host pointers
+ * are staged through device buffers owned by the operator.
+ *
+ * Errors follow BART's error() (misc/misc.c:95-128): by default a message is
+ * printed and the process exits with status 1.  After
+ * bart_amd_set_error_mode(BART_AMD_ERRORS_RETURN) the message is kept
+ * (bart_amd_last_error()) and the call returns -- constructors return NULL.
+ */
+#ifndef BART_AMD_H
+#define BART_AMD_H
+
+#include <complex.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#define BAM_COMPLEX float _Complex
+#else
+#define BAM_COMPLEX _Complex float
+#endif
+
+struct linop_s;
+
+/* noncart/nufft.h:7-29 (field order kept: the struct is passed by value) */
+struct nufft_conf_s {
+
+	bool toeplitz;
+	bool pcycle;
+	bool periodic;
+	bool lowmem;
+	unsigned long flags;
+	unsigned long cfft;
+	bool decomp;
+	bool nopsf;
+	bool upper_triag;
+	bool real;
+	bool compress_psf;
+	bool decomposed_psf;
+
+	bool precomp;
+	bool precomp_linphase;
+	bool precomp_fftmod;
+	bool precomp_roll;
+	bool zero_overhead;
+
+	float width;
+	float os;
+};
+
+/* noncart/nufft.c:53-75 */
+extern const struct nufft_conf_s nufft_conf_defaults;
+
+/* noncart/nufft.h:48-57 -- replaces nufft_create2 (noncart/nufft.c:1174-1276).
+ * Supported: os = 2, decomp = true, basis = NULL, kernel width <= 14,
+ * 1 to 3 transformed dims (flags = 7), samples in dims 1-2 and coils in
+ * dim 3.  Modes that only change memory use (lowmem, zero_overhead,
+ * precomp*, pcycle = false) give the same results and are accepted. */
+struct linop_s* nufft_create2(int N, const long ksp_dims[], const long cim_dims[], const long traj_dims[],
+			      const BAM_COMPLEX* traj, const long wgh_dims[], const BAM_COMPLEX* weights,
+			      const long bas_dims[], const BAM_COMPLEX* basis, struct nufft_conf_s conf);
+
+/* noncart/nufft.h:40-46 -- replaces nufft_create (noncart/nufft.c:1315-1327) */
+struct linop_s* nufft_create(int N, const long ksp_dims[], const long cim_dims[], const long traj_dims[],
+			     const BAM_COMPLEX* traj, const BAM_COMPLEX* weights, struct nufft_conf_s conf);
+
+/* sense/modelnc.h -- replaces sense_nc_init (sense/modelnc.c:28-67): the
+ * linear operator image -> k-space  A = NUFFT S  with normal operator
+ * S^H T S (Toeplitz) when conf->toeplitz.  One set of maps (MAPS dim 1). */
+const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims[16], const BAM_COMPLEX* maps,
+				    const long ksp_dims[16], const long traj_dims[16], const BAM_COMPLEX* traj,
+				    const struct nufft_conf_s* conf, const long wgs_dims[16], const BAM_COMPLEX* weights,
+				    const long basis_dims[16], const BAM_COMPLEX* basis,
+				    const struct linop_s** fft_opp, unsigned long shared_img_dims);
+
+/* linops/linop.h:66-72 -- replace linop_forward/adjoint/normal
+ * (linops/linop.c:406-525).  dst is overwritten. */
+void linop_forward(const struct linop_s* op, int DN, const long ddims[], BAM_COMPLEX* dst, int SN,
+		   const long sdims[], const BAM_COMPLEX* src);
+void linop_adjoint(const struct linop_s* op, int DN, const long ddims[], BAM_COMPLEX* dst, int SN,
+		   const long sdims[], const BAM_COMPLEX* src);
+void linop_normal(const struct linop_s* op, int N, const long dims[], BAM_COMPLEX* dst, const BAM_COMPLEX* src);
+
+/* linops/linop.h:55 -- replaces linop_free (linops/linop.c:998) */
+void linop_free(const struct linop_s* op);
+
+/* noncart/nufft.h:95-100 -- replace nufft_get_psf_dims / nufft_get_psf /
+ * nufft_update_psf (noncart/nufft.c:2008-2073); PSF dims are N + 1 with the
+ * decomposition index last. */
+int nufft_get_psf_dims(const struct linop_s* nufft, int N, long psf_dims[]);
+void nufft_get_psf(const struct linop_s* nufft, int N, const long psf_dims[], BAM_COMPLEX* psf);
+void nufft_update_psf(const struct linop_s* nufft, int ND, const long psf_dims[], const BAM_COMPLEX* psf);
+
+/* num/fft.h -- replace fft/ifft (num/fft.c:387-417): unnormalised FFT,
+ * sign -1 (fft) / +1 (ifft) over the dims in flags (subset of dims 0-2);
+ * dims above 2 are batch.  dst may equal src. */
+void fft(int D, const long dimensions[], unsigned long flags, BAM_COMPLEX* dst, const BAM_COMPLEX* src);
+void ifft(int D, const long dimensions[], unsigned long flags, BAM_COMPLEX* dst, const BAM_COMPLEX* src);
+
+/* Conjugate gradients on (A^H A + l2lambda I) x = b with A^H A the normal
+ * operator of `op` (BART's lsqr2 -> iter2_conjgrad -> conjgrad chain:
+ * iter/lsqr.c:52-170, iter/iter2.c:227-270, iter/italgos.c:497-580).
+ * Stops after maxiter iterations or when ||r|| <= tol ||b|| (tol = 0 as in
+ * BART: fixed iteration count).  x holds the initial guess when warmstart
+ * (pics -W, iter/itop.c:54-66), else zero is used.  With warm_fallback the
+ * zero start is used whenever the warm start's residual exceeds ||b||.
+ * Returns the number of iterations, or -1 on error. */
+struct cg_stats_s {
+
+	int iterations;
+	int status;        /* 1 maxiter, 2 converged, 3 pAp = 0, 4 nothing to do */
+	float resid;       /* ||r|| at exit */
+	float bnorm;       /* ||b|| */
+	float time_ms;     /* device time of the solve */
+	bool fell_back;
+};
+
+int lsqr_conjgrad(const struct linop_s* op, int maxiter, float l2lambda, float tol, int N, const long img_dims[],
+		  BAM_COMPLEX* x, const BAM_COMPLEX* b, bool warmstart, bool warm_fallback, float* resid_trace,
+		  struct cg_stats_s* stats);
+
+/* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
+enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };
+void bart_amd_set_error_mode(int mode);
+const char* bart_amd_last_error(void);
+void bart_amd_clear_error(void);
+
+/* Library version string. */
+const char* bart_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#undef BAM_COMPLEX
+#endif /* BART_AMD_H */

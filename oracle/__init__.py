@@ -1,0 +1,298 @@
+"""CPU oracle for the NUFFT + CG-SENSE path (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package; the product path in ``bart_amd`` never does.
+
+Two oracles live here:
+
+* ``liboracle.so`` (``bart_oracle.c``): a C/OpenMP restatement of BART's CPU
+  NUFFT (noncart/nufft.c, noncart/grid.c), Toeplitz PSF and CG
+  (iter/italgos.c:497-580).  Wrapped by :class:`Nufft` below.
+* :func:`nudft_forward` / :func:`nudft_adjoint`: FP64 direct NUDFT with BART's
+  sign and centring conventions (noncart/nudft.c:17-190).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+KB_SIZE = 1000
+
+
+def build():
+    """Compile liboracle.so from bart_oracle.c (gcc + OpenMP)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        c_long3 = ctypes.c_long * 3
+        vp = ctypes.c_void_p
+        L.orc_bessel_i0.restype = ctypes.c_double
+        L.orc_bessel_i0.argtypes = [ctypes.c_double]
+        L.orc_calc_beta.restype = ctypes.c_double
+        L.orc_calc_beta.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.orc_kb_table.argtypes = [ctypes.c_double, vp]
+        L.orc_rolloff.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_double, c_long3, vp]
+        L.orc_fft.argtypes = [c_long3, ctypes.c_long, ctypes.c_int, vp]
+        L.orc_fftmod_table.argtypes = [c_long3, ctypes.c_bool, vp]
+        L.orc_linphase_table.argtypes = [c_long3, ctypes.c_int, vp, vp]
+        L.orc_nufft_create.restype = vp
+        L.orc_nufft_create.argtypes = [c_long3, ctypes.c_long, vp, vp, ctypes.c_int,
+                                       ctypes.c_bool, ctypes.c_bool, ctypes.c_float, ctypes.c_float]
+        L.orc_nufft_free.argtypes = [vp]
+        L.orc_nufft_forward.argtypes = [vp, vp, vp]
+        L.orc_nufft_adjoint.argtypes = [vp, vp, vp]
+        L.orc_nufft_normal.argtypes = [vp, vp, vp]
+        L.orc_nufft_get_psf.argtypes = [vp, vp]
+        L.orc_nufft_set_psf.argtypes = [vp, vp]
+        L.orc_nufft_get_linphase.argtypes = [vp, vp]
+        L.orc_nufft_subgrids.restype = ctypes.c_int
+        L.orc_nufft_subgrids.argtypes = [vp]
+        L.orc_nufft_compute_psf.argtypes = [c_long3, ctypes.c_long, vp, vp, vp]
+        L.orc_sense_forward.argtypes = [vp, vp, vp, vp]
+        L.orc_sense_adjoint.argtypes = [vp, vp, vp, vp]
+        L.orc_sense_normal.argtypes = [vp, vp, vp, vp]
+        L.orc_cgsense.restype = ctypes.c_int
+        L.orc_cgsense.argtypes = [vp, vp, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+                                  vp, vp, vp, vp]
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.complex64)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _dims3(dims):
+    d = list(dims) + [1] * (3 - len(dims))
+    return (ctypes.c_long * 3)(*d)
+
+
+def bessel_i0(x):
+    return lib().orc_bessel_i0(float(x))
+
+
+def calc_beta(os_=2.0, width=6.0):
+    return lib().orc_calc_beta(os_, width)
+
+
+def kb_table(beta):
+    t = np.zeros(KB_SIZE + 1, np.float32)
+    lib().orc_kb_table(beta, _ptr(t))
+    return t
+
+
+def rolloff(dims, os_=2.0, width=6.0, beta=None):
+    if beta is None:
+        beta = calc_beta(os_, width)
+    d = list(dims) + [1] * (3 - len(dims))
+    out = np.zeros(d[::-1], np.complex64)
+    lib().orc_rolloff(os_, width, beta, _dims3(d), _ptr(out))
+    return out
+
+
+def fft(x, sign=-1):
+    """Unnormalised FFT over the last (up to 3) axes of a C-ordered array whose
+    trailing axes are (Z, Y, X); leading axes are batch."""
+    x = _c64(x).copy()
+    shp = x.shape
+    dims = list(shp[::-1][:3]) + [1] * max(0, 3 - len(shp))
+    batch = int(np.prod(shp[:-3])) if len(shp) > 3 else 1
+    lib().orc_fft(_dims3(dims), batch, sign, _ptr(x))
+    return x
+
+
+def fftmod_table(dims, inv=False):
+    d = list(dims) + [1] * (3 - len(dims))
+    out = np.zeros(d[::-1], np.complex64)
+    lib().orc_fftmod_table(_dims3(d), inv, _ptr(out))
+    return out
+
+
+def linphase_table(dims, s, roll=None):
+    d = list(dims) + [1] * (3 - len(dims))
+    out = np.zeros(d[::-1], np.complex64)
+    r = _c64(roll) if roll is not None else None
+    lib().orc_linphase_table(_dims3(d), s, _ptr(r), _ptr(out))
+    return out
+
+
+def compute_psf(dims, traj, weights=None):
+    """PSF [S, Z, Y, X] as BART's compute_psf2 (noncart/nufft.c:727-776).
+    traj: complex64 [M, 3] (real part = coordinate)."""
+    d = list(dims) + [1] * (3 - len(dims))
+    S = 2 ** sum(1 for v in d if v > 1)
+    tr = _c64(traj).reshape(-1, 3)
+    w = _c64(weights).reshape(-1) if weights is not None else None
+    out = np.zeros([S] + d[::-1], np.complex64)
+    lib().orc_nufft_compute_psf(_dims3(d), tr.shape[0], _ptr(tr), _ptr(w), _ptr(out))
+    return out
+
+
+class Nufft:
+    """BART-style NUFFT on the CPU.
+
+    dims: image dims (X, Y, Z); traj: complex64 [M, 3]; coil images are
+    arrays [C, Z, Y, X]; k-space arrays are [C, M].
+    """
+
+    def __init__(self, dims, traj, ncoils, weights=None, toeplitz=True, periodic=False,
+                 width=6.0, os_=2.0):
+        self.dims = tuple(list(dims) + [1] * (3 - len(dims)))
+        self.traj = _c64(traj).reshape(-1, 3)
+        self.M = self.traj.shape[0]
+        self.C = int(ncoils)
+        self.weights = _c64(weights).reshape(-1) if weights is not None else None
+        self.toeplitz = toeplitz
+        self.h = lib().orc_nufft_create(_dims3(self.dims), self.M, _ptr(self.traj),
+                                        _ptr(self.weights), self.C, toeplitz, periodic,
+                                        width, os_)
+        if not self.h:
+            raise ValueError("oracle supports os = 2 only")
+        self.S = lib().orc_nufft_subgrids(self.h)
+        self.N = int(np.prod(self.dims))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_nufft_free(self.h)
+            self.h = None
+
+    @property
+    def cim_shape(self):
+        return (self.C,) + self.dims[::-1]
+
+    def forward(self, cim):
+        cim = _c64(cim).reshape(self.cim_shape)
+        out = np.zeros((self.C, self.M), np.complex64)
+        lib().orc_nufft_forward(self.h, _ptr(out), _ptr(cim))
+        return out
+
+    def adjoint(self, ksp):
+        ksp = _c64(ksp).reshape(self.C, self.M)
+        out = np.zeros(self.cim_shape, np.complex64)
+        lib().orc_nufft_adjoint(self.h, _ptr(out), _ptr(ksp))
+        return out
+
+    def normal(self, cim):
+        assert self.toeplitz
+        cim = _c64(cim).reshape(self.cim_shape)
+        out = np.zeros(self.cim_shape, np.complex64)
+        lib().orc_nufft_normal(self.h, _ptr(out), _ptr(cim))
+        return out
+
+    def psf(self):
+        out = np.zeros((self.S,) + self.dims[::-1], np.complex64)
+        lib().orc_nufft_get_psf(self.h, _ptr(out))
+        return out
+
+    def set_psf(self, psf):
+        psf = _c64(psf)
+        lib().orc_nufft_set_psf(self.h, _ptr(psf))
+
+    def linphase(self):
+        out = np.zeros((self.S,) + self.dims[::-1], np.complex64)
+        lib().orc_nufft_get_linphase(self.h, _ptr(out))
+        return out
+
+    # SENSE model (maps: [C, Z, Y, X], image [Z, Y, X])
+    def sense_forward(self, maps, img):
+        maps = _c64(maps).reshape(self.cim_shape)
+        img = _c64(img).reshape(self.dims[::-1])
+        out = np.zeros((self.C, self.M), np.complex64)
+        lib().orc_sense_forward(self.h, _ptr(maps), _ptr(out), _ptr(img))
+        return out
+
+    def sense_adjoint(self, maps, ksp):
+        maps = _c64(maps).reshape(self.cim_shape)
+        ksp = _c64(ksp).reshape(self.C, self.M)
+        out = np.zeros(self.dims[::-1], np.complex64)
+        lib().orc_sense_adjoint(self.h, _ptr(maps), _ptr(out), _ptr(ksp))
+        return out
+
+    def sense_normal(self, maps, img):
+        maps = _c64(maps).reshape(self.cim_shape)
+        img = _c64(img).reshape(self.dims[::-1])
+        out = np.zeros(self.dims[::-1], np.complex64)
+        lib().orc_sense_normal(self.h, _ptr(maps), _ptr(out), _ptr(img))
+        return out
+
+    def cgsense(self, maps, b, lam, maxiter, tol=0.0, x0=None):
+        """CG on (A^H A + lam I) x = b as BART's conjgrad.  Returns
+        (x, iterations, residual trace, final residual norm)."""
+        maps = _c64(maps).reshape(self.cim_shape)
+        b = _c64(b).reshape(self.dims[::-1])
+        x = np.zeros(self.dims[::-1], np.complex64) if x0 is None else _c64(x0).reshape(self.dims[::-1]).copy()
+        trace = np.zeros(max(1, maxiter), np.float32)
+        fin = np.zeros(1, np.float32)
+        it = lib().orc_cgsense(self.h, _ptr(maps), lam, maxiter, tol, _ptr(x), _ptr(b),
+                               _ptr(trace), _ptr(fin))
+        return x, it, trace[:it], float(fin[0])
+
+
+# --------------------------------------------------------------------------
+# FP64 direct NUDFT (noncart/nudft.c:17-190): coordinates are negated in
+# get_coord, the phase is linear_phase centred at dims/2 (num/filter.c:241-265)
+# and the result is scaled by 1/sqrt(N).
+# --------------------------------------------------------------------------
+
+def _phase_matrix(dims, traj, sign):
+    d = list(dims) + [1] * (3 - len(dims))
+    k = np.real(np.asarray(traj, np.complex128)).reshape(-1, 3)
+    ph = np.zeros((k.shape[0],) + tuple(d[::-1]), np.float64)
+    grids = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in d[::-1]], indexing="ij")
+    for ax in range(3):
+        n = d[ax]
+        if n == 1:
+            continue
+        pos = grids[2 - ax] - n / 2.0
+        ph += (k[:, ax] / n).reshape((-1, 1, 1, 1)) * pos[None]
+    return np.exp(sign * 2j * np.pi * ph)
+
+
+def nudft_forward(dims, traj, img, chunk=2048):
+    """img [..., Z, Y, X] -> ksp [..., M] in float64/complex128."""
+    d = list(dims) + [1] * (3 - len(dims))
+    img = np.asarray(img, np.complex128).reshape((-1,) + tuple(d[::-1]))
+    tr = np.asarray(traj).reshape(-1, 3)
+    out = np.zeros((img.shape[0], tr.shape[0]), np.complex128)
+    N = int(np.prod(d))
+    for i in range(0, tr.shape[0], chunk):
+        E = _phase_matrix(d, tr[i:i + chunk], -1.0).reshape(-1, N)
+        out[:, i:i + chunk] = img.reshape(img.shape[0], N) @ E.T
+    return out / np.sqrt(N)
+
+
+def nudft_adjoint(dims, traj, ksp, chunk=2048):
+    """ksp [..., M] -> img [..., Z, Y, X] in complex128."""
+    d = list(dims) + [1] * (3 - len(dims))
+    tr = np.asarray(traj).reshape(-1, 3)
+    ksp = np.asarray(ksp, np.complex128).reshape(-1, tr.shape[0])
+    N = int(np.prod(d))
+    out = np.zeros((ksp.shape[0], N), np.complex128)
+    for i in range(0, tr.shape[0], chunk):
+        E = _phase_matrix(d, tr[i:i + chunk], +1.0).reshape(-1, N)
+        out += ksp[:, i:i + chunk] @ E
+    return (out / np.sqrt(N)).reshape((ksp.shape[0],) + tuple(d[::-1]))
+
+
+def nrmse(ref, x):
+    """BART's nrmse (src/nrmse.c, md_znrmse): ||ref - x|| / ||ref||."""
+    ref = np.asarray(ref, np.complex128)
+    x = np.asarray(x, np.complex128)
+    return float(np.linalg.norm((ref - x).ravel()) / np.linalg.norm(ref.ravel()))

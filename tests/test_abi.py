@@ -1,0 +1,53 @@
+"""CPU checks of the C ABI: the library builds, loads without a GPU and
+exports every function include/bart_amd.h declares; the conf struct layout
+matches the header; the host-side mirror fails loudly without the library."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import bart_amd as ba
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "bart_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", src)
+    return sorted(set(n for n in names if n not in ("if", "while", "sizeof")))
+
+
+def test_header_declares_the_boundary():
+    fns = header_functions()
+    for f in ["nufft_create2", "nufft_create", "sense_nc_init", "linop_forward", "linop_adjoint", "linop_normal",
+              "linop_free", "nufft_get_psf_dims", "nufft_get_psf", "nufft_update_psf", "fft", "ifft",
+              "lsqr_conjgrad"]:
+        assert f in fns
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(ba.library_path()):
+        ba.build()
+    L = ctypes.CDLL(ba.library_path())
+    missing = [f for f in header_functions() if not hasattr(L, f)]
+    assert not missing, missing
+    # data symbol
+    assert ctypes.c_bool.in_dll(L, "nufft_conf_defaults").value is True
+
+
+def test_conf_defaults_layout():
+    L = ctypes.CDLL(ba.library_path())
+    c = ba.NufftConf.in_dll(L, "nufft_conf_defaults")
+    ref = ba.NufftConf.defaults()
+    for name, _ in ba.NufftConf._fields_:
+        assert getattr(c, name) == getattr(ref, name), name
+    assert c.width == 6.0 and c.os == 2.0 and c.flags == 7
+
+
+def test_missing_library_raises(monkeypatch, tmp_path):
+    monkeypatch.setattr(ba, "_LIB", None)
+    monkeypatch.setattr(ba, "_LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(ba.BartError):
+        ba.lib()

@@ -1,0 +1,291 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle (a C
+restatement of BART's NUFFT path) on identical seeded inputs.
+
+Tolerances: relative L2 <= 1e-5 for single operator applications (the
+north-star bar against BART), looser where FP32 error compounds over many CG
+iterations (stated per test).
+"""
+import numpy as np
+import pytest
+
+import bart_amd as ba
+import oracle as O
+from bart_amd import inputs as I
+from conftest import crandn, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def dims16(*d):
+    return list(d) + [1] * (16 - len(d))
+
+
+def make_nufft(img, C, traj, R, S, weights=None, **conf):
+    X, Y, Z = img
+    c = ba.NufftConf.defaults(**conf)
+    return ba.nufft_create2(dims16(1, R, S, C), dims16(X, Y, Z, C), dims16(3, R, S), ba.traj_array(traj),
+                            dims16(1, R, S) if weights is not None else None, weights, c)
+
+
+# --------------------------------------------------------------------------
+# FFT
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("n", [2, 8, 12, 30, 64, 128, 192, 256, 320, 384, 512, 640, 768, 56, 286, 1024])
+def test_fft_1d_matches_numpy(n):
+    x = crandn((5, n), n)
+    y = ba.fft([n, 1, 1, 5], 1, x)
+    ref = np.fft.fft(x.astype(np.complex128), axis=1)
+    assert relerr(y.reshape(5, n), ref) < 2e-6
+    z = ba.ifft([n, 1, 1, 5], 1, x)
+    refi = np.fft.ifft(x.astype(np.complex128), axis=1) * n
+    assert relerr(z.reshape(5, n), refi) < 2e-6
+
+
+@pytest.mark.parametrize("shape,flags", [((16, 24, 1), 3), ((64, 64, 1), 3), ((256, 256, 1), 3),
+                                         ((12, 10, 8), 7), ((32, 32, 32), 7), ((40, 1, 20), 5),
+                                         ((16, 24, 8), 2), ((16, 24, 8), 4)])
+def test_fft_nd_matches_numpy(shape, flags):
+    X, Y, Z = shape
+    x = crandn((3, Z, Y, X), X + Y + Z)
+    y = ba.fft([X, Y, Z, 3], flags, x).reshape(3, Z, Y, X)
+    axes = tuple(3 - a for a in range(3) if flags & (1 << a))
+    ref = np.fft.fftn(x.astype(np.complex128), axes=axes)
+    assert relerr(y, ref) < 3e-6
+
+
+# --------------------------------------------------------------------------
+# NUFFT forward / adjoint
+# --------------------------------------------------------------------------
+
+CASES_2D = [
+    dict(n=64, C=3, R=128, S=21, periodic=False),
+    dict(n=64, C=1, R=128, S=33, periodic=True),
+    dict(n=96, C=4, R=192, S=17, periodic=False),
+]
+
+
+@pytest.mark.parametrize("case", CASES_2D)
+def test_nufft_2d_forward_adjoint(case):
+    n, C, R, S = case["n"], case["C"], case["R"], case["S"]
+    traj = I.radial_traj(R, S)
+    cim = crandn((C, 1, n, n), 1)
+    ksp = crandn((C, R * S), 2)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=False, periodic=case["periodic"])
+    op = make_nufft((n, n, 1), C, traj, R, S, toeplitz=False, periodic=case["periodic"])
+    k = op.forward(cim).reshape(C, R * S)
+    assert relerr(k, ref.forward(cim)) < 1e-5
+    x = op.adjoint(ksp).reshape(C, 1, n, n)
+    assert relerr(x, ref.adjoint(ksp)) < 1e-5
+
+
+def test_nufft_2d_weights():
+    n, C, R, S = 64, 2, 128, 15
+    traj = I.radial_traj(R, S)
+    w = (0.5 + np.random.default_rng(3).random(R * S)).astype(np.complex64)
+    cim = crandn((C, 1, n, n), 4)
+    ksp = crandn((C, R * S), 5)
+    ref = O.Nufft((n, n, 1), traj, C, weights=w, toeplitz=False)
+    op = make_nufft((n, n, 1), C, traj, R, S, weights=w, toeplitz=False)
+    assert relerr(op.forward(cim).reshape(C, -1), ref.forward(cim)) < 1e-5
+    assert relerr(op.adjoint(ksp).reshape(C, 1, n, n), ref.adjoint(ksp)) < 1e-5
+
+
+def test_nufft_3d_forward_adjoint():
+    n, C = 24, 2
+    traj = I.phyllotaxis_kooshball(spokes=64, readout=48)
+    R, S = 48, 64
+    cim = crandn((C, n, n, n), 6)
+    ksp = crandn((C, R * S), 7)
+    ref = O.Nufft((n, n, n), traj, C, toeplitz=False)
+    op = make_nufft((n, n, n), C, traj, R, S, toeplitz=False)
+    assert relerr(op.forward(cim).reshape(C, -1), ref.forward(cim)) < 1e-5
+    assert relerr(op.adjoint(ksp).reshape(C, n, n, n), ref.adjoint(ksp)) < 1e-5
+
+
+def test_nufft_nonsquare_and_1d():
+    traj = I.radial_traj(64, 9)
+    traj[:, 1] *= 0.5
+    op = make_nufft((64, 32, 1), 2, traj, 64, 9, toeplitz=False)
+    ref = O.Nufft((64, 32, 1), traj, 2, toeplitz=False)
+    cim = crandn((2, 1, 32, 64), 8)
+    assert relerr(op.forward(cim).reshape(2, -1), ref.forward(cim)) < 1e-5
+    t1 = np.zeros((40, 3), np.complex64)
+    t1[:, 0] = np.linspace(-15.5, 15.5, 40)
+    op1 = make_nufft((32, 1, 1), 1, t1, 40, 1, toeplitz=False)
+    ref1 = O.Nufft((32, 1, 1), t1, 1, toeplitz=False)
+    x = crandn((1, 1, 1, 32), 9)
+    assert relerr(op1.forward(x).reshape(1, -1), ref1.forward(x)) < 1e-5
+
+
+def test_nufft_cartesian_is_unitary_centred_dft():
+    # tests/nufft.mk:28-36: Cartesian trajectory, periodic -> fftuc (nrmse 5e-5)
+    n = 64
+    traj = I.cartesian_traj(n, n)
+    img = I.ellipse_phantom(n, seed=11)
+    op = make_nufft((n, n, 1), 1, traj, n * n, 1, toeplitz=False, periodic=True)
+    k = op.forward(img).reshape(n, n)
+    F = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(img[0].astype(np.complex128)))) / n
+    assert relerr(k, F) < 5e-5
+
+
+def test_nufft_vs_fp64_nudft():
+    # tests/nufft.mk:41-49: NUFFT vs NUDFT on a radial trajectory (nrmse 2e-5)
+    n, R, S = 64, 128, 12
+    traj = I.radial_traj(R, S)
+    img = I.ellipse_phantom(n, seed=12)
+    op = make_nufft((n, n, 1), 1, traj, R, S, toeplitz=False, periodic=True)
+    k = op.forward(img).reshape(1, -1)
+    ref = O.nudft_forward((n, n, 1), traj, img)
+    assert relerr(k, ref) < 2e-5
+    a = op.adjoint(k.astype(np.complex64)).reshape(1, 1, n, n)
+    refa = O.nudft_adjoint((n, n, 1), traj, k)
+    assert relerr(a, refa) < 2e-5
+
+
+def test_nufft_adjoint_dot_test():
+    # utests/test_nufft.c:108-121 / tests/nufft.mk:90-104
+    n, C, R, S = 128, 2, 256, 64
+    traj = I.radial_traj(R, S)
+    op = make_nufft((n, n, 1), C, traj, R, S, toeplitz=False)
+    x = crandn((C, 1, n, n), 13)
+    y = crandn((C, R * S), 14)
+    Ax = op.forward(x)
+    Ahy = op.adjoint(y)
+    s1 = np.vdot(y.ravel().astype(np.complex128), Ax.astype(np.complex128))
+    s2 = np.vdot(Ahy.astype(np.complex128), x.ravel().astype(np.complex128))
+    assert abs(s1 - s2) / abs(s1) < 1e-5
+
+
+# --------------------------------------------------------------------------
+# PSF and Toeplitz normal operator
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("img,R,S,C,weighted", [((64, 64, 1), 128, 21, 2, False), ((48, 48, 1), 96, 13, 3, True),
+                                                ((20, 20, 20), 40, 48, 2, False)])
+def test_psf_and_toeplitz_normal(img, R, S, C, weighted):
+    X, Y, Z = img
+    traj = I.radial_traj(R, S) if Z == 1 else I.phyllotaxis_kooshball(spokes=S, readout=R)
+    w = (0.5 + np.random.default_rng(4).random(R * S)).astype(np.complex64) if weighted else None
+    ref = O.Nufft(img, traj, C, weights=w, toeplitz=True)
+    op = make_nufft(img, C, traj, R, S, weights=w, toeplitz=True)
+    psf = op.get_psf().reshape(ref.psf().shape)
+    assert relerr(psf, ref.psf()) < 1e-5
+    x = crandn((C, Z, Y, X), 15)
+    y = op.normal(x).reshape(C, Z, Y, X)
+    assert relerr(y, ref.normal(x)) < 1e-5
+
+
+def test_psf_update_roundtrip():
+    n, R, S = 32, 64, 11
+    traj = I.radial_traj(R, S)
+    op = make_nufft((n, n, 1), 1, traj, R, S, toeplitz=True)
+    psf = op.get_psf()
+    x = crandn((1, 1, n, n), 16)
+    y0 = op.normal(x)
+    op.update_psf(2.0 * psf)
+    y1 = op.normal(x)
+    assert relerr(y1, 2.0 * y0) < 1e-6
+
+
+# --------------------------------------------------------------------------
+# SENSE model (sense_nc_init) and CG
+# --------------------------------------------------------------------------
+
+def make_sense(n, C, R, S, Z=1, seed=0, lam_weights=None):
+    if Z == 1:
+        traj = I.radial_traj(R, S)
+        img = I.ellipse_phantom(n, seed=seed)
+    else:
+        traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
+        img = I.ellipsoid_phantom(n, nell=6, seed=seed)
+    maps = I.gaussian_coils((Z, n, n), C, seed=seed)
+    max_dims = dims16(n, n, Z, C)
+    op = ba.sense_nc_init(max_dims, dims16(n, n, Z, C), maps, dims16(1, R, S, C), dims16(3, R, S),
+                          ba.traj_array(traj))
+    ref = O.Nufft((n, n, Z), traj, C, toeplitz=True)
+    return op, ref, maps, img, traj
+
+
+def test_sense_forward_adjoint_normal():
+    n, C, R, S = 64, 4, 128, 31
+    op, ref, maps, img, traj = make_sense(n, C, R, S, seed=21)
+    k = op.forward(img).reshape(C, -1)
+    kr = ref.sense_forward(maps, img)
+    assert relerr(k, kr) < 1e-5
+    y = crandn((C, R * S), 22)
+    a = op.adjoint(y).reshape(1, n, n)
+    assert relerr(a, ref.sense_adjoint(maps, y)) < 1e-5
+    x = crandn((1, n, n), 23)
+    nn = op.normal(x).reshape(1, n, n)
+    assert relerr(nn, ref.sense_normal(maps, x)) < 1e-5
+
+
+def test_cgsense_matches_oracle():
+    # pics -S -w 1 -r 0.01 -i 30 -t: CG on (A^H A + lambda I) x = A^H y
+    n, C, R, S = 64, 4, 128, 41
+    op, ref, maps, img, traj = make_sense(n, C, R, S, seed=31)
+    y = I.add_noise(ref.sense_forward(maps, img), 1e-3, seed=31)
+    b = ref.sense_adjoint(maps, y)
+    bg = op.adjoint(y).reshape(1, n, n)
+    assert relerr(bg, b) < 1e-5
+    lam = 0.01
+    xr, itr, trr, _ = ref.cgsense(maps, b, lam, 30, 0.0)
+    x, st, tr = ba.lsqr_conjgrad(op, b, 30, lam, 0.0)
+    assert st.iterations == itr == 30
+    err = relerr(x.reshape(1, n, n), xr)
+    print(f"cg-sense 30 it: rel err vs oracle {err:.2e}")
+    assert err < 1e-4
+    # the recursively updated residual norms of FP32 CG follow each other
+    # closely only for the first iterations of an ill-conditioned problem
+    assert np.max(np.abs(tr[:6] / trr[:6] - 1)) < 1e-4
+    # deterministic: a second solve gives bitwise the same result
+    x2, st2, tr2 = ba.lsqr_conjgrad(op, b, 30, lam, 0.0)
+    assert np.array_equal(x2, x) and np.array_equal(tr2, tr)
+
+
+def test_cgsense_tolerance_and_warm_start():
+    n, C, R, S = 64, 4, 128, 41
+    op, ref, maps, img, traj = make_sense(n, C, R, S, seed=32)
+    y = ref.sense_forward(maps, img)
+    b = op.adjoint(y).reshape(1, n, n)
+    x, st, tr = ba.lsqr_conjgrad(op, b, 100, 0.01, 1e-4)
+    assert st.status == 2 and st.iterations < 100
+    assert st.resid <= 1e-4 * st.bnorm * 1.0001
+    # warm start from the solution converges immediately (pics -W)
+    x2, st2, _ = ba.lsqr_conjgrad(op, b, 100, 0.01, 1e-4, x0=x.reshape(1, n, n))
+    assert st2.iterations <= 2
+    # 0 iterations return the warm start unchanged (tests/pics.mk:184-190)
+    x3, st3, _ = ba.lsqr_conjgrad(op, b, 0, 0.01, 0.0, x0=x.reshape(1, n, n))
+    assert np.array_equal(x3.reshape(-1), x.reshape(-1))
+    # fallback: a bad warm start is replaced by zero
+    bad = (1e3 * crandn((1, n, n), 33)).astype(np.complex64)
+    x4, st4, _ = ba.lsqr_conjgrad(op, b, 100, 0.01, 1e-4, x0=bad, fallback=True)
+    assert st4.fell_back and st4.status == 2
+    assert np.array_equal(x4.reshape(-1), x.reshape(-1))
+
+
+def test_device_pointers_with_torch():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("torch's bundled HIP runtime was initialised after this library's")
+    n, C, R, S = 64, 2, 128, 21
+    traj = I.radial_traj(R, S)
+    op = make_nufft((n, n, 1), C, traj, R, S, toeplitz=True)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    x = crandn((C, 1, n, n), 41)
+    xd = torch.from_numpy(x.reshape(-1)).to("cuda")
+    k = op.forward(xd)
+    assert k.is_cuda
+    assert relerr(k.cpu().numpy().reshape(C, -1), ref.forward(x)) < 1e-5
+    nn = op.normal(xd)
+    assert relerr(nn.cpu().numpy().reshape(x.shape), ref.normal(x)) < 1e-5
+
+
+def test_errors_are_reported():
+    with pytest.raises(ba.BartError):
+        ba.nufft_create2(dims16(1, 10, 1, 2), dims16(16, 16, 1, 3), dims16(3, 10, 1),
+                         np.zeros(30, np.complex64))
+    with pytest.raises(ba.BartError):
+        ba.nufft_create2(dims16(1, 10, 1, 1), dims16(16, 16, 1, 1), dims16(3, 10, 1),
+                         np.zeros(30, np.complex64), conf=ba.NufftConf.defaults(os=1.5))

@@ -1,0 +1,215 @@
+"""CPU tests that pin the oracle (oracle/bart_oracle.c) before it is trusted.
+
+Each check is chosen so that a plausible mistake in the restatement -- a
+dropped factor, a wrong sign or centre, a transposed index -- fails it:
+closed forms (KB table, beta, rolloff, fftmod), library routines (numpy FFT),
+the FP64 direct NUDFT (BART's own cross-check, tests/nufft.mk:41-49), the
+Cartesian special case (tests/nufft.mk:28-36), adjoint dot-tests
+(utests/test_nufft.c:108-121), linearity (tests/nufft.mk:108-135), the DC
+value of utests/test_nufft.c:82-104, Toeplitz = exact normal operator, and CG
+against a direct FP64 solve.
+"""
+import numpy as np
+import pytest
+from scipy.special import i0
+
+import oracle as O
+from bart_amd import inputs as I
+from conftest import crandn, relerr
+
+
+def test_beta_closed_form():
+    # noncart/grid.c:516-519 at os 2, width 6 (decomposed width 3)
+    assert abs(O.calc_beta(2.0, 6.0) - np.pi * np.sqrt(4.5 ** 2 - 0.8)) < 1e-12
+    assert abs(O.calc_beta(2.0, 6.0) - 13.8551) < 1e-4
+
+
+def test_bessel_i0_matches_scipy():
+    for x in [0.0, 0.5, 3.0, 7.74, 7.76, 13.8551, 25.0]:
+        assert abs(O.bessel_i0(x) / i0(x) - 1.0) < 1e-14
+
+
+def test_kb_table_closed_form():
+    beta = O.calc_beta(2.0, 6.0)
+    t = O.kb_table(beta)
+    x = np.arange(1001) / 999.0 / 2.0
+    ref = np.where(np.abs(x) >= 0.5, 0.0, i0(beta * np.sqrt(np.clip(1 - (2 * x) ** 2, 0, None))) / i0(beta))
+    assert t[0] == 1.0 and t[999] == 0.0 and t[1000] == 0.0
+    assert np.max(np.abs(t - ref.astype(np.float32))) < 1e-7
+    assert np.all(np.diff(t[:1000]) <= 0)
+
+
+def test_rolloff_closed_form():
+    beta = O.calc_beta(2.0, 6.0)
+    n = 16
+    r = O.rolloff((n, 1, 1))[0, 0]
+    od = 2 * n
+    x = (np.arange(n) + (od // 2 - n // 2) - od // 2) / od
+    a = beta ** 2 - (np.pi * x * 6.0) ** 2
+    ftkb = np.sinh(np.sqrt(a)) / np.sqrt(a) / i0(beta)
+    assert relerr(r, 1.0 / ftkb / 6.0) < 1e-6
+
+
+@pytest.mark.parametrize("n", [8, 12, 6, 7])
+def test_fftmod_closed_form(n):
+    f = O.fftmod_table((n, 1, 1))[0, 0]
+    c = n // 2
+    ph = (np.arange(n) - c / 2.0) * (c / n)
+    assert relerr(f, np.exp(2j * np.pi * ph)) < 1e-6
+    if n % 4 == 0:
+        # checkerboard (num/fft.c:92-129)
+        s = -1.0 if n % 8 else 1.0
+        assert np.array_equal(f.real, s * (-1.0) ** np.arange(n)) and np.all(f.imag == 0)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 30), (1, 12, 20), (8, 6, 10), (1, 1, 256), (1, 64, 64)])
+def test_fft_matches_numpy(shape):
+    x = crandn((2,) + shape, 1)
+    axes = tuple(ax for ax in (1, 2, 3) if x.shape[ax] > 1)
+    assert relerr(O.fft(x, -1), np.fft.fftn(x.astype(np.complex128), axes=axes)) < 1e-6
+    assert relerr(O.fft(x, +1), np.fft.ifftn(x.astype(np.complex128), axes=axes) * np.prod([x.shape[a] for a in axes])) < 1e-6
+
+
+def test_cartesian_nufft_is_fftuc():
+    n = 32
+    img = I.ellipse_phantom(n, seed=1)
+    nf = O.Nufft((n, n, 1), I.cartesian_traj(n, n), 1, toeplitz=False, periodic=True)
+    k = nf.forward(img[None]).reshape(n, n)
+    F = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(img[0].astype(np.complex128)))) / n
+    assert relerr(k, F) < 5e-5
+
+
+@pytest.mark.parametrize("periodic,tol", [(True, 2e-5), (False, 1e-3)])
+def test_radial_nufft_vs_nudft(periodic, tol):
+    # BART compares with the periodic NUFFT (nufft -P, tests/nufft.mk:41-49);
+    # the non-periodic one drops the taps beyond the grid edge at |k| ~ n/2
+    n, R, S = 64, 128, 12
+    traj = I.radial_traj(R, S)
+    img = I.ellipse_phantom(n, seed=3)
+    nf = O.Nufft((n, n, 1), traj, 1, toeplitz=False, periodic=periodic)
+    k = nf.forward(img[None])
+    assert relerr(k, O.nudft_forward((n, n, 1), traj, img)) < tol
+    a = nf.adjoint(k)
+    assert relerr(a, O.nudft_adjoint((n, n, 1), traj, k)) < tol
+
+
+def test_3d_nufft_vs_nudft():
+    n = 16
+    traj = I.phyllotaxis_kooshball(spokes=40, readout=32)
+    img = crandn((n, n, n), 2)
+    nf = O.Nufft((n, n, n), traj, 1, toeplitz=False, periodic=True)
+    assert relerr(nf.forward(img[None]), O.nudft_forward((n, n, n), traj, img)) < 2e-5
+
+
+def test_odd_size_nufft_vs_centred_dft():
+    # tests/nufft.mk:516-527 (test-nufft-odd-forward2): for odd sizes BART's
+    # decomposed NUFFT agrees with the unitary centred FFT only to nrmse 0.02
+    n = 127
+    img = I.ellipse_phantom(n, seed=1, phase=False)
+    nf = O.Nufft((n, n, 1), I.cartesian_traj(n, n), 1, toeplitz=False)
+    k = nf.forward(img[None]).reshape(n, n)
+    c = n // 2
+    j = np.arange(n)
+    E = np.exp(-2j * np.pi * np.outer(j - c, j - c) / n) / np.sqrt(n)
+    assert relerr(k, E @ img[0].astype(np.complex128) @ E.T) < 0.02
+
+
+def test_dc_of_ones():
+    # utests/test_nufft.c:82-104: forward of an all-ones 8x8 image at k = 0 is 8
+    traj = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0]], np.complex64)
+    nf = O.Nufft((8, 8, 1), traj, 1, toeplitz=False)
+    k = nf.forward(np.ones((1, 1, 8, 8), np.complex64))
+    assert abs(k[0, 0] - 8.0) < 0.02
+
+
+def test_adjoint_dot_test():
+    n, C = 64, 2
+    traj = I.radial_traj(128, 64)
+    nf = O.Nufft((n, n, 1), traj, C, toeplitz=False)
+    x = crandn((C, 1, n, n), 4)
+    y = crandn((C, nf.M), 5)
+    s1 = np.vdot(y.astype(np.complex128), nf.forward(x))
+    s2 = np.vdot(nf.adjoint(y).astype(np.complex128), x)
+    assert abs(s1 - s2) / abs(s1) < 1e-6
+
+
+def test_adjoint_linearity_over_spokes():
+    # tests/nufft.mk:108-135: adjoint of the whole trajectory = sum of parts
+    n = 64
+    traj = I.radial_traj(128, 40)
+    k = crandn((1, traj.shape[0]), 6)
+    whole = O.Nufft((n, n, 1), traj, 1, toeplitz=False).adjoint(k)
+    parts = 0
+    for a, b in [(0, 1280), (1280, 2560), (2560, 5120)]:
+        parts = parts + O.Nufft((n, n, 1), traj[a:b], 1, toeplitz=False).adjoint(k[:, a:b])
+    assert relerr(parts, whole) < 1e-6
+
+
+def test_weights_enter_as_w_and_conj_w():
+    n = 32
+    traj = I.radial_traj(64, 9)
+    w = (0.3 + np.random.default_rng(7).random(traj.shape[0]) + 0.2j).astype(np.complex64)
+    x = crandn((1, 1, n, n), 8)
+    y = crandn((1, traj.shape[0]), 9)
+    nw = O.Nufft((n, n, 1), traj, 1, weights=w, toeplitz=False)
+    n0 = O.Nufft((n, n, 1), traj, 1, toeplitz=False)
+    assert relerr(nw.forward(x), n0.forward(x) * w[None]) < 1e-6
+    assert relerr(nw.adjoint(y), n0.adjoint(y * np.conj(w)[None])) < 1e-6
+
+
+def test_psf_matches_exact_psf():
+    # the PSF is the NUFFT-gridded version of the exact one (doc/nufft.md:13-43):
+    # P(2n grid) = fftuc(NUDFT^H_2n(ones; 2 traj)), decomposed
+    n = 16
+    traj = I.radial_traj(32, 7)
+    psf = O.compute_psf((n, n, 1), traj)
+    m = 2 * n
+    a = O.nudft_adjoint((m, m, 1), 2 * np.real(traj).astype(np.float64), np.ones((1, traj.shape[0])))[0, 0]
+    F = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(a))) / m
+    ref = np.stack([F[r1::2, r0::2] for r1 in (0, 1) for r0 in (0, 1)])
+    ref = ref[[0, 1, 2, 3]]
+    assert relerr(psf[:, 0], ref) < 2e-5
+
+
+def test_toeplitz_equals_exact_normal():
+    n = 32
+    traj = I.radial_traj(64, 23)
+    nf = O.Nufft((n, n, 1), traj, 1, toeplitz=True)
+    x = crandn((1, 1, n, n), 10)
+    ref = O.nudft_adjoint((n, n, 1), traj, O.nudft_forward((n, n, 1), traj, x))
+    assert relerr(nf.normal(x), ref) < 1e-5
+
+
+def test_cgsense_matches_direct_solve():
+    n, C = 16, 3
+    traj = I.radial_traj(32, 15)
+    maps = I.gaussian_coils((1, n, n), C, seed=2)
+    img = I.ellipse_phantom(n, seed=2)
+    nf = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    y = nf.sense_forward(maps, img)
+    b = nf.sense_adjoint(maps, y)
+    lam = 0.05
+    # dense FP64 normal matrix via the (accurate) Toeplitz operator
+    N = n * n
+    A = np.zeros((N, N), np.complex128)
+    for j in range(N):
+        e = np.zeros(N, np.complex64)
+        e[j] = 1
+        A[:, j] = nf.sense_normal(maps, e.reshape(1, n, n)).ravel()
+    xs = np.linalg.solve(A + lam * np.eye(N), b.ravel().astype(np.complex128))
+    x, it, tr, res = nf.cgsense(maps, b, lam, 200, 1e-6)
+    assert it < 200
+    assert relerr(x.ravel(), xs) < 1e-4
+    assert res <= 1e-6 * np.linalg.norm(b) * 1.001
+    assert np.all(np.diff(tr) < 1e3)
+
+
+def test_cg_zero_iterations_returns_warm_start():
+    n, C = 16, 2
+    traj = I.radial_traj(32, 9)
+    maps = I.gaussian_coils((1, n, n), C, seed=3)
+    nf = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    x0 = crandn((1, n, n), 11)
+    b = crandn((1, n, n), 12)
+    x, it, _, _ = nf.cgsense(maps, b, 0.01, 0, 0.0, x0=x0)
+    assert it == 0 and np.array_equal(x, x0)
