@@ -13,6 +13,7 @@
 //     once into a hipGraph and replayed; after convergence every kernel sees
 //     the device stop flag and returns immediately.
 #include "cg.hpp"
+#include "prof.hpp"
 
 #include <cmath>
 
@@ -277,6 +278,7 @@ void CgSolver::drop_graph()
 void CgSolver::enqueue_iteration(const NormalFn& normal, float lambda)
 {
 	normal((float2*)Ap.ptr, (const float2*)p.ptr, &st.ptr->done);
+	ProfScope prof("cg_vector_ops", stream);
 	hipLaunchKernelGGL(k_cg_apdot, dim3(nblocks), dim3(CG_THREADS), 0, stream, n, lambda, p.ptr, Ap.ptr, part.ptr, st.ptr);
 	hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(CG_THREADS), 0, stream, nblocks, part.ptr, st.ptr);
 	hipLaunchKernelGGL(k_cg_xr, dim3(nblocks), dim3(CG_THREADS), 0, stream, n, x.ptr, r.ptr, p.ptr, Ap.ptr, part.ptr, st.ptr);
@@ -344,10 +346,18 @@ CgResult CgSolver::solve(const NormalFn& normal, float lambda, int maxiter, floa
 
 			drop_graph();
 			BAM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+			prof_set_capturing(true);
 
-			for (int i = 0; i < maxiter; i++)
-				enqueue_iteration(normal, lambda);
+			try {
+				for (int i = 0; i < maxiter; i++)
+					enqueue_iteration(normal, lambda);
+			} catch (...) {
+				prof_set_capturing(false);
+				(void)hipStreamEndCapture(stream, &graph);
+				throw;
+			}
 
+			prof_set_capturing(false);
 			BAM_HIP(hipStreamEndCapture(stream, &graph));
 			BAM_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
 			graph_iters = maxiter;

@@ -87,6 +87,27 @@ FftLaunchCfg fft_config(const AxisPlan& plan, long total, bool contig, int P_min
 	FftLaunchCfg c;
 	const int n = plan.n;
 
+	if (ct_supported(n) && ct_block_p(n) >= P_min) {
+
+		const int P = ct_block_p(n);
+		int logP = 0;
+		while ((1 << logP) < P)
+			logP++;
+
+		c.L.plan = plan;
+		c.L.P = P;
+		c.L.logP = logP;
+		c.L.contig = contig;
+		c.L.total = total;
+		c.L.sign1 = sign1;
+		c.L.sign2 = sign2;
+		c.blocks = (unsigned)((total + P - 1) / P);
+		c.lds_bytes = (size_t)(P + 1) * n * sizeof(float2);
+		c.ept = 0;
+		c.fixed = true;
+		return c;
+	}
+
 	// transforms per block: aim at 2048-4096 elements; strided geometry
 	// needs >= 16 adjacent columns for 128-byte coalescing
 	int P = 1;

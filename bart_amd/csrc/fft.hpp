@@ -16,6 +16,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "prof.hpp"
 
 namespace bam {
 
@@ -352,6 +353,192 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftLaunch L, Ld ld, Md
 	st.store(lds, L, t0);
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time lengths.  For the sizes of the image and oversampled grids the
+// radix plan, the butterfly index arithmetic and the block shape are
+// constants, and the twiddles are staged in LDS.
+// ---------------------------------------------------------------------------
+
+constexpr int ct_next_radix(int m)
+{
+	return (0 == m % 8) ? 8 : ((0 == m % 4) ? 4 : ((0 == m % 2) ? 2 : ((0 == m % 3) ? 3 : ((0 == m % 5) ? 5 : m))));
+}
+
+constexpr int ct_radix(int n, int idx)
+{
+	int m = n;
+	for (int i = 0; i < idx; i++)
+		m /= ct_next_radix(m);
+	return ct_next_radix(m);
+}
+
+constexpr int ct_num_radices(int n)
+{
+	int m = n, c = 0;
+	while (m > 1) {
+		m /= ct_next_radix(m);
+		c++;
+	}
+	return c;
+}
+
+// transforms per block for a compile-time length (a power of two): up to
+// 4096 elements, at least 16 columns (strided geometry) up to n = 512
+constexpr int ct_block_p(int n)
+{
+	int p = 1;
+	while (2 * p * n <= 4096)
+		p *= 2;
+	return (n <= 512 && p < 16) ? 16 : ((p < 8) ? 8 : p);
+}
+
+template <int N, int R, int NS, int P>
+__device__ inline void ct_pass(float2* lds, const float2* tw, float sgn)
+{
+	constexpr int nR = N / R;
+	constexpr int NB = P * nR;
+	constexpr int PER = (NB + FFT_THREADS - 1) / FFT_THREADS;
+	constexpr int TSTEP = N / (NS * R);
+	float2 v[PER][R];
+	int dst[PER];
+
+#pragma unroll
+	for (int i = 0; i < PER; i++) {
+
+		const int b = threadIdx.x + i * FFT_THREADS;
+		dst[i] = -1;
+
+		if (0 == NB % FFT_THREADS || b < NB) {
+
+			const int p = b / nR;
+			const int j = b - p * nR;
+			const int k = j % NS;
+			const float2* src = lds + p * N + j;
+
+#pragma unroll
+			for (int r = 0; r < R; r++) {
+
+				float2 x = src[r * nR];
+
+				if (r > 0 && NS > 1) {
+					float2 w = tw[r * k * TSTEP];
+					if (sgn > 0.f)
+						w.y = -w.y;
+					x = cmul(x, w);
+				}
+
+				v[i][r] = x;
+			}
+
+			dft_any<R>(v[i], sgn, tw, N);
+			dst[i] = p * N + (j / NS) * NS * R + k;
+		}
+	}
+
+	__syncthreads();
+
+#pragma unroll
+	for (int i = 0; i < PER; i++)
+		if (0 == NB % FFT_THREADS || dst[i] >= 0)
+#pragma unroll
+			for (int q = 0; q < R; q++)
+				lds[dst[i] + q * NS] = v[i][q];
+
+	__syncthreads();
+}
+
+template <int N, int P, int IDX, int NS>
+__device__ inline void ct_passes(float2* lds, const float2* tw, float sgn)
+{
+	if constexpr (IDX < ct_num_radices(N)) {
+		constexpr int R = ct_radix(N, IDX);
+		ct_pass<N, R, NS, P>(lds, tw, sgn);
+		ct_passes<N, P, IDX + 1, NS * R>(lds, tw, sgn);
+	}
+}
+
+template <int N, class Ld, class Md, class St>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, Md md, St st)
+{
+	constexpr int P = ct_block_p(N);
+	constexpr int TOT = P * N;
+	constexpr int LOGP = (P >= 64) ? 6 : ((P >= 32) ? 5 : ((P >= 16) ? 4 : ((P >= 8) ? 3 : ((P >= 4) ? 2 : ((P >= 2) ? 1 : 0)))));
+
+	extern __shared__ float2 lds[];
+	float2* tw = lds + TOT;
+
+	if (nullptr != L.skip && 0 != *L.skip)
+		return;
+
+	const long t0 = (long)blockIdx.x * P;
+
+	for (int i = threadIdx.x; i < N; i += FFT_THREADS)
+		tw[i] = L.plan.tw[i];
+
+	if (L.contig) {
+
+#pragma unroll 4
+		for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+			const int p = i / N;
+			const int k = i - p * N;
+			const long t = t0 + p;
+			lds[i] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+		}
+
+	} else {
+
+#pragma unroll 4
+		for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+			const int p = i & (P - 1);
+			const int k = i >> LOGP;
+			const long t = t0 + p;
+			lds[p * N + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+		}
+	}
+
+	__syncthreads();
+
+	ct_passes<N, P, 0, 1>(lds, tw, L.sign1);
+
+	if constexpr (Md::active) {
+
+		for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+
+			int p, k;
+
+			if (L.contig) {
+				p = i / N;
+				k = i - p * N;
+			} else {
+				p = i & (P - 1);
+				k = i >> LOGP;
+			}
+
+			const long t = t0 + p;
+
+			if (t < L.total)
+				lds[p * N + k] = md(t, k, lds[p * N + k]);
+		}
+
+		__syncthreads();
+
+		ct_passes<N, P, 0, 1>(lds, tw, L.sign2);
+	}
+
+	st.store(lds, L, t0);
+}
+
+// lengths with a compile-time kernel
+inline bool ct_supported(int n)
+{
+	switch (n) {
+	case 16: case 32: case 64: case 128: case 192: case 256: case 320: case 384: case 512: case 640: case 768:
+		return true;
+	default:
+		return false;
+	}
+}
+
 // Helper for element-wise stores: calls f(t, k, value) for every element of
 // the block in an order that is coalesced for the launch's geometry.
 template <class F>
@@ -388,6 +575,8 @@ struct FftLaunchCfg {
 	unsigned blocks;
 	size_t lds_bytes;
 	int ept;
+	bool fixed = false;  // compile-time length kernel
+	const char* name = "fft_axis";
 };
 
 FftLaunchCfg fft_config(const AxisPlan& plan, long total, bool contig, int P_min, float sign1, float sign2);
@@ -397,6 +586,35 @@ void launch_fft_axis(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, 
 {
 	if (0 == cfg.blocks)
 		return;
+
+	ProfScope prof(cfg.name, stream);
+
+	if (cfg.fixed) {
+
+		const dim3 gb(cfg.blocks), tb(FFT_THREADS);
+		const size_t lds = cfg.lds_bytes;
+
+#define BAM_FFT_N(NN) case NN: hipLaunchKernelGGL((k_fft_axis_n<NN, Ld, Md, St>), gb, tb, lds, stream, cfg.L, ld, md, st); break
+		switch (cfg.L.plan.n) {
+			BAM_FFT_N(16);
+			BAM_FFT_N(32);
+			BAM_FFT_N(64);
+			BAM_FFT_N(128);
+			BAM_FFT_N(192);
+			BAM_FFT_N(256);
+			BAM_FFT_N(320);
+			BAM_FFT_N(384);
+			BAM_FFT_N(512);
+			BAM_FFT_N(640);
+			BAM_FFT_N(768);
+		default:
+			break;
+		}
+#undef BAM_FFT_N
+
+		BAM_HIP(hipGetLastError());
+		return;
+	}
 
 	bool gen = false;
 

@@ -1,18 +1,21 @@
 // grid.hip -- Kaiser-Bessel gridding for gfx950.
 //
 // Maths of noncart/grid.c:153-253 and :403-512 (tap range, clamp or wrap,
-// separable weights from the 1001-entry KB table with linear interpolation),
-// with a different execution strategy from gpu_grid.cu:
-//   * samples are bin-sorted into tiles of the grid once per trajectory
-//     (the plan lives with the operator; BART rebuilds it per call);
-//   * adjoint (k-space -> grid): one work-group per (tile chunk, sub-grid,
-//     coil chunk) accumulates into an LDS-resident window (tile + halo) with
-//     LDS float atomics, then adds the window to the grid with global
-//     atomics -- the only global atomics are these dense tile flushes;
-//   * forward (grid -> k-space): one thread per sorted sample gathers all
-//     taps of all sub-grids, so neighbouring lanes read neighbouring grid
-//     lines; no atomics.
+// separable weights from the 1001-entry KB table with linear interpolation,
+// weight product in the order z, y, x).  Execution differs from
+// gpu_grid.cu:
+//   * samples are sorted once per trajectory by the cell (2^d grid points)
+//     that holds their position; the plan lives with the operator;
+//   * adjoint (k-space -> grid) is output-stationary: a work-group owns a tile
+//     of the sub-grids for a group of coils, accumulates every tap that lands
+//     in the tile from the samples of the tile's cells and the ring of
+//     neighbouring cells into LDS, and stores the tile with plain coalesced
+//     writes.  There are no global atomics and no zero-fill of the grid;
+//   * forward (grid -> k-space): a work-group stages the tile plus halo of one
+//     sub-grid and a group of coils in LDS and gathers the taps of the tile's
+//     own samples from there.
 #include "nufft_impl.hpp"
+#include "prof.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -67,8 +70,10 @@ __device__ inline float kb_lookup(const float* table, float x)
 }
 
 // ---------------------------------------------------------------------------
-// Binning plan
+// Plan: cell sort and tilings
 // ---------------------------------------------------------------------------
+
+constexpr int MAX_RANGES = 256;
 
 // sample position on the grid, pos = os (k + shift) + n / 2, rounded after
 // each operation as in noncart/grid.c:181-187
@@ -81,65 +86,165 @@ static float host_pos(const GridGeom& g, int s, int d, float k)
 }
 #pragma clang fp contract(on)
 
-void GridPlan::build(const GridGeom& geom, const float* traj, int tile_override)
-{
-	g = geom;
+// LDS of one adjoint work-group (gfx950 has 160 KiB per CU)
+static const size_t LDS_BUDGET = 144 * 1024;
 
+static void finish_tiling(const GridGeom& g, GridTiling& t)
+{
+	t.ntiles = 1;
+	t.Wtot = 1;
+
+	for (int d = 0; d < 3; d++) {
+		t.nt[d] = (g.n[d] + t.T[d] - 1) / t.T[d];
+		t.ntiles *= t.nt[d];
+		t.lo[d] = 0;
+		t.W[d] = t.T[d];
+		t.Wtot *= t.W[d];
+	}
+
+	t.lds = (size_t)(t.Wtot * t.sg + 2) * sizeof(float2) * t.cc + 6144;
+}
+
+// Adjoint tiling: a work-group has one lane per (tile point, sub-grid), at
+// most 1024, and accumulates up to 16 coils per lane in registers; 3-d tiles
+// are 8 x 4 x 4, 2-d 16 x 8, 1-d 128.
+static GridTiling choose_tiling(const GridGeom& g)
+{
 	int nd = 0;
 	for (int d = 0; d < 3; d++)
+		nd += (g.n[d] > 1);
+
+	const int Tdef3[3] = { 8, 4, 4 };
+	const int Tdef2[3] = { 16, 8, 1 };
+
+	GridTiling t = {};
+	int k = 0;
+
+	for (int d = 0; d < 3; d++) {
+
+		if (1 == g.n[d]) {
+			t.T[d] = 1;
+			continue;
+		}
+
+		int want = (3 == nd) ? Tdef3[k] : ((2 == nd) ? Tdef2[k] : 128);
+		k++;
+		t.T[d] = std::min(want, g.n[d]);
+	}
+
+	t.sg = g.S;
+	t.cc = (g.C <= 4) ? 4 : ((g.C <= 8) ? 8 : 16);
+	finish_tiling(g, t);
+
+	while (t.Wtot * g.S > 1024) {
+		int dm = 0;
+		for (int d = 1; d < 3; d++)
+			if (t.T[d] > t.T[dm])
+				dm = d;
+		t.T[dm] = std::max(1, t.T[dm] / 2);
+		finish_tiling(g, t);
+	}
+
+	// cell rows visited by one work-group must fit the range list
+	long rows = 2;
+	for (int d = 1; d < 3; d++)
 		if (g.n[d] > 1)
-			nd++;
+			rows *= (long)ceil((t.T[d] + g.reach_lo + g.reach_hi) / g.CS[d]) + 2;
+	BAM_CHECK(rows <= MAX_RANGES, "gridding: tile too large for the cell range list");
 
-	// tile size: 2-d 16 x 16, 3-d 8 x 8 x 8, 1-d 64 (overridable)
-	int Tdef = (3 == nd) ? 8 : ((2 == nd) ? 16 : 64);
+	return t;
+}
 
-	if (tile_override > 0)
-		Tdef = tile_override;
+// cells [clo, chi] of one dim as up to two spans {a0, n0, a1, n1} (wrapped
+// for periodic grids, clamped otherwise, every cell at most once); the
+// device twin is cell_span()
+static void host_cell_span(const GridGeom& g, int d, int clo, int chi, int sp[4])
+{
+	const int nc = g.ncell[d];
+	sp[2] = 0;
+	sp[3] = 0;
+
+	if (chi - clo + 1 >= nc) {
+		sp[0] = 0;
+		sp[1] = nc;
+		return;
+	}
+
+	if (!g.periodic) {
+		clo = std::max(clo, 0);
+		chi = std::min(chi, nc - 1);
+		sp[0] = clo;
+		sp[1] = std::max(0, chi - clo + 1);
+		return;
+	}
+
+	int lo = ((clo % nc) + nc) % nc;
+	int len = chi - clo + 1;
+	sp[0] = lo;
+
+	if (lo + len <= nc) {
+		sp[1] = len;
+	} else {
+		sp[1] = nc - lo;
+		sp[3] = len - sp[1];
+	}
+}
+
+void GridPlan::build(const GridGeom& geom, const float* traj)
+{
+	g = geom;
 
 	float maxshift = 0.f;
 	for (int s = 0; s < g.S; s++)
 		for (int d = 0; d < 3; d++)
 			maxshift = std::max(maxshift, fabsf(g.shift[s][d] - g.shift[0][d]));
 
-	g.Wtot = 1;
+	g.reach_lo = 0.5f * g.width + g.os * maxshift;
+	g.reach_hi = 0.5f * g.width;
 
+	// distinct shift values per dim (the decomposition uses two: 0 and -1/2,
+	// plus the odd-size correction) and the variant of every sub-grid
 	for (int d = 0; d < 3; d++) {
 
-		if (g.n[d] > 1) {
+		g.shv[d][0] = g.shift[0][d];
+		g.shv[d][1] = g.shift[0][d];
+		g.nshv[d] = 1;
 
-			g.T[d] = std::min(Tdef, g.n[d]);
-			g.nt[d] = (g.n[d] + g.T[d] - 1) / g.T[d];
-			g.lo[d] = (int)ceil(0.5 * g.width + g.os * maxshift) + 1;
-			g.hi[d] = (int)ceil(0.5 * g.width) + 1;
-			g.W[d] = g.T[d] + g.lo[d] + g.hi[d];
+		for (int s = 0; s < g.S; s++) {
 
-		} else {
-
-			g.T[d] = 1;
-			g.nt[d] = 1;
-			g.lo[d] = 0;
-			g.hi[d] = 0;
-			g.W[d] = 1;
+			if (g.shift[s][d] == g.shv[d][0]) {
+				g.sbit[s][d] = 0;
+			} else if (1 == g.nshv[d] || g.shift[s][d] == g.shv[d][1]) {
+				g.shv[d][1] = g.shift[s][d];
+				g.nshv[d] = 2;
+				g.sbit[s][d] = 1;
+			} else {
+				BAM_CHECK(false, "gridding: more than two shifts per dim");
+			}
 		}
-
-		g.Wtot *= g.W[d];
 	}
 
-	long ntiles = (long)g.nt[0] * g.nt[1] * g.nt[2];
-	std::vector<int> tile(g.M);
-	std::vector<long> count(ntiles + 1, 0);
+	long ncells = 1;
 
+	for (int d = 0; d < 3; d++) {
+		g.CS[d] = (g.n[d] > 1) ? 2 : 1;
+		g.ncell[d] = (g.n[d] + g.CS[d] - 1) / g.CS[d];
+		ncells *= g.ncell[d];
+	}
+
+	std::vector<int> cell(g.M);
 	std::vector<int> wraps(g.M);
+	std::vector<int> count(ncells + 1, 0);
 
 	for (long m = 0; m < g.M; m++) {
 
-		int tb[3];
+		int cb[3];
 		int code = 0;
 
 		for (int d = 0; d < 3; d++) {
 
 			if (1 == g.n[d]) {
-				tb[d] = 0;
+				cb[d] = 0;
 				code |= 8 << (4 * d);
 				continue;
 			}
@@ -158,25 +263,25 @@ void GridPlan::build(const GridGeom& geom, const float* traj, int tile_override)
 
 			code |= (q + 8) << (4 * d);
 
-			int b = (int)floorf(p / (float)g.T[d]);
-			tb[d] = std::min(std::max(b, 0), g.nt[d] - 1);
+			int b = (int)floorf(p / (float)g.CS[d]);
+			cb[d] = std::min(std::max(b, 0), g.ncell[d] - 1);
 		}
 
 		wraps[m] = code;
-		tile[m] = tb[0] + g.nt[0] * (tb[1] + g.nt[1] * tb[2]);
-		count[tile[m] + 1]++;
+		cell[m] = cb[0] + g.ncell[0] * (cb[1] + g.ncell[1] * cb[2]);
+		count[cell[m] + 1]++;
 	}
 
-	for (long t = 0; t < ntiles; t++)
-		count[t + 1] += count[t];
+	for (long c = 0; c < ncells; c++)
+		count[c + 1] += count[c];
 
-	std::vector<int> hsm(g.M);
-	std::vector<float4> hsk(g.M);
-	std::vector<long> pos(count.begin(), count.end() - 1);
+	std::vector<int> hsm(std::max<long>(g.M, 1));
+	std::vector<float4> hsk(std::max<long>(g.M, 1));
+	std::vector<int> pos(count.begin(), count.end() - 1);
 
 	for (long m = 0; m < g.M; m++) {
 
-		long i = pos[tile[m]]++;
+		long i = pos[cell[m]]++;
 		hsm[i] = (int)m;
 		int code = wraps[m];
 		float fcode;
@@ -184,255 +289,585 @@ void GridPlan::build(const GridGeom& geom, const float* traj, int tile_override)
 		hsk[i] = make_float4(traj[3 * m + 0], traj[3 * m + 1], traj[3 * m + 2], fcode);
 	}
 
-	// work items: tiles split into chunks of at most CH samples
-	const long CH = (3 == nd) ? 512 : 1024;
-	std::vector<int4> hw;
-
-	for (long t = 0; t < ntiles; t++)
-		for (long b = count[t]; b < count[t + 1]; b += CH)
-			hw.push_back(make_int4((int)t, (int)b, (int)std::min(b + CH, count[t + 1]), 0));
-
-	nwork = (int)hw.size();
-
-	// coils per adjoint work-group: keep the LDS window <= 56 KiB
-	cc_adj = std::max(1, std::min(g.C, (int)((56 * 1024) / ((long)g.Wtot * 8))));
-	cc_fwd = std::min(g.C, 16);
-
-	BAM_CHECK((long)g.Wtot * 8 <= 56 * 1024, "gridding window too large (%d points)", g.Wtot);
 	BAM_CHECK(g.M < (1l << 31), "too many samples");
 
-	hipStream_t st = 0;
+	ta = choose_tiling(g);
+
 	sm.alloc(std::max<long>(g.M, 1));
 	sk.alloc(std::max<long>(g.M, 1));
-	work.alloc(std::max<int>(nwork, 1));
+	cstart.alloc(ncells + 1);
 
-	if (g.M > 0) {
-		BAM_HIP(hipMemcpyAsync(sm.ptr, hsm.data(), sizeof(int) * g.M, hipMemcpyHostToDevice, st));
-		BAM_HIP(hipMemcpyAsync(sk.ptr, hsk.data(), sizeof(float4) * g.M, hipMemcpyHostToDevice, st));
+	BAM_HIP(hipMemcpy(sm.ptr, hsm.data(), sizeof(int) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
+	BAM_HIP(hipMemcpy(sk.ptr, hsk.data(), sizeof(float4) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
+	BAM_HIP(hipMemcpy(cstart.ptr, count.data(), sizeof(int) * (ncells + 1), hipMemcpyHostToDevice));
+
+	int nch = (g.C + ta.cc - 1) / ta.cc;
+	kspS.alloc((size_t)nch * ta.cc * std::max<long>(g.M, 1));
+
+	// adjoint work list: the visits of a tile (its cells and their ring) in
+	// chunks of at most MAXV; tiles with more than one chunk are split
+	const long MAXV = 2048;
+	std::vector<int4> hw;
+	std::vector<int> hsplit;
+	const GridTiling& t = ta;
+
+	for (int tile = 0; tile < t.ntiles; tile++) {
+
+		int tb[3] = { tile % t.nt[0], (tile / t.nt[0]) % t.nt[1], tile / (t.nt[0] * t.nt[1]) };
+		int span[3][4];
+
+		for (int d = 0; d < 3; d++) {
+
+			int clo = 0, chi = 0;
+
+			if (g.n[d] > 1) {
+				int t0 = tb[d] * t.T[d];
+				clo = (int)floorf(((float)t0 - g.reach_hi) / (float)g.CS[d]);
+				chi = (int)floorf(((float)(t0 + t.T[d] - 1) + g.reach_lo) / (float)g.CS[d]);
+			}
+
+			host_cell_span(g, d, clo, chi, span[d]);
+		}
+
+		long F = 0;
+
+		for (int zs = 0; zs < 2; zs++)
+			for (int cz = span[2][2 * zs]; cz < span[2][2 * zs] + span[2][2 * zs + 1]; cz++)
+				for (int ys = 0; ys < 2; ys++)
+					for (int cy = span[1][2 * ys]; cy < span[1][2 * ys] + span[1][2 * ys + 1]; cy++)
+						for (int xs = 0; xs < 2; xs++) {
+							long row = (long)g.ncell[0] * (cy + (long)g.ncell[1] * cz);
+							long a = row + span[0][2 * xs];
+							F += count[a + span[0][2 * xs + 1]] - count[a];
+						}
+
+		if (F <= MAXV) {
+			hw.push_back(make_int4(tile, 0, (int)F, 0));
+		} else {
+			hsplit.push_back(tile);
+			for (long b = 0; b < F; b += MAXV)
+				hw.push_back(make_int4(tile, (int)b, (int)std::min(F, b + MAXV), 1));
+		}
 	}
-	if (nwork > 0)
-		BAM_HIP(hipMemcpyAsync(work.ptr, hw.data(), sizeof(int4) * nwork, hipMemcpyHostToDevice, st));
 
-	BAM_HIP(hipStreamSynchronize(st));
+	nwork = (int)hw.size();
+	nsplit = (int)hsplit.size();
+	work.alloc(std::max(nwork, 1));
+	split_tiles.alloc(std::max(nsplit, 1));
+	BAM_HIP(hipMemcpy(work.ptr, hw.data(), sizeof(int4) * nwork, hipMemcpyHostToDevice));
+	if (nsplit > 0)
+		BAM_HIP(hipMemcpy(split_tiles.ptr, hsplit.data(), sizeof(int) * nsplit, hipMemcpyHostToDevice));
+}
+
+// taps per dim: floor(p + w/2) - ceil(p - w/2) + 1 <= floor(w) + 1
+static int taps_bound(const GridGeom& g)
+{
+	int tp = (int)floor(g.width) + 1;
+	BAM_CHECK(tp <= MAX_TAPS, "gridding: kernel width %f too large", g.width);
+	return (tp <= 4) ? 4 : 8;
 }
 
 // ---------------------------------------------------------------------------
-// Taps of one sample along one dim (noncart/grid.c:415-457)
+// Adjoint input permutation: kspS[(ch M + i) cc + cl] = ksp[(ch cc + cl) M + m(i)] conj(w(m))
+// (the weights of noncart/nufft.c:1474-1479 are applied here)
 // ---------------------------------------------------------------------------
 
-struct Taps {
+__global__ void k_permute_ksp(long M, int C, int cc, const int* __restrict__ sm, const float2* __restrict__ ksp,
+			      const float2* __restrict__ weights, float2* __restrict__ kspS)
+{
+	const int nch = (C + cc - 1) / cc;
+	const long tot = (long)nch * M;
+
+	for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < tot; j += (long)gridDim.x * blockDim.x) {
+
+		long ch = j / M;
+		long i = j - ch * M;
+		int m = sm[i];
+		float2 w = (nullptr != weights) ? weights[m] : make_float2(1.f, 0.f);
+
+		for (int cl = 0; cl < cc; cl++) {
+
+			int c = (int)ch * cc + cl;
+			float2 v = make_float2(0.f, 0.f);
+
+			if (c < C)
+				v = (nullptr != weights) ? cmulc(ksp[(long)c * M + m], w) : ksp[(long)c * M + m];
+
+			kspS[(ch * M + i) * cc + cl] = v;
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Cell ranges of a work-group: the sorted samples of the cells [clo, chi] per
+// dim (wrapped for periodic grids, clamped otherwise; every cell at most
+// once) as contiguous index ranges, one per row of cells along x (two when x
+// wraps).
+// ---------------------------------------------------------------------------
+
+struct CellRanges {
+	int start[MAX_RANGES];
+	int pre[MAX_RANGES + 1];
 	int n;
-	int first;      // first tap index (unwrapped, after clamping)
-	float w[MAX_TAPS];
+	int pad[2];
 };
 
-__device__ inline void taps_dim(Taps& t, float pos, int dim, bool periodic, float width, const float* table)
-{
-	int sti = (int)ceil((double)pos - 0.5 * (double)width);
-	int eni = (int)floor((double)pos + 0.5 * (double)width);
+// dynamic LDS layout: KB table, cell ranges, then the float2 data (16-byte aligned)
+constexpr size_t SMEM_TABLE = (KB_SIZE + 4) * sizeof(float);
+constexpr size_t SMEM_DATA = ((SMEM_TABLE + sizeof(CellRanges) + 15) / 16) * 16;
 
-	if (!periodic) {
-		sti = max(sti, 0);
-		eni = min(eni, dim - 1);
+__device__ inline void cell_span(const GridGeom& g, int d, int clo, int chi, int& a0, int& n0, int& a1, int& n1)
+{
+	const int nc = g.ncell[d];
+	a1 = 0;
+	n1 = 0;
+
+	if (chi - clo + 1 >= nc) {
+		a0 = 0;
+		n0 = nc;
+		return;
 	}
 
-	int cnt = eni - sti + 1;
-	cnt = min(cnt, MAX_TAPS);
-	t.n = max(cnt, 0);
-	t.first = sti;
+	if (!g.periodic) {
+		clo = max(clo, 0);
+		chi = min(chi, nc - 1);
+		a0 = clo;
+		n0 = max(0, chi - clo + 1);
+		return;
+	}
+
+	int lo = ((clo % nc) + nc) % nc;
+	int len = chi - clo + 1;
+
+	a0 = lo;
+
+	if (lo + len <= nc) {
+		n0 = len;
+	} else {
+		n0 = nc - lo;
+		n1 = len - n0;
+	}
+}
+
+// builds the ranges in LDS (thread 0); returns the total sample count
+__device__ inline int build_ranges(const GridGeom& g, const int* __restrict__ cstart, const int clo[3],
+				   const int chi[3], CellRanges& R)
+{
+	if (0 == threadIdx.x) {
+
+		int xa[2], xn[2], ya[2], yn[2], za[2], zn[2];
+		cell_span(g, 0, clo[0], chi[0], xa[0], xn[0], xa[1], xn[1]);
+		cell_span(g, 1, clo[1], chi[1], ya[0], yn[0], ya[1], yn[1]);
+		cell_span(g, 2, clo[2], chi[2], za[0], zn[0], za[1], zn[1]);
+
+		int n = 0;
+		int tot = 0;
+		R.pre[0] = 0;
+
+		for (int zs = 0; zs < 2; zs++)
+			for (int cz = za[zs]; cz < za[zs] + zn[zs]; cz++)
+				for (int ys = 0; ys < 2; ys++)
+					for (int cy = ya[ys]; cy < ya[ys] + yn[ys]; cy++)
+						for (int xs = 0; xs < 2; xs++) {
+
+							if (0 == xn[xs] || n >= MAX_RANGES)
+								continue;
+
+							long row = (long)g.ncell[0] * (cy + (long)g.ncell[1] * cz);
+							int a = cstart[row + xa[xs]];
+							int b = cstart[row + xa[xs] + xn[xs]];
+
+							if (b > a) {
+								R.start[n] = a;
+								tot += b - a;
+								R.pre[++n] = tot;
+							}
+						}
+
+		R.n = n;
+	}
+
+	__syncthreads();
+	return R.pre[R.n];
+}
+
+// ---------------------------------------------------------------------------
+// Taps of one sample along one dim (noncart/grid.c:415-457), computed once
+// per sample and shift variant and reused by every sub-grid and coil.
+// ---------------------------------------------------------------------------
+
+__device__ inline float sample_pos_v(const GridGeom& g, int d, int v, float k)
+{
+	return __fadd_rn(__fmul_rn(g.os, __fadd_rn(k, g.shv[d][v])), (float)(g.n[d] / 2));
+}
+
+template <int TP> struct Taps1 {
+	int start;  // first tap (grid index before wrapping; clamped when not periodic)
+	int cnt;    // number of taps
+	float w[TP];
+};
+
+template <int TP>
+__device__ inline void taps_raw(const GridGeom& g, int d, int v, float k, const float* table, Taps1<TP>& t)
+{
+	if (1 == g.n[d]) {
+		t.start = 0;
+		t.cnt = 1;
+#pragma unroll
+		for (int j = 0; j < TP; j++)
+			t.w[j] = (0 == j) ? 1.f : 0.f;
+		return;
+	}
+
+	const float pos = sample_pos_v(g, d, v, k);
+	int sti = (int)ceil((double)pos - 0.5 * (double)g.width);
+	int eni = (int)floor((double)pos + 0.5 * (double)g.width);
+
+	if (!g.periodic) {
+		sti = max(sti, 0);
+		eni = min(eni, g.n[d] - 1);
+	}
+
+	t.start = sti;
+	t.cnt = min(max(eni - sti + 1, 0), TP);
 
 #pragma unroll
-	for (int j = 0; j < MAX_TAPS; j++) {
-		if (j < t.n) {
-			float frac = fabsf((float)(sti + j) - pos);
-			t.w[j] = kb_lookup(table, frac / width);
+	for (int j = 0; j < TP; j++)
+		t.w[j] = (j < t.cnt) ? kb_lookup(table, fabsf((float)(sti + j) - pos) / g.width) : 0.f;
+}
+
+// local index of tap j relative to a tile or window starting at org with
+// extent range, or -1 (periodic grids: modulo n)
+__device__ inline int tap_local(const GridGeom& g, int d, int w, int org, int range)
+{
+	int l;
+
+	if (g.periodic) {
+		const int nn = g.n[d];
+		int gw = w % nn;
+		if (gw < 0)
+			gw += nn;
+		l = gw - org;
+		if (l < 0)
+			l += nn;
+		else if (l >= nn)
+			l -= nn;
+	} else {
+		l = w - org;
+	}
+
+	return (l >= 0 && l < range) ? l : -1;
+}
+
+template <int TP>
+__device__ inline void select_taps(const Taps1<TP>& a, const Taps1<TP>& b, bool useb, const GridGeom& g, int d,
+				   int org, int range, int (&li)[TP], float (&wt)[TP], bool& any)
+{
+	const int start = useb ? b.start : a.start;
+	const int cnt = useb ? b.cnt : a.cnt;
+	bool an = false;
+
+#pragma unroll
+	for (int j = 0; j < TP; j++) {
+		int l = (j < cnt) ? tap_local(g, d, start + j, org, range) : -1;
+		li[j] = l;
+		wt[j] = useb ? b.w[j] : a.w[j];
+		an = an || (l >= 0);
+	}
+
+	any = an;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint gridding (k-space -> grid), output-stationary and atomic-free.
+// grid layout [S][C][N]; launch (nwork, ceil(C / cc)) with Wtot x S lanes.
+//
+// Every lane owns one point of the tile in one sub-grid and accumulates the
+// cc coils of that output in registers.  The visits of the tile (samples of
+// its cells and of the ring of neighbouring cells) are staged through LDS in
+// batches: per visit the 2 x 3 tap sets (3 dims, 2 shift variants) and the
+// coil values; each lane then adds v_c * wz * wy * wx for the taps that hit
+// its point.  Waves whose 64 points are missed by a sample skip it.  No
+// atomics except the final add of tiles whose visits are split over several
+// work-groups, and the summation order of a point is the visit order.
+// ---------------------------------------------------------------------------
+
+constexpr int ADJ_BATCH = 64;
+
+template <int TP>
+struct TapSet {
+	int start;
+	int cnt;
+	float w[TP];
+};
+
+template <int TP, int CC>
+__global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, const float* __restrict__ kbt,
+						   const float4* __restrict__ sk, const int* __restrict__ cstart,
+						   const int4* __restrict__ work, const float2* __restrict__ kspS,
+						   float2* __restrict__ grid)
+{
+	extern __shared__ float4 smem4[];
+	char* smem = (char*)smem4;
+	float* table = (float*)smem;
+	CellRanges& R = *(CellRanges*)(smem + SMEM_TABLE);
+	TapSet<TP>* tbuf = (TapSet<TP>*)(smem + SMEM_DATA);                         // [BATCH][6]
+	float2* vbuf = (float2*)(smem + SMEM_DATA + sizeof(TapSet<TP>) * 6 * ADJ_BATCH);  // [BATCH][CC]
+
+	const int4 wk = work[blockIdx.x];  // (tile, first visit, end visit, split)
+	const int tile = wk.x;
+	const int ch = blockIdx.y;
+	const int cc = t.cc;
+	const int c0 = ch * cc;
+	const int ncc = min(cc, g.C - c0);
+	const int nthr = blockDim.x;
+
+	int tb[3], t0[3], clo[3], chi[3];
+	tb[0] = tile % t.nt[0];
+	tb[1] = (tile / t.nt[0]) % t.nt[1];
+	tb[2] = tile / (t.nt[0] * t.nt[1]);
+
+	for (int d = 0; d < 3; d++) {
+		t0[d] = tb[d] * t.T[d];
+		if (1 == g.n[d]) {
+			clo[d] = 0;
+			chi[d] = 0;
+		} else {
+			clo[d] = (int)floorf(((float)t0[d] - g.reach_hi) / (float)g.CS[d]);
+			chi[d] = (int)floorf(((float)(t0[d] + t.T[d] - 1) + g.reach_lo) / (float)g.CS[d]);
 		}
 	}
-}
 
-__device__ inline float sample_pos(const GridGeom& g, int s, int d, float k)
-{
-	return __fadd_rn(__fmul_rn(g.os, __fadd_rn(k, g.shift[s][d])), (float)(g.n[d] / 2));
-}
-
-// ---------------------------------------------------------------------------
-// Adjoint gridding (k-space -> grid), LDS-tiled.
-// grid layout: [S][C][N]; ksp: [C][M]; launch (nwork, S, ceil(C / cc)).
-// ---------------------------------------------------------------------------
-
-__global__ __launch_bounds__(256) void k_grid_adj(GridGeom g, int cc, const float* __restrict__ kbt,
-						  const int* __restrict__ sm, const float4* __restrict__ sk,
-						  const int4* __restrict__ work, const float2* __restrict__ ksp,
-						  const float2* __restrict__ weights, float2* __restrict__ grid)
-{
-	extern __shared__ float smem[];
-	float* table = smem;                             // KB_SIZE + 1 (padded to 1004)
-	float2* win = (float2*)(smem + 1004);            // cc * Wtot
-
-	const int4 wk = work[blockIdx.x];
-	const int s = blockIdx.y;
-	const int c0 = blockIdx.z * cc;
-	const int ncc = min(cc, g.C - c0);
-
-	for (int i = threadIdx.x; i < KB_SIZE + 1; i += blockDim.x)
+	for (int i = threadIdx.x; i < KB_SIZE + 1; i += nthr)
 		table[i] = kbt[i];
 
-	for (int i = threadIdx.x; i < ncc * g.Wtot; i += blockDim.x)
-		win[i] = make_float2(0.f, 0.f);
+	const int Ftile = build_ranges(g, cstart, clo, chi, R);
+	const int fb = min(wk.y, Ftile);
+	const int fe = min(wk.z, Ftile);
 
-	int tb[3];
-	tb[0] = wk.x % g.nt[0];
-	tb[1] = (wk.x / g.nt[0]) % g.nt[1];
-	tb[2] = wk.x / (g.nt[0] * g.nt[1]);
+	// this lane's output: point pnt of sub-grid s
+	const int me = threadIdx.x;
+	const bool mine = me < t.Wtot * g.S;
+	const int s = mine ? me / t.Wtot : 0;
+	const int pnt = mine ? me - s * t.Wtot : 0;
+	const int lp[3] = { pnt % t.W[0], (pnt / t.W[0]) % t.W[1], pnt / (t.W[0] * t.W[1]) };
+	const int gp[3] = { t0[0] + lp[0], t0[1] + lp[1], t0[2] + lp[2] };
+	const bool inside = mine && gp[0] < g.n[0] && gp[1] < g.n[1] && gp[2] < g.n[2];
+	const int vb[3] = { g.sbit[s][0], g.sbit[s][1], g.sbit[s][2] };
 
-	int org[3];
-	for (int d = 0; d < 3; d++)
-		org[d] = tb[d] * g.T[d] - g.lo[d];
+	float2 acc[CC];
+#pragma unroll
+	for (int c = 0; c < CC; c++)
+		acc[c] = make_float2(0.f, 0.f);
 
-	__syncthreads();
+	int r = 0;
 
-	for (int i = wk.y + threadIdx.x; i < wk.z; i += blockDim.x) {
+	for (int base = fb; base < fe; base += ADJ_BATCH) {
 
-		const int m = sm[i];
-		const float4 kc = sk[i];
-		const float kk[3] = { kc.x, kc.y, kc.z };
+		const int nb = min(ADJ_BATCH, fe - base);
 
-		Taps t[3];
-		int base[3];
-		bool any = true;
+		__syncthreads();
 
-		for (int d = 0; d < 3; d++) {
+		// stage nb visits: thread (v, k), k < 16: tap set k (< 6), coil value k (< cc)
+		for (int j = threadIdx.x; j < 16 * ADJ_BATCH; j += nthr) {
 
-			if (1 == g.n[d]) {
-				t[d].n = 1;
-				t[d].first = 0;
-				t[d].w[0] = 1.f;
-				base[d] = 0;
+			const int v = j >> 4;
+			const int k = j & 15;
+
+			if (v >= nb)
 				continue;
+
+			const int f = base + v;
+			int rr = 0;
+			while (f >= R.pre[rr + 1])
+				rr++;
+			const int i = R.start[rr] + (f - R.pre[rr]);
+
+			if (k < 6) {
+				const float4 kc = sk[i];
+				const float kk[3] = { kc.x, kc.y, kc.z };
+				const int d = k >> 1;
+				const int var = k & 1;
+				Taps1<TP> tt;
+				taps_raw<TP>(g, d, (var < g.nshv[d]) ? var : 0, kk[d], table, tt);
+				TapSet<TP>& ts = tbuf[v * 6 + k];
+				ts.start = tt.start;
+				ts.cnt = tt.cnt;
+#pragma unroll
+				for (int q = 0; q < TP; q++)
+					ts.w[q] = tt.w[q];
 			}
 
-			float pos = sample_pos(g, s, d, kk[d]);
-			taps_dim(t[d], pos, g.n[d], g.periodic, g.width, table);
-
-			int q = ((__float_as_int(kc.w) >> (4 * d)) & 15) - 8;
-
-			base[d] = t[d].first - q * g.n[d] - org[d];
-			any = any && (t[d].n > 0);
+			if (k < CC)
+				vbuf[v * CC + k] = (k < ncc) ? kspS[((long)ch * g.M + i) * cc + k] : make_float2(0.f, 0.f);
 		}
 
-		if (!any)
-			continue;
+		__syncthreads();
 
-		float2 w = make_float2(1.f, 0.f);
+		for (int v = 0; v < nb; v++) {
 
-		if (nullptr != weights)
-			w = weights[m];
+			float w = 1.f;
+			bool hit = inside;
 
-		for (int c = 0; c < ncc; c++) {
+			// z, y, x: the weight of the tap at this lane's point (BART order)
+#pragma unroll
+			for (int dd = 2; dd >= 0; dd--) {
 
-			float2 v = cmulc(ksp[(long)(c0 + c) * g.M + m], w);
+				const TapSet<TP>& ts = tbuf[v * 6 + 2 * dd + vb[dd]];
+				int j;
 
-			if (0.f == v.x && 0.f == v.y)
+				if (g.periodic) {
+					j = (gp[dd] - ts.start) % g.n[dd];
+					if (j < 0)
+						j += g.n[dd];
+				} else {
+					j = gp[dd] - ts.start;
+				}
+
+				const bool ok = (j >= 0) && (j < ts.cnt);
+				float wj = 0.f;
+
+#pragma unroll
+				for (int q = 0; q < TP; q++)
+					if (q == j)
+						wj = ts.w[q];
+
+				w = (2 == dd) ? (1.f * wj) : (w * wj);
+				hit = hit && ok;
+			}
+
+			if (!__any(hit))
 				continue;
 
-			float2* wc = win + c * g.Wtot;
+			if (!hit)
+				w = 0.f;
 
-			for (int iz = 0; iz < t[2].n; iz++) {
-
-				int lz = base[2] + iz;
-				if (lz < 0 || lz >= g.W[2])
-					continue;
-				float dz = 1.f * t[2].w[iz];
-
-				for (int iy = 0; iy < t[1].n; iy++) {
-
-					int ly = base[1] + iy;
-					if (ly < 0 || ly >= g.W[1])
-						continue;
-					float dy = dz * t[1].w[iy];
-					float2* row = wc + g.W[0] * (ly + g.W[1] * lz);
-
-					for (int ix = 0; ix < t[0].n; ix++) {
-
-						int lx = base[0] + ix;
-						if (lx < 0 || lx >= g.W[0])
-							continue;
-						float dd = dy * t[0].w[ix];
-						atomicAdd(&row[lx].x, v.x * dd);
-						atomicAdd(&row[lx].y, v.y * dd);
-					}
-				}
+#pragma unroll
+			for (int c = 0; c < CC; c++) {
+				const float2 x = vbuf[v * CC + c];
+				acc[c].x += x.x * w;
+				acc[c].y += x.y * w;
 			}
 		}
 	}
 
-	__syncthreads();
+	if (!inside)
+		return;
 
-	// flush the window: wrap (periodic) or drop (non-periodic) out-of-range points
-	for (int j = threadIdx.x; j < ncc * g.Wtot; j += blockDim.x) {
+	const long idx = gp[0] + (long)g.n[0] * (gp[1] + (long)g.n[1] * gp[2]);
 
-		float2 v = win[j];
+#pragma unroll
+	for (int c = 0; c < CC; c++) {
 
-		if (0.f == v.x && 0.f == v.y)
-			continue;
-
-		int c = j / g.Wtot;
-		int r = j - c * g.Wtot;
-		int l[3];
-		l[0] = r % g.W[0];
-		l[1] = (r / g.W[0]) % g.W[1];
-		l[2] = r / (g.W[0] * g.W[1]);
-
-		long idx = 0;
-		long str = 1;
-		bool ok = true;
-
-		for (int d = 0; d < 3; d++) {
-
-			int gi = org[d] + l[d];
-
-			if (g.periodic) {
-				gi %= g.n[d];
-				if (gi < 0)
-					gi += g.n[d];
-			} else if (gi < 0 || gi >= g.n[d]) {
-				ok = false;
-			}
-
-			idx += gi * str;
-			str *= g.n[d];
-		}
-
-		if (!ok)
+		if (c >= ncc)
 			continue;
 
 		float2* dst = grid + ((long)s * g.C + c0 + c) * g.N + idx;
-		unsafeAtomicAdd(&dst->x, v.x);
-		unsafeAtomicAdd(&dst->y, v.y);
+
+		if (0 == wk.w) {
+			*dst = acc[c];
+		} else {
+			// tile split over several work-groups (dense region): add into
+			// the pre-zeroed tile
+			if (0.f != acc[c].x)
+				unsafeAtomicAdd(&dst->x, acc[c].x);
+			if (0.f != acc[c].y)
+				unsafeAtomicAdd(&dst->y, acc[c].y);
+		}
+	}
+}
+
+// zero the split tiles (all sub-grids and coils) before their partial sums
+__global__ void k_zero_tiles(GridGeom g, GridTiling t, const int* __restrict__ tiles, int ntl, float2* grid)
+{
+	const int per = t.Wtot;
+	const long tot = (long)ntl * g.S * g.C * per;
+
+	for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < tot; j += (long)gridDim.x * blockDim.x) {
+
+		long q = j / per;
+		int pnt = (int)(j - q * per);
+		int sc = (int)(q % ((long)g.S * g.C));
+		int tl = (int)(q / ((long)g.S * g.C));
+		int tile = tiles[tl];
+		int tb0 = tile % t.nt[0], tb1 = (tile / t.nt[0]) % t.nt[1], tb2 = tile / (t.nt[0] * t.nt[1]);
+		int gx = tb0 * t.T[0] + pnt % t.W[0];
+		int gy = tb1 * t.T[1] + (pnt / t.W[0]) % t.W[1];
+		int gz = tb2 * t.T[2] + pnt / (t.W[0] * t.W[1]);
+
+		if (gx >= g.n[0] || gy >= g.n[1] || gz >= g.n[2])
+			continue;
+
+		grid[(long)sc * g.N + gx + (long)g.n[0] * (gy + (long)g.n[1] * gz)] = make_float2(0.f, 0.f);
 	}
 }
 
 void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float2* ksp, const float2* weights,
 		  hipStream_t st)
 {
-	if (0 == gp.nwork || 0 == gp.g.C)
+	const GridGeom& g = gp.g;
+	const GridTiling& t = gp.ta;
+
+	if (0 == g.C)
 		return;
 
-	dim3 blocks(gp.nwork, gp.g.S, ceil_div(gp.g.C, gp.cc_adj));
-	size_t lds = 1004 * sizeof(float) + (size_t)gp.cc_adj * gp.g.Wtot * sizeof(float2);
+	BAM_CHECK(t.cc <= 16, "gridding: at most 16 coils per work-group");
 
-	hipLaunchKernelGGL(k_grid_adj, blocks, dim3(256), lds, st, gp.g, gp.cc_adj, kb, gp.sm.ptr, gp.sk.ptr,
-			   gp.work.ptr, ksp, weights, grid);
+	{
+		ProfScope prof("permute_ksp", st);
+		int nch = (g.C + t.cc - 1) / t.cc;
+		long tot = (long)nch * g.M;
+		unsigned blocks = (unsigned)std::max<long>(1, std::min<long>((tot + 255) / 256, 16384));
+		if (tot > 0)
+			hipLaunchKernelGGL(k_permute_ksp, dim3(blocks), dim3(256), 0, st, g.M, g.C, t.cc, gp.sm.ptr, ksp,
+					   weights, gp.kspS.ptr);
+		BAM_HIP(hipGetLastError());
+	}
+
+	ProfScope prof("grid_adjoint", st);
+
+	if (gp.nsplit > 0) {
+		long tot = (long)gp.nsplit * g.S * g.C * t.Wtot;
+		unsigned zb = (unsigned)std::min<long>((tot + 255) / 256, 8192);
+		hipLaunchKernelGGL(k_zero_tiles, dim3(zb), dim3(256), 0, st, g, t, gp.split_tiles.ptr, gp.nsplit, grid);
+		BAM_HIP(hipGetLastError());
+	}
+
+	const int tp = taps_bound(g);
+	const int nthr = ((t.Wtot * g.S + 63) / 64) * 64;
+	BAM_CHECK(nthr <= 1024, "gridding: tile too large (%d outputs)", t.Wtot * g.S);
+	dim3 blocks(gp.nwork, (g.C + t.cc - 1) / t.cc);
+
+#define BAM_ADJ(TP, CC)                                                                                          \
+	do {                                                                                                     \
+		size_t lds = SMEM_DATA + sizeof(TapSet<TP>) * 6 * ADJ_BATCH + sizeof(float2) * CC * ADJ_BATCH;  \
+		hipLaunchKernelGGL((k_grid_adj<TP, CC>), blocks, dim3(nthr), lds, st, g, t, kb, gp.sk.ptr,       \
+				   gp.cstart.ptr, gp.work.ptr, gp.kspS.ptr, grid);                               \
+	} while (0)
+
+	if (4 == tp) {
+		if (t.cc <= 4) BAM_ADJ(4, 4);
+		else if (t.cc <= 8) BAM_ADJ(4, 8);
+		else BAM_ADJ(4, 16);
+	} else {
+		if (t.cc <= 4) BAM_ADJ(8, 4);
+		else if (t.cc <= 8) BAM_ADJ(8, 8);
+		else BAM_ADJ(8, 16);
+	}
+#undef BAM_ADJ
+
 	BAM_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
 // Forward gridding (grid -> k-space): one thread per sorted sample and coil
-// chunk; sums the sub-grids in order s = 0 .. S-1 (noncart/nufft.c:1444-1445)
-// and multiplies by the weights (:1456-1457).
+// chunk gathers the taps of all sub-grids from the grid (neighbouring lanes
+// read neighbouring grid lines); sub-grids are summed in order s = 0 .. S-1
+// (noncart/nufft.c:1444-1445) and the result is multiplied by the weights
+// (:1456-1457).  Launch (ceil(M / 256), ceil(C / CC)).
 // ---------------------------------------------------------------------------
 
-template <int CC>
+template <int TP, int CC>
 __global__ __launch_bounds__(256) void k_grid_fwd(GridGeom g, const float* __restrict__ kbt,
 						  const int* __restrict__ sm, const float4* __restrict__ sk,
 						  const float2* __restrict__ grid, const float2* __restrict__ weights,
@@ -440,95 +875,89 @@ __global__ __launch_bounds__(256) void k_grid_fwd(GridGeom g, const float* __res
 {
 	__shared__ float table[KB_SIZE + 1];
 
-	for (int i = threadIdx.x; i < KB_SIZE + 1; i += blockDim.x)
+	for (int i = threadIdx.x; i < KB_SIZE + 1; i += 256)
 		table[i] = kbt[i];
 
 	__syncthreads();
 
-	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long i = (long)blockIdx.x * 256 + threadIdx.x;
 
 	if (i >= g.M)
 		return;
 
 	const int c0 = blockIdx.y * CC;
 	const int ncc = min(CC, g.C - c0);
-	const int m = sm[i];
 	const float4 kc = sk[i];
 	const float kk[3] = { kc.x, kc.y, kc.z };
 
-	float2 acc[CC];
+	Taps1<TP> tv[3][2];
 
 #pragma unroll
-	for (int c = 0; c < CC; c++)
-		acc[c] = make_float2(0.f, 0.f);
+	for (int d = 0; d < 3; d++) {
+		taps_raw<TP>(g, d, 0, kk[d], table, tv[d][0]);
+		if (g.nshv[d] > 1)
+			taps_raw<TP>(g, d, 1, kk[d], table, tv[d][1]);
+		else
+			tv[d][1] = tv[d][0];
+	}
+
+	float2 acc[CC];
+#pragma unroll
+	for (int cl = 0; cl < CC; cl++)
+		acc[cl] = make_float2(0.f, 0.f);
 
 	for (int s = 0; s < g.S; s++) {
 
-		Taps t[3];
-		bool any = true;
+		int li0[TP], li1[TP], li2[TP];
+		float wt0[TP], wt1[TP], wt2[TP];
+		bool a0, a1, a2;
 
-		for (int d = 0; d < 3; d++) {
+		select_taps<TP>(tv[0][0], tv[0][1], 0 != g.sbit[s][0], g, 0, 0, g.n[0], li0, wt0, a0);
+		select_taps<TP>(tv[1][0], tv[1][1], 0 != g.sbit[s][1], g, 1, 0, g.n[1], li1, wt1, a1);
+		select_taps<TP>(tv[2][0], tv[2][1], 0 != g.sbit[s][2], g, 2, 0, g.n[2], li2, wt2, a2);
 
-			if (1 == g.n[d]) {
-				t[d].n = 1;
-				t[d].first = 0;
-				t[d].w[0] = 1.f;
-				continue;
-			}
-
-			float pos = sample_pos(g, s, d, kk[d]);
-			taps_dim(t[d], pos, g.n[d], g.periodic, g.width, table);
-			any = any && (t[d].n > 0);
-		}
-
-		if (!any)
+		if (!(a0 && a1 && a2))
 			continue;
 
 		float2 val[CC];
-
 #pragma unroll
-		for (int c = 0; c < CC; c++)
-			val[c] = make_float2(0.f, 0.f);
+		for (int cl = 0; cl < CC; cl++)
+			val[cl] = make_float2(0.f, 0.f);
 
 		const float2* gs = grid + ((long)s * g.C + c0) * g.N;
 
-		for (int iz = 0; iz < t[2].n; iz++) {
+#pragma unroll
+		for (int iz = 0; iz < TP; iz++) {
 
-			int z = t[2].first + iz;
-			if (g.periodic) {
-				z %= g.n[2];
-				if (z < 0)
-					z += g.n[2];
-			}
-			float dz = 1.f * t[2].w[iz];
+			if (li2[iz] < 0)
+				continue;
 
-			for (int iy = 0; iy < t[1].n; iy++) {
-
-				int y = t[1].first + iy;
-				if (g.periodic) {
-					y %= g.n[1];
-					if (y < 0)
-						y += g.n[1];
-				}
-				float dy = dz * t[1].w[iy];
-				const float2* row = gs + (long)g.n[0] * (y + (long)g.n[1] * z);
-
-				for (int ix = 0; ix < t[0].n; ix++) {
-
-					int x = t[0].first + ix;
-					if (g.periodic) {
-						x %= g.n[0];
-						if (x < 0)
-							x += g.n[0];
-					}
-					float dd = dy * t[0].w[ix];
+			float dz = 1.f * wt2[iz];
 
 #pragma unroll
-					for (int c = 0; c < CC; c++) {
-						if (c < ncc) {
-							float2 v = row[(long)c * g.N + x];
-							val[c].x += v.x * dd;
-							val[c].y += v.y * dd;
+			for (int iy = 0; iy < TP; iy++) {
+
+				if (li1[iy] < 0)
+					continue;
+
+				float dy = dz * wt1[iy];
+				const float2* row = gs + (long)g.n[0] * (li1[iy] + (long)g.n[1] * li2[iz]);
+
+#pragma unroll
+				for (int ix = 0; ix < TP; ix++) {
+
+					if (li0[ix] < 0)
+						continue;
+
+					float dd = dy * wt0[ix];
+					const float2* p = row + li0[ix];
+
+#pragma unroll
+					for (int cl = 0; cl < CC; cl++) {
+						if (cl < ncc) {
+							float2 x = p[(long)cl * g.N];
+							val[cl].x += x.x * dd;
+							val[cl].y += x.y * dd;
 						}
 					}
 				}
@@ -536,35 +965,45 @@ __global__ __launch_bounds__(256) void k_grid_fwd(GridGeom g, const float* __res
 		}
 
 #pragma unroll
-		for (int c = 0; c < CC; c++)
-			acc[c] = cadd(acc[c], val[c]);
+		for (int cl = 0; cl < CC; cl++)
+			acc[cl] = cadd(acc[cl], val[cl]);
 	}
 
-	float2 w = make_float2(1.f, 0.f);
-
-	if (nullptr != weights)
-		w = weights[m];
+	const int m = sm[i];
+	const float2 w = (nullptr != weights) ? weights[m] : make_float2(1.f, 0.f);
 
 #pragma unroll
-	for (int c = 0; c < CC; c++)
-		if (c < ncc)
-			ksp[(long)(c0 + c) * g.M + m] = (nullptr != weights) ? cmul(acc[c], w) : acc[c];
+	for (int cl = 0; cl < CC; cl++)
+		if (cl < ncc)
+			ksp[(long)(c0 + cl) * g.M + m] = (nullptr != weights) ? cmul(acc[cl], w) : acc[cl];
 }
 
 void grid_forward(const GridPlan& gp, const float* kb, float2* ksp, const float2* grid, const float2* weights,
 		  hipStream_t st)
 {
-	if (0 == gp.g.M || 0 == gp.g.C)
+	const GridGeom& g = gp.g;
+
+	if (0 == g.M || 0 == g.C)
 		return;
 
-	int cc = gp.cc_fwd;
+	ProfScope prof("grid_forward", st);
+	const int tp = taps_bound(g);
+	const int cc = (g.C <= 4) ? 4 : ((g.C <= 8) ? 8 : 16);
+	dim3 blocks(ceil_div(g.M, 256), ceil_div(g.C, cc));
 
-	if (cc <= 4)
-		hipLaunchKernelGGL((k_grid_fwd<4>), dim3(ceil_div(gp.g.M, 256), ceil_div(gp.g.C, 4)), dim3(256), 0, st, gp.g, kb, gp.sm.ptr, gp.sk.ptr, grid, weights, ksp);
-	else if (cc <= 8)
-		hipLaunchKernelGGL((k_grid_fwd<8>), dim3(ceil_div(gp.g.M, 256), ceil_div(gp.g.C, 8)), dim3(256), 0, st, gp.g, kb, gp.sm.ptr, gp.sk.ptr, grid, weights, ksp);
-	else
-		hipLaunchKernelGGL((k_grid_fwd<16>), dim3(ceil_div(gp.g.M, 256), ceil_div(gp.g.C, 16)), dim3(256), 0, st, gp.g, kb, gp.sm.ptr, gp.sk.ptr, grid, weights, ksp);
+#define BAM_FWD(TP, CC) hipLaunchKernelGGL((k_grid_fwd<TP, CC>), blocks, dim3(256), 0, st, g, kb, gp.sm.ptr, \
+					    gp.sk.ptr, grid, weights, ksp)
+
+	if (4 == tp) {
+		if (4 == cc) BAM_FWD(4, 4);
+		else if (8 == cc) BAM_FWD(4, 8);
+		else BAM_FWD(4, 16);
+	} else {
+		if (4 == cc) BAM_FWD(8, 4);
+		else if (8 == cc) BAM_FWD(8, 8);
+		else BAM_FWD(8, 16);
+	}
+#undef BAM_FWD
 
 	BAM_HIP(hipGetLastError());
 }

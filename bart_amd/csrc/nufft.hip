@@ -10,6 +10,7 @@
 // product with the coil maps (sense/modelnc.c, linops/fmac.c) and the sum
 // over coils of S^H are fused there as well.
 #include "nufft_impl.hpp"
+#include "prof.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -489,6 +490,7 @@ __global__ void k_sum_groups(float2* __restrict__ dst, const float2* __restrict_
 
 static void sum_groups(float2* dst, const float2* part, long N, int G, hipStream_t st, const int* skip)
 {
+	ProfScope prof("sum_coil_groups", st);
 	unsigned blocks = (unsigned)std::min<long>((N + 255) / 256, 4096);
 	hipLaunchKernelGGL(k_sum_groups, dim3(blocks), dim3(256), 0, st, dst, part, N, G, skip);
 	BAM_HIP(hipGetLastError());
@@ -498,6 +500,7 @@ void fill(float2* p, long n, float2 v, hipStream_t st, const int* skip)
 {
 	if (n <= 0)
 		return;
+	ProfScope prof("fill", st);
 	unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
 	hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, v, skip);
 	BAM_HIP(hipGetLastError());
@@ -508,6 +511,7 @@ static FftLaunchCfg fft_config_skip(const int* skip, const AxisPlan& plan, long 
 {
 	FftLaunchCfg c = fft_config(plan, total, contig, P_min, sign1, sign2);
 	c.L.skip = skip;
+	c.name = "fft_toeplitz";
 	return c;
 }
 
@@ -597,7 +601,7 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 		}
 	}
 
-	gp.build(g, traj, 0);
+	gp.build(g, traj);
 
 	grid.alloc((size_t)S * C * N);
 
@@ -625,6 +629,7 @@ void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
 		int cc = coil_group(S, C, n[a0], nullptr != sens);
 		GeomS gs = GeomS::make(n, a0, S, C, cc);
 		FftLaunchCfg cfg = fft_config(axis[0], gs.va.transforms(), true, S * cc, -1.f, -1.f);
+		cfg.name = "fft_fwd_first";
 		LdImgLin ld{ gs, tabs, src, sens, true };
 		StGridS st{ gs, tabs, grid.ptr, 1 == nact };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
@@ -634,6 +639,7 @@ void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
 
 		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
 		FftLaunchCfg cfg = fft_config(axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
+		cfg.name = "fft_fwd_axis";
 		LdGrid ld{ va, tabs, grid.ptr, false };
 		StGrid st{ va, tabs, grid.ptr, j == nact - 1 };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
@@ -644,13 +650,14 @@ void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
 
 void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 {
-	fill(grid.ptr, (long)S * C * N, make_float2(0.f, 0.f), stream);
+	// the output-stationary adjoint gridding writes every grid point
 	grid_adjoint(gp, kb.ptr, grid.ptr, ksp, has_weights ? weights.ptr : nullptr, stream);
 
 	for (int j = nact - 1; j >= 1; j--) {
 
 		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
 		FftLaunchCfg cfg = fft_config(axis[j], va.transforms(), va.contig(), 1, +1.f, +1.f);
+		cfg.name = "fft_adj_axis";
 		LdGrid ld{ va, tabs, grid.ptr, j == nact - 1 };
 		StGrid st{ va, tabs, grid.ptr, false };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
@@ -660,6 +667,7 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 	const int G = C / cc;
 	GeomS gs = GeomS::make(n, act[0], S, C, cc);
 	FftLaunchCfg cfg = fft_config(axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
+	cfg.name = "fft_adj_last";
 	LdGridS ld{ gs, tabs, grid.ptr, 1 == nact };
 	float2* out = dst;
 
@@ -862,6 +870,7 @@ void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host)
 		int a = p2.act[j];
 		VolAxis va = VolAxis::make(n2, a, 1);
 		FftLaunchCfg cfg = fft_config(p2.axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
+		cfg.name = "fft_psf";
 		LdFm2 ld{ va, p2.tabs, psft.ptr, 0 == j };
 		StDecomp st{ va, p2.tabs, (a == last) ? psf.ptr : psft.ptr, a == last, false, scale,
 			     { n[0], n[1], n[2] }, N };

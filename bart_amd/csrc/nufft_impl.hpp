@@ -10,41 +10,66 @@ namespace bam {
 constexpr int KB_SIZE = 1000;
 constexpr int MAX_TAPS = 8;  // taps per dimension (kernel width <= 7)
 
-// Gridding configuration of one grid2/grid2H call or of one decomposition
-// sub-grid (noncart/grid.h:8-16 struct grid_conf_s, noncart/nufft.c:147-188).
+// Gridding configuration of the decomposition sub-grids
+// (noncart/grid.h:8-16 struct grid_conf_s, noncart/nufft.c:147-188).
 struct GridGeom {
 
-	int n[3];         // grid dims
-	long N;           // grid points per coil
-	int C;            // coils
-	long M;           // samples per coil
-	int S;            // sub-grids (shift sets) handled by one launch
+	int n[3];           // grid dims
+	long N;             // grid points per coil
+	int C;              // coils
+	long M;             // samples per coil
+	int S;              // sub-grids (shift sets)
 	float os;
 	float width;
 	bool periodic;
-	float shift[8][3];  // per sub-grid shift (already including odd-size correction)
+	float shift[8][3];  // per sub-grid shift (including the odd-size correction)
 
-	// binning
-	int T[3];         // tile size
-	int nt[3];        // tiles per dim
-	int lo[3], hi[3]; // window margins
-	int W[3];         // window dims
+	// the distinct shift values per dim (at most 2) and which one each
+	// sub-grid uses
+	float shv[3][2];
+	int nshv[3];
+	int sbit[8][3];
+
+	// reach of a sample: taps w lie in [p - reach_lo, p + reach_hi] for its
+	// shift-0 position p (union over the sub-grid shifts)
+	float reach_lo, reach_hi;
+
+	// cells: samples are sorted by the cell (CS^d grid points) that holds
+	// their shift-0 position
+	int CS[3];
+	int ncell[3];
+};
+
+// Tiling of the adjoint gridding: a work-group owns a tile of T points per
+// dim of sg sub-grids and cc coils.
+struct GridTiling {
+
+	int T[3];
+	int nt[3];
+	int ntiles;
+	int sg;   // sub-grids per work-group
+	int cc;   // coils per work-group
+	int W[3]; // forward: window dims (T + lo + hi); adjoint: = T
+	int lo[3];
 	int Wtot;
+	size_t lds;
 };
 
 struct GridPlan {
 
 	GridGeom g;
-	DevBuf<int> sm;      // sorted sample -> original sample index
-	DevBuf<float4> sk;   // sorted sample coordinates (kx, ky, kz, 0)
-	DevBuf<int4> work;   // (tile, begin, end, 0)
+	DevBuf<int> sm;          // sorted sample -> original sample index
+	DevBuf<float4> sk;       // sorted coordinates (kx, ky, kz) + wrap code
+	DevBuf<int> cstart;      // cell -> first sorted sample (ncells + 1)
+	GridTiling ta;           // adjoint tiling
+	DevBuf<float2> kspS;     // adjoint input permuted to sorted order, coil chunks interleaved
+	DevBuf<int4> work;       // adjoint work items (tile, first visit, end visit, split)
+	DevBuf<int> split_tiles; // tiles whose visits are spread over several work items
 	int nwork = 0;
-	int cc_adj = 1;      // coils per adjoint work-group
-	int cc_fwd = 1;      // coils per forward thread
+	int nsplit = 0;
 
-	// traj: host, M x 3 floats (real parts); shift0: sub-grid 0 shift used
-	// for binning
-	void build(const GridGeom& geom, const float* traj, int tile_override);
+	// traj: host, M x 3 floats (real parts)
+	void build(const GridGeom& geom, const float* traj);
 };
 
 // Separable per-dimension tables for the element-wise factors of the NUFFT:

@@ -1,0 +1,245 @@
+"""Benchmark of the NUFFT + CG-SENSE hot path on one MI355X.
+
+Default workload (one GPU): 3-D radial kooshball, 128^3 image, 16 coils,
+8192 phyllotaxis spokes x 256 samples (2,097,152 samples per coil), 2x
+oversampled grid (decomposed into 8 sub-grids of 128^3), Kaiser-Bessel width 6.
+One step = adjoint NUFFT (k-space -> coil images), forward NUFFT (coil images ->
+k-space) and one CG-SENSE solve (A^H y, then up to 20 Toeplitz iterations,
+lambda = 0.01, stop at relative residual 1e-4).
+
+Prints one JSON line.  `value` is the NUFFT adjoint+forward throughput in
+million coil-samples per second (2 C M per adj+fwd pair / time of the pair).
+Inputs are resident in HBM before the timed region.  `roofline` is for the
+kernel with the largest share of the timed region, timed live with HIP
+events on the stream it runs on; `cpu_baseline` is the CPU oracle (a C/OpenMP
+restatement of BART's CPU algorithm) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def dims16(*d):
+    return list(d) + [1] * (16 - len(d))
+
+
+def workload(name):
+    from bart_amd import inputs as I
+    if name == "kooshball":
+        n, C, spokes, R = 128, 16, 8192, 256
+        traj = I.phyllotaxis_kooshball(spokes=spokes, readout=R)
+        img = I.ellipsoid_phantom(n, nell=16, seed=2)
+        maps = I.gaussian_coils((n, n, n), C, seed=2)
+        return dict(name="3d_radial_kooshball_128^3_16coils_8192x256", img=(n, n, n), C=C, R=R, S=spokes,
+                    traj=traj, x=img, maps=maps, lam=0.01, iters=20)
+    if name == "radial2d":
+        n, C, spokes, R = 256, 8, 201, 512
+        traj = I.radial_traj(R, spokes)
+        img = I.ellipse_phantom(n, nell=10, seed=1)
+        maps = I.gaussian_coils((1, n, n), C, seed=1)
+        return dict(name="2d_golden_radial_256^2_8coils_201x512", img=(n, n, 1), C=C, R=R, S=spokes,
+                    traj=traj, x=img, maps=maps, lam=0.01, iters=30)
+    if name == "spiral":
+        n, C = 320, 32
+        traj = I.vd_spiral(interleaves=24, samples=4096, n=n)
+        img = I.ellipse_phantom(n, nell=10, seed=3)
+        maps = I.gaussian_coils((1, n, n), C, seed=3)
+        return dict(name="2d_vd_spiral_320^2_32coils_24x4096", img=(n, n, 1), C=C, R=4096, S=24,
+                    traj=traj, x=img, maps=maps, lam=0.01, iters=30)
+    raise ValueError(name)
+
+
+def algorithmic_bytes(kernel, w):
+    """Compulsory HBM bytes of one launch of a kernel class (read once, write once)."""
+    X, Y, Z = w["img"]
+    N = X * Y * Z
+    C = w["C"]
+    M = w["R"] * w["S"]
+    d = sum(1 for v in w["img"] if v > 1)
+    S = 2 ** d
+    grid = 8 * S * C * N
+    if kernel == "grid_adjoint":      # k-space + coordinates in, sub-grids out
+        return 8 * C * M + 4 * d * M + grid
+    if kernel == "grid_forward":      # sub-grids + coordinates in, k-space out
+        return grid + 4 * d * M + 8 * C * M
+    if kernel in ("fft_fwd_axis", "fft_adj_axis"):
+        return 2 * grid
+    if kernel == "fft_fwd_first":     # coil images in, sub-grids out
+        return 8 * C * N + grid
+    if kernel == "fft_adj_last":      # sub-grids in, coil images out
+        return grid + 8 * C * N
+    return None
+
+
+def cpu_baseline(w, threads):
+    """Oracle NUFFT adj+fwd on a bounded sample: 2 coils of the same
+    trajectory and image grid."""
+    import oracle as O
+    O.lib().orc_set_num_threads(threads)
+    C = min(2, w["C"])
+    S = w["S"]
+    M = w["R"] * S
+    traj = w["traj"][:M]
+    nf = O.Nufft(w["img"], traj, C, toeplitz=False)
+    rng = np.random.default_rng(0)
+    X, Y, Z = w["img"]
+    ksp = (rng.standard_normal((C, M)) + 1j * rng.standard_normal((C, M))).astype(np.complex64)
+    t0 = time.perf_counter()
+    cim = nf.adjoint(ksp)
+    nf.forward(cim)
+    dt = time.perf_counter() - t0
+    return dict(value=2 * C * M / dt / 1e6, unit="Msamples/s", cores=threads, kind="port",
+                sample=f"oracle adjoint+forward NUFFT, {C} coils x {S} spokes x {w['R']} samples on the "
+                       f"{X}x{Y}x{Z} grid ({dt:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="kooshball", choices=["kooshball", "radial2d", "spiral"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ctypes
+    import bart_amd as ba
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    dev = torch.device("cuda:0")
+    L = ba.lib()
+    L.bart_amd_profile.argtypes = [ctypes.c_int]
+    L.bart_amd_profile_report.restype = ctypes.c_int
+    L.bart_amd_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
+
+    t_gen = time.perf_counter()
+    w = workload(args.config)
+    X, Y, Z = w["img"]
+    C, R, S = w["C"], w["R"], w["S"]
+    M = R * S
+    N = X * Y * Z
+    print(f"# workload {w['name']}: generated in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    traj = ba.traj_array(w["traj"])
+    conf = ba.NufftConf.defaults(toeplitz=False)
+    t0 = time.perf_counter()
+    nufft = ba.nufft_create2(dims16(1, R, S, C), dims16(X, Y, Z, C), dims16(3, R, S), traj, conf=conf)
+    sense = ba.sense_nc_init(dims16(X, Y, Z, C), dims16(X, Y, Z, C), w["maps"], dims16(1, R, S, C),
+                             dims16(3, R, S), traj)
+    t_plan = time.perf_counter() - t0
+    print(f"# plans (binning + PSF) in {t_plan:.2f} s", file=sys.stderr, flush=True)
+
+    x = torch.from_numpy(w["x"].reshape(-1)).to(dev)
+    y = sense.forward(x)
+    scale = float(y.abs().max())
+    g = torch.Generator(device="cpu").manual_seed(0)
+    noise = (torch.randn(M * C, generator=g) + 1j * torch.randn(M * C, generator=g)).to(torch.complex64)
+    y = (y + (1e-3 * scale / np.sqrt(2.0)) * noise.to(dev)).contiguous()
+    cim = torch.empty(C * N, dtype=torch.complex64, device=dev)
+    ksp2 = torch.empty(C * M, dtype=torch.complex64, device=dev)
+    b = torch.empty(N, dtype=torch.complex64, device=dev)
+    torch.cuda.synchronize()
+
+    def step(rec):
+        t0 = time.perf_counter()
+        nufft.adjoint(y, out=cim)
+        nufft.forward(cim, out=ksp2)
+        t1 = time.perf_counter()
+        sense.adjoint(y, out=b)
+        xs, st, tr = ba.lsqr_conjgrad(sense, b, w["iters"], w["lam"], 1e-4)
+        t2 = time.perf_counter()
+        rec.append((t1 - t0, t2 - t1, st.iterations, st.resid / max(st.bnorm, 1e-30)))
+
+    warm = []
+    for _ in range(args.warmup):
+        step(warm)
+    torch.cuda.synchronize()
+
+    if not args.no_profile:
+        L.bart_amd_profile(1)
+    rec = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(rec)
+        print(f"# step {i}: nufft adj+fwd {rec[-1][0] * 1e3:.2f} ms, cg-sense {rec[-1][1] * 1e3:.2f} ms "
+              f"({rec[-1][2]} it, rel-res {rec[-1][3]:.2e})", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    L.bart_amd_profile(0)
+
+    prof = {}
+    if not args.no_profile:
+        n = L.bart_amd_profile_report(None, 0)
+        buf = ctypes.create_string_buffer(n + 16)
+        L.bart_amd_profile_report(buf, n + 16)
+        prof = json.loads(buf.value.decode())
+
+    t_nufft = sum(r[0] for r in rec) / len(rec)
+    t_cg = sum(r[1] for r in rec) / len(rec)
+    its = rec[-1][2]
+    relres = rec[-1][3]
+    value = 2 * C * M / t_nufft / 1e6
+
+    roofline = None
+    if prof:
+        dom = max(prof.items(), key=lambda kv: kv[1][1])
+        name, (cnt, tot, mn, mx) = dom
+        avg_ms = tot / cnt
+        bts = algorithmic_bytes(name, w)
+        if bts is not None:
+            ach = bts / (avg_ms * 1e-3) / 1e9
+            roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                            frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=name,
+                            avg_ms=round(avg_ms, 4), launches=cnt, bytes_per_launch=bts)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = cpu_baseline(w, threads)
+
+    out = {
+        "metric": "NUFFT adj+fwd Msamples/s; CG-SENSE solves/s to rel-res 1e-4, warm vs cold",
+        "value": round(value, 2),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 complex; fp64 CG reductions)",
+        "data": "synthetic",
+        "config": {"workload": w["name"], "coils": C, "samples_per_coil": M, "image": list(w["img"]),
+                   "oversampling": 2.0, "kb_width": 6, "cg_max_iters": w["iters"], "lambda": w["lam"]},
+        "nufft_adj_fwd_ms": round(t_nufft * 1e3, 3),
+        "cg_sense_solves_per_s_cold": round(1.0 / t_cg, 3),
+        "cg_sense_ms_per_solve_cold": round(t_cg * 1e3, 3),
+        "cg_iterations": its,
+        "cg_rel_res": relres,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_ms": {k: round(v[1] / max(1, args.steps), 4) for k, v in sorted(prof.items())},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -133,6 +133,14 @@ void bart_amd_set_error_mode(int mode);
 const char* bart_amd_last_error(void);
 void bart_amd_clear_error(void);
 
+/* Per-kernel timing with HIP events on the launching stream (no counterpart
+ * in BART): bart_amd_profile(1) starts recording, the report returns
+ * {"kernel": [count, total_ms, min_ms, max_ms], ...} as JSON and returns the
+ * length needed; the records are cleared only when the report fits into buf.
+ * Launches inside stream capture are not recorded. */
+void bart_amd_profile(int enable);
+int bart_amd_profile_report(char* buf, int len);
+
 /* Library version string. */
 const char* bart_amd_version(void);
 
