@@ -290,21 +290,29 @@ def nufft_create(ksp_dims, cim_dims, traj_dims, traj, weights=None, conf=None):
     return Linop(h, ksp_dims, cim_dims)
 
 
-def sense_nc_init(max_dims, map_dims, maps, ksp_dims, traj_dims, traj, conf=None, wgs_dims=None, weights=None):
-    """sense/modelnc.c:28-67: A = NUFFT S, image -> k-space."""
+def sense_nc_init(max_dims, map_dims, maps, ksp_dims, traj_dims, traj, conf=None, wgs_dims=None, weights=None,
+                  nufft_op=False):
+    """sense/modelnc.c:28-67: A = NUFFT S, image -> k-space.  With nufft_op
+    the NUFFT operator sharing the plan (BART's fft_opp) is returned too."""
     conf = NufftConf.defaults() if conf is None else conf
     maps = _host_c64(maps) if not _is_torch(maps) else maps
     traj = _host_c64(traj) if not _is_torch(traj) else traj
     w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
     if weights is not None and wgs_dims is None:
         wgs_dims = [1] + list(traj_dims[1:3]) + [1] * 13
+    fop = ctypes.c_void_p(0)
     h = lib().sense_nc_init(_dims(max_dims), _dims(map_dims), _ptr(maps), _dims(ksp_dims), _dims(traj_dims),
                             _ptr(traj), ctypes.byref(conf), _dims(wgs_dims) if wgs_dims is not None else None,
-                            _ptr(w), None, None, None, 0)
+                            _ptr(w), None, None, ctypes.byref(fop) if nufft_op else None, 0)
     img_dims = list(max_dims) + [1] * (DIMS - len(max_dims))
     img_dims[3] = 1
     img_dims[4] = 1
-    return Linop(h, ksp_dims, img_dims)
+    op = Linop(h, ksp_dims, img_dims)
+    if not nufft_op:
+        return op
+    cim_dims = list(max_dims) + [1] * (DIMS - len(max_dims))
+    cim_dims[4] = 1
+    return op, Linop(fop.value, ksp_dims, cim_dims)
 
 
 def lsqr_conjgrad(op, b, maxiter, lam=0.0, tol=0.0, x0=None, fallback=False):

@@ -1,0 +1,228 @@
+/*
+ * nufft.c -- "bart nufft" on the gfx950 library (src/nufft.c:44-330).
+ *
+ *   nufft [-a | -i] [-x x:y:z] [-t | -r] [-l lambda] [-m iter] [-P] [-g]
+ *         [-p pattern] [-o 2] [-w width] [--lowmem] [--zero-mem]
+ *         [--no-precomp] traj input output
+ *
+ * forward: image [X Y Z C ...] -> k-space [1 R S C ...]
+ * -a: adjoint; -i: inverse (CG on (A^H A + lambda I) x = A^H y, Toeplitz
+ * normal operator unless -r, 50 iterations by default as
+ * iter_conjgrad_defaults, iter/iter.c:51-60).
+ * -g is accepted (every path runs on the GPU).  -s (NUDFT), -c
+ * (preconditioner), -1 (non-decomposed grid), -B (basis), -F/-T (B0) are
+ * rejected with an error.
+ */
+#include "tools.h"
+#include "cfl.h"
+#include "bart_amd.h"
+
+#include <complex.h>
+#include <getopt.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static void usage(void)
+{
+	fprintf(stderr,
+		"Usage: nufft [-a,-i] [-x x:y:z] [-t,-r] [-l lambda] [-m iter] [-P] [-g] [-p <pattern>]\n"
+		"             [-o os] [-w width] [--lowmem] [--zero-mem] [--no-precomp] <traj> <input> <output>\n"
+		"\n"
+		"Perform non-uniform Fast Fourier Transform.\n");
+}
+
+int main_nufft(int argc, char* argv[])
+{
+	bool adjoint = false;
+	bool inverse = false;
+	bool precomp = true;
+	long coilim_vec[3] = { 0, 0, 0 };
+	float lambda = 0.f;
+	int maxiter = 50;
+	const char* pattern_file = NULL;
+	struct nufft_conf_s conf = nufft_conf_defaults;
+
+	enum { O_LOWMEM = 256, O_ZEROMEM, O_NOPRECOMP, O_NUFFTCONF };
+
+	static const struct option lopts[] = {
+		{ "lowmem", no_argument, NULL, O_LOWMEM },
+		{ "zero-mem", no_argument, NULL, O_ZEROMEM },
+		{ "no-precomp", no_argument, NULL, O_NOPRECOMP },
+		{ "oversampling", required_argument, NULL, 'o' },
+		{ "width", required_argument, NULL, 'w' },
+		{ "nufft-conf", required_argument, NULL, O_NUFFTCONF },
+		{ NULL, 0, NULL, 0 },
+	};
+
+	optind = 1;
+	int c;
+
+	while (-1 != (c = getopt_long(argc, argv, "aix:d:D:trcl:m:Psg1B:p:o:w:F:T:h", lopts, NULL))) {
+
+		switch (c) {
+		case 'a': adjoint = true; break;
+		case 'i': inverse = true; break;
+		case 'x':
+		case 'd':
+		case 'D': parse_vec3(optarg, coilim_vec, "-x"); break;
+		case 't': conf.toeplitz = true; break;
+		case 'r': conf.toeplitz = false; break;
+		case 'l': lambda = parse_float(optarg, "-l"); break;
+		case 'm': maxiter = parse_int(optarg, "-m"); break;
+		case 'P': conf.periodic = true; break;
+		case 'g': break;
+		case 'p': pattern_file = optarg; break;
+		case 'o': conf.os = parse_float(optarg, "-o"); break;
+		case 'w': conf.width = parse_float(optarg, "-w"); break;
+		case O_LOWMEM: conf.lowmem = true; break;
+		case O_ZEROMEM: conf.zero_overhead = true; break;
+		case O_NOPRECOMP: precomp = false; break;
+		case 'c': unsupported_option("nufft", "-c"); break;
+		case 's': unsupported_option("nufft", "-s"); break;
+		case '1': unsupported_option("nufft", "-1"); break;
+		case 'B': unsupported_option("nufft", "-B"); break;
+		case 'F': unsupported_option("nufft", "-F"); break;
+		case 'T': unsupported_option("nufft", "-T"); break;
+		case O_NUFFTCONF: unsupported_option("nufft", "--nufft-conf"); break;
+		case 'h': usage(); return 0;
+		default: usage(); return 1;
+		}
+	}
+
+	if (3 != argc - optind) {
+		usage();
+		cli_error("nufft: expected 3 file arguments");
+	}
+
+	const char* traj_file = argv[optind];
+	const char* in_file = argv[optind + 1];
+	const char* out_file = argv[optind + 2];
+
+	if (maxiter < 0)
+		cli_error("nufft: -m needs a non-negative iteration count");
+
+	if (adjoint && inverse)
+		cli_error("Adjoint and inverse requested at the same time.\n");
+
+	if (!precomp) {
+		conf.lowmem = true;
+		conf.precomp_linphase = false;
+		conf.precomp_fftmod = false;
+		conf.precomp_roll = false;
+	}
+
+	// avoid computing the PSF if not necessary (src/nufft.c:134-136)
+	if (!inverse)
+		conf.toeplitz = false;
+
+	long traj_dims[DIMS];
+	complex float* traj = cfl_load(traj_file, DIMS, traj_dims);
+
+	if (3 != traj_dims[0])
+		cli_error("nufft: trajectory must have 3 coordinates in dim 0");
+
+	long coilest[3];
+	estimate_im_dims(traj_dims, traj, coilest);
+
+	if (8 >= coilest[0] * coilest[1] * coilest[2])
+		debug_printf(1, "\tThe estimated image size %ldx%ldx%ld is very small.\n"
+				"\tDid you scale your trajectory correctly?\n"
+				"\tThe unit of measurement is pixel_size / FOV.\n",
+			     coilest[0], coilest[1], coilest[2]);
+
+	long pattern_dims[DIMS];
+	complex float* pattern = NULL;
+
+	if (NULL != pattern_file)
+		pattern = cfl_load(pattern_file, DIMS, pattern_dims);
+
+	if (inverse || adjoint) {
+
+		long ksp_dims[DIMS];
+		complex float* ksp = cfl_load(in_file, DIMS, ksp_dims);
+
+		if (1 != ksp_dims[0])
+			cli_error("nufft: k-space must have size 1 in dim 0");
+
+		for (int i = 1; i < 3; i++)
+			if (ksp_dims[i] != traj_dims[i] && 1 != ksp_dims[i] && 1 != traj_dims[i])
+				cli_error("nufft: k-space and trajectory do not match in dim %d", i);
+
+		long coilim_dims[DIMS];
+
+		for (int i = 0; i < DIMS; i++)
+			coilim_dims[i] = 1;
+
+		if (0 == coilim_vec[0] * coilim_vec[1] * coilim_vec[2]) {
+
+			for (int i = 0; i < 3; i++)
+				coilim_dims[i] = coilest[i];
+
+			debug_printf(2, "Est. image size: %ldx%ldx%ld\n", coilim_dims[0], coilim_dims[1], coilim_dims[2]);
+
+		} else {
+
+			for (int i = 0; i < 3; i++)
+				coilim_dims[i] = coilim_vec[i];
+		}
+
+		for (int i = 3; i < DIMS; i++)
+			coilim_dims[i] = ksp_dims[i];
+
+		long n = cfl_size(DIMS, coilim_dims);
+		complex float* img = calloc((size_t)n, sizeof(complex float));
+
+		const struct linop_s* op = nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
+							 pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
+
+		if (inverse) {
+
+			complex float* b = calloc((size_t)n, sizeof(complex float));
+			linop_adjoint(op, DIMS, coilim_dims, b, DIMS, ksp_dims, ksp);
+
+			struct cg_stats_s st;
+			lsqr_conjgrad(op, maxiter, lambda, 0.f, DIMS, coilim_dims, img, b, false, false, NULL, &st);
+			debug_printf(3, "CG: %d iterations, |r| / |b| = %e, %.3f ms\n", st.iterations,
+				     st.bnorm > 0.f ? st.resid / st.bnorm : 0.f, st.time_ms);
+			free(b);
+
+		} else {
+
+			linop_adjoint(op, DIMS, coilim_dims, img, DIMS, ksp_dims, ksp);
+		}
+
+		linop_free(op);
+		cfl_write(out_file, DIMS, coilim_dims, img);
+		free(img);
+		free(ksp);
+
+	} else {
+
+		long coilim_dims[DIMS];
+		complex float* img = cfl_load(in_file, DIMS, coilim_dims);
+
+		long ksp_dims[DIMS];
+
+		for (int i = 0; i < DIMS; i++)
+			ksp_dims[i] = (i < 3) ? ((0 == i) ? 1 : traj_dims[i]) : coilim_dims[i];
+
+		long m = cfl_size(DIMS, ksp_dims);
+		complex float* ksp = calloc((size_t)m, sizeof(complex float));
+
+		const struct linop_s* op = nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
+							 pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
+
+		linop_forward(op, DIMS, ksp_dims, ksp, DIMS, coilim_dims, img);
+		linop_free(op);
+
+		cfl_write(out_file, DIMS, ksp_dims, ksp);
+		free(ksp);
+		free(img);
+	}
+
+	free(traj);
+	free(pattern);
+
+	return 0;
+}

@@ -86,9 +86,6 @@ static float host_pos(const GridGeom& g, int s, int d, float k)
 }
 #pragma clang fp contract(on)
 
-// LDS of one adjoint work-group (gfx950 has 160 KiB per CU)
-static const size_t LDS_BUDGET = 144 * 1024;
-
 static void finish_tiling(const GridGeom& g, GridTiling& t)
 {
 	t.ntiles = 1;
@@ -659,8 +656,6 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 #pragma unroll
 	for (int c = 0; c < CC; c++)
 		acc[c] = make_float2(0.f, 0.f);
-
-	int r = 0;
 
 	for (int base = fb; base < fe; base += ADJ_BATCH) {
 

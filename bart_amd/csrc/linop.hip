@@ -90,6 +90,16 @@ static bool on_device(const void* p)
 
 enum OpKind { OP_NUFFT = 1, OP_SENSE = 2 };
 
+// stream and lock of one NUFFT plan, shared by every handle that applies it
+struct Queue {
+
+	hipStream_t stream = 0;
+	std::mutex mtx;
+
+	Queue() { BAM_HIP(hipStreamCreate(&stream)); }
+	~Queue() { (void)hipStreamDestroy(stream); }
+};
+
 struct Op {
 
 	int kind = 0;
@@ -97,18 +107,19 @@ struct Op {
 	long odims[32];
 	long idims[32];
 	long osize = 0, isize = 0;
+	std::shared_ptr<Queue> q;
 	std::shared_ptr<Nufft> nufft;
 	std::shared_ptr<DevBuf<float2>> sens;
 	hipStream_t stream = 0;
-	std::mutex mtx;
+	std::mutex& mtx() { return q->mtx; }
 	DevBuf<float2> stage_in, stage_out, kbuf;
 	std::unique_ptr<CgSolver> cg;
 
 	~Op()
 	{
 		cg.reset();
-		if (0 != stream)
-			(void)hipStreamDestroy(stream);
+		nufft.reset();
+		q.reset();
 	}
 };
 
@@ -147,7 +158,7 @@ template <class T> static std::vector<T> to_host(const T* p, long n)
 template <class F>
 static void apply_staged(Op* op, float2* dst, long dn, const float2* src, long sn, F&& f)
 {
-	std::lock_guard<std::mutex> lock(op->mtx);
+	std::lock_guard<std::mutex> lock(op->mtx());
 
 	bool dd = on_device(dst);
 	bool sd = on_device(src);
@@ -269,7 +280,8 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 	memcpy(op->idims, cim_dims, sizeof(long) * N);
 	op->osize = md_size(N, ksp_dims);
 	op->isize = md_size(N, cim_dims);
-	BAM_HIP(hipStreamCreate(&op->stream));
+	op->q = std::make_shared<Queue>();
+	op->stream = op->q->stream;
 
 	try {
 		op->nufft = std::make_shared<Nufft>();
@@ -366,7 +378,6 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		nufft_conf_s conf = *_conf;
 		check_conf(conf, nullptr != basis);
 		BAM_CHECK(0 == shared_img_dims, "sense_nc_init: shared image dims are not supported");
-		BAM_CHECK(nullptr == fft_opp, "sense_nc_init: fft_opp is not supported");
 		BAM_CHECK(1 == max_dims[4] && 1 == map_dims[4], "sense_nc_init: only one set of maps is supported");
 
 		long coilim_dims[16], img_dims[16];
@@ -400,6 +411,23 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		} catch (...) {
 			delete op;
 			throw;
+		}
+
+		if (nullptr != fft_opp) {
+
+			// the NUFFT alone (coil images <-> k-space), sharing the plan,
+			// stream and lock of the SENSE operator; freed separately
+			Op* nop = new Op();
+			nop->kind = OP_NUFFT;
+			nop->N = N;
+			memcpy(nop->odims, ksp2, sizeof(long) * N);
+			memcpy(nop->idims, coilim_dims, sizeof(long) * N);
+			nop->osize = md_size(N, ksp2);
+			nop->isize = md_size(N, coilim_dims);
+			nop->q = op->q;
+			nop->stream = op->stream;
+			nop->nufft = op->nufft;
+			*fft_opp = (const struct linop_s*)nop;
 		}
 
 		return (const struct linop_s*)op;
@@ -576,9 +604,9 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		Op* op = (Op*)_op;
 		BAM_CHECK(nullptr != op, "lsqr_conjgrad: NULL operator");
 		check_dims("lsqr_conjgrad", N, img_dims, op->N, op->idims);
-		BAM_CHECK(op->nufft->toeplitz && op->nufft->has_psf, "lsqr_conjgrad: needs a Toeplitz normal operator");
+		BAM_CHECK(!op->nufft->toeplitz || op->nufft->has_psf, "lsqr_conjgrad: Toeplitz operator without PSF");
 
-		std::lock_guard<std::mutex> lock(op->mtx);
+		std::lock_guard<std::mutex> lock(op->mtx());
 
 		long ne = op->isize;
 
@@ -587,8 +615,25 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 
 		const float2* sens = (OP_SENSE == op->kind) ? op->sens->ptr : nullptr;
 		Nufft* nf = op->nufft.get();
+		NormalFn normal;
 
-		NormalFn normal = [nf, sens](float2* d, const float2* s, const int* skip) { nf->normal(d, s, sens, skip); };
+		if (nf->toeplitz) {
+
+			normal = [nf, sens](float2* d, const float2* s, const int* skip) { nf->normal(d, s, sens, skip); };
+
+		} else {
+
+			// explicit normal operator A^H A (linops/linop.c:489-525); the
+			// k-space buffer exists before the iterations are captured
+			if ((long)op->kbuf.n < op->osize)
+				op->kbuf.alloc(op->osize);
+
+			float2* kb = op->kbuf.ptr;
+			normal = [nf, sens, kb](float2* d, const float2* s, const int*) {
+				nf->forward(kb, s, sens);
+				nf->adjoint(d, kb, sens);
+			};
+		}
 
 		bool xd = on_device(x), bd = on_device(b);
 		float2* xdev = (float2*)x;

@@ -74,7 +74,9 @@ struct linop_s* nufft_create(int N, const long ksp_dims[], const long cim_dims[]
 
 /* sense/modelnc.h -- replaces sense_nc_init (sense/modelnc.c:28-67): the
  * linear operator image -> k-space  A = NUFFT S  with normal operator
- * S^H T S (Toeplitz) when conf->toeplitz.  One set of maps (MAPS dim 1). */
+ * S^H T S (Toeplitz) when conf->toeplitz.  One set of maps (MAPS dim 1).
+ * When fft_opp is not NULL it receives the NUFFT operator (coil images <->
+ * k-space) sharing the same plan and PSF; free it with linop_free. */
 const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims[16], const BAM_COMPLEX* maps,
 				    const long ksp_dims[16], const long traj_dims[16], const BAM_COMPLEX* traj,
 				    const struct nufft_conf_s* conf, const long wgs_dims[16], const BAM_COMPLEX* weights,
@@ -106,7 +108,8 @@ void fft(int D, const long dimensions[], unsigned long flags, BAM_COMPLEX* dst, 
 void ifft(int D, const long dimensions[], unsigned long flags, BAM_COMPLEX* dst, const BAM_COMPLEX* src);
 
 /* Conjugate gradients on (A^H A + l2lambda I) x = b with A^H A the normal
- * operator of `op` (BART's lsqr2 -> iter2_conjgrad -> conjgrad chain:
+ * operator of `op` -- Toeplitz when the operator was created with
+ * conf.toeplitz, else adjoint(forward(x)) (BART's lsqr2 -> iter2_conjgrad -> conjgrad chain:
  * iter/lsqr.c:52-170, iter/iter2.c:227-270, iter/italgos.c:497-580).
  * Stops after maxiter iterations or when ||r|| <= tol ||b|| (tol = 0 as in
  * BART: fixed iteration count).  x holds the initial guess when warmstart

@@ -1,0 +1,142 @@
+"""GPU parity of the command-line drop-in: ``bart nufft`` and ``bart pics``
+through .cfl files against the CPU oracle on the same inputs.
+
+Expected values follow src/nufft.c and src/pics.c: forward/adjoint/inverse
+NUFFT; for pics the pattern of non-zero samples, the data scale of
+sense/optcom.c:74-109 computed from |A^H y|, CG on the scaled data and -S.
+Tolerances: 1e-5 for single operators, 1e-4 after CG iterations (as the
+library's CG-SENSE test).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from bart_amd import inputs as I
+from cli_util import dims16, read_raw, run_bart, write_raw
+from conftest import crandn, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def scaling_norm(x):
+    """sense/optcom.c:74-109 with rescale 1, compat false, p = -1."""
+    a = np.sort(np.abs(x.reshape(-1).astype(np.complex64)))
+    n = a.size
+    median, p90, mx = a[n // 2], a[int(np.trunc(n * 0.9))], a[n - 1]
+    return p90 if (mx - p90) < 2 * (p90 - median) else mx
+
+
+def write_traj(path, traj, R, S):
+    write_raw(path, dims16(3, R, S), traj.reshape(-1))
+
+
+def test_cli_nufft_forward_adjoint(tmp_path):
+    n, C, R, S = 64, 3, 128, 31
+    traj = I.radial_traj(R, S)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=False)
+    x = crandn((C, 1, n, n), 71)
+    write_traj(str(tmp_path / "t"), traj, R, S)
+    write_raw(str(tmp_path / "x"), dims16(n, n, 1, C), x)
+    run_bart(["nufft", "t", "x", "k"], tmp_path)
+    dims, k = read_raw(str(tmp_path / "k"))
+    assert dims == dims16(1, R, S, C)
+    assert relerr(k, ref.forward(x).reshape(-1)) < 1e-5
+
+    y = crandn((C, R * S), 72)
+    write_raw(str(tmp_path / "y"), dims16(1, R, S, C), y)
+    # image size estimated from the trajectory: 2 ceil(max |k|) = 64
+    run_bart(["nufft", "-a", "t", "y", "a"], tmp_path)
+    dims, a = read_raw(str(tmp_path / "a"))
+    assert dims == dims16(n, n, 1, C)
+    assert relerr(a, ref.adjoint(y).reshape(-1)) < 1e-5
+    run_bart(["nufft", "-a", "-x", f"{n}:{n}:1", "--lowmem", "t", "y", "a2"], tmp_path)
+    assert np.array_equal(read_raw(str(tmp_path / "a2"))[1], a)
+
+
+def test_cli_nufft_inverse(tmp_path):
+    # nufft -i: CG on (A^H A + l I) x = A^H y; one coil equals CG-SENSE with
+    # unit maps
+    n, R, S, lam, it = 64, 128, 41, 0.01, 20
+    traj = I.radial_traj(R, S)
+    ref = O.Nufft((n, n, 1), traj, 1, toeplitz=True)
+    img = I.ellipse_phantom(n, seed=5).reshape(1, 1, n, n)
+    ones = np.ones((1, 1, n, n), np.complex64)
+    y = I.add_noise(ref.sense_forward(ones, img), 1e-3, seed=5)
+    b = ref.sense_adjoint(ones, y)
+    xr, _, _, _ = ref.cgsense(ones, b, lam, it, 0.0)
+    write_traj(str(tmp_path / "t"), traj, R, S)
+    write_raw(str(tmp_path / "y"), dims16(1, R, S, 1), y)
+    run_bart(["nufft", "-i", "-l", lam, "-m", it, "t", "y", "x"], tmp_path)
+    _, x = read_raw(str(tmp_path / "x"))
+    err = relerr(x, xr.reshape(-1))
+    print(f"nufft -i vs oracle: {err:.2e}")
+    assert err < 1e-4
+    # -r: explicit A^H A instead of the Toeplitz operator (same system up to
+    # the accuracy of the Toeplitz embedding)
+    run_bart(["nufft", "-i", "-r", "-l", lam, "-m", it, "t", "y", "xr"], tmp_path)
+    _, x2 = read_raw(str(tmp_path / "xr"))
+    err2 = relerr(x2, xr.reshape(-1))
+    print(f"nufft -i -r vs Toeplitz oracle: {err2:.2e}")
+    assert err2 < 2e-3
+
+
+def make_pics_case(tmp_path, n=64, C=4, R=128, S=41, seed=41):
+    traj = I.radial_traj(R, S)
+    img = I.ellipse_phantom(n, seed=seed)
+    maps = I.gaussian_coils((1, n, n), C, seed=seed)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    y = I.add_noise(ref.sense_forward(maps, img), 1e-3, seed=seed)
+    write_traj(str(tmp_path / "t"), traj, R, S)
+    write_raw(str(tmp_path / "k"), dims16(1, R, S, C), y)
+    write_raw(str(tmp_path / "s"), dims16(n, n, 1, C), maps)
+    return ref, maps, y
+
+
+def test_cli_pics_matches_oracle(tmp_path):
+    ref, maps, y = make_pics_case(tmp_path)
+    lam, it = 0.01, 30
+    scale = scaling_norm(ref.sense_adjoint(maps, y))
+    ys = (y * np.complex64(1.0 / scale)).astype(np.complex64)
+    xr, _, _, _ = ref.cgsense(maps, ref.sense_adjoint(maps, ys), lam, it, 0.0)
+    # pics -l2 -r 0.01 -i 30 -t traj (automatic scaling, result left scaled)
+    run_bart(["pics", "-l2", "-r", lam, "-i", it, "-t", "t", "k", "s", "x"], tmp_path)
+    dims, x = read_raw(str(tmp_path / "x"))
+    assert dims == dims16(64, 64, 1)
+    err = relerr(x, xr.reshape(-1))
+    print(f"pics vs oracle: {err:.2e}")
+    assert err < 1e-4
+    # -S rescales the result by the data scale
+    run_bart(["pics", "-S", "-r", lam, "-i", it, "-t", "t", "k", "s", "xs"], tmp_path)
+    _, xs = read_raw(str(tmp_path / "xs"))
+    assert relerr(xs, xr.reshape(-1) * np.float32(scale)) < 1e-4
+    # -w 1 (no scaling) is the benchmark's pics -S -w 1 -r 0.01 -i 30 -t
+    run_bart(["pics", "-S", "-w", 1, "-r", lam, "-i", it, "-t", "t", "k", "s", "x1"], tmp_path)
+    _, x1 = read_raw(str(tmp_path / "x1"))
+    xr1, _, _, _ = ref.cgsense(maps, ref.sense_adjoint(maps, y), lam, it, 0.0)
+    assert relerr(x1, xr1.reshape(-1)) < 1e-4
+
+
+def test_cli_pics_pattern_psf_and_warm_start(tmp_path):
+    ref, maps, y = make_pics_case(tmp_path, seed=42)
+    lam, it = 0.01, 10
+    # --psf_export writes the PSF (dims N + 1, decomposition index last)
+    run_bart(["pics", "-r", lam, "-i", it, "--psf_export", "p", "-t", "t", "k", "s", "x"], tmp_path)
+    dims, psf = read_raw(str(tmp_path / "p"))
+    assert dims == [64, 64, 1] + [1] * 13 + [4]
+    assert relerr(psf, ref.psf().reshape(-1)) < 1e-5
+    # --psf_import of that PSF gives bitwise the same reconstruction
+    run_bart(["pics", "-r", lam, "-i", it, "--psf_import", "p", "-t", "t", "k", "s", "x2"], tmp_path)
+    assert np.array_equal(read_raw(str(tmp_path / "x2"))[1], read_raw(str(tmp_path / "x"))[1])
+    # an explicit all-ones pattern equals the estimated one for noisy data
+    write_raw(str(tmp_path / "pat"), dims16(1, 128, 41), np.ones(128 * 41, np.complex64))
+    run_bart(["pics", "-r", lam, "-i", it, "-p", "pat", "-t", "t", "k", "s", "x3"], tmp_path)
+    assert np.array_equal(read_raw(str(tmp_path / "x3"))[1], read_raw(str(tmp_path / "x"))[1])
+    # -W with 0 iterations returns the warm start (tests/pics.mk:184-190)
+    w0 = crandn((1, 64, 64), 43)
+    write_raw(str(tmp_path / "w"), dims16(64, 64, 1), w0)
+    run_bart(["pics", "-i", 0, "-W", "w", "-t", "t", "k", "s", "x4"], tmp_path)
+    assert np.array_equal(read_raw(str(tmp_path / "x4"))[1], w0.reshape(-1))
+    # --tol stops early at the relative residual
+    run_bart(["pics", "-r", lam, "-i", 200, "--tol", 1e-3, "-d", 3, "-t", "t", "k", "s", "x5"], tmp_path)
+    _, x5 = read_raw(str(tmp_path / "x5"))
+    assert np.all(np.isfinite(x5))
