@@ -382,6 +382,14 @@ constexpr int ct_num_radices(int n)
 	return c;
 }
 
+constexpr int ct_log2(int p)
+{
+	int l = 0;
+	while ((1 << l) < p)
+		l++;
+	return l;
+}
+
 // transforms per block for a compile-time length (a power of two): up to
 // 4096 elements, at least 16 columns (strided geometry) up to n = 512
 constexpr int ct_block_p(int n)
@@ -402,10 +410,14 @@ __device__ inline void ct_pass(float2* lds, const float2* tw, float sgn)
 	float2 v[PER][R];
 	int dst[PER];
 
+	// see slice_pass: butterfly addresses are computed per pass
+	int lane = threadIdx.x;
+	asm volatile("" : "+v"(lane));
+
 #pragma unroll
 	for (int i = 0; i < PER; i++) {
 
-		const int b = threadIdx.x + i * FFT_THREADS;
+		const int b = lane + i * FFT_THREADS;
 		dst[i] = -1;
 
 		if (0 == NB % FFT_THREADS || b < NB) {
@@ -462,7 +474,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 {
 	constexpr int P = ct_block_p(N);
 	constexpr int TOT = P * N;
-	constexpr int LOGP = (P >= 64) ? 6 : ((P >= 32) ? 5 : ((P >= 16) ? 4 : ((P >= 8) ? 3 : ((P >= 4) ? 2 : ((P >= 2) ? 1 : 0)))));
+	constexpr int LOGP = ct_log2(P);
+	static_assert((1 << LOGP) == P, "transforms per block must be a power of two");
 
 	extern __shared__ float2 lds[];
 	float2* tw = lds + TOT;
@@ -526,6 +539,88 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 	}
 
 	st.store(lds, L, t0);
+}
+
+// ---------------------------------------------------------------------------
+// Strided multi-transform passes in LDS for whole 2-D slices: P transforms
+// of length N, element j of transform p at lds[p TS + j ES].  Butterflies
+// are numbered with p fastest so that neighbouring lanes touch neighbouring
+// transforms (consecutive addresses for the column transforms of a
+// row-major slice, row pitch TS apart for the row transforms).  NTH threads.
+// ---------------------------------------------------------------------------
+
+template <int N, int R, int NS, int P, int ES, int TS, int NTH>
+__device__ inline void slice_pass(float2* lds, const float2* tw, float sgn)
+{
+	constexpr int nR = N / R;
+	constexpr int NB = P * nR;
+	constexpr int PER = (NB + NTH - 1) / NTH;
+	constexpr int TSTEP = N / (NS * R);
+	float2 v[PER][R];
+	int dst[PER];
+
+	// the lane index passes through an empty asm so that the butterfly
+	// addresses are computed here rather than hoisted out of the caller's
+	// loops (they would occupy registers across all passes)
+	int lane = threadIdx.x;
+	asm volatile("" : "+v"(lane));
+
+#pragma unroll
+	for (int i = 0; i < PER; i++) {
+
+		const int b = lane + i * NTH;
+		dst[i] = -1;
+
+		if (0 == NB % NTH || b < NB) {
+
+			const int p = b % P;
+			const int j = b / P;
+			const int k = j % NS;
+			const float2* src = lds + p * TS + j * ES;
+
+#pragma unroll
+			for (int r = 0; r < R; r++) {
+
+				float2 x = src[r * nR * ES];
+
+				if (r > 0 && NS > 1) {
+					float2 w = tw[r * k * TSTEP];
+					if (sgn > 0.f)
+						w.y = -w.y;
+					x = cmul(x, w);
+				}
+
+				v[i][r] = x;
+			}
+
+			dft_any<R>(v[i], sgn, tw, N);
+			dst[i] = p * TS + ((j / NS) * NS * R + k) * ES;
+		}
+
+		// one butterfly's loads at a time (register pressure)
+		__builtin_amdgcn_sched_barrier(0);
+	}
+
+	__syncthreads();
+
+#pragma unroll
+	for (int i = 0; i < PER; i++)
+		if (0 == NB % NTH || dst[i] >= 0)
+#pragma unroll
+			for (int q = 0; q < R; q++)
+				lds[dst[i] + q * NS * ES] = v[i][q];
+
+	__syncthreads();
+}
+
+template <int N, int P, int ES, int TS, int NTH, int IDX = 0, int NS = 1>
+__device__ inline void slice_passes(float2* lds, const float2* tw, float sgn)
+{
+	if constexpr (IDX < ct_num_radices(N)) {
+		constexpr int R = ct_radix(N, IDX);
+		slice_pass<N, R, NS, P, ES, TS, NTH>(lds, tw, sgn);
+		slice_passes<N, P, ES, TS, NTH, IDX + 1, NS * R>(lds, tw, sgn);
+	}
 }
 
 // lengths with a compile-time kernel

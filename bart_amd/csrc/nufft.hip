@@ -458,6 +458,356 @@ struct StReduceS {
 };
 
 // ---------------------------------------------------------------------------
+// Slice kernels for 3-D grids whose x-y slice fits in LDS (NX x NY <= 128 x
+// 128): one work-group transforms whole slices along x and y, so that the
+// sub-grid stack makes one round trip through HBM for both in-plane axes.
+// LDS holds the slice row-major with a row pitch of NX + 1 (bank spread of
+// the column transforms) and the two twiddle tables.
+//
+// k_slice_fwd: work-group (z, c).  The image-side values of slice z (coil
+// image c, or image x map c; x rolloff when roll) are read once into
+// registers; for every sub-grid s they are multiplied by linphase_s,
+// transformed along x and y and stored to grid[s][c][z].
+//
+// k_slice_inv: work-group (z, coil group).  For every coil c of the group
+// and every s: grid[s][c][z] is inverse-transformed along y and x and
+// conj(linphase_s) x value is summed over s in order (x rolloff when roll);
+// with maps the coils are summed in order as conj(map_c) x value into the
+// group's partial image, else coil image c is stored.  No atomics.
+// ---------------------------------------------------------------------------
+
+// threads of a slice work-group: a multiple of NX (every lane keeps its x)
+template <int NX>
+constexpr int slice_threads()
+{
+	return (1024 / NX) * NX;
+}
+
+template <int NX, int NY>
+constexpr int slice_threads_inv()
+{
+	return slice_threads<NX>();
+}
+
+struct SliceGeom {
+	int nz, S, C;
+	long N;
+};
+
+template <int NX, int NY>
+constexpr size_t slice_lds_bytes()
+{
+	return sizeof(float2) * ((size_t)NY * (NX + 1) + NX + NY);
+}
+
+// separable factors of one lane: x and z fixed, y = y0 + i dy
+struct SliceLane {
+	int x, y0, dy;
+};
+
+__device__ inline float2 lin_xyz(const SepTables& T, int s, int x, int y, int z)
+{
+	// same product order as lin_val: ((x) y) z
+	float2 v = T.lin[0][(s >> T.bitpos[0]) & 1][x];
+	v = cmul(v, T.lin[1][(s >> T.bitpos[1]) & 1][y]);
+	return cmul(v, T.lin[2][(s >> T.bitpos[2]) & 1][z]);
+}
+
+__device__ inline float roll_xyz(const SepTables& T, int x, int y, int z)
+{
+	return T.roll[0][x] * T.roll[1][y] * T.roll[2][z];
+}
+
+template <int NX, int NY>
+__global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg, SepTables T,
+								    const float2* __restrict__ src,
+								    const float2* __restrict__ sens, bool roll,
+								    float2* __restrict__ grid,
+								    const float2* __restrict__ twx_g,
+								    const float2* __restrict__ twy_g, const int* skip)
+{
+	constexpr int NTH = slice_threads<NX>();
+	constexpr int DY = NTH / NX;
+	constexpr int EPT = (NY + DY - 1) / DY;
+	constexpr int NP = NX * NY;
+	constexpr int LD = NX + 1;
+
+	extern __shared__ float2 lds[];
+	float2* twx = lds + NY * LD;
+	float2* twy = twx + NX;
+
+	if (nullptr != skip && 0 != *skip)
+		return;
+
+	const int z = blockIdx.x;
+	const int c = blockIdx.y;
+	const int x = threadIdx.x % NX;
+	const int y0 = threadIdx.x / NX;
+
+	for (int i = threadIdx.x; i < NX; i += NTH)
+		twx[i] = twx_g[i];
+
+	for (int i = threadIdx.x; i < NY; i += NTH)
+		twy[i] = twy_g[i];
+
+	const long zoff = (long)z * NP;
+	const float2* img = (nullptr != sens) ? src + zoff : src + (long)c * sg.N + zoff;
+	const float2* map = (nullptr != sens) ? sens + (long)c * sg.N + zoff : nullptr;
+
+	for (int s = 0; s < sg.S; s++) {
+
+		// per-iteration copies of the lane coordinates (keeps the element
+		// addresses out of registers across the loop)
+		int xs = x, ys = y0;
+		asm volatile("" : "+v"(xs), "+v"(ys));
+
+		__syncthreads();
+
+		// the image-side values are read again for every sub-grid (cache
+		// hits after the first) instead of being held in registers
+#pragma unroll 4
+		for (int i = 0; i < EPT; i++) {
+
+			const int x = xs;
+			const int y = ys + i * DY;
+
+			if (0 == NY % DY || y < NY) {
+
+				const int e = y * NX + x;
+				float2 v = (nullptr != map) ? cmul(img[e], map[e]) : img[e];
+
+				if (roll)
+					v = cscale(v, roll_xyz(T, x, y, z));
+
+				lds[y * LD + x] = cmul(v, lin_xyz(T, s, x, y, z));
+			}
+		}
+
+		__syncthreads();
+
+		slice_passes<NX, NY, 1, LD, NTH>(lds, twx, -1.f);
+		slice_passes<NY, NX, LD, 1, NTH>(lds, twy, -1.f);
+
+		float2* out = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
+
+		xs = x;
+		ys = y0;
+		asm volatile("" : "+v"(xs), "+v"(ys));
+
+#pragma unroll 4
+		for (int i = 0; i < EPT; i++) {
+
+			const int y = ys + i * DY;
+
+			if (0 == NY % DY || y < NY)
+				out[y * NX + xs] = lds[y * LD + xs];
+		}
+	}
+}
+
+template <int NX, int NY>
+__global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(SliceGeom sg, SepTables T,
+								    const float2* __restrict__ grid,
+								    const float2* __restrict__ sens, bool roll, int cc,
+								    float2* __restrict__ dst,
+								    const float2* __restrict__ twx_g,
+								    const float2* __restrict__ twy_g, const int* skip)
+{
+	constexpr int NTH = slice_threads_inv<NX, NY>();
+	constexpr int DY = NTH / NX;
+	constexpr int EPT = (NY + DY - 1) / DY;
+	constexpr int NP = NX * NY;
+	constexpr int LD = NX + 1;
+
+	extern __shared__ float2 lds[];
+	float2* twx = lds + NY * LD;
+	float2* twy = twx + NX;
+
+	if (nullptr != skip && 0 != *skip)
+		return;
+
+	const int z = blockIdx.x;
+	const int gi = blockIdx.y;
+	const long zoff = (long)z * NP;
+	const int x = threadIdx.x % NX;
+	const int y0 = threadIdx.x / NX;
+
+	for (int i = threadIdx.x; i < NX; i += NTH)
+		twx[i] = twx_g[i];
+
+	for (int i = threadIdx.x; i < NY; i += NTH)
+		twy[i] = twy_g[i];
+
+	float2 acc[EPT];
+
+#pragma unroll
+	for (int i = 0; i < EPT; i++)
+		acc[i] = make_float2(0.f, 0.f);
+
+	for (int cl = 0; cl < cc; cl++) {
+
+		const int c = gi * cc + cl;
+		const float2* map = (nullptr != sens) ? sens + (long)c * sg.N + zoff : nullptr;
+
+		for (int s = 0; s < sg.S; s++) {
+
+			const float2* in = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
+
+			// per-iteration copies of the lane coordinates (keeps the
+			// element addresses out of registers across the loop)
+			int xs = x, ys = y0;
+			asm volatile("" : "+v"(xs), "+v"(ys));
+
+			__syncthreads();
+
+#pragma unroll 4
+			for (int i = 0; i < EPT; i++) {
+
+				const int y = ys + i * DY;
+
+				if (0 == NY % DY || y < NY)
+					lds[y * LD + xs] = in[y * NX + xs];
+			}
+
+			__syncthreads();
+
+			slice_passes<NY, NX, LD, 1, NTH>(lds, twy, +1.f);
+			slice_passes<NX, NY, 1, LD, NTH>(lds, twx, +1.f);
+
+			// sum over s of conj(linphase_s) x value; with maps the factor
+			// conj(map_c) (and the rolloff) is applied per term so that one
+			// accumulator carries the sums over s and coils
+			xs = x;
+			ys = y0;
+			asm volatile("" : "+v"(xs), "+v"(ys));
+
+#pragma unroll
+			for (int i = 0; i < EPT; i++) {
+
+				const int x = xs;
+				const int y = ys + i * DY;
+
+				if (0 == NY % DY || y < NY) {
+
+					float2 f = lin_xyz(T, s, x, y, z);
+
+					if (nullptr != map) {
+						f = cmul(f, map[y * NX + x]);
+						if (roll)
+							f = cscale(f, roll_xyz(T, x, y, z));
+					}
+
+					acc[i] = cadd(acc[i], cmulc(lds[y * LD + x], f));
+				}
+			}
+		}
+
+		if (nullptr != map)
+			continue;
+
+		// coil image c
+		int xs = x, ys = y0;
+		asm volatile("" : "+v"(xs), "+v"(ys));
+
+#pragma unroll
+		for (int i = 0; i < EPT; i++) {
+
+			const int y = ys + i * DY;
+
+			if (0 == NY % DY || y < NY)
+				dst[(long)c * sg.N + zoff + y * NX + xs] = roll ? cscale(acc[i], roll_xyz(T, xs, y, z)) : acc[i];
+
+			acc[i] = make_float2(0.f, 0.f);
+		}
+	}
+
+	if (nullptr == sens)
+		return;
+
+	int xs = x, ys = y0;
+	asm volatile("" : "+v"(xs), "+v"(ys));
+
+#pragma unroll
+	for (int i = 0; i < EPT; i++) {
+
+		const int y = ys + i * DY;
+
+		if (0 == NY % DY || y < NY)
+			dst[(long)gi * sg.N + zoff + y * NX + xs] = acc[i];
+	}
+}
+
+// slice sizes with a kernel (square slices)
+static bool slice_supported(const int n[3])
+{
+	if (n[0] != n[1] || n[2] < 2)
+		return false;
+
+	switch (n[0]) {
+	case 32: case 64: case 96: case 128:
+		return true;
+	default:
+		return false;
+	}
+}
+
+// coils per work-group of the inverse slice kernel: enough work-groups to
+// fill the chip (>= 512) while keeping the partial images few
+static int slice_coil_group(int nz, int C)
+{
+	int cc = C;
+
+	while (cc > 1 && 0 == cc % 2 && (long)nz * (C / cc) < 512)
+		cc /= 2;
+
+	return cc;
+}
+
+static void launch_slice_fwd(const Nufft& p, const float2* src, const float2* sens, bool roll, const int* skip,
+			     const char* name)
+{
+	ProfScope prof(name, p.stream);
+	SliceGeom sg{ p.n[2], p.S, p.C, p.N };
+	dim3 blocks(p.n[2], p.C);
+
+#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_fwd<NN, NN>), blocks, dim3(slice_threads<NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, src, sens, roll, p.grid.ptr, p.axis[0].tw, p.axis[1].tw,   				   skip);                                                                       		break
+
+	switch (p.n[0]) {
+		BAM_SLICE(32);
+		BAM_SLICE(64);
+		BAM_SLICE(96);
+		BAM_SLICE(128);
+	default:
+		fail("slice FFT: unsupported size %d", p.n[0]);
+	}
+#undef BAM_SLICE
+
+	BAM_HIP(hipGetLastError());
+}
+
+static void launch_slice_inv(const Nufft& p, float2* dst, const float2* sens, bool roll, int cc, const int* skip,
+			     const char* name)
+{
+	ProfScope prof(name, p.stream);
+	SliceGeom sg{ p.n[2], p.S, p.C, p.N };
+	dim3 blocks(p.n[2], p.C / cc);
+
+#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_inv<NN, NN>), blocks, dim3(slice_threads_inv<NN, NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, p.grid.ptr, sens, roll, cc, dst, p.axis[0].tw,             				   p.axis[1].tw, skip);                                                         		break
+
+	switch (p.n[0]) {
+		BAM_SLICE(32);
+		BAM_SLICE(64);
+		BAM_SLICE(96);
+		BAM_SLICE(128);
+	default:
+		fail("slice FFT: unsupported size %d", p.n[0]);
+	}
+#undef BAM_SLICE
+
+	BAM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Small element-wise kernels
 // ---------------------------------------------------------------------------
 
@@ -562,6 +912,8 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	for (int j = 0; j < nact; j++)
 		axis[j] = make_axis_plan(n[act[j]]);
 
+	slice = (3 == nact) && slice_supported(n);
+
 	has_weights = (nullptr != weights_host);
 
 	if (has_weights) {
@@ -607,7 +959,7 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 
 	// partial images of the coil groups of S^H (allocated here so that no
 	// allocation happens inside a captured CG iteration)
-	const int G = C / coil_group(S, C, n[act[0]], true);
+	const int G = C / (slice ? slice_coil_group(n[2], C) : coil_group(S, C, n[act[0]], true));
 
 	if (G > 1)
 		part.alloc((size_t)G * N);
@@ -620,8 +972,26 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 // Forward / adjoint / normal
 // ---------------------------------------------------------------------------
 
+// z axis of the sub-grid stack (slice path)
+static void z_pass(const Nufft& p, float sign, bool conj_fm, bool fm, const char* name)
+{
+	VolAxis va = VolAxis::make(p.n, 2, (long)p.S * p.C);
+	FftLaunchCfg cfg = fft_config(p.axis[2], va.transforms(), va.contig(), 1, sign, sign);
+	cfg.name = name;
+	LdGrid ld{ va, p.tabs, p.grid.ptr, conj_fm };
+	StGrid st{ va, p.tabs, p.grid.ptr, fm };
+	launch_fft_axis(cfg, p.stream, ld, NoMid{}, st);
+}
+
 void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
 {
+	if (slice) {
+		launch_slice_fwd(*this, src, sens, true, nullptr, "fft_fwd_slice");
+		z_pass(*this, -1.f, false, true, "fft_fwd_axis");
+		grid_forward(gp, kb.ptr, ksp, grid.ptr, has_weights ? weights.ptr : nullptr, stream);
+		return;
+	}
+
 	const int a0 = act[0];
 
 	// first axis: image side -> grid (x roll x linphase), FFT
@@ -652,6 +1022,22 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 {
 	// the output-stationary adjoint gridding writes every grid point
 	grid_adjoint(gp, kb.ptr, grid.ptr, ksp, has_weights ? weights.ptr : nullptr, stream);
+
+	if (slice) {
+
+		z_pass(*this, +1.f, true, false, "fft_adj_axis");
+
+		const int cc = (nullptr != sens) ? slice_coil_group(n[2], C) : 1;
+		const int G = C / cc;
+		float2* out = (nullptr != sens && G > 1) ? part.ptr : dst;
+
+		launch_slice_inv(*this, out, sens, true, cc, nullptr, "fft_adj_slice");
+
+		if (out != dst)
+			sum_groups(dst, part.ptr, N, G, stream, nullptr);
+
+		return;
+	}
 
 	for (int j = nact - 1; j >= 1; j--) {
 
@@ -687,6 +1073,29 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int* skip)
 {
 	BAM_CHECK(has_psf, "Toeplitz normal operator without PSF");
+
+	if (slice) {
+
+		launch_slice_fwd(*this, src, sens, false, skip, "fft_toeplitz_slice_fwd");
+
+		VolAxis va = VolAxis::make(n, 2, (long)S * C);
+		FftLaunchCfg cfg = fft_config_skip(skip, axis[2], va.transforms(), va.contig(), 1, -1.f, +1.f);
+		LdGrid ld{ va, tabs, grid.ptr, false };
+		StGrid st{ va, tabs, grid.ptr, false };
+		MdPsf md{ va, C, psf.ptr };
+		launch_fft_axis(cfg, stream, ld, md, st);
+
+		const int cc = (nullptr != sens) ? slice_coil_group(n[2], C) : 1;
+		const int G = C / cc;
+		float2* out = (nullptr != sens && G > 1) ? part.ptr : dst;
+
+		launch_slice_inv(*this, out, sens, false, cc, skip, "fft_toeplitz_slice_inv");
+
+		if (out != dst)
+			sum_groups(dst, part.ptr, N, G, stream, skip);
+
+		return;
+	}
 
 	const int cc = coil_group(S, C, n[act[0]], nullptr != sens);
 	const int G = C / cc;

