@@ -121,6 +121,8 @@ def lib():
     L.nufft_create.argtypes = [ctypes.c_int, lp, lp, lp, vp, vp, NufftConf]
     L.sense_nc_init.restype = vp
     L.sense_nc_init.argtypes = [lp, lp, vp, lp, lp, vp, ctypes.POINTER(NufftConf), lp, vp, lp, vp, vp, ctypes.c_ulong]
+    L.nudft_create.restype = vp
+    L.nudft_create.argtypes = [ctypes.c_int, ctypes.c_ulong, lp, lp, lp, vp, lp, vp, lp, vp]
     for f in (L.linop_forward, L.linop_adjoint):
         f.restype = None
         f.argtypes = [vp, ctypes.c_int, lp, vp, ctypes.c_int, lp, vp]
@@ -287,6 +289,14 @@ def nufft_create(ksp_dims, cim_dims, traj_dims, traj, weights=None, conf=None):
     traj = _host_c64(traj) if not _is_torch(traj) else traj
     w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
     h = lib().nufft_create(DIMS, _dims(ksp_dims), _dims(cim_dims), _dims(traj_dims), _ptr(traj), _ptr(w), conf)
+    return Linop(h, ksp_dims, cim_dims)
+
+
+def nudft_create(ksp_dims, cim_dims, traj_dims, traj):
+    """noncart/nudft.c:283-348: the direct non-uniform DFT (nufft -s)."""
+    traj = _host_c64(traj) if not _is_torch(traj) else traj
+    h = lib().nudft_create(DIMS, 7, _dims(ksp_dims), _dims(cim_dims), _dims(traj_dims), _ptr(traj), None, None,
+                           None, None)
     return Linop(h, ksp_dims, cim_dims)
 
 

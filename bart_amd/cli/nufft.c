@@ -9,9 +9,9 @@
  * -a: adjoint; -i: inverse (CG on (A^H A + lambda I) x = A^H y, Toeplitz
  * normal operator unless -r, 50 iterations by default as
  * iter_conjgrad_defaults, iter/iter.c:51-60).
- * -g is accepted (every path runs on the GPU).  -s (NUDFT), -c
- * (preconditioner), -1 (non-decomposed grid), -B (basis), -F/-T (B0) are
- * rejected with an error.
+ * -s: the direct NUDFT instead of the NUFFT (noncart/nudft.c).
+ * -g is accepted (every path runs on the GPU).  -c (preconditioner), -1
+ * (non-decomposed grid), -B (basis), -F/-T (B0) are rejected with an error.
  */
 #include "tools.h"
 #include "cfl.h"
@@ -26,7 +26,7 @@
 static void usage(void)
 {
 	fprintf(stderr,
-		"Usage: nufft [-a,-i] [-x x:y:z] [-t,-r] [-l lambda] [-m iter] [-P] [-g] [-p <pattern>]\n"
+		"Usage: nufft [-a,-i] [-x x:y:z] [-t,-r] [-l lambda] [-m iter] [-P] [-s] [-g] [-p <pattern>]\n"
 		"             [-o os] [-w width] [--lowmem] [--zero-mem] [--no-precomp] <traj> <input> <output>\n"
 		"\n"
 		"Perform non-uniform Fast Fourier Transform.\n");
@@ -36,6 +36,7 @@ int main_nufft(int argc, char* argv[])
 {
 	bool adjoint = false;
 	bool inverse = false;
+	bool dft = false;
 	bool precomp = true;
 	long coilim_vec[3] = { 0, 0, 0 };
 	float lambda = 0.f;
@@ -79,7 +80,7 @@ int main_nufft(int argc, char* argv[])
 		case O_ZEROMEM: conf.zero_overhead = true; break;
 		case O_NOPRECOMP: precomp = false; break;
 		case 'c': unsupported_option("nufft", "-c"); break;
-		case 's': unsupported_option("nufft", "-s"); break;
+		case 's': dft = true; break;
 		case '1': unsupported_option("nufft", "-1"); break;
 		case 'B': unsupported_option("nufft", "-B"); break;
 		case 'F': unsupported_option("nufft", "-F"); break;
@@ -137,6 +138,9 @@ int main_nufft(int argc, char* argv[])
 	if (NULL != pattern_file)
 		pattern = cfl_load(pattern_file, DIMS, pattern_dims);
 
+	if (dft && NULL != pattern)
+		cli_error("nufft: -s does not take a pattern (-p)");
+
 	if (inverse || adjoint) {
 
 		long ksp_dims[DIMS];
@@ -173,8 +177,10 @@ int main_nufft(int argc, char* argv[])
 		long n = cfl_size(DIMS, coilim_dims);
 		complex float* img = calloc((size_t)n, sizeof(complex float));
 
-		const struct linop_s* op = nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
-							 pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
+		const struct linop_s* op = dft ? nudft_create(DIMS, 7, ksp_dims, coilim_dims, traj_dims, traj, NULL, NULL,
+							      NULL, NULL)
+					       : nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
+							       pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
 
 		if (inverse) {
 
@@ -210,8 +216,10 @@ int main_nufft(int argc, char* argv[])
 		long m = cfl_size(DIMS, ksp_dims);
 		complex float* ksp = calloc((size_t)m, sizeof(complex float));
 
-		const struct linop_s* op = nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
-							 pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
+		const struct linop_s* op = dft ? nudft_create(DIMS, 7, ksp_dims, coilim_dims, traj_dims, traj, NULL, NULL,
+							      NULL, NULL)
+					       : nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
+							       pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
 
 		linop_forward(op, DIMS, ksp_dims, ksp, DIMS, coilim_dims, img);
 		linop_free(op);

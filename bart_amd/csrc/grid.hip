@@ -296,6 +296,16 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 
 	BAM_HIP(hipMemcpy(sm.ptr, hsm.data(), sizeof(int) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
 	BAM_HIP(hipMemcpy(sk.ptr, hsk.data(), sizeof(float4) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
+
+	{
+		std::vector<float4> hko(std::max<long>(g.M, 1));
+
+		for (long m = 0; m < g.M; m++)
+			hko[m] = make_float4(traj[3 * m + 0], traj[3 * m + 1], traj[3 * m + 2], 0.f);
+
+		ko.alloc(std::max<long>(g.M, 1));
+		BAM_HIP(hipMemcpy(ko.ptr, hko.data(), sizeof(float4) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
+	}
 	BAM_HIP(hipMemcpy(cstart.ptr, count.data(), sizeof(int) * (ncells + 1), hipMemcpyHostToDevice));
 
 	int nch = (g.C + ta.cc - 1) / ta.cc;
@@ -305,8 +315,9 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 	// chunks of at most MAXV; tiles with more than one chunk are split
 	const long MAXV = 2048;
 	std::vector<int4> hw;
-	std::vector<int> hsplit;
+	std::vector<int4> hsplit;
 	const GridTiling& t = ta;
+	int slots = 0;
 
 	for (int tile = 0; tile < t.ntiles; tile++) {
 
@@ -341,19 +352,22 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 		if (F <= MAXV) {
 			hw.push_back(make_int4(tile, 0, (int)F, 0));
 		} else {
-			hsplit.push_back(tile);
+			const int first = slots;
 			for (long b = 0; b < F; b += MAXV)
-				hw.push_back(make_int4(tile, (int)b, (int)std::min(F, b + MAXV), 1));
+				hw.push_back(make_int4(tile, (int)b, (int)std::min(F, b + MAXV), 1 + slots++));
+			hsplit.push_back(make_int4(tile, first, slots - first, 0));
 		}
 	}
 
 	nwork = (int)hw.size();
 	nsplit = (int)hsplit.size();
+	nslot = slots;
 	work.alloc(std::max(nwork, 1));
 	split_tiles.alloc(std::max(nsplit, 1));
 	BAM_HIP(hipMemcpy(work.ptr, hw.data(), sizeof(int4) * nwork, hipMemcpyHostToDevice));
 	if (nsplit > 0)
-		BAM_HIP(hipMemcpy(split_tiles.ptr, hsplit.data(), sizeof(int) * nsplit, hipMemcpyHostToDevice));
+		BAM_HIP(hipMemcpy(split_tiles.ptr, hsplit.data(), sizeof(int4) * nsplit, hipMemcpyHostToDevice));
+	spart.alloc((size_t)std::max(nslot, 1) * g.S * g.C * t.Wtot);
 }
 
 // taps per dim: floor(p + w/2) - ceil(p - w/2) + 1 <= floor(w) + 1
@@ -591,10 +605,11 @@ __device__ inline void select_taps(const Taps1<TP>& a, const Taps1<TP>& b, bool 
 
 constexpr int ADJ_BATCH = 64;
 
+// taps of one sample, dim and shift variant: sc = start * 256 + count
+// (start may be negative on periodic grids; count <= TP < 256)
 template <int TP>
 struct TapSet {
-	int start;
-	int cnt;
+	int sc;
 	float w[TP];
 };
 
@@ -602,7 +617,7 @@ template <int TP, int CC>
 __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, const float* __restrict__ kbt,
 						   const float4* __restrict__ sk, const int* __restrict__ cstart,
 						   const int4* __restrict__ work, const float2* __restrict__ kspS,
-						   float2* __restrict__ grid)
+						   float2* __restrict__ grid, float2* __restrict__ spart)
 {
 	extern __shared__ float4 smem4[];
 	char* smem = (char*)smem4;
@@ -611,7 +626,7 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 	TapSet<TP>* tbuf = (TapSet<TP>*)(smem + SMEM_DATA);                         // [BATCH][6]
 	float2* vbuf = (float2*)(smem + SMEM_DATA + sizeof(TapSet<TP>) * 6 * ADJ_BATCH);  // [BATCH][CC]
 
-	const int4 wk = work[blockIdx.x];  // (tile, first visit, end visit, split)
+	const int4 wk = work[blockIdx.x];  // (tile, first visit, end visit, partial slot + 1 or 0)
 	const int tile = wk.x;
 	const int ch = blockIdx.y;
 	const int cc = t.cc;
@@ -686,8 +701,7 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 				Taps1<TP> tt;
 				taps_raw<TP>(g, d, (var < g.nshv[d]) ? var : 0, kk[d], table, tt);
 				TapSet<TP>& ts = tbuf[v * 6 + k];
-				ts.start = tt.start;
-				ts.cnt = tt.cnt;
+				ts.sc = tt.start * 256 + tt.cnt;
 #pragma unroll
 				for (int q = 0; q < TP; q++)
 					ts.w[q] = tt.w[q];
@@ -709,23 +723,21 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 			for (int dd = 2; dd >= 0; dd--) {
 
 				const TapSet<TP>& ts = tbuf[v * 6 + 2 * dd + vb[dd]];
+				const int sc = ts.sc;
+				const int start = sc >> 8;
+				const int cnt = sc & 255;
 				int j;
 
 				if (g.periodic) {
-					j = (gp[dd] - ts.start) % g.n[dd];
+					j = (gp[dd] - start) % g.n[dd];
 					if (j < 0)
 						j += g.n[dd];
 				} else {
-					j = gp[dd] - ts.start;
+					j = gp[dd] - start;
 				}
 
-				const bool ok = (j >= 0) && (j < ts.cnt);
-				float wj = 0.f;
-
-#pragma unroll
-				for (int q = 0; q < TP; q++)
-					if (q == j)
-						wj = ts.w[q];
+				const bool ok = (j >= 0) && (j < cnt);
+				const float wj = ok ? ts.w[min(max(j, 0), TP - 1)] : 0.f;
 
 				w = (2 == dd) ? (1.f * wj) : (w * wj);
 				hit = hit && ok;
@@ -757,43 +769,47 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 		if (c >= ncc)
 			continue;
 
-		float2* dst = grid + ((long)s * g.C + c0 + c) * g.N + idx;
-
 		if (0 == wk.w) {
-			*dst = acc[c];
+			grid[((long)s * g.C + c0 + c) * g.N + idx] = acc[c];
 		} else {
-			// tile split over several work-groups (dense region): add into
-			// the pre-zeroed tile
-			if (0.f != acc[c].x)
-				unsafeAtomicAdd(&dst->x, acc[c].x);
-			if (0.f != acc[c].y)
-				unsafeAtomicAdd(&dst->y, acc[c].y);
+			// tile split over several work-groups (dense region): the
+			// partial tile goes to its own slot, summed by k_sum_split
+			spart[(((long)(wk.w - 1) * g.S + s) * g.C + c0 + c) * t.Wtot + pnt] = acc[c];
 		}
 	}
 }
 
-// zero the split tiles (all sub-grids and coils) before their partial sums
-__global__ void k_zero_tiles(GridGeom g, GridTiling t, const int* __restrict__ tiles, int ntl, float2* grid)
+// tiles split over several work items: sum of their partial tiles in slot
+// order (deterministic, no atomics); thread per (split tile, s, c, point)
+__global__ void k_sum_split(GridGeom g, GridTiling t, const int4* __restrict__ tiles, int ntl,
+			    const float2* __restrict__ spart, float2* __restrict__ grid)
 {
 	const int per = t.Wtot;
-	const long tot = (long)ntl * g.S * g.C * per;
+	const long SC = (long)g.S * g.C;
+	const long tot = (long)ntl * SC * per;
 
 	for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < tot; j += (long)gridDim.x * blockDim.x) {
 
-		long q = j / per;
-		int pnt = (int)(j - q * per);
-		int sc = (int)(q % ((long)g.S * g.C));
-		int tl = (int)(q / ((long)g.S * g.C));
-		int tile = tiles[tl];
-		int tb0 = tile % t.nt[0], tb1 = (tile / t.nt[0]) % t.nt[1], tb2 = tile / (t.nt[0] * t.nt[1]);
-		int gx = tb0 * t.T[0] + pnt % t.W[0];
-		int gy = tb1 * t.T[1] + (pnt / t.W[0]) % t.W[1];
-		int gz = tb2 * t.T[2] + pnt / (t.W[0] * t.W[1]);
+		const long q = j / per;
+		const int pnt = (int)(j - q * per);
+		const long sc = q % SC;
+		const int tl = (int)(q / SC);
+		const int4 st = tiles[tl];
+		const int tile = st.x;
+		const int tb0 = tile % t.nt[0], tb1 = (tile / t.nt[0]) % t.nt[1], tb2 = tile / (t.nt[0] * t.nt[1]);
+		const int gx = tb0 * t.T[0] + pnt % t.W[0];
+		const int gy = tb1 * t.T[1] + (pnt / t.W[0]) % t.W[1];
+		const int gz = tb2 * t.T[2] + pnt / (t.W[0] * t.W[1]);
 
 		if (gx >= g.n[0] || gy >= g.n[1] || gz >= g.n[2])
 			continue;
 
-		grid[(long)sc * g.N + gx + (long)g.n[0] * (gy + (long)g.n[1] * gz)] = make_float2(0.f, 0.f);
+		float2 a = make_float2(0.f, 0.f);
+
+		for (int k = 0; k < st.z; k++)
+			a = cadd(a, spart[((long)(st.y + k) * SC + sc) * per + pnt]);
+
+		grid[sc * g.N + gx + (long)g.n[0] * (gy + (long)g.n[1] * gz)] = a;
 	}
 }
 
@@ -821,13 +837,6 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 
 	ProfScope prof("grid_adjoint", st);
 
-	if (gp.nsplit > 0) {
-		long tot = (long)gp.nsplit * g.S * g.C * t.Wtot;
-		unsigned zb = (unsigned)std::min<long>((tot + 255) / 256, 8192);
-		hipLaunchKernelGGL(k_zero_tiles, dim3(zb), dim3(256), 0, st, g, t, gp.split_tiles.ptr, gp.nsplit, grid);
-		BAM_HIP(hipGetLastError());
-	}
-
 	const int tp = taps_bound(g);
 	const int nthr = ((t.Wtot * g.S + 63) / 64) * 64;
 	BAM_CHECK(nthr <= 1024, "gridding: tile too large (%d outputs)", t.Wtot * g.S);
@@ -837,7 +846,7 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 	do {                                                                                                     \
 		size_t lds = SMEM_DATA + sizeof(TapSet<TP>) * 6 * ADJ_BATCH + sizeof(float2) * CC * ADJ_BATCH;  \
 		hipLaunchKernelGGL((k_grid_adj<TP, CC>), blocks, dim3(nthr), lds, st, g, t, kb, gp.sk.ptr,       \
-				   gp.cstart.ptr, gp.work.ptr, gp.kspS.ptr, grid);                               \
+				   gp.cstart.ptr, gp.work.ptr, gp.kspS.ptr, grid, gp.spart.ptr);                 \
 	} while (0)
 
 	if (4 == tp) {
@@ -852,21 +861,31 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 #undef BAM_ADJ
 
 	BAM_HIP(hipGetLastError());
+
+	if (gp.nsplit > 0) {
+		ProfScope prof2("grid_adjoint_sum_split", st);
+		long tot = (long)gp.nsplit * g.S * g.C * t.Wtot;
+		unsigned zb = (unsigned)std::min<long>((tot + 255) / 256, 16384);
+		hipLaunchKernelGGL(k_sum_split, dim3(zb), dim3(256), 0, st, g, t, gp.split_tiles.ptr, gp.nsplit,
+				   gp.spart.ptr, grid);
+		BAM_HIP(hipGetLastError());
+	}
 }
 
 // ---------------------------------------------------------------------------
-// Forward gridding (grid -> k-space): one thread per sorted sample and coil
-// chunk gathers the taps of all sub-grids from the grid (neighbouring lanes
-// read neighbouring grid lines); sub-grids are summed in order s = 0 .. S-1
-// (noncart/nufft.c:1444-1445) and the result is multiplied by the weights
-// (:1456-1457).  Launch (ceil(M / 256), ceil(C / CC)).
+// Forward gridding (grid -> k-space): one thread per sample and coil chunk
+// gathers the taps of all sub-grids from the grid; sub-grids are summed in
+// order s = 0 .. S-1 (noncart/nufft.c:1444-1445) and the result is
+// multiplied by the weights (:1456-1457).  Samples are taken in their
+// original order: along a readout neighbouring samples share most taps (the
+// grid reads hit the caches) and the k-space stores are coalesced.
+// Launch (ceil(M / 256), ceil(C / CC)).
 // ---------------------------------------------------------------------------
 
 template <int TP, int CC>
 __global__ __launch_bounds__(256) void k_grid_fwd(GridGeom g, const float* __restrict__ kbt,
-						  const int* __restrict__ sm, const float4* __restrict__ sk,
-						  const float2* __restrict__ grid, const float2* __restrict__ weights,
-						  float2* __restrict__ ksp)
+						  const float4* __restrict__ ko, const float2* __restrict__ grid,
+						  const float2* __restrict__ weights, float2* __restrict__ ksp)
 {
 	__shared__ float table[KB_SIZE + 1];
 
@@ -882,7 +901,7 @@ __global__ __launch_bounds__(256) void k_grid_fwd(GridGeom g, const float* __res
 
 	const int c0 = blockIdx.y * CC;
 	const int ncc = min(CC, g.C - c0);
-	const float4 kc = sk[i];
+	const float4 kc = ko[i];
 	const float kk[3] = { kc.x, kc.y, kc.z };
 
 	Taps1<TP> tv[3][2];
@@ -964,7 +983,7 @@ __global__ __launch_bounds__(256) void k_grid_fwd(GridGeom g, const float* __res
 			acc[cl] = cadd(acc[cl], val[cl]);
 	}
 
-	const int m = sm[i];
+	const long m = i;
 	const float2 w = (nullptr != weights) ? weights[m] : make_float2(1.f, 0.f);
 
 #pragma unroll
@@ -986,8 +1005,8 @@ void grid_forward(const GridPlan& gp, const float* kb, float2* ksp, const float2
 	const int cc = (g.C <= 4) ? 4 : ((g.C <= 8) ? 8 : 16);
 	dim3 blocks(ceil_div(g.M, 256), ceil_div(g.C, cc));
 
-#define BAM_FWD(TP, CC) hipLaunchKernelGGL((k_grid_fwd<TP, CC>), blocks, dim3(256), 0, st, g, kb, gp.sm.ptr, \
-					    gp.sk.ptr, grid, weights, ksp)
+#define BAM_FWD(TP, CC) hipLaunchKernelGGL((k_grid_fwd<TP, CC>), blocks, dim3(256), 0, st, g, kb, gp.ko.ptr, \
+					    grid, weights, ksp)
 
 	if (4 == tp) {
 		if (4 == cc) BAM_FWD(4, 4);

@@ -4,6 +4,7 @@
 #include "../../include/bart_amd.h"
 
 #include "cg.hpp"
+#include "nudft.hpp"
 #include "nufft_impl.hpp"
 
 #include <atomic>
@@ -88,7 +89,7 @@ static bool on_device(const void* p)
 // operator handle
 // ---------------------------------------------------------------------------
 
-enum OpKind { OP_NUFFT = 1, OP_SENSE = 2 };
+enum OpKind { OP_NUFFT = 1, OP_SENSE = 2, OP_NUDFT = 3 };
 
 // stream and lock of one NUFFT plan, shared by every handle that applies it
 struct Queue {
@@ -109,6 +110,7 @@ struct Op {
 	long osize = 0, isize = 0;
 	std::shared_ptr<Queue> q;
 	std::shared_ptr<Nufft> nufft;
+	std::shared_ptr<Nudft> nudft;
 	std::shared_ptr<DevBuf<float2>> sens;
 	hipStream_t stream = 0;
 	std::mutex& mtx() { return q->mtx; }
@@ -119,7 +121,53 @@ struct Op {
 	{
 		cg.reset();
 		nufft.reset();
+		nudft.reset();
 		q.reset();
+	}
+
+	const float2* maps() const { return (OP_SENSE == kind) ? sens->ptr : nullptr; }
+
+	// device pointers, on this operator's stream
+	void fwd(float2* d, const float2* s)
+	{
+		if (OP_NUDFT == kind)
+			nudft->forward(d, s);
+		else
+			nufft->forward(d, s, maps());
+	}
+
+	void adj(float2* d, const float2* s)
+	{
+		if (OP_NUDFT == kind)
+			nudft->adjoint(d, s);
+		else
+			nufft->adjoint(d, s, maps());
+	}
+
+	bool toeplitz() const { return (OP_NUDFT != kind) && nufft->toeplitz; }
+
+	// A^H A: Toeplitz when available, else adjoint(forward(x)) through kbuf
+	// (linops/linop.c:489-525); kbuf must be allocated (prepare_normal)
+	void nrm(float2* d, const float2* s, const int* skip)
+	{
+		if (toeplitz()) {
+			nufft->normal(d, s, maps(), skip);
+		} else {
+			fwd(kbuf.ptr, s);
+			adj(d, kbuf.ptr);
+		}
+	}
+
+	void prepare_normal()
+	{
+		if (!toeplitz() && (long)kbuf.n < osize)
+			kbuf.alloc(osize);
+	}
+
+	Nufft& plan(const char* what)
+	{
+		BAM_CHECK(OP_NUDFT != kind && nullptr != nufft, "%s: not a NUFFT operator", what);
+		return *nufft;
 	}
 };
 
@@ -365,6 +413,48 @@ struct linop_s* nufft_create(int N, const long ksp_dims[], const long cim_dims[]
 	return nufft_create2(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, nullptr, nullptr, conf);
 }
 
+const struct linop_s* nudft_create(int N, unsigned long flags, const long odims[], const long idims[],
+				   const long tdims[], const float _Complex* traj, const long fmdims[],
+				   const float _Complex* fieldmap, const long tmdims[], const float _Complex* timemap)
+{
+	(void)fmdims;
+	(void)tmdims;
+
+	return guarded([&]() -> const struct linop_s* {
+		BAM_CHECK(7 == flags, "nudft: only flags = 7 (dims 0-2) are supported");
+		BAM_CHECK(nullptr == fieldmap && nullptr == timemap, "nudft: the B0 term is not supported");
+		BAM_CHECK(nullptr != traj, "nudft: trajectory is NULL");
+
+		NufftDims nd = parse_nufft_dims(N, odims, idims, tdims);
+		std::vector<float2> th = to_host((const float2*)traj, 3 * nd.M);
+		std::vector<float> tr(3 * nd.M);
+
+		for (long i = 0; i < 3 * nd.M; i++)
+			tr[i] = th[i].x;
+
+		auto op = new Op();
+		op->kind = OP_NUDFT;
+		op->N = N;
+		memcpy(op->odims, odims, sizeof(long) * N);
+		memcpy(op->idims, idims, sizeof(long) * N);
+		op->osize = md_size(N, odims);
+		op->isize = md_size(N, idims);
+		op->q = std::make_shared<Queue>();
+		op->stream = op->q->stream;
+
+		try {
+			op->nudft = std::make_shared<Nudft>();
+			op->nudft->stream = op->stream;
+			op->nudft->create(nd.img, nd.C, nd.M, tr.data());
+		} catch (...) {
+			delete op;
+			throw;
+		}
+
+		return (const struct linop_s*)op;
+	}, nullptr);
+}
+
 const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims[16], const float _Complex* maps,
 				    const long ksp_dims[16], const long traj_dims[16], const float _Complex* traj,
 				    const struct nufft_conf_s* _conf, const long wgs_dims[16],
@@ -454,9 +544,8 @@ void linop_forward(const struct linop_s* _op, int DN, const long ddims[], float 
 		check_dims("linop_forward (output)", DN, ddims, op->N, op->odims);
 		check_dims("linop_forward (input)", SN, sdims, op->N, op->idims);
 
-		apply_staged(op, (float2*)dst, op->osize, (const float2*)src, op->isize, [&](float2* d, const float2* s) {
-			op->nufft->forward(d, s, (OP_SENSE == op->kind) ? op->sens->ptr : nullptr);
-		});
+		apply_staged(op, (float2*)dst, op->osize, (const float2*)src, op->isize,
+			     [&](float2* d, const float2* s) { op->fwd(d, s); });
 	});
 }
 
@@ -469,9 +558,8 @@ void linop_adjoint(const struct linop_s* _op, int DN, const long ddims[], float 
 		check_dims("linop_adjoint (output)", DN, ddims, op->N, op->idims);
 		check_dims("linop_adjoint (input)", SN, sdims, op->N, op->odims);
 
-		apply_staged(op, (float2*)dst, op->isize, (const float2*)src, op->osize, [&](float2* d, const float2* s) {
-			op->nufft->adjoint(d, s, (OP_SENSE == op->kind) ? op->sens->ptr : nullptr);
-		});
+		apply_staged(op, (float2*)dst, op->isize, (const float2*)src, op->osize,
+			     [&](float2* d, const float2* s) { op->adj(d, s); });
 	});
 }
 
@@ -482,19 +570,9 @@ void linop_normal(const struct linop_s* _op, int N, const long dims[], float _Co
 		BAM_CHECK(nullptr != op, "linop_normal: NULL operator");
 		check_dims("linop_normal", N, dims, op->N, op->idims);
 
-		apply_staged(op, (float2*)dst, op->isize, (const float2*)src, op->isize, [&](float2* d, const float2* s) {
-			const float2* sens = (OP_SENSE == op->kind) ? op->sens->ptr : nullptr;
-
-			if (op->nufft->toeplitz) {
-				op->nufft->normal(d, s, sens);
-			} else {
-				// no normal operator: adjoint(forward(x)) (linops/linop.c:489-525)
-				if ((long)op->kbuf.n < op->osize)
-					op->kbuf.alloc(op->osize);
-				op->nufft->forward(op->kbuf.ptr, s, sens);
-				op->nufft->adjoint(d, op->kbuf.ptr, sens);
-			}
-		});
+		op->prepare_normal();
+		apply_staged(op, (float2*)dst, op->isize, (const float2*)src, op->isize,
+			     [&](float2* d, const float2* s) { op->nrm(d, s, nullptr); });
 	});
 }
 
@@ -502,7 +580,7 @@ int nufft_get_psf_dims(const struct linop_s* _op, int N, long psf_dims[])
 {
 	return guarded([&]() -> int {
 		Op* op = (Op*)_op;
-		BAM_CHECK(nullptr != op && op->nufft->has_psf, "nufft_get_psf_dims: operator has no PSF");
+		BAM_CHECK(nullptr != op && op->plan("nufft_get_psf_dims").has_psf, "nufft_get_psf_dims: operator has no PSF");
 
 		if (N > 0) {
 			BAM_CHECK(N == op->N + 1, "nufft_get_psf_dims: need %d dims", op->N + 1);
@@ -534,7 +612,7 @@ void nufft_update_psf(const struct linop_s* _op, int ND, const long psf_dims[], 
 	guarded_void([&]() {
 		Op* op = (Op*)_op;
 		BAM_CHECK(nullptr != op, "nufft_update_psf: NULL operator");
-		BAM_CHECK(op->nufft->toeplitz, "nufft_update_psf: operator is not in Toeplitz mode");
+		BAM_CHECK(op->plan("nufft_update_psf").toeplitz, "nufft_update_psf: operator is not in Toeplitz mode");
 		long want[33];
 		BAM_CHECK(ND == op->N + 1, "nufft_update_psf: dims");
 		for (int i = 0; i < op->N; i++)
@@ -604,7 +682,7 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		Op* op = (Op*)_op;
 		BAM_CHECK(nullptr != op, "lsqr_conjgrad: NULL operator");
 		check_dims("lsqr_conjgrad", N, img_dims, op->N, op->idims);
-		BAM_CHECK(!op->nufft->toeplitz || op->nufft->has_psf, "lsqr_conjgrad: Toeplitz operator without PSF");
+		BAM_CHECK(!op->toeplitz() || op->nufft->has_psf, "lsqr_conjgrad: Toeplitz operator without PSF");
 
 		std::lock_guard<std::mutex> lock(op->mtx());
 
@@ -613,27 +691,10 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		if (!op->cg)
 			op->cg.reset(new CgSolver(2 * ne, op->stream));
 
-		const float2* sens = (OP_SENSE == op->kind) ? op->sens->ptr : nullptr;
-		Nufft* nf = op->nufft.get();
-		NormalFn normal;
-
-		if (nf->toeplitz) {
-
-			normal = [nf, sens](float2* d, const float2* s, const int* skip) { nf->normal(d, s, sens, skip); };
-
-		} else {
-
-			// explicit normal operator A^H A (linops/linop.c:489-525); the
-			// k-space buffer exists before the iterations are captured
-			if ((long)op->kbuf.n < op->osize)
-				op->kbuf.alloc(op->osize);
-
-			float2* kb = op->kbuf.ptr;
-			normal = [nf, sens, kb](float2* d, const float2* s, const int*) {
-				nf->forward(kb, s, sens);
-				nf->adjoint(d, kb, sens);
-			};
-		}
+		// the k-space buffer of an explicit normal operator exists before
+		// the iterations are captured
+		op->prepare_normal();
+		NormalFn normal = [op](float2* d, const float2* s, const int* skip) { op->nrm(d, s, skip); };
 
 		bool xd = on_device(x), bd = on_device(b);
 		float2* xdev = (float2*)x;

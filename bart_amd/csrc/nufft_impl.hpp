@@ -60,11 +60,14 @@ struct GridPlan {
 	GridGeom g;
 	DevBuf<int> sm;          // sorted sample -> original sample index
 	DevBuf<float4> sk;       // sorted coordinates (kx, ky, kz) + wrap code
+	DevBuf<float4> ko;       // coordinates in the original sample order (forward)
 	DevBuf<int> cstart;      // cell -> first sorted sample (ncells + 1)
 	GridTiling ta;           // adjoint tiling
 	DevBuf<float2> kspS;     // adjoint input permuted to sorted order, coil chunks interleaved
-	DevBuf<int4> work;       // adjoint work items (tile, first visit, end visit, split)
-	DevBuf<int> split_tiles; // tiles whose visits are spread over several work items
+	DevBuf<int4> work;       // adjoint work items (tile, first visit, end visit, partial slot + 1 or 0)
+	DevBuf<int4> split_tiles; // tiles whose visits are spread over several work items: (tile, first slot, slots, 0)
+	DevBuf<float2> spart;    // partial tiles of the split work items [slot][S][C][Wtot]
+	int nslot = 0;
 	int nwork = 0;
 	int nsplit = 0;
 

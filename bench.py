@@ -39,27 +39,32 @@ def workload(name):
         traj = I.phyllotaxis_kooshball(spokes=spokes, readout=R)
         img = I.ellipsoid_phantom(n, nell=16, seed=2)
         maps = I.gaussian_coils((n, n, n), C, seed=2)
+        # groups: coil groups of the SENSE normal operator's S^H sum
         return dict(name="3d_radial_kooshball_128^3_16coils_8192x256", img=(n, n, n), C=C, R=R, S=spokes,
-                    traj=traj, x=img, maps=maps, lam=0.01, iters=20)
+                    traj=traj, x=img, maps=maps, lam=0.01, iters=20, groups=4)
     if name == "radial2d":
         n, C, spokes, R = 256, 8, 201, 512
         traj = I.radial_traj(R, spokes)
         img = I.ellipse_phantom(n, nell=10, seed=1)
         maps = I.gaussian_coils((1, n, n), C, seed=1)
         return dict(name="2d_golden_radial_256^2_8coils_201x512", img=(n, n, 1), C=C, R=R, S=spokes,
-                    traj=traj, x=img, maps=maps, lam=0.01, iters=30)
+                    traj=traj, x=img, maps=maps, lam=0.01, iters=30, groups=2)
     if name == "spiral":
         n, C = 320, 32
         traj = I.vd_spiral(interleaves=24, samples=4096, n=n)
         img = I.ellipse_phantom(n, nell=10, seed=3)
         maps = I.gaussian_coils((1, n, n), C, seed=3)
         return dict(name="2d_vd_spiral_320^2_32coils_24x4096", img=(n, n, 1), C=C, R=4096, S=24,
-                    traj=traj, x=img, maps=maps, lam=0.01, iters=30)
+                    traj=traj, x=img, maps=maps, lam=0.01, iters=30, groups=16)
     raise ValueError(name)
 
 
 def algorithmic_bytes(kernel, w):
-    """Compulsory HBM bytes of one launch of a kernel class (read once, write once)."""
+    """Compulsory HBM bytes of one launch of a kernel class: each input read
+    once, each output written once (complex64 = 8 B, coordinates 4 B/dim).
+    The NUFFT operator of the step works on coil images (no maps); the
+    Toeplitz classes belong to the SENSE normal operator (maps read, one
+    image in and out, coil-group partial images)."""
     X, Y, Z = w["img"]
     N = X * Y * Z
     C = w["C"]
@@ -67,25 +72,39 @@ def algorithmic_bytes(kernel, w):
     d = sum(1 for v in w["img"] if v > 1)
     S = 2 ** d
     grid = 8 * S * C * N
-    if kernel == "grid_adjoint":      # k-space + coordinates in, sub-grids out
-        return 8 * C * M + 4 * d * M + grid
-    if kernel == "grid_forward":      # sub-grids + coordinates in, k-space out
-        return grid + 4 * d * M + 8 * C * M
-    if kernel in ("fft_fwd_axis", "fft_adj_axis"):
-        return 2 * grid
-    if kernel == "fft_fwd_first":     # coil images in, sub-grids out
-        return 8 * C * N + grid
-    if kernel == "fft_adj_last":      # sub-grids in, coil images out
-        return grid + 8 * C * N
-    return None
+    G = w.get("groups", 1)
+    table = {
+        # k-space (x conj(weights)) -> sorted, coil-interleaved copy
+        "permute_ksp": 8 * C * M + 8 * C * M,
+        # sorted k-space + coordinates in, sub-grid stack out
+        "grid_adjoint": 8 * C * M + 4 * d * M + grid,
+        # sub-grid stack + coordinates in, k-space out
+        "grid_forward": grid + 4 * d * M + 8 * C * M,
+        # one axis of the sub-grid stack: read + write
+        "fft_fwd_axis": 2 * grid,
+        "fft_adj_axis": 2 * grid,
+        # coil images in, sub-grid stack out / back
+        "fft_fwd_first": 8 * C * N + grid,
+        "fft_adj_last": grid + 8 * C * N,
+        "fft_fwd_slice": 8 * C * N + grid,
+        "fft_adj_slice": grid + 8 * C * N,
+        # SENSE normal operator: image + maps in, stack out
+        "fft_toeplitz_slice_fwd": 8 * N + 8 * C * N + grid,
+        # z axis forward, x PSF, inverse: stack read + write, PSF read
+        "fft_toeplitz": 2 * grid + 8 * S * N,
+        # stack + maps in, coil-group partial images out
+        "fft_toeplitz_slice_inv": grid + 8 * C * N + 8 * G * N,
+        "sum_coil_groups": 8 * G * N + 8 * N,
+    }
+    return table.get(kernel)
 
 
 def cpu_baseline(w, threads):
-    """Oracle NUFFT adj+fwd on a bounded sample: 2 coils of the same
-    trajectory and image grid."""
+    """Oracle NUFFT adj+fwd on a bounded sample: 8 coils of the same
+    trajectory and image grid (a few seconds on 16 cores)."""
     import oracle as O
     O.lib().orc_set_num_threads(threads)
-    C = min(2, w["C"])
+    C = min(8, w["C"])
     S = w["S"]
     M = w["R"] * S
     traj = w["traj"][:M]

@@ -83,6 +83,15 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 				    const long basis_dims[16], const BAM_COMPLEX* basis,
 				    const struct linop_s** fft_opp, unsigned long shared_img_dims);
 
+/* noncart/nudft.h -- replaces nudft_create (noncart/nudft.c:283-348): the
+ * direct non-uniform DFT (nufft -s), ksp[c][m] = s sum_x img[c][x]
+ * exp(-2 pi i k_m (x - n/2) / n), s = 1 / sqrt(size of the coil-image
+ * array) as BART.  flags = 7; no B0 term (fieldmap and timemap NULL).
+ * Its normal operator is adjoint(forward(x)). */
+const struct linop_s* nudft_create(int N, unsigned long flags, const long odims[], const long idims[],
+				   const long tdims[], const BAM_COMPLEX* traj, const long fmdims[],
+				   const BAM_COMPLEX* fieldmap, const long tmdims[], const BAM_COMPLEX* timemap);
+
 /* linops/linop.h:66-72 -- replace linop_forward/adjoint/normal
  * (linops/linop.c:406-525).  dst is overwritten. */
 void linop_forward(const struct linop_s* op, int DN, const long ddims[], BAM_COMPLEX* dst, int SN,

@@ -72,7 +72,7 @@ def test_header_with_extra_sections(tmp_path):
 
 
 @pytest.mark.parametrize("args,msg", [
-    (["nufft", "-s", "t", "i", "o"], "-s"),
+    (["nufft", "-s", "-p", "p", "t", "i", "o"], "ERROR"),
     (["nufft", "-a", "-i", "t", "i", "o"], "Adjoint and inverse"),
     (["nufft", "-B", "b", "t", "i", "o"], "-B"),
     (["pics", "-l1", "-t", "t", "k", "s", "o"], "-l1"),

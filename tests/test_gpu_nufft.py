@@ -156,6 +156,25 @@ def test_nufft_vs_fp64_nudft():
     assert relerr(a, refa) < 2e-5
 
 
+@pytest.mark.parametrize("img,C", [((32, 32, 1), 1), ((32, 32, 1), 2), ((12, 10, 8), 2)])
+def test_nudft_matches_fp64(img, C):
+    # nufft -s (noncart/nudft.c): BART scales by 1/sqrt(size of the coil-image
+    # array), i.e. 1/sqrt(N C); the FP64 reference scales by 1/sqrt(N)
+    X, Y, Z = img
+    R, S = 2 * X, 9
+    traj = I.radial_traj(R, S) if Z == 1 else I.phyllotaxis_kooshball(spokes=S, readout=R)
+    op = ba.nudft_create(dims16(1, R, S, C), dims16(X, Y, Z, C), dims16(3, R, S), ba.traj_array(traj))
+    x = crandn((C, Z, Y, X), 90)
+    k = op.forward(x).reshape(C, -1)
+    assert relerr(k, O.nudft_forward(img, traj, x) / np.sqrt(C)) < 1e-5
+    y = crandn((C, R * S), 91)
+    a = op.adjoint(y).reshape(C, Z, Y, X)
+    assert relerr(a, O.nudft_adjoint(img, traj, y) / np.sqrt(C)) < 1e-5
+    # normal operator = adjoint(forward(x))
+    nrm = op.normal(x).reshape(C, Z, Y, X)
+    assert relerr(nrm, op.adjoint(op.forward(x)).reshape(C, Z, Y, X)) < 1e-6
+
+
 def test_nufft_adjoint_dot_test():
     # utests/test_nufft.c:108-121 / tests/nufft.mk:90-104
     n, C, R, S = 128, 2, 256, 64
