@@ -5,6 +5,8 @@
 #include "volume.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 
@@ -87,7 +89,13 @@ FftLaunchCfg fft_config(const AxisPlan& plan, long total, bool contig, int P_min
 	FftLaunchCfg c;
 	const int n = plan.n;
 
-	if (ct_supported(n) && ct_block_p(n) >= P_min) {
+	// BART_AMD_FFT_FIXED=off: runtime-plan kernels for every length
+	static const bool fixed_on = [] {
+		const char* e = getenv("BART_AMD_FFT_FIXED");
+		return !(nullptr != e && 0 == strcmp(e, "off"));
+	}();
+
+	if (fixed_on && ct_supported(n) && ct_block_p(n) >= P_min) {
 
 		const int P = ct_block_p(n);
 		int logP = 0;
@@ -165,9 +173,79 @@ struct StPlain {
 	}
 };
 
+// x-y slices of square size NX transformed in LDS (rows, then columns),
+// work-group per slice; lanes keep their x (NTH a multiple of NX).  src may
+// equal dst: a slice is read completely before it is written.
+template <int NX>
+__global__ __launch_bounds__((1024 / NX) * NX) void k_slice_plain(const float2* src, float2* dst, float sign,
+								  float scale, const float2* __restrict__ tw_g)
+{
+	constexpr int NTH = (1024 / NX) * NX;
+	constexpr int DY = NTH / NX;
+	constexpr int LD = NX + 1;
+	constexpr int NP = NX * NX;
+
+	extern __shared__ float2 lds[];
+	float2* tw = lds + NX * LD;
+
+	const long off = (long)blockIdx.x * NP;
+	const int x = threadIdx.x % NX;
+	const int y0 = threadIdx.x / NX;
+
+	for (int i = threadIdx.x; i < NX; i += NTH)
+		tw[i] = tw_g[i];
+
+	for (int y = y0; y < NX; y += DY)
+		lds[y * LD + x] = src[off + y * NX + x];
+
+	__syncthreads();
+
+	slice_passes<NX, NX, 1, LD, NTH>(lds, tw, sign);
+	slice_passes<NX, NX, LD, 1, NTH>(lds, tw, sign);
+
+	for (int y = y0; y < NX; y += DY)
+		dst[off + y * NX + x] = cscale(lds[y * LD + x], scale);
+}
+
+static bool slice_plain_size(int n)
+{
+	return 32 == n || 64 == n || 96 == n || 128 == n;
+}
+
 void fft_nd(const int dims[3], long batch, unsigned flags, float sign, float2* dst, const float2* src,
 	    float scale, hipStream_t stream)
 {
+	// square x-y slices of a 3-d array: both in-plane axes in one pass
+	if (7 == (flags & 7) && dims[0] == dims[1] && dims[2] > 1 && slice_plain_size(dims[0])) {
+
+		const int n = dims[0];
+		AxisPlan plan = make_axis_plan(n);
+		const long planes = (long)dims[2] * batch;
+		const size_t lds = sizeof(float2) * ((size_t)n * (n + 1) + n);
+
+		{
+			ProfScope prof("fft_slice", stream);
+			const unsigned nth = (unsigned)((1024 / n) * n);
+
+			switch (n) {
+			case 32: hipLaunchKernelGGL(k_slice_plain<32>, dim3(planes), dim3(nth), lds, stream, src, dst, sign, 1.f, plan.tw); break;
+			case 64: hipLaunchKernelGGL(k_slice_plain<64>, dim3(planes), dim3(nth), lds, stream, src, dst, sign, 1.f, plan.tw); break;
+			case 96: hipLaunchKernelGGL(k_slice_plain<96>, dim3(planes), dim3(nth), lds, stream, src, dst, sign, 1.f, plan.tw); break;
+			default: hipLaunchKernelGGL(k_slice_plain<128>, dim3(planes), dim3(nth), lds, stream, src, dst, sign, 1.f, plan.tw); break;
+			}
+
+			BAM_HIP(hipGetLastError());
+		}
+
+		VolAxis g = VolAxis::make(dims, 2, batch);
+		AxisPlan pz = make_axis_plan(dims[2]);
+		FftLaunchCfg cfg = fft_config(pz, g.transforms(), g.contig(), 1, sign, sign);
+		LdPlain ld{ g, dst };
+		StPlain st{ g, dst, scale };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+		return;
+	}
+
 	bool first = true;
 	int last = -1;
 

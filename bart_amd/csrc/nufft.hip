@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
+#include <cstring>
 
 namespace bam {
 
@@ -912,7 +914,10 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	for (int j = 0; j < nact; j++)
 		axis[j] = make_axis_plan(n[act[j]]);
 
-	slice = (3 == nact) && slice_supported(n);
+	// BART_AMD_SLICE=off selects the per-axis passes also where the x-y
+	// slices fit in LDS
+	const char* sl = getenv("BART_AMD_SLICE");
+	slice = (3 == nact) && slice_supported(n) && !(nullptr != sl && 0 == strcmp(sl, "off"));
 
 	has_weights = (nullptr != weights_host);
 
@@ -935,6 +940,9 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	g.os = 1.f;
 	g.width = width / 2.f;
 	g.periodic = periodic;
+
+	for (int d = 0; d < 3; d++)
+		g.fm[d] = tabs.fm[d];
 
 	for (int s = 0; s < S; s++) {
 
@@ -1025,7 +1033,7 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 
 	if (slice) {
 
-		z_pass(*this, +1.f, true, false, "fft_adj_axis");
+		z_pass(*this, +1.f, false, false, "fft_adj_axis");
 
 		const int cc = (nullptr != sens) ? slice_coil_group(n[2], C) : 1;
 		const int G = C / cc;
@@ -1044,7 +1052,7 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
 		FftLaunchCfg cfg = fft_config(axis[j], va.transforms(), va.contig(), 1, +1.f, +1.f);
 		cfg.name = "fft_adj_axis";
-		LdGrid ld{ va, tabs, grid.ptr, j == nact - 1 };
+		LdGrid ld{ va, tabs, grid.ptr, false };
 		StGrid st{ va, tabs, grid.ptr, false };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 	}
@@ -1054,7 +1062,7 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 	GeomS gs = GeomS::make(n, act[0], S, C, cc);
 	FftLaunchCfg cfg = fft_config(axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
 	cfg.name = "fft_adj_last";
-	LdGridS ld{ gs, tabs, grid.ptr, 1 == nact };
+	LdGridS ld{ gs, tabs, grid.ptr, false };
 	float2* out = dst;
 
 	if (nullptr != sens && G > 1) {

@@ -38,6 +38,11 @@ struct GridGeom {
 	// their shift-0 position
 	int CS[3];
 	int ncell[3];
+
+	// per-dim fftmod tables: the adjoint gridding stores grid x conj(fm)
+	// (the first step of the adjoint chain, noncart/nufft.c:1462-1540);
+	// null: no factor
+	const float2* fm[3];
 };
 
 // Tiling of the adjoint gridding: a work-group owns a tile of T points per
@@ -60,7 +65,6 @@ struct GridPlan {
 	GridGeom g;
 	DevBuf<int> sm;          // sorted sample -> original sample index
 	DevBuf<float4> sk;       // sorted coordinates (kx, ky, kz) + wrap code
-	DevBuf<float4> ko;       // coordinates in the original sample order (forward)
 	DevBuf<int> cstart;      // cell -> first sorted sample (ncells + 1)
 	GridTiling ta;           // adjoint tiling
 	DevBuf<float2> kspS;     // adjoint input permuted to sorted order, coil chunks interleaved
@@ -68,6 +72,7 @@ struct GridPlan {
 	DevBuf<int4> split_tiles; // tiles whose visits are spread over several work items: (tile, first slot, slots, 0)
 	DevBuf<float2> spart;    // partial tiles of the split work items [slot][S][C][Wtot]
 	int nslot = 0;
+
 	int nwork = 0;
 	int nsplit = 0;
 

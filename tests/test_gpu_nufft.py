@@ -44,14 +44,19 @@ def test_fft_1d_matches_numpy(n):
 
 @pytest.mark.parametrize("shape,flags", [((16, 24, 1), 3), ((64, 64, 1), 3), ((256, 256, 1), 3),
                                          ((12, 10, 8), 7), ((32, 32, 32), 7), ((40, 1, 20), 5),
-                                         ((16, 24, 8), 2), ((16, 24, 8), 4)])
+                                         ((16, 24, 8), 2), ((16, 24, 8), 4),
+                                         ((64, 64, 8), 7), ((96, 96, 4), 7), ((128, 128, 6), 7)])
 def test_fft_nd_matches_numpy(shape, flags):
+    # square x-y slices of 3-d arrays go through the LDS slice passes
     X, Y, Z = shape
     x = crandn((3, Z, Y, X), X + Y + Z)
     y = ba.fft([X, Y, Z, 3], flags, x).reshape(3, Z, Y, X)
     axes = tuple(3 - a for a in range(3) if flags & (1 << a))
     ref = np.fft.fftn(x.astype(np.complex128), axes=axes)
     assert relerr(y, ref) < 3e-6
+    z = ba.ifft([X, Y, Z, 3], flags, x).reshape(3, Z, Y, X)
+    refi = np.fft.ifftn(x.astype(np.complex128), axes=axes) * np.prod([shape[a] for a in range(3) if flags & (1 << a)])
+    assert relerr(z, refi) < 3e-6
 
 
 # --------------------------------------------------------------------------
@@ -105,7 +110,8 @@ def test_nufft_3d_forward_adjoint():
 
 @pytest.mark.parametrize("n,C,S", [(32, 3, 200), (64, 1, 300)])
 def test_nufft_3d_slice_path(n, C, S):
-    # x-y slices transformed in LDS (square slices up to 128 x 128)
+    # x-y slices transformed in LDS (square slices up to 128 x 128); the dense
+    # centre tiles are split over several work-groups in the tile kernel
     R = 2 * n
     traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
     cim = crandn((C, n, n, n), 60 + n)
@@ -114,6 +120,35 @@ def test_nufft_3d_slice_path(n, C, S):
     op = make_nufft((n, n, n), C, traj, R, S, toeplitz=False)
     assert relerr(op.forward(cim).reshape(C, -1), ref.forward(cim)) < 1e-5
     assert relerr(op.adjoint(ksp).reshape(C, n, n, n), ref.adjoint(ksp)) < 1e-5
+
+
+def test_nufft_3d_slice_vs_axis_passes(monkeypatch):
+    # the same 32^3 operator with the slice kernels and with per-axis passes
+    n, C, R, S = 32, 2, 64, 60
+    traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
+    cim = crandn((C, n, n, n), 66)
+    ksp = crandn((C, R * S), 67)
+    a = make_nufft((n, n, n), C, traj, R, S, toeplitz=False)
+    monkeypatch.setenv("BART_AMD_SLICE", "off")
+    b = make_nufft((n, n, n), C, traj, R, S, toeplitz=False)
+    ref = O.Nufft((n, n, n), traj, C, toeplitz=False)
+    fa, fb = a.forward(cim), b.forward(cim)
+    aa, ab = a.adjoint(ksp), b.adjoint(ksp)
+    print(f"forward slice vs axis {relerr(fa, fb):.2e}, adjoint slice vs axis {relerr(aa, ab):.2e}, "
+          f"axis vs oracle {relerr(ab.reshape(C, -1), ref.adjoint(ksp).reshape(C, -1)):.2e}")
+    assert relerr(fa, fb) < 1e-5
+    assert relerr(ab.reshape(C, -1), ref.adjoint(ksp).reshape(C, -1)) < 1e-5
+    assert relerr(aa, ab) < 1e-5
+
+
+def test_nufft_2d_dense_adjoint():
+    # 2-D radial with a centre dense enough to split tiles (> 2048 visits)
+    n, C, R, S = 64, 2, 128, 201
+    traj = I.radial_traj(R, S)
+    ksp = crandn((C, R * S), 64)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=False)
+    op = make_nufft((n, n, 1), C, traj, R, S, toeplitz=False)
+    assert relerr(op.adjoint(ksp).reshape(C, 1, n, n), ref.adjoint(ksp)) < 1e-5
 
 
 def test_nufft_nonsquare_and_1d():
