@@ -7,13 +7,13 @@
 //   * samples are sorted once per trajectory by the cell (2^d grid points)
 //     that holds their position; the plan lives with the operator;
 //   * adjoint (k-space -> grid) is output-stationary: a work-group owns a tile
-//     of the sub-grids for a group of coils, accumulates every tap that lands
-//     in the tile from the samples of the tile's cells and the ring of
-//     neighbouring cells into LDS, and stores the tile with plain coalesced
-//     writes.  There are no global atomics and no zero-fill of the grid;
-//   * forward (grid -> k-space): a work-group stages the tile plus halo of one
-//     sub-grid and a group of coils in LDS and gathers the taps of the tile's
-//     own samples from there.
+//     of all sub-grids for a group of coils, each wave a block of every
+//     sub-grid (in 3-d 2 x 2 x 2 points = a 4 x 4 x 4 block of the 2x grid),
+//     each lane one output in registers; a wave walks only the samples that
+//     reach its block, with the coil values as wave-uniform loads.  There
+//     are no atomics and no zero-fill of the grid;
+//   * forward (grid -> k-space): a thread per sample gathers its taps in
+//     cell-sorted order and stores k-space at the original positions.
 #include "nufft_impl.hpp"
 #include "prof.hpp"
 
@@ -87,6 +87,9 @@ __device__ inline float2 apply_conj_fm(const GridGeom& g, float2 v, int x, int y
 
 constexpr int MAX_RANGES = 256;
 
+// visits (samples of a tile's cells and their ring) per adjoint work item
+constexpr int ADJ_MAXV = 2048;
+
 // sample position on the grid, pos = os (k + shift) + n / 2, rounded after
 // each operation as in noncart/grid.c:181-187
 #pragma clang fp contract(off)
@@ -98,25 +101,11 @@ static float host_pos(const GridGeom& g, int s, int d, float k)
 }
 #pragma clang fp contract(on)
 
-static void finish_tiling(const GridGeom& g, GridTiling& t)
-{
-	t.ntiles = 1;
-	t.Wtot = 1;
-
-	for (int d = 0; d < 3; d++) {
-		t.nt[d] = (g.n[d] + t.T[d] - 1) / t.T[d];
-		t.ntiles *= t.nt[d];
-		t.lo[d] = 0;
-		t.W[d] = t.T[d];
-		t.Wtot *= t.W[d];
-	}
-
-	t.lds = (size_t)(t.Wtot * t.sg + 2) * sizeof(float2) * t.cc + 6144;
-}
-
-// Adjoint tiling: a work-group has one lane per (tile point, sub-grid), at
-// most 1024, and accumulates up to 16 coils per lane in registers; 3-d tiles
-// are 8 x 4 x 4, 2-d 16 x 8, 1-d 128.
+// Adjoint tiling.  A wave owns a block of every sub-grid: 2 x 2 x 2 points
+// in 3-d (a 4 x 4 x 4 block of the 2x oversampled grid), 4 x 4 in 2-d, 32 in
+// 1-d, one lane per (sub-grid, point), up to 16 coils per lane in registers.
+// Tiles are 8 x 4 x 4 (16 waves), 16 x 16 (16 waves), 32 (1 wave); a tile
+// dim is cut to the grid rounded up to the block.
 static GridTiling choose_tiling(const GridGeom& g)
 {
 	int nd = 0;
@@ -124,35 +113,42 @@ static GridTiling choose_tiling(const GridGeom& g)
 		nd += (g.n[d] > 1);
 
 	const int Tdef3[3] = { 8, 4, 4 };
-	const int Tdef2[3] = { 16, 8, 1 };
+	const int Tdef2[3] = { 16, 16, 1 };
+	const int bdim = (3 == nd) ? 2 : ((2 == nd) ? 4 : 32);
 
 	GridTiling t = {};
 	int k = 0;
+	int e = 0;
+	t.ntiles = 1;
+	t.Wtot = 1;
+	t.nw = 1;
 
 	for (int d = 0; d < 3; d++) {
 
+		t.eoff[d] = -1;
+
 		if (1 == g.n[d]) {
 			t.T[d] = 1;
-			continue;
+			t.b[d] = 1;
+		} else {
+			int want = (3 == nd) ? Tdef3[k] : ((2 == nd) ? Tdef2[k] : 32);
+			k++;
+			t.b[d] = bdim;
+			t.T[d] = std::min(want, ((g.n[d] + bdim - 1) / bdim) * bdim);
+			t.eoff[d] = e;
+			e += 2 * t.T[d];
 		}
 
-		int want = (3 == nd) ? Tdef3[k] : ((2 == nd) ? Tdef2[k] : 128);
-		k++;
-		t.T[d] = std::min(want, g.n[d]);
+		t.nt[d] = (g.n[d] + t.T[d] - 1) / t.T[d];
+		t.ntiles *= t.nt[d];
+		t.W[d] = t.T[d];
+		t.Wtot *= t.T[d];
+		t.nb[d] = t.T[d] / t.b[d];
+		t.nw *= t.nb[d];
 	}
 
-	t.sg = g.S;
+	BAM_CHECK(e <= 64 && t.nw <= 16 && g.S * t.b[0] * t.b[1] * t.b[2] <= 64, "gridding: bad tiling");
 	t.cc = (g.C <= 4) ? 4 : ((g.C <= 8) ? 8 : 16);
-	finish_tiling(g, t);
-
-	while (t.Wtot * g.S > 1024) {
-		int dm = 0;
-		for (int d = 1; d < 3; d++)
-			if (t.T[d] > t.T[dm])
-				dm = d;
-		t.T[dm] = std::max(1, t.T[dm] / 2);
-		finish_tiling(g, t);
-	}
 
 	// cell rows visited by one work-group must fit the range list
 	long rows = 2;
@@ -316,7 +312,7 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 
 	// adjoint work list: the visits of a tile (its cells and their ring) in
 	// chunks of at most MAXV; tiles with more than one chunk are split
-	const long MAXV = 2048;
+	const long MAXV = ADJ_MAXV;
 	std::vector<int4> hw;
 	std::vector<int4> hsplit;
 	const GridTiling& t = ta;
@@ -463,42 +459,48 @@ __device__ inline void cell_span(const GridGeom& g, int d, int clo, int chi, int
 	}
 }
 
-// builds the ranges in LDS (thread 0); returns the total sample count
+// Builds the ranges in LDS, one per row of cells along x (and x span), in
+// the order z, y, x; empty rows are kept (their prefix entries repeat).
+// Returns the total sample count.
 __device__ inline int build_ranges(const GridGeom& g, const int* __restrict__ cstart, const int clo[3],
 				   const int chi[3], CellRanges& R)
 {
+	int xa[2], xn[2], ya[2], yn[2], za[2], zn[2];
+	cell_span(g, 0, clo[0], chi[0], xa[0], xn[0], xa[1], xn[1]);
+	cell_span(g, 1, clo[1], chi[1], ya[0], yn[0], ya[1], yn[1]);
+	cell_span(g, 2, clo[2], chi[2], za[0], zn[0], za[1], zn[1]);
+
+	const int nxs = (xn[0] > 0) + (xn[1] > 0);
+	const int ny = yn[0] + yn[1];
+	const int nrow = min(nxs * ny * (zn[0] + zn[1]), MAX_RANGES);  // the plan checks the bound
+
+	for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
+
+		const int xs = (xn[0] > 0) ? (r % nxs) : 1;
+		const int q = r / nxs;
+		const int yi = q % ny;
+		const int zi = q / ny;
+		const int cy = (yi < yn[0]) ? (ya[0] + yi) : (ya[1] + yi - yn[0]);
+		const int cz = (zi < zn[0]) ? (za[0] + zi) : (za[1] + zi - zn[0]);
+		const long row = (long)g.ncell[0] * (cy + (long)g.ncell[1] * cz) + xa[xs];
+		const int a = cstart[row];
+		R.start[r] = a;
+		R.pre[r + 1] = cstart[row + xn[xs]] - a;
+	}
+
+	__syncthreads();
+
 	if (0 == threadIdx.x) {
 
-		int xa[2], xn[2], ya[2], yn[2], za[2], zn[2];
-		cell_span(g, 0, clo[0], chi[0], xa[0], xn[0], xa[1], xn[1]);
-		cell_span(g, 1, clo[1], chi[1], ya[0], yn[0], ya[1], yn[1]);
-		cell_span(g, 2, clo[2], chi[2], za[0], zn[0], za[1], zn[1]);
-
-		int n = 0;
 		int tot = 0;
 		R.pre[0] = 0;
 
-		for (int zs = 0; zs < 2; zs++)
-			for (int cz = za[zs]; cz < za[zs] + zn[zs]; cz++)
-				for (int ys = 0; ys < 2; ys++)
-					for (int cy = ya[ys]; cy < ya[ys] + yn[ys]; cy++)
-						for (int xs = 0; xs < 2; xs++) {
+		for (int r = 0; r < nrow; r++) {
+			tot += R.pre[r + 1];
+			R.pre[r + 1] = tot;
+		}
 
-							if (0 == xn[xs] || n >= MAX_RANGES)
-								continue;
-
-							long row = (long)g.ncell[0] * (cy + (long)g.ncell[1] * cz);
-							int a = cstart[row + xa[xs]];
-							int b = cstart[row + xa[xs] + xn[xs]];
-
-							if (b > a) {
-								R.start[n] = a;
-								tot += b - a;
-								R.pre[++n] = tot;
-							}
-						}
-
-		R.n = n;
+		R.n = nrow;
 	}
 
 	__syncthreads();
@@ -594,30 +596,28 @@ __device__ inline void select_taps(const Taps1<TP>& a, const Taps1<TP>& b, bool 
 
 // ---------------------------------------------------------------------------
 // Adjoint gridding (k-space -> grid), output-stationary and atomic-free.
-// grid layout [S][C][N]; launch (nwork, ceil(C / cc)) with Wtot x S lanes.
+// grid layout [S][C][N]; launch (nwork, ceil(C / cc)) with nw waves.
 //
-// Every lane owns one point of the tile in one sub-grid and accumulates the
-// cc coils of that output in registers.  The visits of the tile (samples of
-// its cells and of the ring of neighbouring cells) are staged through LDS in
-// batches: per visit the 2 x 3 tap sets (3 dims, 2 shift variants) and the
-// coil values; each lane then adds v_c * wz * wy * wx for the taps that hit
-// its point.  Waves whose 64 points are missed by a sample skip it.  No
-// atomics except the final add of tiles whose visits are split over several
-// work-groups, and the summation order of a point is the visit order.
+// A wave owns a block of b points per dim in every sub-grid, one lane per
+// (sub-grid, point); the lane accumulates the cc coils of its output in
+// registers.  The visits of the tile (samples of its cells and of the ring
+// of neighbouring cells) are taken in batches of 64:
+//   1. one wave per visit evaluates, for every active dim, shift variant and
+//      tile row, the tap weight (0 off the taps) into LDS and, from the tap
+//      membership bits, which waves' blocks the sample reaches;
+//   2. every wave walks only the visits that reach its block (a scalar loop
+//      over a ballot mask); the coil values of the sample are wave-uniform
+//      loads, and each lane adds v_c * wz * wy * wx.
+// The summation order of a point is the visit order, as in the sequential
+// gridding of noncart/grid.c:240-250.  Tiles whose visits are split over
+// several work items store partial tiles, summed in order by k_sum_split.
 // ---------------------------------------------------------------------------
 
 constexpr int ADJ_BATCH = 64;
-
-// taps of one sample, dim and shift variant: sc = start * 256 + count
-// (start may be negative on periodic grids; count <= TP < 256)
-template <int TP>
-struct TapSet {
-	int sc;
-	float w[TP];
-};
+constexpr int ADJ_WROW = 65;  // weight table row: 64 entries and a constant 1 (inactive dims)
 
 template <int TP, int CC>
-__global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, const float* __restrict__ kbt,
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_grid_adj(GridGeom g, GridTiling t, const float* __restrict__ kbt,
 						   const float4* __restrict__ sk, const int* __restrict__ cstart,
 						   const int4* __restrict__ work, const float2* __restrict__ kspS,
 						   float2* __restrict__ grid, float2* __restrict__ spart)
@@ -626,16 +626,19 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 	char* smem = (char*)smem4;
 	float* table = (float*)smem;
 	CellRanges& R = *(CellRanges*)(smem + SMEM_TABLE);
-	TapSet<TP>* tbuf = (TapSet<TP>*)(smem + SMEM_DATA);                         // [BATCH][6]
-	float2* vbuf = (float2*)(smem + SMEM_DATA + sizeof(TapSet<TP>) * 6 * ADJ_BATCH);  // [BATCH][CC]
+	int* sidx = (int*)(smem + SMEM_DATA);               // [ADJ_MAXV] sorted sample of each visit
+	float* wtab = (float*)(sidx + ADJ_MAXV);            // [ADJ_BATCH][ADJ_WROW] tap weights per visit
+	unsigned* hitm = (unsigned*)(wtab + ADJ_WROW * ADJ_BATCH);  // [ADJ_BATCH] waves reached per visit
 
 	const int4 wk = work[blockIdx.x];  // (tile, first visit, end visit, partial slot + 1 or 0)
 	const int tile = wk.x;
 	const int ch = blockIdx.y;
-	const int cc = t.cc;
-	const int c0 = ch * cc;
-	const int ncc = min(cc, g.C - c0);
+	const int c0 = ch * CC;
+	const int ncc = min(CC, g.C - c0);
 	const int nthr = blockDim.x;
+	const int lane = threadIdx.x & 63;
+	const int wid = threadIdx.x >> 6;
+	const int nw = t.nw;
 
 	int tb[3], t0[3], clo[3], chi[3];
 	tb[0] = tile % t.nt[0];
@@ -656,107 +659,158 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 	for (int i = threadIdx.x; i < KB_SIZE + 1; i += nthr)
 		table[i] = kbt[i];
 
+	for (int v = threadIdx.x; v < ADJ_BATCH; v += nthr)
+		wtab[v * ADJ_WROW + 64] = 1.f;
+
 	const int Ftile = build_ranges(g, cstart, clo, chi, R);
 	const int fb = min(wk.y, Ftile);
 	const int fe = min(wk.z, Ftile);
 
-	// this lane's output: point pnt of sub-grid s
-	const int me = threadIdx.x;
-	const bool mine = me < t.Wtot * g.S;
-	const int s = mine ? me / t.Wtot : 0;
-	const int pnt = mine ? me - s * t.Wtot : 0;
-	const int lp[3] = { pnt % t.W[0], (pnt / t.W[0]) % t.W[1], pnt / (t.W[0] * t.W[1]) };
+	// sorted sample of every visit of this work item (fe - fb <= ADJ_MAXV)
+	for (int f = fb + threadIdx.x; f < fe; f += nthr) {
+		int lo = 0, hi = R.n;
+		while (hi - lo > 1) {
+			const int mid = (lo + hi) >> 1;
+			if (R.pre[mid] <= f)
+				lo = mid;
+			else
+				hi = mid;
+		}
+		sidx[f - fb] = R.start[lo] + (f - R.pre[lo]);
+	}
+
+	// this lane's output: point lp of the wave's block in sub-grid s
+	const int B = t.b[0] * t.b[1] * t.b[2];
+	const bool mine = lane < g.S * B;
+	const int s = mine ? lane / B : 0;
+	const int cb = lane - s * B;
+	const int wb[3] = { wid % t.nb[0], (wid / t.nb[0]) % t.nb[1], wid / (t.nb[0] * t.nb[1]) };
+	const int lp[3] = { wb[0] * t.b[0] + cb % t.b[0], wb[1] * t.b[1] + (cb / t.b[0]) % t.b[1],
+			    wb[2] * t.b[2] + cb / (t.b[0] * t.b[1]) };
 	const int gp[3] = { t0[0] + lp[0], t0[1] + lp[1], t0[2] + lp[2] };
 	const bool inside = mine && gp[0] < g.n[0] && gp[1] < g.n[1] && gp[2] < g.n[2];
-	const int vb[3] = { g.sbit[s][0], g.sbit[s][1], g.sbit[s][2] };
+	const int pnt = lp[0] + t.W[0] * (lp[1] + t.W[1] * lp[2]);
+
+	// the lane's weight entries per dim (inactive dim: the constant 1)
+	int wo[3];
+#pragma unroll
+	for (int d = 0; d < 3; d++)
+		wo[d] = (t.eoff[d] < 0) ? 64 : (t.eoff[d] + g.sbit[s][d] * t.T[d] + lp[d]);
+
+	// staging role of this lane: weight entry (dim ed, variant ev, tile row el)
+	int ed = -1, ev = 0, el = 0;
+#pragma unroll
+	for (int d = 0; d < 3; d++)
+		if (t.eoff[d] >= 0 && lane >= t.eoff[d] && lane < t.eoff[d] + 2 * t.T[d]) {
+			ed = d;
+			ev = (lane - t.eoff[d]) / t.T[d];
+			el = lane - t.eoff[d] - ev * t.T[d];
+		}
+	const int edd = max(ed, 0);
+	const int en = g.n[edd];
+	const float eshift = g.shv[edd][(ev < g.nshv[edd]) ? ev : 0];
+	const int egp = ((0 == edd) ? t0[0] : ((1 == edd) ? t0[1] : t0[2])) + el;
+
+	// reach test of wave `lane` (lanes < nw): block bits of every active dim
+	unsigned long long qmask[3];
+	int qT[3];
+	const int q = min(lane, nw - 1);
+#pragma unroll
+	for (int d = 0; d < 3; d++) {
+		const int qb = (0 == d) ? (q % t.nb[0]) : ((1 == d) ? ((q / t.nb[0]) % t.nb[1]) : (q / (t.nb[0] * t.nb[1])));
+		qT[d] = t.T[d];
+		qmask[d] = (t.eoff[d] < 0) ? 0ull : ((((1ull << t.b[d]) - 1ull) << (qb * t.b[d])) << t.eoff[d]);
+	}
 
 	float2 acc[CC];
 #pragma unroll
 	for (int c = 0; c < CC; c++)
 		acc[c] = make_float2(0.f, 0.f);
 
-	for (int base = fb; base < fe; base += ADJ_BATCH) {
+	const float2* kch = kspS + (long)ch * g.M * CC;
 
-		const int nb = min(ADJ_BATCH, fe - base);
+	for (int base = 0; base < fe - fb; base += ADJ_BATCH) {
+
+		const int nb = min(ADJ_BATCH, fe - fb - base);
 
 		__syncthreads();
 
-		// stage nb visits: thread (v, k), k < 16: tap set k (< 6), coil value k (< cc)
-		for (int j = threadIdx.x; j < 16 * ADJ_BATCH; j += nthr) {
+		// 1. weights and reached waves of the batch, one wave per visit
+		for (int v = wid; v < nb; v += nw) {
 
-			const int v = j >> 4;
-			const int k = j & 15;
+			const int i = sidx[base + v];
+			const float4 kc = sk[i];
+			bool mem = false;
 
-			if (v >= nb)
-				continue;
+			if (ed >= 0) {
 
-			const int f = base + v;
-			int rr = 0;
-			while (f >= R.pre[rr + 1])
-				rr++;
-			const int i = R.start[rr] + (f - R.pre[rr]);
+				const float k = (0 == ed) ? kc.x : ((1 == ed) ? kc.y : kc.z);
+				const float pos = __fadd_rn(__fmul_rn(g.os, __fadd_rn(k, eshift)), (float)(en / 2));
+				int sti = (int)ceil((double)pos - 0.5 * (double)g.width);
+				int eni = (int)floor((double)pos + 0.5 * (double)g.width);
 
-			if (k < 6) {
-				const float4 kc = sk[i];
-				const float kk[3] = { kc.x, kc.y, kc.z };
-				const int d = k >> 1;
-				const int var = k & 1;
-				Taps1<TP> tt;
-				taps_raw<TP>(g, d, (var < g.nshv[d]) ? var : 0, kk[d], table, tt);
-				TapSet<TP>& ts = tbuf[v * 6 + k];
-				ts.sc = tt.start * 256 + tt.cnt;
-#pragma unroll
-				for (int q = 0; q < TP; q++)
-					ts.w[q] = tt.w[q];
+				if (!g.periodic) {
+					sti = max(sti, 0);
+					eni = min(eni, en - 1);
+				}
+
+				const int cnt = min(max(eni - sti + 1, 0), TP);
+				int j;
+
+				if (g.periodic) {
+					j = (egp - sti) % en;
+					if (j < 0)
+						j += en;
+				} else {
+					j = egp - sti;
+				}
+
+				mem = (egp < en) && (j >= 0) && (j < cnt);
+				wtab[v * ADJ_WROW + lane] = mem ? kb_lookup(table, fabsf((float)(sti + j) - pos) / g.width) : 0.f;
 			}
 
-			if (k < CC)
-				vbuf[v * CC + k] = (k < ncc) ? kspS[((long)ch * g.M + i) * cc + k] : make_float2(0.f, 0.f);
+			const unsigned long long mb = __ballot(mem);
+			bool reach = lane < nw;
+
+#pragma unroll
+			for (int d = 0; d < 3; d++)
+				if (0ull != qmask[d])
+					reach = reach && (0ull != ((mb | (mb >> qT[d])) & qmask[d]));
+
+			const unsigned long long rw = __ballot(reach);
+
+			if (0 == lane)
+				hitm[v] = (unsigned)rw;
 		}
 
 		__syncthreads();
 
-		for (int v = 0; v < nb; v++) {
+		// 2. the visits that reach this wave's block
+		unsigned long long mh = __ballot(lane < nb && 0u != ((hitm[min(lane, nb - 1)] >> wid) & 1u));
+		const int myi = sidx[base + min(lane, nb - 1)];
 
-			float w = 1.f;
-			bool hit = inside;
+		while (0ull != mh) {
 
-			// z, y, x: the weight of the tap at this lane's point (BART order)
-#pragma unroll
-			for (int dd = 2; dd >= 0; dd--) {
+			const int v = __builtin_ctzll(mh);
+			mh &= mh - 1ull;
 
-				const TapSet<TP>& ts = tbuf[v * 6 + 2 * dd + vb[dd]];
-				const int sc = ts.sc;
-				const int start = sc >> 8;
-				const int cnt = sc & 255;
-				int j;
+			const int i = __builtin_amdgcn_readlane(myi, v);
+			const float* wv = wtab + v * ADJ_WROW;
 
-				if (g.periodic) {
-					j = (gp[dd] - start) % g.n[dd];
-					if (j < 0)
-						j += g.n[dd];
-				} else {
-					j = gp[dd] - start;
-				}
+			// z, y, x (BART order)
+			float w = wv[wo[2]];
+			w = w * wv[wo[1]];
+			w = w * wv[wo[0]];
 
-				const bool ok = (j >= 0) && (j < cnt);
-				const float wj = ok ? ts.w[min(max(j, 0), TP - 1)] : 0.f;
-
-				w = (2 == dd) ? (1.f * wj) : (w * wj);
-				hit = hit && ok;
-			}
-
-			if (!__any(hit))
-				continue;
-
-			if (!hit)
-				w = 0.f;
+			const float4* xp = (const float4*)(kch + (long)i * CC);
 
 #pragma unroll
-			for (int c = 0; c < CC; c++) {
-				const float2 x = vbuf[v * CC + c];
-				acc[c].x += x.x * w;
-				acc[c].y += x.y * w;
+			for (int c = 0; c < CC / 2; c++) {
+				const float4 x = xp[c];
+				acc[2 * c].x += x.x * w;
+				acc[2 * c].y += x.y * w;
+				acc[2 * c + 1].x += x.z * w;
+				acc[2 * c + 1].y += x.w * w;
 			}
 		}
 	}
@@ -775,7 +829,7 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 		if (0 == wk.w) {
 			grid[((long)s * g.C + c0 + c) * g.N + idx] = apply_conj_fm(g, acc[c], gp[0], gp[1], gp[2]);
 		} else {
-			// tile split over several work-groups (dense region): the
+			// tile split over several work items (dense region): the
 			// partial tile goes to its own slot, summed by k_sum_split
 			spart[(((long)(wk.w - 1) * g.S + s) * g.C + c0 + c) * t.Wtot + pnt] = acc[c];
 		}
@@ -841,16 +895,14 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 	ProfScope prof("grid_adjoint", st);
 
 	const int tp = taps_bound(g);
-	const int nthr = ((t.Wtot * g.S + 63) / 64) * 64;
-	BAM_CHECK(nthr <= 1024, "gridding: tile too large (%d outputs)", t.Wtot * g.S);
+	const int nthr = 64 * t.nw;
 	dim3 blocks(gp.nwork, (g.C + t.cc - 1) / t.cc);
+	const size_t lds = SMEM_DATA + sizeof(int) * ADJ_MAXV + sizeof(float) * ADJ_WROW * ADJ_BATCH +
+			   sizeof(unsigned) * ADJ_BATCH;
 
-#define BAM_ADJ(TP, CC)                                                                                          \
-	do {                                                                                                     \
-		size_t lds = SMEM_DATA + sizeof(TapSet<TP>) * 6 * ADJ_BATCH + sizeof(float2) * CC * ADJ_BATCH;  \
-		hipLaunchKernelGGL((k_grid_adj<TP, CC>), blocks, dim3(nthr), lds, st, g, t, kb, gp.sk.ptr,       \
-				   gp.cstart.ptr, gp.work.ptr, gp.kspS.ptr, grid, gp.spart.ptr);                 \
-	} while (0)
+#define BAM_ADJ(TP, CC)                                                                                    \
+	hipLaunchKernelGGL((k_grid_adj<TP, CC>), blocks, dim3(nthr), lds, st, g, t, kb, gp.sk.ptr, gp.cstart.ptr, \
+			   gp.work.ptr, gp.kspS.ptr, grid, gp.spart.ptr)
 
 	if (4 == tp) {
 		if (t.cc <= 4) BAM_ADJ(4, 4);

@@ -46,18 +46,22 @@ struct GridGeom {
 };
 
 // Tiling of the adjoint gridding: a work-group owns a tile of T points per
-// dim of sg sub-grids and cc coils.
+// dim of all sub-grids for cc coils.  Each wave owns a block of b points per
+// dim of every sub-grid (S x b0 b1 b2 = 64 lanes); the tile holds nb blocks
+// per dim.  The per-visit weight table has 2 T[d] entries (two shift
+// variants) for every active dim d, starting at eoff[d] (-1: inactive).
 struct GridTiling {
 
 	int T[3];
 	int nt[3];
 	int ntiles;
-	int sg;   // sub-grids per work-group
-	int cc;   // coils per work-group
-	int W[3]; // forward: window dims (T + lo + hi); adjoint: = T
-	int lo[3];
-	int Wtot;
-	size_t lds;
+	int cc;    // coils per work-group
+	int W[3];  // = T
+	int Wtot;  // points per tile and sub-grid
+	int b[3];  // wave block
+	int nb[3]; // blocks per tile
+	int nw;    // waves per work-group
+	int eoff[3];
 };
 
 struct GridPlan {
