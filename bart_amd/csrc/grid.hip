@@ -358,8 +358,18 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 		}
 	}
 
+	// largest items first: the persistent work-groups take them in this order
+	std::stable_sort(hw.begin(), hw.end(), [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+
 	nwork = (int)hw.size();
 	nsplit = (int)hsplit.size();
+	ta.nwork = nwork;
+	queue.alloc(1);
+	{
+		int dev = 0;
+		BAM_HIP(hipGetDevice(&dev));
+		BAM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+	}
 	nslot = slots;
 	work.alloc(std::max(nwork, 1));
 	split_tiles.alloc(std::max(nsplit, 1));
@@ -594,67 +604,75 @@ __device__ inline void select_taps(const Taps1<TP>& a, const Taps1<TP>& b, bool 
 	any = an;
 }
 
-// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int ADJ_BATCH = 256;  // visits per batch (4 groups of 64)
+constexpr int ADJ_WROW = 65;    // weight table row: 64 entries and a constant 1 (inactive dims)
+
+template <typename X>
+__device__ inline X sel3(int d, X a, X b, X c)
+{
+	return (0 == d) ? a : ((1 == d) ? b : c);
+}
+
+// Work-group barrier that orders LDS only: waits for this wave's LDS
+// operations (lgkmcnt(0)) but not for its vector memory loads, so that
+// loads of the next batch stay in flight across it (__syncthreads() waits
+// for them as part of its fence).
+__device__ inline void lds_barrier()
+{
+	__builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+	__atomic_signal_fence(__ATOMIC_SEQ_CST);
+	__builtin_amdgcn_s_barrier();
+	__atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__global__ void k_reset_queue(int* q)
+{
+	if (0 == threadIdx.x)
+		q[0] = 0;
+}
+
 // Adjoint gridding (k-space -> grid), output-stationary and atomic-free.
-// grid layout [S][C][N]; launch (nwork, ceil(C / cc)) with nw waves.
+// grid layout [S][C][N].  Persistent work-groups (one per CU) take work
+// items (tile, visit range, coil chunk) from a queue, largest first.
 //
-// A wave owns a block of b points per dim in every sub-grid, one lane per
-// (sub-grid, point); the lane accumulates the cc coils of its output in
-// registers.  The visits of the tile (samples of its cells and of the ring
-// of neighbouring cells) are taken in batches of 64:
-//   1. one wave per visit evaluates, for every active dim, shift variant and
-//      tile row, the tap weight (0 off the taps) into LDS and, from the tap
-//      membership bits, which waves' blocks the sample reaches;
-//   2. every wave walks only the visits that reach its block (a scalar loop
-//      over a ballot mask); the coil values of the sample are wave-uniform
-//      loads, and each lane adds v_c * wz * wy * wx.
-// The summation order of a point is the visit order, as in the sequential
-// gridding of noncart/grid.c:240-250.  Tiles whose visits are split over
-// several work items store partial tiles, summed in order by k_sum_split.
-// ---------------------------------------------------------------------------
-
-constexpr int ADJ_BATCH = 64;
-constexpr int ADJ_WROW = 65;  // weight table row: 64 entries and a constant 1 (inactive dims)
-
+// A wave owns a block of b points per dim in every sub-grid (64 outputs:
+// sub-grid s, point).  For each batch of visits (samples of the tile's cells
+// and of the ring of neighbouring cells, in cell order):
+//   1. one lane per (visit, dim, shift variant) evaluates the taps once and
+//      writes the weight of every tile row (0 off the taps) and a bit mask
+//      of the rows hit to LDS; the coil values of the batch, loaded during
+//      the previous batch, go to LDS;
+//   2. every wave tests from the masks which visits reach its block and
+//      walks those two at a time: C[out][j] += sum_k A[out][k] B[k][j] on
+//      the matrix cores (v_mfma_f32_32x32x2_f32), A = tap weight
+//      wz * wy * wx of output out for visit k, B = value j (coil j / 2,
+//      re / im) of visit k.
+// Each product is accumulated in visit order, as in the sequential gridding
+// of noncart/grid.c:240-250.  Tiles whose visits are split over several
+// work items store partial tiles, summed in order by k_sum_split.
 template <int TP, int CC>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_grid_adj(GridGeom g, GridTiling t, const float* __restrict__ kbt,
+__global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, const float* __restrict__ kbt,
 						   const float4* __restrict__ sk, const int* __restrict__ cstart,
-						   const int4* __restrict__ work, const float2* __restrict__ kspS,
-						   float2* __restrict__ grid, float2* __restrict__ spart)
+						   const int4* __restrict__ work, int nitems, int* __restrict__ queue,
+						   const float2* __restrict__ kspS, float2* __restrict__ grid,
+						   float2* __restrict__ spart)
 {
 	extern __shared__ float4 smem4[];
 	char* smem = (char*)smem4;
 	float* table = (float*)smem;
 	CellRanges& R = *(CellRanges*)(smem + SMEM_TABLE);
-	int* sidx = (int*)(smem + SMEM_DATA);               // [ADJ_MAXV] sorted sample of each visit
-	float* wtab = (float*)(sidx + ADJ_MAXV);            // [ADJ_BATCH][ADJ_WROW] tap weights per visit
-	unsigned* hitm = (unsigned*)(wtab + ADJ_WROW * ADJ_BATCH);  // [ADJ_BATCH] waves reached per visit
+	float4* skall = (float4*)(smem + SMEM_DATA);          // [ADJ_MAXV] coordinates of each visit
+	float2* vbuf = (float2*)(skall + ADJ_MAXV);           // [ADJ_BATCH][CC] coil values per visit
+	int* sidx = (int*)(vbuf + ADJ_BATCH * CC);            // [ADJ_MAXV] sorted sample of each visit
+	float* wtab = (float*)(sidx + ADJ_MAXV);              // [ADJ_BATCH][ADJ_WROW] tap weights per visit
+	unsigned* rows = (unsigned*)(wtab + ADJ_WROW * ADJ_BATCH);  // [ADJ_BATCH][6] tile rows hit per dim, variant
+	int* qitem = (int*)(rows + 6 * ADJ_BATCH);
 
-	const int4 wk = work[blockIdx.x];  // (tile, first visit, end visit, partial slot + 1 or 0)
-	const int tile = wk.x;
-	const int ch = blockIdx.y;
-	const int c0 = ch * CC;
-	const int ncc = min(CC, g.C - c0);
 	const int nthr = blockDim.x;
 	const int lane = threadIdx.x & 63;
-	const int wid = threadIdx.x >> 6;
-	const int nw = t.nw;
-
-	int tb[3], t0[3], clo[3], chi[3];
-	tb[0] = tile % t.nt[0];
-	tb[1] = (tile / t.nt[0]) % t.nt[1];
-	tb[2] = tile / (t.nt[0] * t.nt[1]);
-
-	for (int d = 0; d < 3; d++) {
-		t0[d] = tb[d] * t.T[d];
-		if (1 == g.n[d]) {
-			clo[d] = 0;
-			chi[d] = 0;
-		} else {
-			clo[d] = (int)floorf(((float)t0[d] - g.reach_hi) / (float)g.CS[d]);
-			chi[d] = (int)floorf(((float)(t0[d] + t.T[d] - 1) + g.reach_lo) / (float)g.CS[d]);
-		}
-	}
+	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+	const int nwork = t.nwork;
 
 	for (int i = threadIdx.x; i < KB_SIZE + 1; i += nthr)
 		table[i] = kbt[i];
@@ -662,177 +680,298 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 	for (int v = threadIdx.x; v < ADJ_BATCH; v += nthr)
 		wtab[v * ADJ_WROW + 64] = 1.f;
 
-	const int Ftile = build_ranges(g, cstart, clo, chi, R);
-	const int fb = min(wk.y, Ftile);
-	const int fe = min(wk.z, Ftile);
+	if (0 == threadIdx.x)
+		qitem[1] = 0;
 
-	// sorted sample of every visit of this work item (fe - fb <= ADJ_MAXV)
-	for (int f = fb + threadIdx.x; f < fe; f += nthr) {
-		int lo = 0, hi = R.n;
-		while (hi - lo > 1) {
-			const int mid = (lo + hi) >> 1;
-			if (R.pre[mid] <= f)
-				lo = mid;
-			else
-				hi = mid;
-		}
-		sidx[f - fb] = R.start[lo] + (f - R.pre[lo]);
-	}
-
-	// this lane's output: point lp of the wave's block in sub-grid s
+	// the wave's block; C rows this lane feeds with weights: l % 32 and 32 + l % 32
 	const int B = t.b[0] * t.b[1] * t.b[2];
-	const bool mine = lane < g.S * B;
-	const int s = mine ? lane / B : 0;
-	const int cb = lane - s * B;
 	const int wb[3] = { wid % t.nb[0], (wid / t.nb[0]) % t.nb[1], wid / (t.nb[0] * t.nb[1]) };
-	const int lp[3] = { wb[0] * t.b[0] + cb % t.b[0], wb[1] * t.b[1] + (cb / t.b[0]) % t.b[1],
-			    wb[2] * t.b[2] + cb / (t.b[0] * t.b[1]) };
-	const int gp[3] = { t0[0] + lp[0], t0[1] + lp[1], t0[2] + lp[2] };
-	const bool inside = mine && gp[0] < g.n[0] && gp[1] < g.n[1] && gp[2] < g.n[2];
-	const int pnt = lp[0] + t.W[0] * (lp[1] + t.W[1] * lp[2]);
-
-	// the lane's weight entries per dim (inactive dim: the constant 1)
-	int wo[3];
+	const int jcol = lane & 31;
+	int wo1[3], wo2[3];
+	{
+		const int r1 = lane & 31;
+		const int r2 = 32 + (lane & 31);
+		const int s1 = min(r1 / B, g.S - 1), s2 = min(r2 / B, g.S - 1);
+		const int cb1 = r1 - (r1 / B) * B, cb2 = r2 - (r2 / B) * B;
+		const int p1[3] = { wb[0] * t.b[0] + cb1 % t.b[0], wb[1] * t.b[1] + (cb1 / t.b[0]) % t.b[1],
+				    wb[2] * t.b[2] + cb1 / (t.b[0] * t.b[1]) };
+		const int p2[3] = { wb[0] * t.b[0] + cb2 % t.b[0], wb[1] * t.b[1] + (cb2 / t.b[0]) % t.b[1],
+				    wb[2] * t.b[2] + cb2 / (t.b[0] * t.b[1]) };
 #pragma unroll
-	for (int d = 0; d < 3; d++)
-		wo[d] = (t.eoff[d] < 0) ? 64 : (t.eoff[d] + g.sbit[s][d] * t.T[d] + lp[d]);
-
-	// staging role of this lane: weight entry (dim ed, variant ev, tile row el)
-	int ed = -1, ev = 0, el = 0;
-#pragma unroll
-	for (int d = 0; d < 3; d++)
-		if (t.eoff[d] >= 0 && lane >= t.eoff[d] && lane < t.eoff[d] + 2 * t.T[d]) {
-			ed = d;
-			ev = (lane - t.eoff[d]) / t.T[d];
-			el = lane - t.eoff[d] - ev * t.T[d];
+		for (int d = 0; d < 3; d++) {
+			wo1[d] = (t.eoff[d] < 0) ? 64 : (t.eoff[d] + g.sbit[s1][d] * t.T[d] + min(p1[d], t.T[d] - 1));
+			wo2[d] = (t.eoff[d] < 0) ? 64 : (t.eoff[d] + g.sbit[s2][d] * t.T[d] + min(p2[d], t.T[d] - 1));
 		}
-	const int edd = max(ed, 0);
-	const int en = g.n[edd];
-	const float eshift = g.shv[edd][(ev < g.nshv[edd]) ? ev : 0];
-	const int egp = ((0 == edd) ? t0[0] : ((1 == edd) ? t0[1] : t0[2])) + el;
-
-	// reach test of wave `lane` (lanes < nw): block bits of every active dim
-	unsigned long long qmask[3];
-	int qT[3];
-	const int q = min(lane, nw - 1);
-#pragma unroll
-	for (int d = 0; d < 3; d++) {
-		const int qb = (0 == d) ? (q % t.nb[0]) : ((1 == d) ? ((q / t.nb[0]) % t.nb[1]) : (q / (t.nb[0] * t.nb[1])));
-		qT[d] = t.T[d];
-		qmask[d] = (t.eoff[d] < 0) ? 0ull : ((((1ull << t.b[d]) - 1ull) << (qb * t.b[d])) << t.eoff[d]);
 	}
 
-	float2 acc[CC];
-#pragma unroll
-	for (int c = 0; c < CC; c++)
-		acc[c] = make_float2(0.f, 0.f);
+	constexpr int VPT = (ADJ_BATCH * CC + 1023) / 1024;  // coil values carried per lane (1024 lanes)
 
-	const float2* kch = kspS + (long)ch * g.M * CC;
+	while (true) {
 
-	for (int base = 0; base < fe - fb; base += ADJ_BATCH) {
+		// next work item (the barrier also ends every wave's use of the
+		// previous item's LDS data)
+		__syncthreads();
 
-		const int nb = min(ADJ_BATCH, fe - fb - base);
+		if (0 == threadIdx.x)
+			qitem[0] = atomicAdd(queue, 1);
 
 		__syncthreads();
 
-		// 1. weights and reached waves of the batch, one wave per visit
-		for (int v = wid; v < nb; v += nw) {
+		const int item = qitem[0];
 
-			const int i = sidx[base + v];
-			const float4 kc = sk[i];
-			bool mem = false;
+		if (item >= nitems)
+			break;
 
-			if (ed >= 0) {
+		const int ch = item / nwork;
+		const int4 wk = work[item - ch * nwork];  // (tile, first visit, end visit, partial slot + 1 or 0)
+		const int tile = wk.x;
+		const int c0 = ch * CC;
+		const int ncc = min(CC, g.C - c0);
 
-				const float k = (0 == ed) ? kc.x : ((1 == ed) ? kc.y : kc.z);
-				const float pos = __fadd_rn(__fmul_rn(g.os, __fadd_rn(k, eshift)), (float)(en / 2));
+		int tb[3], t0[3], clo[3], chi[3];
+		tb[0] = tile % t.nt[0];
+		tb[1] = (tile / t.nt[0]) % t.nt[1];
+		tb[2] = tile / (t.nt[0] * t.nt[1]);
+
+		for (int d = 0; d < 3; d++) {
+			t0[d] = tb[d] * t.T[d];
+			if (1 == g.n[d]) {
+				clo[d] = 0;
+				chi[d] = 0;
+			} else {
+				clo[d] = (int)floorf(((float)t0[d] - g.reach_hi) / (float)g.CS[d]);
+				chi[d] = (int)floorf(((float)(t0[d] + t.T[d] - 1) + g.reach_lo) / (float)g.CS[d]);
+			}
+		}
+
+		const int Ftile = build_ranges(g, cstart, clo, chi, R);
+		const int fb = min(wk.y, Ftile);
+		const int fe = min(wk.z, Ftile);
+		const int F = fe - fb;
+
+		// sorted sample and coordinates of every visit of this item (F <= ADJ_MAXV)
+		for (int f = fb + threadIdx.x; f < fe; f += nthr) {
+			int lo = 0, hi = R.n;
+			while (hi - lo > 1) {
+				const int mid = (lo + hi) >> 1;
+				if (R.pre[mid] <= f)
+					lo = mid;
+				else
+					hi = mid;
+			}
+			const int i = R.start[lo] + (f - R.pre[lo]);
+			sidx[f - fb] = i;
+			skall[f - fb] = sk[i];
+		}
+
+		__syncthreads();
+
+		// coil values: lane carries VPT float2 of the next batch (entries
+		// threadIdx.x + 1024 q: visit e / CC, coil e % CC) through the current one
+		const float2* kch = kspS + (long)ch * g.M * CC;
+		float2 vnext[VPT];
+
+#pragma unroll
+		for (int q = 0; q < VPT; q++) {
+			const int e = threadIdx.x + 1024 * q;
+			vnext[q] = (e < ADJ_BATCH * CC && F > 0) ? kch[(long)sidx[min(e / CC, F - 1)] * CC + (e % CC)]
+								 : make_float2(0.f, 0.f);
+		}
+
+		f32x16 C1 = {}, C2 = {};
+
+		for (int base = 0; base < F; base += ADJ_BATCH) {
+
+			const int nb = min(ADJ_BATCH, F - base);
+			const float4* skc = skall + base;
+
+			// the previous batch is done with the LDS buffers
+			lds_barrier();
+
+#pragma unroll
+			for (int q = 0; q < VPT; q++) {
+				const int e = threadIdx.x + 1024 * q;
+				if (e < ADJ_BATCH * CC)
+					vbuf[e] = vnext[q];
+			}
+
+			// work-groups of fewer than 1024 lanes load the rest now
+			for (int e = threadIdx.x + nthr; e < ADJ_BATCH * CC; e += nthr)
+				if (e % 1024 >= nthr)
+					vbuf[e] = kch[(long)sidx[base + min(e / CC, nb - 1)] * CC + (e % CC)];
+
+			// the next batch's values are loaded while this one is processed
+			if (base + ADJ_BATCH < F) {
+#pragma unroll
+				for (int q = 0; q < VPT; q++) {
+					const int e = threadIdx.x + 1024 * q;
+					if (e < ADJ_BATCH * CC)
+						vnext[q] = kch[(long)sidx[min(base + ADJ_BATCH + e / CC, F - 1)] * CC + (e % CC)];
+				}
+			}
+
+			// 1. taps of the batch: lane (visit v, dim d, variant var)
+			for (int jj = threadIdx.x; jj < 6 * nb; jj += nthr) {
+
+				const int v = jj / 6;
+				const int k = jj - 6 * v;
+				const int d = k >> 1;
+				const int var = k & 1;
+
+				// per-dim parameters by selection (no indexed kernel-argument loads)
+				const int eo = sel3(d, t.eoff[0], t.eoff[1], t.eoff[2]);
+
+				if (eo < 0)
+					continue;
+
+				const float4 kc = skc[v];
+				const float kd = sel3(d, kc.x, kc.y, kc.z);
+				const int n = sel3(d, g.n[0], g.n[1], g.n[2]);
+				const bool v1 = (1 == var) && (sel3(d, g.nshv[0], g.nshv[1], g.nshv[2]) > 1);
+				const float sh = v1 ? sel3(d, g.shv[0][1], g.shv[1][1], g.shv[2][1])
+						    : sel3(d, g.shv[0][0], g.shv[1][0], g.shv[2][0]);
+				const float pos = __fadd_rn(__fmul_rn(g.os, __fadd_rn(kd, sh)), (float)(n / 2));
 				int sti = (int)ceil((double)pos - 0.5 * (double)g.width);
 				int eni = (int)floor((double)pos + 0.5 * (double)g.width);
 
 				if (!g.periodic) {
 					sti = max(sti, 0);
-					eni = min(eni, en - 1);
+					eni = min(eni, n - 1);
 				}
 
 				const int cnt = min(max(eni - sti + 1, 0), TP);
-				int j;
+				float wq[TP];
 
-				if (g.periodic) {
-					j = (egp - sti) % en;
-					if (j < 0)
-						j += en;
-				} else {
-					j = egp - sti;
+#pragma unroll
+				for (int q = 0; q < TP; q++)
+					wq[q] = (q < cnt) ? kb_lookup(table, fabsf((float)(sti + q) - pos) / g.width) : 0.f;
+
+				const int T = sel3(d, t.T[0], t.T[1], t.T[2]);
+				const int td = sel3(d, t0[0], t0[1], t0[2]);
+				float* wr = wtab + v * ADJ_WROW + eo + var * T;
+				unsigned bits = 0u;
+
+				for (int l = 0; l < T; l++) {
+
+					const int gl = td + l;
+					int j;
+
+					if (g.periodic) {
+						j = (gl - sti) % n;
+						if (j < 0)
+							j += n;
+					} else {
+						j = gl - sti;
+					}
+
+					const bool mem = (gl < n) && (j >= 0) && (j < cnt);
+					float wl = 0.f;
+
+#pragma unroll
+					for (int q = 0; q < TP; q++)
+						wl = (mem && j == q) ? wq[q] : wl;
+
+					wr[l] = wl;
+					bits |= mem ? (1u << l) : 0u;
 				}
 
-				mem = (egp < en) && (j >= 0) && (j < cnt);
-				wtab[v * ADJ_WROW + lane] = mem ? kb_lookup(table, fabsf((float)(sti + j) - pos) / g.width) : 0.f;
+				rows[v * 6 + k] = bits;
 			}
 
-			const unsigned long long mb = __ballot(mem);
-			bool reach = lane < nw;
+			lds_barrier();
+
+			// 2. per group of 64 visits: those that reach this wave's block
+			// (taps in its rows in every active dim, either variant), two at
+			// a time
+			for (int gb = 0; gb < nb; gb += 64) {
+
+				const int vl = gb + min(lane, nb - gb - 1);
+				bool reach = lane < nb - gb;
 
 #pragma unroll
-			for (int d = 0; d < 3; d++)
-				if (0ull != qmask[d])
-					reach = reach && (0ull != ((mb | (mb >> qT[d])) & qmask[d]));
+				for (int d = 0; d < 3; d++)
+					if (t.eoff[d] >= 0) {
+						const unsigned long long m = rows[vl * 6 + 2 * d] | rows[vl * 6 + 2 * d + 1];
+						reach = reach && (0ull != ((m >> (wb[d] * t.b[d])) & ((1ull << t.b[d]) - 1ull)));
+					}
 
-			const unsigned long long rw = __ballot(reach);
+				unsigned long long mh = __ballot(reach);
 
-			if (0 == lane)
-				hitm[v] = (unsigned)rw;
-		}
+				while (0ull != mh) {
 
-		__syncthreads();
+					const int vA = gb + __builtin_ctzll(mh);
+					mh &= mh - 1ull;
+					const bool hasB = 0ull != mh;
+					const int vB = hasB ? (gb + __builtin_ctzll(mh)) : vA;
+					if (hasB)
+						mh &= mh - 1ull;
 
-		// 2. the visits that reach this wave's block
-		unsigned long long mh = __ballot(lane < nb && 0u != ((hitm[min(lane, nb - 1)] >> wid) & 1u));
-		const int myi = sidx[base + min(lane, nb - 1)];
+					const bool use = (lane < 32) || hasB;
+					const int v = (lane < 32) ? vA : vB;
+					const float* wv = wtab + v * ADJ_WROW;
 
-		while (0ull != mh) {
+					// z, y, x (BART order)
+					float a1 = wv[wo1[2]];
+					a1 = a1 * wv[wo1[1]];
+					a1 = a1 * wv[wo1[0]];
+					float a2 = wv[wo2[2]];
+					a2 = a2 * wv[wo2[1]];
+					a2 = a2 * wv[wo2[0]];
+					float bv = (jcol < 2 * CC) ? ((const float*)vbuf)[v * 2 * CC + jcol] : 0.f;
 
-			const int v = __builtin_ctzll(mh);
-			mh &= mh - 1ull;
+					if (!use) {
+						a1 = 0.f;
+						a2 = 0.f;
+						bv = 0.f;
+					}
 
-			const int i = __builtin_amdgcn_readlane(myi, v);
-			const float* wv = wtab + v * ADJ_WROW;
-
-			// z, y, x (BART order)
-			float w = wv[wo[2]];
-			w = w * wv[wo[1]];
-			w = w * wv[wo[0]];
-
-			const float4* xp = (const float4*)(kch + (long)i * CC);
-
-#pragma unroll
-			for (int c = 0; c < CC / 2; c++) {
-				const float4 x = xp[c];
-				acc[2 * c].x += x.x * w;
-				acc[2 * c].y += x.y * w;
-				acc[2 * c + 1].x += x.z * w;
-				acc[2 * c + 1].y += x.w * w;
+					C1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bv, C1, 0, 0, 0);
+					C2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, bv, C2, 0, 0, 0);
+				}
 			}
 		}
-	}
 
-	if (!inside)
-		return;
-
-	const long idx = gp[0] + (long)g.n[0] * (gp[1] + (long)g.n[1] * gp[2]);
+		// C row i (of the wave's 64 outputs), column j = 2 coil + (re, im):
+		// register r of lane l holds i = 8 (r / 4) + 4 (l / 32) + r % 4 (+ 32
+		// in C2), j = l % 32.  (zero is 0, read from LDS here so that the
+		// row decoding below is not hoisted out of the item loop, where its
+		// 32 copies would stay live.)
+		const int zero = qitem[1];
+		const int c = (jcol + zero) >> 1;
+		const int lhalf = (lane + zero) >> 5;
 
 #pragma unroll
-	for (int c = 0; c < CC; c++) {
+		for (int h = 0; h < 2; h++)
+#pragma unroll
+			for (int r = 0; r < 16; r++) {
 
-		if (c >= ncc)
-			continue;
+				const float x = (0 == h) ? C1[r] : C2[r];
+				const float y = __shfl_xor(x, 1);
+				const int i = 32 * h + 8 * (r / 4) + 4 * lhalf + (r % 4);
+				const int rs = i / B;
+				const int rc = i - rs * B;
+				const int q0 = wb[0] * t.b[0] + rc % t.b[0];
+				const int q1 = wb[1] * t.b[1] + (rc / t.b[0]) % t.b[1];
+				const int q2 = wb[2] * t.b[2] + rc / (t.b[0] * t.b[1]);
+				const int x0 = t0[0] + q0, x1 = t0[1] + q1, x2 = t0[2] + q2;
 
-		if (0 == wk.w) {
-			grid[((long)s * g.C + c0 + c) * g.N + idx] = apply_conj_fm(g, acc[c], gp[0], gp[1], gp[2]);
-		} else {
-			// tile split over several work items (dense region): the
-			// partial tile goes to its own slot, summed by k_sum_split
-			spart[(((long)(wk.w - 1) * g.S + s) * g.C + c0 + c) * t.Wtot + pnt] = acc[c];
-		}
+				if (0 != (lane & 1) || i >= g.S * B || c >= ncc || x0 >= g.n[0] || x1 >= g.n[1] ||
+				    x2 >= g.n[2])
+					continue;
+
+				const float2 val = make_float2(x, y);
+
+				if (0 == wk.w) {
+					const long idx = x0 + (long)g.n[0] * (x1 + (long)g.n[1] * x2);
+					grid[((long)rs * g.C + c0 + c) * g.N + idx] = apply_conj_fm(g, val, x0, x1, x2);
+				} else {
+					// tile split over several work items (dense region):
+					// the partial tile goes to its own slot, summed by
+					// k_sum_split
+					const int pq = q0 + t.W[0] * (q1 + t.W[1] * q2);
+					spart[(((long)(wk.w - 1) * g.S + rs) * g.C + c0 + c) * t.Wtot + pq] = val;
+				}
+			}
 	}
 }
 
@@ -896,13 +1035,18 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 
 	const int tp = taps_bound(g);
 	const int nthr = 64 * t.nw;
-	dim3 blocks(gp.nwork, (g.C + t.cc - 1) / t.cc);
-	const size_t lds = SMEM_DATA + sizeof(int) * ADJ_MAXV + sizeof(float) * ADJ_WROW * ADJ_BATCH +
-			   sizeof(unsigned) * ADJ_BATCH;
+	const int nch = (g.C + t.cc - 1) / t.cc;
+	const int nitems = gp.nwork * nch;
+	const unsigned nwg = (unsigned)std::max(1, std::min(nitems, 2 * gp.ncu));
+	const size_t lds = SMEM_DATA + sizeof(float4) * ADJ_MAXV + sizeof(float2) * ADJ_BATCH * t.cc +
+			   sizeof(int) * ADJ_MAXV + sizeof(float) * ADJ_WROW * ADJ_BATCH + sizeof(unsigned) * 6 * ADJ_BATCH +
+			   sizeof(int) * 4;
 
-#define BAM_ADJ(TP, CC)                                                                                    \
-	hipLaunchKernelGGL((k_grid_adj<TP, CC>), blocks, dim3(nthr), lds, st, g, t, kb, gp.sk.ptr, gp.cstart.ptr, \
-			   gp.work.ptr, gp.kspS.ptr, grid, gp.spart.ptr)
+	hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, st, gp.queue.ptr);
+
+#define BAM_ADJ(TP, CC)                                                                                     \
+	hipLaunchKernelGGL((k_grid_adj<TP, CC>), dim3(nwg), dim3(nthr), lds, st, g, t, kb, gp.sk.ptr, gp.cstart.ptr, \
+			   gp.work.ptr, nitems, gp.queue.ptr, gp.kspS.ptr, grid, gp.spart.ptr)
 
 	if (4 == tp) {
 		if (t.cc <= 4) BAM_ADJ(4, 4);

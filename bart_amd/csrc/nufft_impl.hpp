@@ -62,6 +62,7 @@ struct GridTiling {
 	int nb[3]; // blocks per tile
 	int nw;    // waves per work-group
 	int eoff[3];
+	int nwork; // work items per coil chunk
 };
 
 struct GridPlan {
@@ -79,6 +80,8 @@ struct GridPlan {
 
 	int nwork = 0;
 	int nsplit = 0;
+	DevBuf<int> queue;       // adjoint work queue head
+	int ncu = 1;             // compute units of the device
 
 	// traj: host, M x 3 floats (real parts)
 	void build(const GridGeom& geom, const float* traj);
