@@ -471,11 +471,11 @@ struct StReduceS {
 // registers; for every sub-grid s they are multiplied by linphase_s,
 // transformed along x and y and stored to grid[s][c][z].
 //
-// k_slice_inv: work-group (z, coil group).  For every coil c of the group
-// and every s: grid[s][c][z] is inverse-transformed along y and x and
-// conj(linphase_s) x value is summed over s in order (x rolloff when roll);
-// with maps the coils are summed in order as conj(map_c) x value into the
-// group's partial image, else coil image c is stored.  No atomics.
+// k_slice_inv: work-group (z, c).  For every s, grid[s][c][z] is
+// inverse-transformed along y and x and conj(linphase_s) x value is summed
+// over s in order; the sum is scaled by the rolloff (when roll) and, with
+// maps, multiplied by conj(map_c) into partial image c (the coils are summed
+// in order by sum_groups), else stored as coil image c.  No atomics.
 // ---------------------------------------------------------------------------
 
 // threads of a slice work-group: a multiple of NX (every lane keeps its x)
@@ -556,6 +556,27 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 	const float2* img = (nullptr != sens) ? src + zoff : src + (long)c * sg.N + zoff;
 	const float2* map = (nullptr != sens) ? sens + (long)c * sg.N + zoff : nullptr;
 
+	// image-side values of this lane's points, held across the sub-grids
+	float2 vin[EPT];
+
+#pragma unroll
+	for (int i = 0; i < EPT; i++) {
+
+		const int y = y0 + i * DY;
+		vin[i] = make_float2(0.f, 0.f);
+
+		if (0 == NY % DY || y < NY) {
+
+			const int e = y * NX + x;
+			float2 v = (nullptr != map) ? cmul(img[e], map[e]) : img[e];
+
+			if (roll)
+				v = cscale(v, roll_xyz(T, x, y, z));
+
+			vin[i] = v;
+		}
+	}
+
 	for (int s = 0; s < sg.S; s++) {
 
 		// per-iteration copies of the lane coordinates (keeps the element
@@ -565,24 +586,13 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 
 		__syncthreads();
 
-		// the image-side values are read again for every sub-grid (cache
-		// hits after the first) instead of being held in registers
-#pragma unroll 4
+#pragma unroll
 		for (int i = 0; i < EPT; i++) {
 
-			const int x = xs;
 			const int y = ys + i * DY;
 
-			if (0 == NY % DY || y < NY) {
-
-				const int e = y * NX + x;
-				float2 v = (nullptr != map) ? cmul(img[e], map[e]) : img[e];
-
-				if (roll)
-					v = cscale(v, roll_xyz(T, x, y, z));
-
-				lds[y * LD + x] = cmul(v, lin_xyz(T, s, x, y, z));
-			}
+			if (0 == NY % DY || y < NY)
+				lds[y * LD + xs] = cmul(vin[i], lin_xyz(T, s, xs, y, z));
 		}
 
 		__syncthreads();
@@ -610,7 +620,7 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 template <int NX, int NY>
 __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(SliceGeom sg, SepTables T,
 								    const float2* __restrict__ grid,
-								    const float2* __restrict__ sens, bool roll, int cc,
+								    const float2* __restrict__ sens, bool roll,
 								    float2* __restrict__ dst,
 								    const float2* __restrict__ twx_g,
 								    const float2* __restrict__ twy_g, const int* skip)
@@ -629,7 +639,7 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 		return;
 
 	const int z = blockIdx.x;
-	const int gi = blockIdx.y;
+	const int c = blockIdx.y;
 	const long zoff = (long)z * NP;
 	const int x = threadIdx.x % NX;
 	const int y0 = threadIdx.x / NX;
@@ -646,69 +656,34 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 	for (int i = 0; i < EPT; i++)
 		acc[i] = make_float2(0.f, 0.f);
 
-	for (int cl = 0; cl < cc; cl++) {
+	for (int s = 0; s < sg.S; s++) {
 
-		const int c = gi * cc + cl;
-		const float2* map = (nullptr != sens) ? sens + (long)c * sg.N + zoff : nullptr;
+		const float2* in = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
 
-		for (int s = 0; s < sg.S; s++) {
+		// per-iteration copies of the lane coordinates (keeps the element
+		// addresses out of registers across the loop)
+		int xs = x, ys = y0;
+		asm volatile("" : "+v"(xs), "+v"(ys));
 
-			const float2* in = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
-
-			// per-iteration copies of the lane coordinates (keeps the
-			// element addresses out of registers across the loop)
-			int xs = x, ys = y0;
-			asm volatile("" : "+v"(xs), "+v"(ys));
-
-			__syncthreads();
+		__syncthreads();
 
 #pragma unroll 4
-			for (int i = 0; i < EPT; i++) {
+		for (int i = 0; i < EPT; i++) {
 
-				const int y = ys + i * DY;
+			const int y = ys + i * DY;
 
-				if (0 == NY % DY || y < NY)
-					lds[y * LD + xs] = in[y * NX + xs];
-			}
-
-			__syncthreads();
-
-			slice_passes<NY, NX, LD, 1, NTH>(lds, twy, +1.f);
-			slice_passes<NX, NY, 1, LD, NTH>(lds, twx, +1.f);
-
-			// sum over s of conj(linphase_s) x value; with maps the factor
-			// conj(map_c) (and the rolloff) is applied per term so that one
-			// accumulator carries the sums over s and coils
-			xs = x;
-			ys = y0;
-			asm volatile("" : "+v"(xs), "+v"(ys));
-
-#pragma unroll
-			for (int i = 0; i < EPT; i++) {
-
-				const int x = xs;
-				const int y = ys + i * DY;
-
-				if (0 == NY % DY || y < NY) {
-
-					float2 f = lin_xyz(T, s, x, y, z);
-
-					if (nullptr != map) {
-						f = cmul(f, map[y * NX + x]);
-						if (roll)
-							f = cscale(f, roll_xyz(T, x, y, z));
-					}
-
-					acc[i] = cadd(acc[i], cmulc(lds[y * LD + x], f));
-				}
-			}
+			if (0 == NY % DY || y < NY)
+				lds[y * LD + xs] = in[y * NX + xs];
 		}
 
-		if (nullptr != map)
-			continue;
+		__syncthreads();
 
-		// coil image c
-		int xs = x, ys = y0;
+		slice_passes<NY, NX, LD, 1, NTH>(lds, twy, +1.f);
+		slice_passes<NX, NY, 1, LD, NTH>(lds, twx, +1.f);
+
+		// sum over s of conj(linphase_s) x value
+		xs = x;
+		ys = y0;
 		asm volatile("" : "+v"(xs), "+v"(ys));
 
 #pragma unroll
@@ -717,15 +692,12 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 			const int y = ys + i * DY;
 
 			if (0 == NY % DY || y < NY)
-				dst[(long)c * sg.N + zoff + y * NX + xs] = roll ? cscale(acc[i], roll_xyz(T, xs, y, z)) : acc[i];
-
-			acc[i] = make_float2(0.f, 0.f);
+				acc[i] = cadd(acc[i], cmulc(lds[y * LD + xs], lin_xyz(T, s, xs, y, z)));
 		}
 	}
 
-	if (nullptr == sens)
-		return;
-
+	// rolloff, then conj(map_c) into partial image c, or coil image c
+	const float2* map = (nullptr != sens) ? sens + (long)c * sg.N + zoff : nullptr;
 	int xs = x, ys = y0;
 	asm volatile("" : "+v"(xs), "+v"(ys));
 
@@ -734,8 +706,15 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 
 		const int y = ys + i * DY;
 
-		if (0 == NY % DY || y < NY)
-			dst[(long)gi * sg.N + zoff + y * NX + xs] = acc[i];
+		if (0 == NY % DY || y < NY) {
+
+			float2 v = roll ? cscale(acc[i], roll_xyz(T, xs, y, z)) : acc[i];
+
+			if (nullptr != map)
+				v = cmulc(v, map[y * NX + xs]);
+
+			dst[(long)c * sg.N + zoff + y * NX + xs] = v;
+		}
 	}
 }
 
@@ -751,18 +730,6 @@ static bool slice_supported(const int n[3])
 	default:
 		return false;
 	}
-}
-
-// coils per work-group of the inverse slice kernel: enough work-groups to
-// fill the chip (>= 512) while keeping the partial images few
-static int slice_coil_group(int nz, int C)
-{
-	int cc = C;
-
-	while (cc > 1 && 0 == cc % 2 && (long)nz * (C / cc) < 512)
-		cc /= 2;
-
-	return cc;
 }
 
 static void launch_slice_fwd(const Nufft& p, const float2* src, const float2* sens, bool roll, const int* skip,
@@ -787,14 +754,14 @@ static void launch_slice_fwd(const Nufft& p, const float2* src, const float2* se
 	BAM_HIP(hipGetLastError());
 }
 
-static void launch_slice_inv(const Nufft& p, float2* dst, const float2* sens, bool roll, int cc, const int* skip,
+static void launch_slice_inv(const Nufft& p, float2* dst, const float2* sens, bool roll, const int* skip,
 			     const char* name)
 {
 	ProfScope prof(name, p.stream);
 	SliceGeom sg{ p.n[2], p.S, p.C, p.N };
-	dim3 blocks(p.n[2], p.C / cc);
+	dim3 blocks(p.n[2], p.C);
 
-#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_inv<NN, NN>), blocks, dim3(slice_threads_inv<NN, NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, p.grid.ptr, sens, roll, cc, dst, p.axis[0].tw,             				   p.axis[1].tw, skip);                                                         		break
+#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_inv<NN, NN>), blocks, dim3(slice_threads_inv<NN, NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, p.grid.ptr, sens, roll, dst, p.axis[0].tw,                 				   p.axis[1].tw, skip);                                                         		break
 
 	switch (p.n[0]) {
 		BAM_SLICE(32);
@@ -967,7 +934,8 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 
 	// partial images of the coil groups of S^H (allocated here so that no
 	// allocation happens inside a captured CG iteration)
-	const int G = C / (slice ? slice_coil_group(n[2], C) : coil_group(S, C, n[act[0]], true));
+	// partial images of the S^H sum: one per coil with the slice kernels
+	const int G = slice ? C : C / coil_group(S, C, n[act[0]], true);
 
 	if (G > 1)
 		part.alloc((size_t)G * N);
@@ -1035,11 +1003,11 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 
 		z_pass(*this, +1.f, false, false, "fft_adj_axis");
 
-		const int cc = (nullptr != sens) ? slice_coil_group(n[2], C) : 1;
-		const int G = C / cc;
+		// with maps: one partial image per coil, summed in coil order
+		const int G = C;
 		float2* out = (nullptr != sens && G > 1) ? part.ptr : dst;
 
-		launch_slice_inv(*this, out, sens, true, cc, nullptr, "fft_adj_slice");
+		launch_slice_inv(*this, out, sens, true, nullptr, "fft_adj_slice");
 
 		if (out != dst)
 			sum_groups(dst, part.ptr, N, G, stream, nullptr);
@@ -1088,16 +1056,19 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 
 		VolAxis va = VolAxis::make(n, 2, (long)S * C);
 		FftLaunchCfg cfg = fft_config_skip(skip, axis[2], va.transforms(), va.contig(), 1, -1.f, +1.f);
+		// the C coil volumes of a sub-grid share its PSF: launch order
+		// column group -> volume, so that the PSF columns come from cache
+		va.group_volumes(cfg.L.P);
 		LdGrid ld{ va, tabs, grid.ptr, false };
 		StGrid st{ va, tabs, grid.ptr, false };
 		MdPsf md{ va, C, psf.ptr };
 		launch_fft_axis(cfg, stream, ld, md, st);
 
-		const int cc = (nullptr != sens) ? slice_coil_group(n[2], C) : 1;
-		const int G = C / cc;
+		// with maps: one partial image per coil, summed in coil order
+		const int G = C;
 		float2* out = (nullptr != sens && G > 1) ? part.ptr : dst;
 
-		launch_slice_inv(*this, out, sens, false, cc, skip, "fft_toeplitz_slice_inv");
+		launch_slice_inv(*this, out, sens, false, skip, "fft_toeplitz_slice_inv");
 
 		if (out != dst)
 			sum_groups(dst, part.ptr, N, G, stream, skip);

@@ -51,6 +51,12 @@ struct VolAxis {
 	long nvol;
 	FastDiv f_inner, f_outer, f_n0, f_n1;
 
+	// volume-interleaved order (set by group_volumes): consecutive groups of
+	// vgrp transforms run through the volumes before the next columns,
+	// t = il + vgrp (v + nvol ih) with i = il + vgrp ih (outer == 1 only)
+	uint32_t vgrp = 0;
+	FastDiv f_vgrp, f_nvol;
+
 	static VolAxis make(const int dims[3], int axis, long nvol)
 	{
 		VolAxis g;
@@ -82,9 +88,32 @@ struct VolAxis {
 	long transforms() const { return (long)inner * outer * nvol; }
 	bool contig() const { return 1 == inner; }
 
+	// Launch blocks of P transforms then take the same P columns of every
+	// volume one after the other (data shared by the volumes, such as a
+	// PSF, is reused from cache).  Returns false (order unchanged) when the
+	// geometry does not allow it.
+	bool group_volumes(int P)
+	{
+		if (1 != outer || P <= 1 || 0 != inner % (uint32_t)P)
+			return false;
+		vgrp = (uint32_t)P;
+		f_vgrp = FastDiv((uint32_t)P);
+		f_nvol = FastDiv((uint32_t)nvol);
+		return true;
+	}
+
 	// (i, o, v) of transform t
 	__host__ __device__ inline void split(long t, uint32_t& i, uint32_t& o, uint32_t& v) const
 	{
+		if (0 != vgrp) {
+			uint32_t r, il, ih;
+			f_vgrp.divmod((uint32_t)t, r, il);
+			f_nvol.divmod(r, ih, v);
+			i = il + vgrp * ih;
+			o = 0;
+			return;
+		}
+
 		uint32_t rest;
 		f_inner.divmod((uint32_t)t, rest, i);
 		f_outer.divmod(rest, v, o);

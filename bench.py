@@ -41,7 +41,7 @@ def workload(name):
         maps = I.gaussian_coils((n, n, n), C, seed=2)
         # groups: coil groups of the SENSE normal operator's S^H sum
         return dict(name="3d_radial_kooshball_128^3_16coils_8192x256", img=(n, n, n), C=C, R=R, S=spokes,
-                    traj=traj, x=img, maps=maps, lam=0.01, iters=20, groups=4)
+                    traj=traj, x=img, maps=maps, lam=0.01, iters=20, groups=16)
     if name == "radial2d":
         n, C, spokes, R = 256, 8, 201, 512
         traj = I.radial_traj(R, spokes)
