@@ -109,8 +109,9 @@ FftLaunchCfg fft_config(const AxisPlan& plan, long total, bool contig, int P_min
 		c.L.total = total;
 		c.L.sign1 = sign1;
 		c.L.sign2 = sign2;
+		c.L.pitch = n + 1;
 		c.blocks = (unsigned)((total + P - 1) / P);
-		c.lds_bytes = (size_t)(P + 1) * n * sizeof(float2);
+		c.lds_bytes = (size_t)(P * (n + 1) + n) * sizeof(float2);
 		c.ept = 0;
 		c.fixed = true;
 		return c;
@@ -145,6 +146,7 @@ FftLaunchCfg fft_config(const AxisPlan& plan, long total, bool contig, int P_min
 	c.L.sign1 = sign1;
 	c.L.sign2 = sign2;
 	c.blocks = (unsigned)((total + P - 1) / P);
+	c.L.pitch = n;
 	c.lds_bytes = (size_t)P * n * sizeof(float2);
 	c.ept = (P * n <= 2048) ? 8 : ((P * n <= 4096) ? 16 : 32);
 

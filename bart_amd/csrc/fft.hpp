@@ -46,6 +46,7 @@ struct FftLaunch {
 	AxisPlan plan;
 	int P = 1;
 	int logP = 0;
+	int pitch = 0;       // LDS distance of consecutive transforms (n or n + 1)
 	bool contig = true;
 	long total = 0;      // number of transforms
 	float sign1 = -1.f;  // first transform
@@ -425,7 +426,7 @@ __device__ inline void ct_pass(float2* lds, const float2* tw, float sgn)
 			const int p = b / nR;
 			const int j = b - p * nR;
 			const int k = j % NS;
-			const float2* src = lds + p * N + j;
+			const float2* src = lds + p * (N + 1) + j;
 
 #pragma unroll
 			for (int r = 0; r < R; r++) {
@@ -443,7 +444,7 @@ __device__ inline void ct_pass(float2* lds, const float2* tw, float sgn)
 			}
 
 			dft_any<R>(v[i], sgn, tw, N);
-			dst[i] = p * N + (j / NS) * NS * R + k;
+			dst[i] = p * (N + 1) + (j / NS) * NS * R + k;
 		}
 	}
 
@@ -475,10 +476,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 	constexpr int P = ct_block_p(N);
 	constexpr int TOT = P * N;
 	constexpr int LOGP = ct_log2(P);
+	constexpr int PN = N + 1;  // transform pitch: strided accesses (p fastest) spread over the banks
 	static_assert((1 << LOGP) == P, "transforms per block must be a power of two");
 
 	extern __shared__ float2 lds[];
-	float2* tw = lds + TOT;
+	float2* tw = lds + P * PN;
 
 	if (nullptr != L.skip && 0 != *L.skip)
 		return;
@@ -495,7 +497,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 			const int p = i / N;
 			const int k = i - p * N;
 			const long t = t0 + p;
-			lds[i] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+			lds[p * PN + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
 		}
 
 	} else {
@@ -505,7 +507,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 			const int p = i & (P - 1);
 			const int k = i >> LOGP;
 			const long t = t0 + p;
-			lds[p * N + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+			lds[p * PN + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
 		}
 	}
 
@@ -530,7 +532,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 			const long t = t0 + p;
 
 			if (t < L.total)
-				lds[p * N + k] = md(t, k, lds[p * N + k]);
+				lds[p * PN + k] = md(t, k, lds[p * PN + k]);
 		}
 
 		__syncthreads();
@@ -658,7 +660,7 @@ __device__ inline void store_each(const float2* lds, const FftLaunch& L, long t0
 		long t = t0 + p;
 
 		if (t < L.total)
-			f(t, k, lds[p * n + k]);
+			f(t, k, lds[p * L.pitch + k]);
 	}
 }
 

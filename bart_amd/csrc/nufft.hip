@@ -440,7 +440,7 @@ struct StReduceS {
 				float2 acc = make_float2(0.f, 0.f);
 
 				for (int j = 0; j < S; j++)
-					acc = cadd(acc, cmulc(lds[(gi * SC + cl * S + j) * n + k], lin_val(T, j, px)));
+					acc = cadd(acc, cmulc(lds[(gi * SC + cl * S + j) * L.pitch + k], lin_val(T, j, px)));
 
 				if (roll)
 					acc = cscale(acc, r);
