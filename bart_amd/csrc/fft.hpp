@@ -15,6 +15,8 @@
 // a radix-R DFT and writes back in auto-sorted order.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "prof.hpp"
 
@@ -269,7 +271,23 @@ __device__ inline void run_passes(float2* lds, const AxisPlan& plan, int P, floa
 // Identity middle stage.
 struct NoMid {
 	static constexpr bool active = false;
+	static constexpr bool colblock = true;
 	__device__ float2 operator()(long, int, float2 v) const { return v; }
+	__device__ bool cb_ok(int) const { return true; }
+	__device__ int base(long) const { return 0; }
+	__device__ float2 at(int, int, int, float2 v) const { return v; }
+};
+
+// functors with column-block access (static constexpr bool colblock = true;
+// cb_ok(P), base(t0), at(base, p, k [, v]))
+template <class F, class = void>
+struct colblock_of {
+	static constexpr bool value = false;
+};
+
+template <class F>
+struct colblock_of<F, std::void_t<decltype(F::colblock)>> {
+	static constexpr bool value = F::colblock;
 };
 
 // ---------------------------------------------------------------------------
@@ -487,6 +505,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 
 	const long t0 = (long)blockIdx.x * P;
 
+	// strided axis with column-block functors: per-block base addresses
+	constexpr bool CBF = colblock_of<Ld>::value && colblock_of<Md>::value && colblock_of<St>::value;
+	bool cb = false;
+
+	if constexpr (CBF)
+		cb = !L.contig && ld.cb_ok(P) && md.cb_ok(P) && st.cb_ok(P);
+
 	for (int i = threadIdx.x; i < N; i += FFT_THREADS)
 		tw[i] = L.plan.tw[i];
 
@@ -502,12 +527,31 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 
 	} else {
 
+		bool done = false;
+
+		if constexpr (CBF) {
+			if (cb) {
+				const auto lb = ld.base(t0);
+
 #pragma unroll 4
-		for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
-			const int p = i & (P - 1);
-			const int k = i >> LOGP;
-			const long t = t0 + p;
-			lds[p * PN + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+				for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+					const int p = i & (P - 1);
+					const int k = i >> LOGP;
+					lds[p * PN + k] = (t0 + p < L.total) ? ld.at(lb, p, k) : make_float2(0.f, 0.f);
+				}
+
+				done = true;
+			}
+		}
+
+		if (!done) {
+#pragma unroll 4
+			for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+				const int p = i & (P - 1);
+				const int k = i >> LOGP;
+				const long t = t0 + p;
+				lds[p * PN + k] = (t < L.total) ? ld(t, k) : make_float2(0.f, 0.f);
+			}
 		}
 	}
 
@@ -517,27 +561,62 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 
 	if constexpr (Md::active) {
 
-		for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+		bool done = false;
 
-			int p, k;
+		if constexpr (CBF) {
+			if (cb) {
+				const auto mb = md.base(t0);
 
-			if (L.contig) {
-				p = i / N;
-				k = i - p * N;
-			} else {
-				p = i & (P - 1);
-				k = i >> LOGP;
+				for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+					const int p = i & (P - 1);
+					const int k = i >> LOGP;
+					if (t0 + p < L.total)
+						lds[p * PN + k] = md.at(mb, p, k, lds[p * PN + k]);
+				}
+
+				done = true;
 			}
+		}
 
-			const long t = t0 + p;
+		if (!done) {
 
-			if (t < L.total)
-				lds[p * PN + k] = md(t, k, lds[p * PN + k]);
+			for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+
+				int p, k;
+
+				if (L.contig) {
+					p = i / N;
+					k = i - p * N;
+				} else {
+					p = i & (P - 1);
+					k = i >> LOGP;
+				}
+
+				const long t = t0 + p;
+
+				if (t < L.total)
+					lds[p * PN + k] = md(t, k, lds[p * PN + k]);
+			}
 		}
 
 		__syncthreads();
 
 		ct_passes<N, P, 0, 1>(lds, tw, L.sign2);
+	}
+
+	if constexpr (CBF) {
+		if (cb) {
+			const auto sb = st.base(t0);
+
+			for (int i = threadIdx.x; i < TOT; i += FFT_THREADS) {
+				const int p = i & (P - 1);
+				const int k = i >> LOGP;
+				if (t0 + p < L.total)
+					st.at(sb, p, k, lds[p * PN + k]);
+			}
+
+			return;
+		}
 	}
 
 	st.store(lds, L, t0);

@@ -323,7 +323,27 @@ struct StGridS {
 };
 
 // Standard-order grid loads / stores for the other axes
+// Column-block access (used by the fixed-length kernel on strided axes):
+// the P transforms of a block are P adjacent columns of one volume when P
+// divides the transforms below the axis, so element (p, k) of the block is
+// at base + p + inner k.
+struct ColBase {
+	long off;     // offset of element (0, 0)
+	uint32_t i0;  // index below the axis of transform 0
+	uint32_t o;   // index above the axis
+	uint32_t v;   // volume
+};
+
+__device__ inline ColBase col_base(const VolAxis& g, long t0)
+{
+	ColBase b;
+	g.split(t0, b.i0, b.o, b.v);
+	b.off = g.offset_iov(b.i0, b.o, b.v, 0);
+	return b;
+}
+
 struct LdGrid {
+	static constexpr bool colblock = true;
 	VolAxis g;
 	SepTables T;
 	const float2* grid;
@@ -343,9 +363,26 @@ struct LdGrid {
 
 		return x;
 	}
+
+	__device__ bool cb_ok(int P) const { return 0 == g.inner % (uint32_t)P; }
+	__device__ ColBase base(long t0) const { return col_base(g, t0); }
+
+	__device__ float2 at(const ColBase& b, int p, int k) const
+	{
+		float2 x = grid[b.off + p + (long)g.inner * k];
+
+		if (conj_fm) {
+			int px[3];
+			g.coords(b.i0 + p, b.o, k, px);
+			x = cmulc(x, fm_val(T, px));
+		}
+
+		return x;
+	}
 };
 
 struct StGrid {
+	static constexpr bool colblock = true;
 	VolAxis g;
 	SepTables T;
 	float2* grid;
@@ -364,12 +401,27 @@ struct StGrid {
 			grid[g.offset_iov(i, o, v, k)] = x;
 		});
 	}
+
+	__device__ bool cb_ok(int P) const { return 0 == g.inner % (uint32_t)P; }
+	__device__ ColBase base(long t0) const { return col_base(g, t0); }
+
+	__device__ void at(const ColBase& b, int p, int k, float2 x) const
+	{
+		if (fm) {
+			int px[3];
+			g.coords(b.i0 + p, b.o, k, px);
+			x = cmul(x, fm_val(T, px));
+		}
+
+		grid[b.off + p + (long)g.inner * k] = x;
+	}
 };
 
 // PSF multiply between forward and inverse transform (standard order,
 // volumes v = c + C s)
 struct MdPsf {
 	static constexpr bool active = true;
+	static constexpr bool colblock = true;
 	VolAxis g;
 	int C;
 	const float2* psf;
@@ -381,6 +433,21 @@ struct MdPsf {
 		uint32_t s = v / (uint32_t)C;
 		long off = g.offset_iov(i, o, 0, k);
 		return cmul(x, psf[(long)s * g.N + off]);
+	}
+
+	__device__ bool cb_ok(int P) const { return 0 == g.inner % (uint32_t)P; }
+
+	// base: the PSF offset of element (0, 0) (sub-grid v / C)
+	__device__ ColBase base(long t0) const
+	{
+		ColBase b = col_base(g, t0);
+		b.off = (long)(b.v / (uint32_t)C) * g.N + g.offset_iov(b.i0, b.o, 0, 0);
+		return b;
+	}
+
+	__device__ float2 at(const ColBase& b, int p, int k, float2 x) const
+	{
+		return cmul(x, psf[b.off + p + (long)g.inner * k]);
 	}
 };
 
