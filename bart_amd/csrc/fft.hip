@@ -200,7 +200,7 @@ __global__ __launch_bounds__((1024 / NX) * NX) void k_slice_plain(const float2* 
 	for (int y = y0; y < NX; y += DY)
 		lds[y * LD + x] = src[off + y * NX + x];
 
-	__syncthreads();
+	lds_barrier();
 
 	slice_passes<NX, NX, 1, LD, NTH>(lds, tw, sign);
 	slice_passes<NX, NX, LD, 1, NTH>(lds, tw, sign);

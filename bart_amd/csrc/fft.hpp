@@ -223,7 +223,7 @@ __device__ inline void stockham_pass(float2* lds, int n, int P, int Ns, const fl
 		}
 	}
 
-	__syncthreads();
+	lds_barrier();
 
 #pragma unroll
 	for (int i = 0; i < NB; i++)
@@ -232,7 +232,7 @@ __device__ inline void stockham_pass(float2* lds, int n, int P, int Ns, const fl
 			for (int q = 0; q < R; q++)
 				lds[dst[i] + q * Ns] = v[i][q];
 
-	__syncthreads();
+	lds_barrier();
 }
 
 // GEN: also instantiate the generic odd-prime radices (7, 11, 13), which
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftLaunch L, Ld ld, Md
 		}
 	}
 
-	__syncthreads();
+	lds_barrier();
 
 	run_passes<EPT, GEN>(lds, L.plan, P, L.sign1);
 
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis(FftLaunch L, Ld ld, Md
 				lds[p * n + k] = md(t, k, lds[p * n + k]);
 		}
 
-		__syncthreads();
+		lds_barrier();
 
 		run_passes<EPT, GEN>(lds, L.plan, P, L.sign2);
 	}
@@ -466,7 +466,7 @@ __device__ inline void ct_pass(float2* lds, const float2* tw, float sgn)
 		}
 	}
 
-	__syncthreads();
+	lds_barrier();
 
 #pragma unroll
 	for (int i = 0; i < PER; i++)
@@ -475,7 +475,7 @@ __device__ inline void ct_pass(float2* lds, const float2* tw, float sgn)
 			for (int q = 0; q < R; q++)
 				lds[dst[i] + q * NS] = v[i][q];
 
-	__syncthreads();
+	lds_barrier();
 }
 
 template <int N, int P, int IDX, int NS>
@@ -555,7 +555,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 		}
 	}
 
-	__syncthreads();
+	lds_barrier();
 
 	ct_passes<N, P, 0, 1>(lds, tw, L.sign1);
 
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 			}
 		}
 
-		__syncthreads();
+		lds_barrier();
 
 		ct_passes<N, P, 0, 1>(lds, tw, L.sign2);
 	}
@@ -682,7 +682,7 @@ __device__ inline void slice_pass(float2* lds, const float2* tw, float sgn)
 		__builtin_amdgcn_sched_barrier(0);
 	}
 
-	__syncthreads();
+	lds_barrier();
 
 #pragma unroll
 	for (int i = 0; i < PER; i++)
@@ -691,7 +691,7 @@ __device__ inline void slice_pass(float2* lds, const float2* tw, float sgn)
 			for (int q = 0; q < R; q++)
 				lds[dst[i] + q * NS * ES] = v[i][q];
 
-	__syncthreads();
+	lds_barrier();
 }
 
 template <int N, int P, int ES, int TS, int NTH, int IDX = 0, int NS = 1>

@@ -614,18 +614,6 @@ __device__ inline X sel3(int d, X a, X b, X c)
 	return (0 == d) ? a : ((1 == d) ? b : c);
 }
 
-// Work-group barrier that orders LDS only: waits for this wave's LDS
-// operations (lgkmcnt(0)) but not for its vector memory loads, so that
-// loads of the next batch stay in flight across it (__syncthreads() waits
-// for them as part of its fence).
-__device__ inline void lds_barrier()
-{
-	__builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-	__atomic_signal_fence(__ATOMIC_SEQ_CST);
-	__builtin_amdgcn_s_barrier();
-	__atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-
 __global__ void k_reset_queue(int* q)
 {
 	if (0 == threadIdx.x)

@@ -651,7 +651,7 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 		int xs = x, ys = y0;
 		asm volatile("" : "+v"(xs), "+v"(ys));
 
-		__syncthreads();
+		lds_barrier();
 
 #pragma unroll
 		for (int i = 0; i < EPT; i++) {
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 				lds[y * LD + xs] = cmul(vin[i], lin_xyz(T, s, xs, y, z));
 		}
 
-		__syncthreads();
+		lds_barrier();
 
 		slice_passes<NX, NY, 1, LD, NTH>(lds, twx, -1.f);
 		slice_passes<NY, NX, LD, 1, NTH>(lds, twy, -1.f);
@@ -732,7 +732,7 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 		int xs = x, ys = y0;
 		asm volatile("" : "+v"(xs), "+v"(ys));
 
-		__syncthreads();
+		lds_barrier();
 
 #pragma unroll 4
 		for (int i = 0; i < EPT; i++) {
@@ -743,7 +743,7 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 				lds[y * LD + xs] = in[y * NX + xs];
 		}
 
-		__syncthreads();
+		lds_barrier();
 
 		slice_passes<NY, NX, LD, 1, NTH>(lds, twy, +1.f);
 		slice_passes<NX, NY, 1, LD, NTH>(lds, twx, +1.f);
