@@ -148,6 +148,17 @@ static GridTiling choose_tiling(const GridGeom& g)
 	}
 
 	BAM_CHECK(e <= 64 && t.nw <= 16 && g.S * t.b[0] * t.b[1] * t.b[2] <= 64, "gridding: bad tiling");
+
+	auto ilog2 = [](int v) {
+		int l = 0;
+		while ((1 << l) < v)
+			l++;
+		BAM_CHECK((1 << l) == v, "gridding: block dims must be powers of two");
+		return l;
+	};
+	t.lb0 = ilog2(t.b[0]);
+	t.lb01 = t.lb0 + ilog2(t.b[1]);
+	t.lb = t.lb01 + ilog2(t.b[2]);
 	t.cc = (g.C <= 4) ? 4 : ((g.C <= 8) ? 8 : 16);
 
 	// cell rows visited by one work-group must fit the range list
@@ -656,6 +667,7 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 	float* wtab = (float*)(sidx + ADJ_MAXV);              // [ADJ_BATCH][ADJ_WROW] tap weights per visit
 	unsigned* rows = (unsigned*)(wtab + ADJ_WROW * ADJ_BATCH);  // [ADJ_BATCH][6] tile rows hit per dim, variant
 	int* qitem = (int*)(rows + 6 * ADJ_BATCH);
+	float2* fml = (float2*)(qitem + 4);                   // [n0 + n1 + n2] fftmod tables (when t.fmlds)
 
 	const int nthr = blockDim.x;
 	const int lane = threadIdx.x & 63;
@@ -670,6 +682,18 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 
 	if (0 == threadIdx.x)
 		qitem[1] = 0;
+
+	const bool fm = (nullptr != g.fm[0]);
+	const bool fmlds = fm && (0 != t.fmlds);
+
+	if (fmlds) {
+		for (int i = threadIdx.x; i < g.n[0]; i += nthr)
+			fml[i] = g.fm[0][i];
+		for (int i = threadIdx.x; i < g.n[1]; i += nthr)
+			fml[g.n[0] + i] = g.fm[1][i];
+		for (int i = threadIdx.x; i < g.n[2]; i += nthr)
+			fml[g.n[0] + g.n[1] + i] = g.fm[2][i];
+	}
 
 	// the wave's block; C rows this lane feeds with weights: l % 32 and 32 + l % 32
 	const int B = t.b[0] * t.b[1] * t.b[2];
@@ -936,11 +960,11 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 				const float x = (0 == h) ? C1[r] : C2[r];
 				const float y = __shfl_xor(x, 1);
 				const int i = 32 * h + 8 * (r / 4) + 4 * lhalf + (r % 4);
-				const int rs = i / B;
-				const int rc = i - rs * B;
-				const int q0 = wb[0] * t.b[0] + rc % t.b[0];
-				const int q1 = wb[1] * t.b[1] + (rc / t.b[0]) % t.b[1];
-				const int q2 = wb[2] * t.b[2] + rc / (t.b[0] * t.b[1]);
+				const int rs = i >> t.lb;
+				const int rc = i & (B - 1);
+				const int q0 = wb[0] * t.b[0] + (rc & (t.b[0] - 1));
+				const int q1 = wb[1] * t.b[1] + ((rc >> t.lb0) & (t.b[1] - 1));
+				const int q2 = wb[2] * t.b[2] + (rc >> t.lb01);
 				const int x0 = t0[0] + q0, x1 = t0[1] + q1, x2 = t0[2] + q2;
 
 				if (0 != (lane & 1) || i >= g.S * B || c >= ncc || x0 >= g.n[0] || x1 >= g.n[1] ||
@@ -951,7 +975,16 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 
 				if (0 == wk.w) {
 					const long idx = x0 + (long)g.n[0] * (x1 + (long)g.n[1] * x2);
-					grid[((long)rs * g.C + c0 + c) * g.N + idx] = apply_conj_fm(g, val, x0, x1, x2);
+					float2 o = val;
+					if (fmlds) {
+						float2 f = fml[x0];
+						f = cmul(f, fml[g.n[0] + x1]);
+						f = cmul(f, fml[g.n[0] + g.n[1] + x2]);
+						o = cmulc(val, f);
+					} else {
+						o = apply_conj_fm(g, val, x0, x1, x2);
+					}
+					grid[((long)rs * g.C + c0 + c) * g.N + idx] = o;
 				} else {
 					// tile split over several work items (dense region):
 					// the partial tile goes to its own slot, summed by
@@ -1026,14 +1059,21 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 	const int nch = (g.C + t.cc - 1) / t.cc;
 	const int nitems = gp.nwork * nch;
 	const unsigned nwg = (unsigned)std::max(1, std::min(nitems, 2 * gp.ncu));
-	const size_t lds = SMEM_DATA + sizeof(float4) * ADJ_MAXV + sizeof(float2) * ADJ_BATCH * t.cc +
-			   sizeof(int) * ADJ_MAXV + sizeof(float) * ADJ_WROW * ADJ_BATCH + sizeof(unsigned) * 6 * ADJ_BATCH +
-			   sizeof(int) * 4;
+	size_t lds = SMEM_DATA + sizeof(float4) * ADJ_MAXV + sizeof(float2) * ADJ_BATCH * t.cc +
+		     sizeof(int) * ADJ_MAXV + sizeof(float) * ADJ_WROW * ADJ_BATCH + sizeof(unsigned) * 6 * ADJ_BATCH +
+		     sizeof(int) * 4;
+
+	// fftmod tables in LDS when they fit (160 KiB per work-group)
+	GridTiling tl = t;
+	const size_t fmb = sizeof(float2) * ((size_t)g.n[0] + g.n[1] + g.n[2]);
+	tl.fmlds = (nullptr != g.fm[0] && lds + fmb <= 160 * 1024) ? 1 : 0;
+	if (tl.fmlds)
+		lds += fmb;
 
 	hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, st, gp.queue.ptr);
 
 #define BAM_ADJ(TP, CC)                                                                                     \
-	hipLaunchKernelGGL((k_grid_adj<TP, CC>), dim3(nwg), dim3(nthr), lds, st, g, t, kb, gp.sk.ptr, gp.cstart.ptr, \
+	hipLaunchKernelGGL((k_grid_adj<TP, CC>), dim3(nwg), dim3(nthr), lds, st, g, tl, kb, gp.sk.ptr, gp.cstart.ptr, \
 			   gp.work.ptr, nitems, gp.queue.ptr, gp.kspS.ptr, grid, gp.spart.ptr)
 
 	if (4 == tp) {

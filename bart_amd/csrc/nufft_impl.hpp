@@ -63,6 +63,8 @@ struct GridTiling {
 	int nw;    // waves per work-group
 	int eoff[3];
 	int nwork; // work items per coil chunk
+	int lb, lb0, lb01;  // log2 of b0 b1 b2, b0 and b0 b1 (all powers of two)
+	int fmlds;          // fftmod tables staged in LDS by the adjoint (they fit)
 };
 
 struct GridPlan {
