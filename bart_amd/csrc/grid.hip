@@ -677,9 +677,6 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 	for (int i = threadIdx.x; i < KB_SIZE + 1; i += nthr)
 		table[i] = kbt[i];
 
-	for (int v = threadIdx.x; v < ADJ_BATCH; v += nthr)
-		wtab[v * ADJ_WROW + 64] = 1.f;
-
 	if (0 == threadIdx.x)
 		qitem[1] = 0;
 
@@ -733,6 +730,11 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 
 		if (item >= nitems)
 			break;
+
+		// the constant-1 weight entry of every row (the output store of the
+		// previous item reused this memory)
+		for (int v = threadIdx.x; v < ADJ_BATCH; v += nthr)
+			wtab[v * ADJ_WROW + 64] = 1.f;
 
 		const int ch = item / nwork;
 		const int4 wk = work[item - ch * nwork];  // (tile, first visit, end visit, partial slot + 1 or 0)
@@ -943,56 +945,79 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 			}
 		}
 
-		// C row i (of the wave's 64 outputs), column j = 2 coil + (re, im):
-		// register r of lane l holds i = 8 (r / 4) + 4 (l / 32) + r % 4 (+ 32
-		// in C2), j = l % 32.  (zero is 0, read from LDS here so that the
-		// row decoding below is not hoisted out of the item loop, where its
-		// 32 copies would stay live.)
-		const int zero = qitem[1];
-		const int c = (jcol + zero) >> 1;
-		const int lhalf = (lane + zero) >> 5;
+		// Store through LDS: the waves put their outputs into the tile
+		// buffer otile[s][c][point] (plane pitch Wtot + 1, spread over the
+		// banks; it reuses the batch buffers), then all lanes write the tile
+		// out row by row.  C row i (of the wave's 64 outputs), column
+		// j = 2 coil + (re, im): register r of lane l holds
+		// i = 8 (r / 4) + 4 (l / 32) + r % 4 (+ 32 in C2), j = l % 32.
+		const int OP = t.Wtot + 1;
+		float* otile = (float*)skall;
+
+		lds_barrier();
+
+		{
+			// (zero is 0, read from LDS here so that the row decoding is
+			// not hoisted out of the item loop, where its 32 copies would
+			// stay live)
+			const int zero = qitem[1];
+			const int c = (jcol + zero) >> 1;
+			const int lhalf = (lane + zero) >> 5;
 
 #pragma unroll
-		for (int h = 0; h < 2; h++)
+			for (int h = 0; h < 2; h++)
 #pragma unroll
-			for (int r = 0; r < 16; r++) {
+				for (int r = 0; r < 16; r++) {
 
-				const float x = (0 == h) ? C1[r] : C2[r];
-				const float y = __shfl_xor(x, 1);
-				const int i = 32 * h + 8 * (r / 4) + 4 * lhalf + (r % 4);
-				const int rs = i >> t.lb;
-				const int rc = i & (B - 1);
-				const int q0 = wb[0] * t.b[0] + (rc & (t.b[0] - 1));
-				const int q1 = wb[1] * t.b[1] + ((rc >> t.lb0) & (t.b[1] - 1));
-				const int q2 = wb[2] * t.b[2] + (rc >> t.lb01);
-				const int x0 = t0[0] + q0, x1 = t0[1] + q1, x2 = t0[2] + q2;
+					const int i = 32 * h + 8 * (r / 4) + 4 * lhalf + (r % 4);
+					const int rs = i >> t.lb;
+					const int rc = i & (B - 1);
+					const int q0 = wb[0] * t.b[0] + (rc & (t.b[0] - 1));
+					const int q1 = wb[1] * t.b[1] + ((rc >> t.lb0) & (t.b[1] - 1));
+					const int q2 = wb[2] * t.b[2] + (rc >> t.lb01);
 
-				if (0 != (lane & 1) || i >= g.S * B || c >= ncc || x0 >= g.n[0] || x1 >= g.n[1] ||
-				    x2 >= g.n[2])
-					continue;
-
-				const float2 val = make_float2(x, y);
-
-				if (0 == wk.w) {
-					const long idx = x0 + (long)g.n[0] * (x1 + (long)g.n[1] * x2);
-					float2 o = val;
-					if (fmlds) {
-						float2 f = fml[x0];
-						f = cmul(f, fml[g.n[0] + x1]);
-						f = cmul(f, fml[g.n[0] + g.n[1] + x2]);
-						o = cmulc(val, f);
-					} else {
-						o = apply_conj_fm(g, val, x0, x1, x2);
-					}
-					grid[((long)rs * g.C + c0 + c) * g.N + idx] = o;
-				} else {
-					// tile split over several work items (dense region):
-					// the partial tile goes to its own slot, summed by
-					// k_sum_split
-					const int pq = q0 + t.W[0] * (q1 + t.W[1] * q2);
-					spart[(((long)(wk.w - 1) * g.S + rs) * g.C + c0 + c) * t.Wtot + pq] = val;
+					if (i < g.S * B && c < CC)
+						otile[(((long)rs * CC + c) * OP + q0 + t.W[0] * (q1 + t.W[1] * q2)) * 2 + (lane & 1)] =
+							(0 == h) ? C1[r] : C2[r];
 				}
+		}
+
+		lds_barrier();
+
+		for (int e = threadIdx.x; e < g.S * CC * t.Wtot; e += nthr) {
+
+			const int pq = e % t.Wtot;
+			const int rest = e / t.Wtot;
+			const int c = rest % CC;
+			const int rs = rest / CC;
+			const int x0 = t0[0] + pq % t.W[0];
+			const int x1 = t0[1] + (pq / t.W[0]) % t.W[1];
+			const int x2 = t0[2] + pq / (t.W[0] * t.W[1]);
+
+			if (c >= ncc || x0 >= g.n[0] || x1 >= g.n[1] || x2 >= g.n[2])
+				continue;
+
+			const float* op = otile + (((long)rs * CC + c) * OP + pq) * 2;
+			const float2 val = make_float2(op[0], op[1]);
+
+			if (0 == wk.w) {
+				const long idx = x0 + (long)g.n[0] * (x1 + (long)g.n[1] * x2);
+				float2 o = val;
+				if (fmlds) {
+					float2 f = fml[x0];
+					f = cmul(f, fml[g.n[0] + x1]);
+					f = cmul(f, fml[g.n[0] + g.n[1] + x2]);
+					o = cmulc(val, f);
+				} else {
+					o = apply_conj_fm(g, val, x0, x1, x2);
+				}
+				grid[((long)rs * g.C + c0 + c) * g.N + idx] = o;
+			} else {
+				// tile split over several work items (dense region): the
+				// partial tile goes to its own slot, summed by k_sum_split
+				spart[(((long)(wk.w - 1) * g.S + rs) * g.C + c0 + c) * t.Wtot + pq] = val;
 			}
+		}
 	}
 }
 
@@ -1062,6 +1087,12 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 	size_t lds = SMEM_DATA + sizeof(float4) * ADJ_MAXV + sizeof(float2) * ADJ_BATCH * t.cc +
 		     sizeof(int) * ADJ_MAXV + sizeof(float) * ADJ_WROW * ADJ_BATCH + sizeof(unsigned) * 6 * ADJ_BATCH +
 		     sizeof(int) * 4;
+
+	// the output tile (plane pitch Wtot + 1) reuses the batch buffers
+	BAM_CHECK((size_t)g.S * t.cc * (t.Wtot + 1) * sizeof(float2) <=
+			  sizeof(float4) * ADJ_MAXV + sizeof(float2) * ADJ_BATCH * t.cc + sizeof(int) * ADJ_MAXV +
+				  sizeof(float) * ADJ_WROW * ADJ_BATCH,
+		  "gridding: output tile larger than the batch buffers");
 
 	// fftmod tables in LDS when they fit (160 KiB per work-group)
 	GridTiling tl = t;

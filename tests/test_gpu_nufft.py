@@ -151,6 +151,20 @@ def test_nufft_2d_dense_adjoint():
     assert relerr(op.adjoint(ksp).reshape(C, 1, n, n), ref.adjoint(ksp)) < 1e-5
 
 
+def test_nufft_2d_adjoint_many_work_items():
+    # 448^2: 784 tiles x 2 coil chunks (20 coils = 16 + 4), more work items
+    # than the persistent work-groups of the adjoint gridding, so that each
+    # work-group runs several items in a row; repeated calls are bitwise equal
+    n, C, R, S = 448, 20, 896, 41
+    traj = I.radial_traj(R, S)
+    ksp = crandn((C, R * S), 65)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=False)
+    op = make_nufft((n, n, 1), C, traj, R, S, toeplitz=False)
+    a = op.adjoint(ksp)
+    assert relerr(a.reshape(C, 1, n, n), ref.adjoint(ksp)) < 1e-5
+    assert np.array_equal(a, op.adjoint(ksp))
+
+
 def test_nufft_nonsquare_and_1d():
     traj = I.radial_traj(64, 9)
     traj[:, 1] *= 0.5
