@@ -86,6 +86,24 @@ static double rolloff_1d(int d, int i, float os, double beta, double width)
 	return 1. / ft / width;
 }
 
+// x-y product tables, computed on the device with the same cmul as the
+// per-element products they replace (bitwise equal)
+__global__ void k_xy_tables(SepTables T, float2* out)
+{
+	const long nxy = (long)T.n[0] * T.n[1];
+
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nxy; e += (long)gridDim.x * blockDim.x) {
+
+		const int x = (int)(e % T.n[0]);
+		const int y = (int)(e / T.n[0]);
+
+		for (int b = 0; b < 4; b++)
+			out[b * nxy + e] = cmul(T.lin[0][b & 1][x], T.lin[1][b >> 1][y]);
+
+		out[4 * nxy + e] = cmul(T.fm[0][x], T.fm[1][y]);
+	}
+}
+
 static void build_tables(Nufft& p)
 {
 	long tot = 0;
@@ -150,6 +168,24 @@ static void build_tables(Nufft& p)
 		p.tabs.lin[d][1] = p.tab2.ptr + o2[d] + p.n[d];
 		p.tabs.fm[d] = p.tab2.ptr + o2[d] + 2 * p.n[d];
 		p.tabs.roll[d] = p.tabr.ptr + orr[d];
+	}
+
+	for (int b = 0; b < 4; b++)
+		p.tabs.linxy[b] = nullptr;
+	p.tabs.fmxy = nullptr;
+
+	if (p.n[0] > 1 && p.n[1] > 1 && p.n[2] > 1) {
+
+		const long nxy = (long)p.n[0] * p.n[1];
+		p.tabxy.alloc(5 * nxy);
+		hipLaunchKernelGGL(k_xy_tables, dim3((unsigned)std::min<long>((nxy + 255) / 256, 4096)), dim3(256), 0,
+				   p.stream, p.tabs, p.tabxy.ptr);
+		BAM_HIP(hipGetLastError());
+		BAM_HIP(hipStreamSynchronize(p.stream));
+
+		for (int b = 0; b < 4; b++)
+			p.tabs.linxy[b] = p.tabxy.ptr + b * nxy;
+		p.tabs.fmxy = p.tabxy.ptr + 4 * nxy;
 	}
 }
 
@@ -408,9 +444,14 @@ struct StGrid {
 	__device__ void at(const ColBase& b, int p, int k, float2 x) const
 	{
 		if (fm) {
-			int px[3];
-			g.coords(b.i0 + p, b.o, k, px);
-			x = cmul(x, fm_val(T, px));
+			if (2 == g.axis && nullptr != T.fmxy) {
+				// fm_val = ((fm0 fm1) fm2), the x-y product from its table
+				x = cmul(x, cmul(T.fmxy[b.i0 + p], T.fm[2][k]));
+			} else {
+				int px[3];
+				g.coords(b.i0 + p, b.o, k, px);
+				x = cmul(x, fm_val(T, px));
+			}
 		}
 
 		grid[b.off + p + (long)g.inner * k] = x;
@@ -576,9 +617,10 @@ struct SliceLane {
 
 __device__ inline float2 lin_xyz(const SepTables& T, int s, int x, int y, int z)
 {
-	// same product order as lin_val: ((x) y) z
-	float2 v = T.lin[0][(s >> T.bitpos[0]) & 1][x];
-	v = cmul(v, T.lin[1][(s >> T.bitpos[1]) & 1][y]);
+	// same product order as lin_val: ((x) y) z; the x-y product from its
+	// table (3-D slice kernels)
+	const int bx = (s >> T.bitpos[0]) & 1, by = (s >> T.bitpos[1]) & 1;
+	const float2 v = T.linxy[bx + 2 * by][x + T.n[0] * y];
 	return cmul(v, T.lin[2][(s >> T.bitpos[2]) & 1][z]);
 }
 

@@ -100,6 +100,11 @@ struct SepTables {
 	const float* roll[3];
 	int bitpos[3];  // decomposition bit of each dim, -1 if the dim is trivial
 	int n[3];
+
+	// x-y products for 3-D grids (null otherwise), each [n1][n0]:
+	// linxy[bx + 2 by] = lin[0][bx] x lin[1][by], fmxy = fm[0] x fm[1]
+	const float2* linxy[4];
+	const float2* fmxy;
 };
 
 struct Nufft {
@@ -120,6 +125,7 @@ struct Nufft {
 	DevBuf<float> kb;
 	DevBuf<float2> tab2;   // separable complex tables
 	DevBuf<float> tabr;    // separable rolloff
+	DevBuf<float2> tabxy;  // x-y product tables (3-D)
 	SepTables tabs;
 	DevBuf<float2> weights;
 	bool has_weights = false;

@@ -325,12 +325,30 @@ def test_cgsense_matches_oracle():
     bg = op.adjoint(y).reshape(1, n, n)
     assert relerr(bg, b) < 1e-5
     lam = 0.01
+    # The oracle's adjoint gridding adds with OpenMP atomics (as BART's CPU
+    # gridding, noncart/grid.c:482-485), so its sums vary in the last bits
+    # from run to run, and FP32 CG amplifies that with the iteration count
+    # and the GPU and CPU operators round differently: on this problem the
+    # error vs the oracle grows about tenfold every two iterations (measured
+    # over four runs: 4 it 7e-7, 6 it 5e-6 .. 1e-5, 8 it 6e-5 .. 1.2e-4,
+    # 30 it 2e-5 .. 2e-4).  The solution is compared after 6 iterations; the
+    # 30-iteration one by the residual it reaches.
+    xr6, _, _, _ = ref.cgsense(maps, b, lam, 6, 0.0)
+    x6, _, _ = ba.lsqr_conjgrad(op, b, 6, lam, 0.0)
+    err6 = relerr(x6.reshape(1, n, n), xr6)
     xr, itr, trr, _ = ref.cgsense(maps, b, lam, 30, 0.0)
     x, st, tr = ba.lsqr_conjgrad(op, b, 30, lam, 0.0)
     assert st.iterations == itr == 30
-    err = relerr(x.reshape(1, n, n), xr)
-    print(f"cg-sense 30 it: rel err vs oracle {err:.2e}")
-    assert err < 1e-4
+
+    def resid(v):
+        v = v.reshape(1, n, n).astype(np.complex64)
+        r = b.astype(np.complex128) - ref.sense_normal(maps, v) - lam * v
+        return np.linalg.norm(r) / np.linalg.norm(b)
+
+    print(f"cg-sense: 6 it rel err vs oracle {err6:.2e}; 30 it residual {resid(x):.2e} (oracle {resid(xr):.2e}),"
+          f" rel err {relerr(x.reshape(1, n, n), xr):.2e}")
+    assert err6 < 1e-4
+    assert resid(x) < 1.5 * resid(xr)
     # the recursively updated residual norms of FP32 CG follow each other
     # closely only for the first iterations of an ill-conditioned problem
     assert np.max(np.abs(tr[:6] / trr[:6] - 1)) < 1e-4
