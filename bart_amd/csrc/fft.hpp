@@ -9,7 +9,7 @@
 // Convention (FFTW's, as used by BART): unnormalised, sign -1 forward,
 // +1 backward.
 //
-// One work-group (256 threads = 4 wave64) holds P transforms of length n in
+// One work-group (512 threads = 8 wave64) holds P transforms of length n in
 // LDS (P * n <= 8192 complex).  Each Stockham pass reads R strided values per
 // butterfly into registers, applies twiddles from an FP64-built table, does
 // a radix-R DFT and writes back in auto-sorted order.
@@ -22,7 +22,7 @@
 
 namespace bam {
 
-constexpr int FFT_THREADS = 256;
+constexpr int FFT_THREADS = 512;
 constexpr int FFT_MAX_ELEMS = 8192;
 constexpr int FFT_MAX_PASSES = 16;
 
