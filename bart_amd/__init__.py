@@ -142,6 +142,10 @@ def lib():
     L.lsqr_conjgrad.restype = ctypes.c_int
     L.lsqr_conjgrad.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, lp, vp, vp,
                                 ctypes.c_bool, ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
+    L.lsqr_conjgrad_batch.restype = ctypes.c_int
+    L.lsqr_conjgrad_batch.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_bool,
+                                      ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
     L.bart_amd_set_error_mode.argtypes = [ctypes.c_int]
     L.bart_amd_last_error.restype = ctypes.c_char_p
     L.bart_amd_clear_error.restype = None
@@ -339,6 +343,31 @@ def lsqr_conjgrad(op, b, maxiter, lam=0.0, tol=0.0, x0=None, fallback=False):
                              trace.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
     _check()
     return x, st, trace[:max(0, it)]
+
+
+def lsqr_conjgrad_batch(ops, bs, maxiter, lam=0.0, tol=0.0, x0s=None, fallback=False):
+    """Independent CG solves (A_i^H A_i + lam I) x_i = bs[i], one operator
+    each, run together on the device.  Returns (xs, stats, traces)."""
+    n = len(ops)
+    assert len(bs) == n and (x0s is None or len(x0s) == n)
+    bs = [_host_c64(b) for b in bs]
+    warm = x0s is not None
+    if warm:
+        xs = [x.clone() if _is_torch(x) else np.array(_host_c64(x), copy=True) for x in x0s]
+    else:
+        xs = [_like(b, op.idims) for b, op in zip(bs, ops)]
+    T = max(1, maxiter)
+    trace = np.zeros((max(1, n), T), np.float32)
+    stats = (CgStats * max(1, n))()
+    vp = ctypes.c_void_p
+    hs = (vp * max(1, n))(*[op.h for op in ops])
+    xp = (vp * max(1, n))(*[_ptr(x).value for x in xs])
+    bp = (vp * max(1, n))(*[_ptr(b).value for b in bs])
+    lib().lsqr_conjgrad_batch(n, hs, maxiter, lam, tol, DIMS, xp, bp, warm, fallback,
+                              trace.ctypes.data_as(vp), stats)
+    _check()
+    st = [stats[i] for i in range(n)]
+    return xs, st, [trace[i, :max(0, st[i].iterations)] for i in range(n)]
 
 
 def fft(dims, flags, x, inverse=False):

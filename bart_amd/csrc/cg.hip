@@ -1,26 +1,33 @@
-// cg.hip -- conjugate gradients on (A^H A + lambda I) x = b, gfx950.
+// cg.hip -- conjugate gradients on (A^H A + lambda I) x = b, gfx950, for one
+// problem or a batch of independent problems.
 //
 // Arithmetic of iter/italgos.c:497-580 (conjgrad) with the stopping rule of
 // iter/iter2.c:227-270 (stop when ||r||^2 <= (tol ||b||)^2; tol = 0 runs
 // maxiter iterations) and the FP64 dot/norm accumulation of
-// num/vecops.c:89-124.  Differences in execution, not in maths:
+// num/vecops.c:89-124.  A batch packs the vectors of its problems into one
+// buffer each and keeps the scalars per problem, as conjgrad_batch
+// (iter/italgos.c:589-680, dot_bat/axpy_bat/xpay_bat of
+// num/gpukrnls_bat.cu) does; every problem stops by its own test, so each
+// one follows exactly the iterations of its single solve.  Differences in
+// execution, not in maths:
 //   * the scalars (rsold, alpha, beta, stop flags) stay on the device; the
 //     host does not read anything until the solve ends (BART's GPU path copies
 //     two scalars to the host every iteration, num/gpukrnls.cu:1847-1858);
-//   * reductions are two-stage and deterministic (per-block FP64 partials,
-//     then a fixed-order sum);
-//   * maxiter iterations -- normal operator and vector updates -- are captured
-//     once into a hipGraph and replayed; after convergence every kernel sees
-//     the device stop flag and returns immediately.
+//   * reductions are two-stage and deterministic (per-work-group FP64
+//     partials, then a fixed-order sum per problem);
+//   * one iteration -- normal operators and vector updates -- is captured
+//     once into a hipGraph and launched maxiter times; after a problem stops,
+//     its kernels see the device stop flag and return immediately.
 #include "cg.hpp"
 #include "prof.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace bam {
 
 constexpr int CG_THREADS = 256;
-constexpr int CG_BLOCKS = 512;
+constexpr int CG_BLOCKS = 512;   // work-groups per problem at most
 
 __device__ inline double block_sum(double v, double* sh)
 {
@@ -55,19 +62,40 @@ __device__ inline double sum_partials(const double* part, int n, double* sh)
 	return block_sum(v, sh);
 }
 
+// the element range of this work-group's problem: first element and stride
+struct SegIt {
+
+	long i0, step, len, off;
+};
+
+__device__ inline SegIt seg_it(const CgSeg& s)
+{
+	SegIt it;
+	it.i0 = (long)(blockIdx.x - s.blk0) * blockDim.x + threadIdx.x;
+	it.step = (long)s.nblk * blockDim.x;
+	it.len = s.len;
+	it.off = s.off;
+	return it;
+}
+
 // Ap += lambda p; partial sums of p . Ap (float products, FP64 accumulation)
-__global__ __launch_bounds__(CG_THREADS) void k_cg_apdot(long n, float lambda, const float* __restrict__ p,
+__global__ __launch_bounds__(CG_THREADS) void k_cg_apdot(const int* __restrict__ bprob, const CgSeg* __restrict__ seg,
+							 float lambda, const float* __restrict__ p,
 							 float* __restrict__ Ap, double* __restrict__ part,
 							 const CgState* st)
 {
 	__shared__ double sh[4];
+	const int q = bprob[blockIdx.x];
 
-	if (0 != st->done)
+	if (0 != st[q].done)
 		return;
 
+	const SegIt it = seg_it(seg[q]);
+	p += it.off;
+	Ap += it.off;
 	double acc = 0.;
 
-	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+	for (long i = it.i0; i < it.len; i += it.step) {
 
 		float pv = p[i];
 		float a = Ap[i] + lambda * pv;
@@ -81,41 +109,51 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_apdot(long n, float lambda, c
 		part[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(CG_THREADS) void k_cg_alpha(int nparts, const double* __restrict__ part, CgState* st)
+// one work-group per problem
+__global__ __launch_bounds__(CG_THREADS) void k_cg_alpha(const CgSeg* __restrict__ seg,
+							 const double* __restrict__ part, CgState* st)
 {
 	__shared__ double sh[4];
+	const int q = blockIdx.x;
 
-	if (0 != st->done)
+	if (0 != st[q].done)
 		return;
 
-	double s = sum_partials(part, nparts, sh);
+	double s = sum_partials(part + seg[q].blk0, seg[q].nblk, sh);
 
 	if (0 == threadIdx.x) {
 
 		float pAp = (float)s;
 
 		if (0.f == pAp) {
-			st->done = 3;     // breakdown: x unchanged (italgos.c:544-545)
+			st[q].done = 3;     // breakdown: x unchanged (italgos.c:544-545)
 		} else {
-			st->alpha = st->rsold / pAp;
+			st[q].alpha = st[q].rsold / pAp;
 		}
 	}
 }
 
 // x += alpha p; r -= alpha Ap; partial sums of r . r
-__global__ __launch_bounds__(CG_THREADS) void k_cg_xr(long n, float* __restrict__ x, float* __restrict__ r,
+__global__ __launch_bounds__(CG_THREADS) void k_cg_xr(const int* __restrict__ bprob, const CgSeg* __restrict__ seg,
+						      float* __restrict__ x, float* __restrict__ r,
 						      const float* __restrict__ p, const float* __restrict__ Ap,
 						      double* __restrict__ part, const CgState* st)
 {
 	__shared__ double sh[4];
+	const int q = bprob[blockIdx.x];
 
-	if (0 != st->done)
+	if (0 != st[q].done)
 		return;
 
-	const float alpha = st->alpha;
+	const float alpha = st[q].alpha;
+	const SegIt it = seg_it(seg[q]);
+	x += it.off;
+	r += it.off;
+	p += it.off;
+	Ap += it.off;
 	double acc = 0.;
 
-	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+	for (long i = it.i0; i < it.len; i += it.step) {
 
 		x[i] += alpha * p[i];
 		float rv = r[i] + (-alpha) * Ap[i];
@@ -129,60 +167,82 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_xr(long n, float* __restrict_
 		part[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(CG_THREADS) void k_cg_beta(int nparts, const double* __restrict__ part, CgState* st,
-							float* __restrict__ trace)
+// one work-group per problem; trace: [B][tstride] residual norms
+__global__ __launch_bounds__(CG_THREADS) void k_cg_beta(const CgSeg* __restrict__ seg,
+							const double* __restrict__ part, CgState* st,
+							float* __restrict__ trace, int tstride)
 {
 	__shared__ double sh[4];
+	const int q = blockIdx.x;
 
-	if (0 != st->done)
+	if (0 != st[q].done)
 		return;
 
-	double s = sum_partials(part, nparts, sh);
+	double s = sum_partials(part + seg[q].blk0, seg[q].nblk, sh);
 
 	if (0 == threadIdx.x) {
 
+		CgState* sq = st + q;
 		double nr = sqrt(s);
 		float rsnew = (float)(nr * nr);
-		int it = st->iter;
+		int it = sq->iter;
 
-		st->beta = rsnew / st->rsold;
-		st->rsold = rsnew;
-		st->rsnew = rsnew;
-		st->iter = it + 1;
+		sq->beta = rsnew / sq->rsold;
+		sq->rsold = rsnew;
+		sq->rsnew = rsnew;
+		sq->iter = it + 1;
 
-		if (nullptr != trace)
-			trace[it] = sqrtf(rsnew);
+		if (it < tstride)
+			trace[(long)q * tstride + it] = sqrtf(rsnew);
 
-		if (rsnew <= st->eps2)
-			st->done = 2;  // converged
-		else if (st->iter >= st->maxiter)
-			st->done = 1;  // iteration budget used
+		if (rsnew <= sq->eps2)
+			sq->done = 2;  // converged
+		else if (sq->iter >= sq->maxiter)
+			sq->done = 1;  // iteration budget used
 	}
 }
 
 // p = beta p + r
-__global__ __launch_bounds__(CG_THREADS) void k_cg_p(long n, float* __restrict__ p, const float* __restrict__ r,
+__global__ __launch_bounds__(CG_THREADS) void k_cg_p(const int* __restrict__ bprob, const CgSeg* __restrict__ seg,
+						     float* __restrict__ p, const float* __restrict__ r,
 						     const CgState* st)
 {
-	if (0 != st->done)
+	const int q = bprob[blockIdx.x];
+
+	if (0 != st[q].done)
 		return;
 
-	const float beta = st->beta;
+	const float beta = st[q].beta;
+	const SegIt it = seg_it(seg[q]);
+	p += it.off;
+	r += it.off;
 
-	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+	for (long i = it.i0; i < it.len; i += it.step)
 		p[i] = beta * p[i] + r[i];
 }
 
-// r = b - (A x + lambda x) [with ax = A x already in r]; p = r; ||r||^2 and ||b||^2 partials
-__global__ __launch_bounds__(CG_THREADS) void k_cg_init(long n, float lambda, const float* __restrict__ x,
+// r = b - (A x + lambda x) [with ax = A x already in r]; p = r; ||r||^2 and
+// ||b||^2 partials.  mask: only the problems with mask[q] != 0 (null: all).
+__global__ __launch_bounds__(CG_THREADS) void k_cg_init(const int* __restrict__ bprob, const CgSeg* __restrict__ seg,
+							float lambda, const float* __restrict__ x,
 							const float* __restrict__ b, float* __restrict__ r,
 							float* __restrict__ p, double* __restrict__ part,
-							double* __restrict__ part_b, bool cold)
+							double* __restrict__ part_b, bool cold, const int* mask)
 {
 	__shared__ double sh[4];
+	const int q = bprob[blockIdx.x];
+
+	if (nullptr != mask && 0 == mask[q])
+		return;
+
+	const SegIt it = seg_it(seg[q]);
+	x += it.off;
+	b += it.off;
+	r += it.off;
+	p += it.off;
 	double acc = 0., accb = 0.;
 
-	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+	for (long i = it.i0; i < it.len; i += it.step) {
 
 		float bv = b[i];
 		float rv;
@@ -210,61 +270,94 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_init(long n, float lambda, co
 	}
 }
 
-__global__ void k_cg_init_fin(int nparts, const double* part, const double* part_b, float tol, int maxiter,
-			      CgState* st)
+// one work-group per problem
+__global__ void k_cg_init_fin(const CgSeg* __restrict__ seg, const double* part, const double* part_b, float tol,
+			      int maxiter, CgState* st, const int* mask)
 {
 	__shared__ double sh[4];
+	const int q = blockIdx.x;
 
-	double s = sum_partials(part, nparts, sh);
+	if (nullptr != mask && 0 == mask[q])
+		return;
+
+	double s = sum_partials(part + seg[q].blk0, seg[q].nblk, sh);
 	__syncthreads();
-	double sb = sum_partials(part_b, nparts, sh);
+	double sb = sum_partials(part_b + seg[q].blk0, seg[q].nblk, sh);
 
 	if (0 == threadIdx.x) {
 
+		CgState* sq = st + q;
 		double nb = sqrt(sb);
 		float eps = (float)nb * tol;
 		double nr = sqrt(s);
 		float rs = (float)(nr * nr);
 
-		st->bnorm = (float)nb;
-		st->r0norm = (float)nr;
-		st->rsold = rs;
-		st->rsnew = rs;
-		st->eps2 = (float)pow((double)eps, 2.);
-		st->iter = 0;
-		st->maxiter = maxiter;
-		st->alpha = 0.f;
-		st->beta = 0.f;
+		sq->bnorm = (float)nb;
+		sq->r0norm = (float)nr;
+		sq->rsold = rs;
+		sq->rsnew = rs;
+		sq->eps2 = (float)pow((double)eps, 2.);
+		sq->iter = 0;
+		sq->maxiter = maxiter;
+		sq->alpha = 0.f;
+		sq->beta = 0.f;
 		// checkeps (iter2.c:65-80) and the early out of italgos.c:526-530
-		st->done = (0. == nb || 0.f == rs || maxiter <= 0) ? 4 : 0;
+		sq->done = (0. == nb || 0.f == rs || maxiter <= 0) ? 4 : 0;
 	}
 }
 
 // ---------------------------------------------------------------------------
 
-CgSolver::CgSolver(long nfloats, hipStream_t s) : n(nfloats), stream(s)
+CgBatch::CgBatch(const std::vector<long>& nfloats, hipStream_t s) : B((int)nfloats.size()), stream(s)
 {
-	x.alloc(n);
-	b.alloc(n);
-	r.alloc(n);
-	p.alloc(n);
-	Ap.alloc(n);
-	part.alloc(CG_BLOCKS);
-	part_b.alloc(CG_BLOCKS);
-	st.alloc(1);
-	trace.alloc(1);
-	nblocks = (int)std::min<long>(CG_BLOCKS, std::max<long>(1, (n + CG_THREADS - 1) / CG_THREADS));
-	BAM_HIP(hipHostMalloc((void**)&host_st, sizeof(CgState), hipHostMallocDefault));
+	BAM_CHECK(B >= 1, "cg: empty batch");
+
+	seg_h.resize(B);
+	std::vector<int> bp;
+
+	for (int i = 0; i < B; i++) {
+
+		long n = nfloats[i];
+		BAM_CHECK(n >= 0, "cg: negative problem size");
+		CgSeg& sg = seg_h[i];
+		sg.off = total;
+		sg.len = n;
+		sg.blk0 = nblk;
+		sg.nblk = (int)std::min<long>(CG_BLOCKS, std::max<long>(1, (n + CG_THREADS - 1) / CG_THREADS));
+		nblk += sg.nblk;
+		total += (n + 63) / 64 * 64;  // 256-byte aligned problems
+		bp.insert(bp.end(), sg.nblk, i);
+	}
+
+	total = std::max<long>(total, 64);
+	x.alloc(total);
+	b.alloc(total);
+	r.alloc(total);
+	p.alloc(total);
+	Ap.alloc(total);
+	part.alloc(nblk);
+	part_b.alloc(nblk);
+	seg.alloc(B);
+	bprob.alloc(nblk);
+	mask.alloc(B);
+	st.alloc(B);
+	trace.alloc(B);
+	trace_len = 1;
+	fell_back.assign(B, 0);
+
+	BAM_HIP(hipMemcpy(seg.ptr, seg_h.data(), sizeof(CgSeg) * B, hipMemcpyHostToDevice));
+	BAM_HIP(hipMemcpy(bprob.ptr, bp.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
+	BAM_HIP(hipHostMalloc((void**)&host_st, sizeof(CgState) * B, hipHostMallocDefault));
 }
 
-CgSolver::~CgSolver()
+CgBatch::~CgBatch()
 {
 	drop_graph();
 	if (nullptr != host_st)
 		(void)hipHostFree(host_st);
 }
 
-void CgSolver::drop_graph()
+void CgBatch::drop_graph()
 {
 	if (nullptr != gexec)
 		(void)hipGraphExecDestroy(gexec);
@@ -272,127 +365,186 @@ void CgSolver::drop_graph()
 		(void)hipGraphDestroy(graph);
 	gexec = nullptr;
 	graph = nullptr;
-	graph_iters = -1;
+	have_graph = false;
 }
 
-void CgSolver::enqueue_iteration(const NormalFn& normal, float lambda)
+void CgBatch::init_kernels(float lambda, float tol, int maxiter, bool cold, bool masked)
 {
-	normal((float2*)Ap.ptr, (const float2*)p.ptr, &st.ptr->done);
-	ProfScope prof("cg_vector_ops", stream);
-	hipLaunchKernelGGL(k_cg_apdot, dim3(nblocks), dim3(CG_THREADS), 0, stream, n, lambda, p.ptr, Ap.ptr, part.ptr, st.ptr);
-	hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(CG_THREADS), 0, stream, nblocks, part.ptr, st.ptr);
-	hipLaunchKernelGGL(k_cg_xr, dim3(nblocks), dim3(CG_THREADS), 0, stream, n, x.ptr, r.ptr, p.ptr, Ap.ptr, part.ptr, st.ptr);
-	hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(CG_THREADS), 0, stream, nblocks, part.ptr, st.ptr, trace.ptr);
-	hipLaunchKernelGGL(k_cg_p, dim3(nblocks), dim3(CG_THREADS), 0, stream, n, p.ptr, r.ptr, st.ptr);
+	const int* m = masked ? mask.ptr : nullptr;
+	hipLaunchKernelGGL(k_cg_init, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, lambda, x.ptr, b.ptr,
+			   r.ptr, p.ptr, part.ptr, part_b.ptr, cold, m);
+	hipLaunchKernelGGL(k_cg_init_fin, dim3(B), dim3(CG_THREADS), 0, stream, seg.ptr, part.ptr, part_b.ptr, tol,
+			   maxiter, st.ptr, m);
 	BAM_HIP(hipGetLastError());
+}
+
+void CgBatch::start(const BatchNormalFn& normal, float lambda, float tol, int maxiter, const float2* const* x0,
+		    const float2* const* b_in, bool warm, bool fallback)
+{
+	std::fill(fell_back.begin(), fell_back.end(), 0);
+
+	// per-problem residual trace, allocated before any capture
+	if (trace_len < std::max(1, maxiter)) {
+		trace_len = std::max(1, maxiter);
+		trace.alloc((size_t)B * trace_len);
+		drop_graph();
+	}
+
+	for (int i = 0; i < B; i++)
+		BAM_HIP(hipMemcpyAsync(bs(i), b_in[i], sizeof(float) * seg_h[i].len, hipMemcpyDeviceToDevice, stream));
+
+	if (warm) {
+
+		for (int i = 0; i < B; i++) {
+			BAM_HIP(hipMemcpyAsync(xs(i), x0[i], sizeof(float) * seg_h[i].len, hipMemcpyDeviceToDevice, stream));
+			normal(i, (float2*)rs(i), (const float2*)xs(i), nullptr);
+		}
+
+	} else {
+
+		fill((float2*)x.ptr, total / 2, make_float2(0.f, 0.f), stream);
+	}
+
+	init_kernels(lambda, tol, maxiter, !warm, false);
+
+	if (warm && fallback) {
+
+		// zero start for every problem whose warm-start residual exceeds ||b||
+		BAM_HIP(hipMemcpyAsync(host_st, st.ptr, sizeof(CgState) * B, hipMemcpyDeviceToHost, stream));
+		BAM_HIP(hipStreamSynchronize(stream));
+
+		std::vector<int> m(B, 0);
+		int any = 0;
+
+		for (int i = 0; i < B; i++) {
+			if (host_st[i].r0norm > host_st[i].bnorm) {
+				m[i] = 1;
+				fell_back[i] = 1;
+				any = 1;
+			}
+		}
+
+		if (any) {
+
+			BAM_HIP(hipMemcpyAsync(mask.ptr, m.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream));
+
+			for (int i = 0; i < B; i++)
+				if (m[i])
+					fill((float2*)xs(i), seg_h[i].len / 2, make_float2(0.f, 0.f), stream);
+
+			init_kernels(lambda, tol, maxiter, true, true);
+			BAM_HIP(hipStreamSynchronize(stream));  // m leaves scope
+		}
+	}
+}
+
+void CgBatch::enqueue_iteration(const BatchNormalFn& normal, float lambda)
+{
+	for (int i = 0; i < B; i++)
+		normal(i, (float2*)(Ap.ptr + seg_h[i].off), (const float2*)(p.ptr + seg_h[i].off), &st.ptr[i].done);
+
+	ProfScope prof("cg_vector_ops", stream);
+	hipLaunchKernelGGL(k_cg_apdot, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, lambda, p.ptr, Ap.ptr,
+			   part.ptr, st.ptr);
+	hipLaunchKernelGGL(k_cg_alpha, dim3(B), dim3(CG_THREADS), 0, stream, seg.ptr, part.ptr, st.ptr);
+	hipLaunchKernelGGL(k_cg_xr, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, x.ptr, r.ptr, p.ptr,
+			   Ap.ptr, part.ptr, st.ptr);
+	hipLaunchKernelGGL(k_cg_beta, dim3(B), dim3(CG_THREADS), 0, stream, seg.ptr, part.ptr, st.ptr, trace.ptr,
+			   trace_len);
+	hipLaunchKernelGGL(k_cg_p, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, p.ptr, r.ptr, st.ptr);
+	BAM_HIP(hipGetLastError());
+}
+
+void CgBatch::iterate(const BatchNormalFn& normal, float lambda, int maxiter, bool use_graph)
+{
+	if (maxiter <= 0)
+		return;
+
+	if (!use_graph) {
+
+		for (int i = 0; i < maxiter; i++)
+			enqueue_iteration(normal, lambda);
+		return;
+	}
+
+	if (!have_graph || graph_lambda != lambda) {
+
+		drop_graph();
+		BAM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+		prof_set_capturing(true);
+
+		try {
+			enqueue_iteration(normal, lambda);
+		} catch (...) {
+			prof_set_capturing(false);
+			(void)hipStreamEndCapture(stream, &graph);
+			throw;
+		}
+
+		prof_set_capturing(false);
+		BAM_HIP(hipStreamEndCapture(stream, &graph));
+		BAM_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+		have_graph = true;
+		graph_lambda = lambda;
+	}
+
+	for (int i = 0; i < maxiter; i++)
+		BAM_HIP(hipGraphLaunch(gexec, stream));
+}
+
+void CgBatch::finish(float2* const* x_out)
+{
+	for (int i = 0; i < B; i++)
+		BAM_HIP(hipMemcpyAsync(x_out[i], xs(i), sizeof(float) * seg_h[i].len, hipMemcpyDeviceToDevice, stream));
+
+	BAM_HIP(hipMemcpyAsync(host_st, st.ptr, sizeof(CgState) * B, hipMemcpyDeviceToHost, stream));
+}
+
+CgResult CgBatch::result(int i) const
+{
+	CgResult res;
+	const CgState& s = host_st[i];
+
+	res.iterations = s.iter;
+	res.resid = sqrtf(s.rsnew);
+	res.bnorm = s.bnorm;
+	res.status = s.done;
+	res.fell_back = 0 != fell_back[i];
+
+	if (res.iterations > 0) {
+		res.trace.resize(res.iterations);
+		BAM_HIP(hipMemcpy(res.trace.data(), trace.ptr + (long)i * trace_len, sizeof(float) * res.iterations,
+				  hipMemcpyDeviceToHost));
+	}
+
+	return res;
 }
 
 CgResult CgSolver::solve(const NormalFn& normal, float lambda, int maxiter, float tol, float2* x_io,
 			 const float2* b_in, bool warm, bool fallback, bool use_graph)
 {
-	CgResult res;
-
-	if ((long)trace.n < std::max(1, maxiter)) {
-		trace.alloc(std::max(1, maxiter));
-		drop_graph();
-	}
+	BatchNormalFn bn = [&normal](int, float2* d, const float2* s, const int* skip) { normal(d, s, skip); };
 
 	hipEvent_t e0, e1;
 	BAM_HIP(hipEventCreate(&e0));
 	BAM_HIP(hipEventCreate(&e1));
-	BAM_HIP(hipEventRecord(e0, stream));
+	BAM_HIP(hipEventRecord(e0, bt.stream));
 
-	BAM_HIP(hipMemcpyAsync(b.ptr, b_in, sizeof(float) * n, hipMemcpyDeviceToDevice, stream));
+	const float2* x0 = x_io;
+	bt.start(bn, lambda, tol, maxiter, &x0, &b_in, warm, fallback);
+	bt.iterate(bn, lambda, maxiter, use_graph);
+	bt.finish(&x_io);
 
-	bool cold = !warm;
-
-	if (warm) {
-
-		BAM_HIP(hipMemcpyAsync(x.ptr, x_io, sizeof(float) * n, hipMemcpyDeviceToDevice, stream));
-		normal((float2*)r.ptr, (const float2*)x.ptr, nullptr);
-
-	} else {
-
-		fill((float2*)x.ptr, n / 2, make_float2(0.f, 0.f), stream);
-	}
-
-	hipLaunchKernelGGL(k_cg_init, dim3(nblocks), dim3(CG_THREADS), 0, stream, n, lambda, x.ptr, b.ptr, r.ptr, p.ptr,
-			   part.ptr, part_b.ptr, cold);
-	hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(CG_THREADS), 0, stream, nblocks, part.ptr, part_b.ptr, tol, maxiter,
-			   st.ptr);
-	BAM_HIP(hipGetLastError());
-
-	if (warm && fallback) {
-
-		// zero start when the warm start's residual is worse than ||b||
-		BAM_HIP(hipMemcpyAsync(host_st, st.ptr, sizeof(CgState), hipMemcpyDeviceToHost, stream));
-		BAM_HIP(hipStreamSynchronize(stream));
-
-		if (host_st->r0norm > host_st->bnorm) {
-
-			res.fell_back = true;
-			fill((float2*)x.ptr, n / 2, make_float2(0.f, 0.f), stream);
-			hipLaunchKernelGGL(k_cg_init, dim3(nblocks), dim3(CG_THREADS), 0, stream, n, lambda, x.ptr, b.ptr,
-					   r.ptr, p.ptr, part.ptr, part_b.ptr, true);
-			hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(CG_THREADS), 0, stream, nblocks, part.ptr, part_b.ptr,
-					   tol, maxiter, st.ptr);
-			BAM_HIP(hipGetLastError());
-		}
-	}
-
-	if (use_graph) {
-
-		if (graph_iters != maxiter || graph_lambda != lambda) {
-
-			drop_graph();
-			BAM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-			prof_set_capturing(true);
-
-			try {
-				for (int i = 0; i < maxiter; i++)
-					enqueue_iteration(normal, lambda);
-			} catch (...) {
-				prof_set_capturing(false);
-				(void)hipStreamEndCapture(stream, &graph);
-				throw;
-			}
-
-			prof_set_capturing(false);
-			BAM_HIP(hipStreamEndCapture(stream, &graph));
-			BAM_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-			graph_iters = maxiter;
-			graph_lambda = lambda;
-		}
-
-		BAM_HIP(hipGraphLaunch(gexec, stream));
-
-	} else {
-
-		for (int i = 0; i < maxiter; i++)
-			enqueue_iteration(normal, lambda);
-	}
-
-	BAM_HIP(hipMemcpyAsync(x_io, x.ptr, sizeof(float) * n, hipMemcpyDeviceToDevice, stream));
-	BAM_HIP(hipMemcpyAsync(host_st, st.ptr, sizeof(CgState), hipMemcpyDeviceToHost, stream));
-	BAM_HIP(hipEventRecord(e1, stream));
-	BAM_HIP(hipStreamSynchronize(stream));
+	BAM_HIP(hipEventRecord(e1, bt.stream));
+	BAM_HIP(hipStreamSynchronize(bt.stream));
 
 	float ms = 0.f;
 	BAM_HIP(hipEventElapsedTime(&ms, e0, e1));
 	BAM_HIP(hipEventDestroy(e0));
 	BAM_HIP(hipEventDestroy(e1));
 
-	res.iterations = host_st->iter;
-	res.resid = sqrtf(host_st->rsnew);
-	res.bnorm = host_st->bnorm;
-	res.status = host_st->done;
+	CgResult res = bt.result(0);
 	res.ms = ms;
-
-	if (res.iterations > 0) {
-		res.trace.resize(res.iterations);
-		BAM_HIP(hipMemcpy(res.trace.data(), trace.ptr, sizeof(float) * res.iterations, hipMemcpyDeviceToHost));
-	}
-
 	return res;
 }
 

@@ -1,4 +1,5 @@
-// cg.hpp -- device-resident conjugate-gradient solver.
+// cg.hpp -- device-resident conjugate-gradient solver for one or many
+// independent problems.
 #pragma once
 
 #include "nufft_impl.hpp"
@@ -33,38 +34,98 @@ struct CgResult {
 	std::vector<float> trace;
 };
 
-// normal(dst, src, skip): enqueue dst = A^H A src on the solver's stream;
-// skip is a device flag (may be null) that turns the launches into no-ops.
+// One problem's part of the packed vectors: floats [off, off + len), reduced
+// by the work-groups [blk0, blk0 + nblk) of a vector launch.
+struct CgSeg {
+
+	long off;
+	long len;
+	int blk0;
+	int nblk;
+};
+
+// normal(i, dst, src, skip): enqueue dst = A_i^H A_i src for problem i on
+// the batch's stream; skip is a device flag (may be null) that turns the
+// launches into no-ops once problem i has stopped.
+using BatchNormalFn = std::function<void(int, float2*, const float2*, const int*)>;
+
+// normal(dst, src, skip) of a single problem
 using NormalFn = std::function<void(float2*, const float2*, const int*)>;
 
-struct CgSolver {
+// Independent problems (A_i^H A_i + lambda I) x_i = b_i solved in lockstep:
+// the vectors of all problems are packed into one buffer each, every vector
+// operation is one launch over all problems, and the scalars (alpha, beta,
+// stop flags) are per problem -- iter/italgos.c:589-680 (conjgrad_batch)
+// with the per-problem stopping rule of the single solve.
+struct CgBatch {
 
-	long n;  // floats (2 x complex elements)
-	hipStream_t stream;
+	int B = 0;
+	long total = 0;    // floats over all problems
+	int nblk = 0;      // work-groups of a vector launch
+	hipStream_t stream = 0;
+	std::vector<CgSeg> seg_h;
 	DevBuf<float> x, b, r, p, Ap, trace;
 	DevBuf<double> part, part_b;
+	DevBuf<CgSeg> seg;
+	DevBuf<int> bprob;   // work-group -> problem
+	DevBuf<int> mask;    // problems to (re)initialise
 	DevBuf<CgState> st;
 	CgState* host_st = nullptr;
-	int nblocks = 1;
+	int trace_len = 0;   // per problem
+	std::vector<char> fell_back;
 
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t gexec = nullptr;
-	int graph_iters = -1;
+	bool have_graph = false;
 	float graph_lambda = 0.f;
 
-	CgSolver(long nfloats, hipStream_t s);
-	~CgSolver();
-	CgSolver(const CgSolver&) = delete;
-	CgSolver& operator=(const CgSolver&) = delete;
+	// nfloats[i]: floats (2 x complex elements) of problem i
+	CgBatch(const std::vector<long>& nfloats, hipStream_t s);
+	~CgBatch();
+	CgBatch(const CgBatch&) = delete;
+	CgBatch& operator=(const CgBatch&) = delete;
+
+	float* xs(int i) const { return x.ptr + seg_h[i].off; }
+	float* bs(int i) const { return b.ptr + seg_h[i].off; }
+	float* rs(int i) const { return r.ptr + seg_h[i].off; }
 
 	void drop_graph();
-	void enqueue_iteration(const NormalFn& normal, float lambda);
+
+	// Enqueues the start: b_i copied in, x_i = warm start or zero, r = b - (A^H A
+	// + lambda I) x, p = r.  With warm && fallback, waits for the device and
+	// restarts from zero every problem whose warm-start residual exceeds ||b_i||.
+	void start(const BatchNormalFn& normal, float lambda, float tol, int maxiter, const float2* const* x0,
+		   const float2* const* b_in, bool warm, bool fallback);
+
+	// Enqueues maxiter iterations: one iteration is captured once as a hipGraph
+	// (per lambda) and launched maxiter times; stopped problems skip their work.
+	void iterate(const BatchNormalFn& normal, float lambda, int maxiter, bool use_graph);
+
+	// Enqueues x_i -> x_out[i] and the state read-back.
+	void finish(float2* const* x_out);
+
+	// After the stream has completed: per-problem results.
+	CgResult result(int i) const;
+
+private:
+	void enqueue_iteration(const BatchNormalFn& normal, float lambda);
+	void init_kernels(float lambda, float tol, int maxiter, bool cold, bool masked);
+};
+
+// Single problem with its solver state kept across calls (graph reuse).
+struct CgSolver {
+
+	CgBatch bt;
+
+	CgSolver(long nfloats, hipStream_t s) : bt(std::vector<long>{ nfloats }, s) {}
 
 	// x_io, b: device pointers of n floats.  warm: x_io holds the initial
 	// guess; fallback: restart from zero when the warm start's residual
 	// exceeds ||b||.
 	CgResult solve(const NormalFn& normal, float lambda, int maxiter, float tol, float2* x_io, const float2* b,
 		       bool warm, bool fallback, bool use_graph);
+
+	void drop_graph() { bt.drop_graph(); }
 };
 
 }  // namespace bam

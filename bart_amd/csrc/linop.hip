@@ -7,6 +7,7 @@
 #include "nudft.hpp"
 #include "nufft_impl.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <cstdarg>
 #include <cstring>
@@ -162,6 +163,25 @@ struct Op {
 	{
 		if (!toeplitz() && (long)kbuf.n < osize)
 			kbuf.alloc(osize);
+	}
+
+	// the plan's launches go to stream s (the operator's own stream by default)
+	void set_plan_stream(hipStream_t s)
+	{
+		if (nufft)
+			nufft->stream = s;
+		if (nudft)
+			nudft->stream = s;
+	}
+
+	const void* plan_id() const { return nufft ? (const void*)nufft.get() : (const void*)nudft.get(); }
+
+	// relative cost of one normal operator: the sub-grid stack it transforms
+	double cost() const
+	{
+		if (OP_NUDFT == kind)
+			return (double)isize * (double)osize;
+		return (double)nufft->S * nufft->C * (double)nufft->N;
 	}
 
 	Nufft& plan(const char* what)
@@ -731,6 +751,206 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		}
 
 		return r.iterations;
+	}, -1);
+}
+
+// Streams the problems of a batched solve are spread over; the device has
+// four hardware queues per process by default.
+constexpr int CG_BATCH_STREAMS = 4;
+
+int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxiter, float l2lambda, float tol, int N,
+			float _Complex* const x[], const float _Complex* const b[], bool warmstart, bool warm_fallback,
+			float* resid_trace, struct cg_stats_s stats[])
+{
+	struct Workers {
+
+		std::vector<hipStream_t> ws;
+		std::vector<hipEvent_t> ev;
+		std::vector<Op*> ops;
+
+		~Workers()
+		{
+			for (hipStream_t s : ws)
+				(void)hipStreamSynchronize(s);
+			for (Op* o : ops)
+				o->set_plan_stream(o->stream);
+			for (hipEvent_t e : ev)
+				(void)hipEventDestroy(e);
+			for (hipStream_t s : ws)
+				(void)hipStreamDestroy(s);
+		}
+	};
+
+	return guarded([&]() -> int {
+		BAM_CHECK(nprob >= 0, "lsqr_conjgrad_batch: negative problem count");
+
+		if (0 == nprob)
+			return 0;
+
+		BAM_CHECK(nullptr != ops && nullptr != x && nullptr != b, "lsqr_conjgrad_batch: NULL argument");
+
+		std::vector<Op*> op(nprob);
+		std::vector<const void*> plans(nprob);
+		std::vector<std::mutex*> locks;
+
+		for (int i = 0; i < nprob; i++) {
+
+			op[i] = (Op*)ops[i];
+			BAM_CHECK(nullptr != op[i], "lsqr_conjgrad_batch: NULL operator (problem %d)", i);
+			BAM_CHECK(N == op[i]->N, "lsqr_conjgrad_batch: dimension count %d != %d (problem %d)", N, op[i]->N, i);
+			BAM_CHECK(!op[i]->toeplitz() || op[i]->nufft->has_psf,
+				  "lsqr_conjgrad_batch: Toeplitz operator without PSF (problem %d)", i);
+			BAM_CHECK(nullptr != x[i] && nullptr != b[i], "lsqr_conjgrad_batch: NULL vector (problem %d)", i);
+			plans[i] = op[i]->plan_id();
+			locks.push_back(&op[i]->mtx());
+		}
+
+		// the problems' normal operators run concurrently: no two may share a plan
+		std::vector<const void*> ps = plans;
+		std::sort(ps.begin(), ps.end());
+		BAM_CHECK(std::adjacent_find(ps.begin(), ps.end()) == ps.end(),
+			  "lsqr_conjgrad_batch: two problems share one NUFFT plan");
+
+		std::sort(locks.begin(), locks.end());
+		locks.erase(std::unique(locks.begin(), locks.end()), locks.end());
+		std::vector<std::unique_lock<std::mutex>> held;
+		for (std::mutex* m : locks)
+			held.emplace_back(*m);
+
+		// device copies of host vectors
+		std::vector<std::unique_ptr<DevBuf<float2>>> stage;
+		std::vector<float2*> xd(nprob);
+		std::vector<const float2*> bd(nprob);
+		std::vector<char> xhost(nprob, 0);
+
+		for (int i = 0; i < nprob; i++) {
+
+			long ne = op[i]->isize;
+			xd[i] = (float2*)x[i];
+			bd[i] = (const float2*)b[i];
+
+			if (!on_device(x[i])) {
+				stage.emplace_back(new DevBuf<float2>(ne));
+				BAM_HIP(hipMemcpy(stage.back()->ptr, x[i], sizeof(float2) * ne, hipMemcpyHostToDevice));
+				xd[i] = stage.back()->ptr;
+				xhost[i] = 1;
+			}
+
+			if (!on_device(b[i])) {
+				stage.emplace_back(new DevBuf<float2>(ne));
+				BAM_HIP(hipMemcpy(stage.back()->ptr, b[i], sizeof(float2) * ne, hipMemcpyHostToDevice));
+				bd[i] = stage.back()->ptr;
+			}
+		}
+
+		// longest-processing-time assignment of the problems to the streams
+		const int K = std::min(nprob, CG_BATCH_STREAMS);
+		std::vector<int> order(nprob);
+		for (int i = 0; i < nprob; i++)
+			order[i] = i;
+		std::stable_sort(order.begin(), order.end(),
+				 [&](int a, int c) { return op[a]->cost() > op[c]->cost(); });
+
+		std::vector<std::vector<int>> members(K);
+		std::vector<double> load(K, 0.);
+
+		for (int i : order) {
+			int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+			members[k].push_back(i);
+			load[k] += op[i]->cost();
+		}
+
+		// (declared first: destroyed after the streams have drained)
+		std::vector<std::unique_ptr<CgBatch>> bt(K);
+		Workers w;
+
+		for (int k = 0; k < K; k++) {
+
+			hipStream_t s;
+			BAM_HIP(hipStreamCreate(&s));
+			w.ws.push_back(s);
+			hipEvent_t e0, e1;
+			BAM_HIP(hipEventCreate(&e0));
+			w.ev.push_back(e0);
+			BAM_HIP(hipEventCreate(&e1));
+			w.ev.push_back(e1);
+
+			std::vector<long> nf;
+
+			for (int i : members[k]) {
+				op[i]->prepare_normal();
+				BAM_HIP(hipStreamSynchronize(op[i]->stream));
+				op[i]->set_plan_stream(s);
+				w.ops.push_back(op[i]);
+				nf.push_back(2 * op[i]->isize);
+			}
+
+			bt[k].reset(new CgBatch(nf, s));
+		}
+
+		std::vector<BatchNormalFn> fn(K);
+
+		for (int k = 0; k < K; k++) {
+
+			const std::vector<int>& mem = members[k];
+			fn[k] = [&op, &mem](int j, float2* d, const float2* s, const int* skip) { op[mem[j]]->nrm(d, s, skip); };
+
+			std::vector<const float2*> x0, bb;
+			for (int i : mem) {
+				x0.push_back(xd[i]);
+				bb.push_back(bd[i]);
+			}
+
+			BAM_HIP(hipEventRecord(w.ev[2 * k], w.ws[k]));
+			bt[k]->start(fn[k], l2lambda, tol, maxiter, x0.data(), bb.data(), warmstart, warm_fallback);
+		}
+
+		for (int k = 0; k < K; k++)
+			bt[k]->iterate(fn[k], l2lambda, maxiter, true);
+
+		for (int k = 0; k < K; k++) {
+
+			std::vector<float2*> xo;
+			for (int i : members[k])
+				xo.push_back(xd[i]);
+
+			bt[k]->finish(xo.data());
+			BAM_HIP(hipEventRecord(w.ev[2 * k + 1], w.ws[k]));
+		}
+
+		int maxit = 0;
+
+		for (int k = 0; k < K; k++) {
+
+			BAM_HIP(hipStreamSynchronize(w.ws[k]));
+			float ms = 0.f;
+			BAM_HIP(hipEventElapsedTime(&ms, w.ev[2 * k], w.ev[2 * k + 1]));
+
+			for (size_t j = 0; j < members[k].size(); j++) {
+
+				const int i = members[k][j];
+				CgResult r = bt[k]->result((int)j);
+				maxit = std::max(maxit, r.iterations);
+
+				if (xhost[i])
+					BAM_HIP(hipMemcpy(x[i], xd[i], sizeof(float2) * op[i]->isize, hipMemcpyDeviceToHost));
+
+				if (nullptr != resid_trace && r.iterations > 0)
+					memcpy(resid_trace + (long)i * std::max(1, maxiter), r.trace.data(),
+					       sizeof(float) * r.iterations);
+
+				if (nullptr != stats) {
+					stats[i].iterations = r.iterations;
+					stats[i].status = r.status;
+					stats[i].resid = r.resid;
+					stats[i].bnorm = r.bnorm;
+					stats[i].time_ms = ms;
+					stats[i].fell_back = r.fell_back;
+				}
+			}
+		}
+
+		return maxit;
 	}, -1);
 }
 

@@ -136,12 +136,148 @@ def cpu_baseline(w, threads):
                        f"{X}x{Y}x{Z} grid, {reps} repetitions ({dt:.1f} s)")
 
 
+def batched_slices(nslices=96, seed=4):
+    """Uneven 2-D CG-SENSE problems: matrix n in U{128,192,256,320,384}, coils
+    in U{8,16,32}, golden-angle radial spokes in U[21,201] with readout 2n;
+    per slice an ellipse phantom and Gaussian coil maps of its own seed."""
+    from bart_amd import inputs as I
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nslices):
+        n = int(rng.choice([128, 192, 256, 320, 384]))
+        C = int(rng.choice([8, 16, 32]))
+        spokes = int(rng.integers(21, 202))
+        out.append(dict(n=n, C=C, R=2 * n, S=spokes, seed=1000 + s))
+    return out
+
+
+def run_batched(args, L, report):
+    """96 independent CG-SENSE solves (30 iterations each, lambda 0.01): the
+    right-hand sides A^H y, then one lsqr_conjgrad_batch call.  value = solves/s."""
+    import torch
+    import bart_amd as ba
+    from bart_amd import inputs as I
+
+    dev = torch.device("cuda:0")
+    sl = batched_slices()
+    iters, lam = 30, 0.01
+    t0 = time.perf_counter()
+    ops, ys, bs = [], [], []
+    for d in sl:
+        n, C, R, S = d["n"], d["C"], d["R"], d["S"]
+        traj = I.radial_traj(R, S)
+        img = I.ellipse_phantom(n, seed=d["seed"])
+        maps = I.gaussian_coils((1, n, n), C, seed=d["seed"])
+        op = ba.sense_nc_init(dims16(n, n, 1, C), dims16(n, n, 1, C), maps, dims16(1, R, S, C), dims16(3, R, S),
+                              ba.traj_array(traj))
+        x = torch.from_numpy(img.reshape(-1)).to(dev)
+        y = op.forward(x)
+        scale = float(y.abs().max())
+        g = torch.Generator(device="cpu").manual_seed(d["seed"])
+        noise = (torch.randn(y.numel(), generator=g) + 1j * torch.randn(y.numel(), generator=g)).to(torch.complex64)
+        ys.append((y + (1e-3 * scale / np.sqrt(2.0)) * noise.to(dev)).contiguous())
+        bs.append(torch.empty(n * n, dtype=torch.complex64, device=dev))
+        ops.append(op)
+    torch.cuda.synchronize()
+    print(f"# {len(sl)} slices: plans and data in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    def step(rec):
+        t0 = time.perf_counter()
+        for op, y, b in zip(ops, ys, bs):
+            op.adjoint(y, out=b)
+        t1 = time.perf_counter()
+        xs, st, tr = ba.lsqr_conjgrad_batch(ops, bs, iters, lam, 0.0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        rec.append((t1 - t0, t2 - t1, max(s.iterations for s in st),
+                    max(s.resid / max(s.bnorm, 1e-30) for s in st)))
+
+    for _ in range(args.warmup):
+        step([])
+    rec = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(rec)
+        print(f"# step {i}: A^H y {rec[-1][0] * 1e3:.1f} ms, batched CG {rec[-1][1] * 1e3:.1f} ms "
+              f"({rec[-1][2]} it, max rel-res {rec[-1][3]:.2e})", file=sys.stderr, flush=True)
+    t_total = time.perf_counter() - t_start
+    t_adj = sum(r[0] for r in rec) / len(rec)
+    t_cg = sum(r[1] for r in rec) / len(rec)
+
+    # the same solves one after another (lsqr_conjgrad per slice)
+    t0 = time.perf_counter()
+    for op, b in zip(ops, bs):
+        ba.lsqr_conjgrad(op, b, iters, lam, 0.0)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+
+    # compulsory HBM bytes of the solves: A^H y (k-space, coordinates, maps
+    # in, image out) + per iteration image in/out, maps, PSF and the CG
+    # vector updates (8 N (9 + C + 4) bytes for a 2-D slice)
+    byts = 0
+    for d in sl:
+        N, C, M = d["n"] ** 2, d["C"], d["R"] * d["S"]
+        byts += 8 * C * M + 8 * M + 8 * N + 8 * C * N + iters * 8 * N * (9 + C + 4)
+    ach = byts / (t_adj + t_cg) / 1e9
+    roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel="whole batched solve (end to end)",
+                    bytes_per_launch=byts)
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.lib().orc_set_num_threads(threads)
+        done, t0 = 0, time.perf_counter()
+        for d in sl:
+            n, C, R, S = d["n"], d["C"], d["R"], d["S"]
+            traj = I.radial_traj(R, S)
+            maps = I.gaussian_coils((1, n, n), C, seed=d["seed"])
+            ref = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+            b = ref.sense_adjoint(maps, ys[done].cpu().numpy())
+            ref.cgsense(maps, b, lam, iters, 0.0)
+            done += 1
+            if time.perf_counter() - t0 >= 10.0:
+                break
+        dt = time.perf_counter() - t0
+        cpu = dict(value=round(done / dt, 4), unit="solves/s", cores=threads, kind="port",
+                   sample=f"oracle PSF + A^H y + {iters} Toeplitz CG iterations for the first {done} slices of "
+                          f"the batch ({dt:.1f} s, plan creation included)")
+
+    tot_samples = sum(d["C"] * d["R"] * d["S"] for d in sl)
+    out = {
+        "metric": "NUFFT adj+fwd Msamples/s; CG-SENSE solves/s to rel-res 1e-4, warm vs cold",
+        "value": round(len(sl) / (t_adj + t_cg), 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (fp32 complex; fp64 CG reductions)",
+        "data": "synthetic",
+        "config": {"workload": "batched_uneven_2d_radial_96_slices", "slices": len(sl),
+                   "matrix": sorted(set(d["n"] for d in sl)), "coils": sorted(set(d["C"] for d in sl)),
+                   "coil_samples": tot_samples, "cg_iters": iters, "lambda": lam},
+        "adjoint_rhs_ms": round(t_adj * 1e3, 3),
+        "batched_cg_ms": round(t_cg * 1e3, 3),
+        "sequential_cg_ms": round(t_seq * 1e3, 3),
+        "cg_iterations": rec[-1][2],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="kooshball", choices=["kooshball", "radial2d", "spiral"])
+    ap.add_argument("--config", default="kooshball", choices=["kooshball", "radial2d", "spiral", "batched"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -164,6 +300,10 @@ def main():
     L.bart_amd_profile.argtypes = [ctypes.c_int]
     L.bart_amd_profile_report.restype = ctypes.c_int
     L.bart_amd_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
+
+    if args.config == "batched":
+        run_batched(args, L, None)
+        return
 
     t_gen = time.perf_counter()
     w = workload(args.config)

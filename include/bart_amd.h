@@ -139,6 +139,25 @@ int lsqr_conjgrad(const struct linop_s* op, int maxiter, float l2lambda, float t
 		  BAM_COMPLEX* x, const BAM_COMPLEX* b, bool warmstart, bool warm_fallback, float* resid_trace,
 		  struct cg_stats_s* stats);
 
+/* Batched CG: nprob independent problems (A_i^H A_i + l2lambda I) x_i = b_i,
+ * each with its own operator ops[i] (image size, coils and trajectory may
+ * differ), solved together on the device.  Replaces BART's tool-level loop
+ * over slices (`bart -l/-p ... pics`, src/bart.c:200-330) around lsqr2; the
+ * vector updates of all problems are one launch each with per-problem
+ * scalars, as conjgrad_batch (iter/italgos.c:589-680) does, and every
+ * problem stops by the rule of lsqr_conjgrad, so x_i, the iteration count
+ * and the residual trace of problem i equal those of lsqr_conjgrad on it.
+ * The problems are spread over several streams; one iteration is captured
+ * as a hipGraph per stream.  No two problems may share a NUFFT plan.  x[i],
+ * b[i]: host or device vectors of ops[i]'s image size (x[i] holds the warm
+ * start when warmstart).  resid_trace: NULL or nprob rows of max(1, maxiter)
+ * floats.  stats: NULL or nprob entries; time_ms is the device time of the
+ * stream that solved the problem.  Returns the largest iteration count, or
+ * -1 on error. */
+int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxiter, float l2lambda, float tol, int N,
+			BAM_COMPLEX* const x[], const BAM_COMPLEX* const b[], bool warmstart, bool warm_fallback,
+			float* resid_trace, struct cg_stats_s stats[]);
+
 /* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
 enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };
 void bart_amd_set_error_mode(int mode);
