@@ -206,6 +206,102 @@ static void host_cell_span(const GridGeom& g, int d, int clo, int chi, int sp[4]
 	}
 }
 
+// Forward gather work list of 3-D grids (non-periodic, kernel width < 4):
+// the samples in home-tile order (cell order within a tile) and work items of
+// at most FW_MAXS samples of one tile.
+constexpr int FW_T = 8;          // tile edge, grid points
+constexpr int FW_SPT = 2;        // samples per thread
+constexpr int FW_THREADS = 512;
+constexpr int FW_MAXS = FW_SPT * FW_THREADS;
+constexpr int FW_CC = 4;         // coils per staged region
+constexpr int FW_E = 12;         // staged region edge: tile + halo of a kernel of width 3
+
+static int taps_bound(const GridGeom& g);
+
+void GridPlan::build_forward_tiles(const GridGeom& gg, const std::vector<int>& hsm, const std::vector<float4>& hsk,
+				   const std::vector<int>& cell)
+{
+	ft = FwdTiling();
+
+	// BART_AMD_FWD_TILES=off keeps the per-sample gather (for comparisons)
+	const char* env = getenv("BART_AMD_FWD_TILES");
+
+	if (nullptr != env && 0 == strcmp(env, "off"))
+		return;
+
+	bool all3 = gg.n[0] > 1 && gg.n[1] > 1 && gg.n[2] > 1;
+
+	if (!all3 || gg.periodic || 4 != taps_bound(gg) || 0 == gg.M || gg.N * FW_CC >= (1l << 31))
+		return;
+
+	ft.T = FW_T;
+	long ntiles = 1;
+
+	for (int d = 0; d < 3; d++) {
+		BAM_CHECK(0 == FW_T % gg.CS[d], "gridding: tile is not a multiple of the cell");
+		ft.nt[d] = (gg.n[d] + FW_T - 1) / FW_T;
+		ft.HL[d] = (int)ceilf(gg.reach_lo);
+		ft.E[d] = FW_T + ft.HL[d] + (int)ceilf(gg.reach_hi);
+		ntiles *= ft.nt[d];
+	}
+
+	ft.Etot = ft.E[0] * ft.E[1] * ft.E[2];
+
+	// the kernel is built for one region edge; the region rows are loaded in
+	// 16-byte pairs of points starting at even x
+	if (FW_E != ft.E[0] || FW_E != ft.E[1] || FW_E != ft.E[2] || 0 != gg.n[0] % 2 || 0 != ft.HL[0] % 2)
+		return;
+
+	// the staged region of FW_CC coils and the KB table fit into 64 KiB
+	if ((size_t)ft.Etot * FW_CC * sizeof(float2) + (KB_SIZE + 4) * sizeof(float) > 64 * 1024)
+		return;
+
+	const int cpt = FW_T / gg.CS[0];  // cells per tile edge (CS is equal in every dim)
+	std::vector<int> tile_of(gg.M);
+	std::vector<long> cnt(ntiles + 1, 0);
+
+	// hsm/hsk are in cell order; cell[] is indexed by the original sample
+	for (long i = 0; i < gg.M; i++) {
+		int c = cell[hsm[i]];
+		int cx = c % gg.ncell[0];
+		int cy = (c / gg.ncell[0]) % gg.ncell[1];
+		int cz = c / (gg.ncell[0] * gg.ncell[1]);
+		int t = cx / cpt + ft.nt[0] * (cy / cpt + ft.nt[1] * (cz / cpt));
+		tile_of[i] = t;
+		cnt[t + 1]++;
+	}
+
+	for (long t = 0; t < ntiles; t++)
+		cnt[t + 1] += cnt[t];
+
+	std::vector<int> osm(gg.M);
+	std::vector<float4> osk(gg.M);
+	std::vector<long> pos(cnt.begin(), cnt.end() - 1);
+
+	for (long i = 0; i < gg.M; i++) {
+		long j = pos[tile_of[i]]++;
+		osm[j] = hsm[i];
+		osk[j] = hsk[i];
+	}
+
+	std::vector<int4> items;
+
+	for (long t = 0; t < ntiles; t++)
+		for (long a = cnt[t]; a < cnt[t + 1]; a += FW_MAXS)
+			items.push_back(make_int4((int)t, (int)a, (int)std::min(cnt[t + 1], a + FW_MAXS), 0));
+
+	std::stable_sort(items.begin(), items.end(), [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+
+	ft.nitems = (int)items.size();
+	fsm.alloc(gg.M);
+	fsk.alloc(gg.M);
+	fwork.alloc(std::max(ft.nitems, 1));
+	BAM_HIP(hipMemcpy(fsm.ptr, osm.data(), sizeof(int) * gg.M, hipMemcpyHostToDevice));
+	BAM_HIP(hipMemcpy(fsk.ptr, osk.data(), sizeof(float4) * gg.M, hipMemcpyHostToDevice));
+	BAM_HIP(hipMemcpy(fwork.ptr, items.data(), sizeof(int4) * ft.nitems, hipMemcpyHostToDevice));
+	ft.on = true;
+}
+
 void GridPlan::build(const GridGeom& geom, const float* traj)
 {
 	g = geom;
@@ -306,6 +402,8 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 	}
 
 	BAM_CHECK(g.M < (1l << 31), "too many samples");
+
+	build_forward_tiles(g, hsm, hsk, cell);
 
 	ta = choose_tiling(g);
 
@@ -1131,7 +1229,8 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 }
 
 // ---------------------------------------------------------------------------
-// Forward gridding (grid -> k-space): one thread per sample and coil chunk
+// Forward gridding (grid -> k-space), per-sample form for the grids the
+// tiled kernel below does not take: one thread per sample and coil chunk
 // gathers the taps of all sub-grids from the grid; sub-grids are summed in
 // order s = 0 .. S-1 (noncart/nufft.c:1444-1445) and the result is
 // multiplied by the weights (:1456-1457).  Samples are taken in cell-sorted
@@ -1252,6 +1351,203 @@ __global__ __launch_bounds__(256) void k_grid_fwd(GridGeom g, const float* __res
 			ksp[(long)(c0 + cl) * g.M + m] = (nullptr != weights) ? cmul(acc[cl], w) : acc[cl];
 }
 
+// ---------------------------------------------------------------------------
+// Tiled forward gather (3-D grids): one work-group per work item, i.e. up to
+// FW_MAXS samples of one home tile.  For every coil chunk and sub-grid the
+// work-group stages the tile with its halo -- E^3 points of FW_CC coils, zero
+// outside the grid -- in LDS with row-wise coalesced loads, and each thread
+// gathers the taps of its samples from there.  The arithmetic is that of
+// k_grid_fwd, term by term in the same order, so both give the same bits.
+// ---------------------------------------------------------------------------
+
+constexpr size_t FW_TABLE_BYTES = ((KB_SIZE + 4) * sizeof(float) + 15) / 16 * 16;
+
+// E: staged region edge (tile + halo); LDS region layout [z][y][x][coil]
+template <int CCH, int E>
+__global__ __launch_bounds__(FW_THREADS) void k_grid_fwd_tile(GridGeom g, FwdTiling ft, const float* __restrict__ kbt,
+							      const int4* __restrict__ work,
+							      const int* __restrict__ fsm,
+							      const float4* __restrict__ fsk,
+							      const float2* __restrict__ grid,
+							      const float2* __restrict__ weights,
+							      float2* __restrict__ ksp)
+{
+	static_assert(CCH == 4 && E <= 16 && 0 == E % 2, "region rows are loaded by 8 lanes, 4 coils per point");
+
+	extern __shared__ __align__(16) unsigned char fw_smem[];
+	float* table = (float*)fw_smem;
+	float2* reg = (float2*)(fw_smem + FW_TABLE_BYTES);
+
+	for (int i = threadIdx.x; i < KB_SIZE + 1; i += FW_THREADS)
+		table[i] = kbt[i];
+
+	const int4 it = work[blockIdx.x];
+	const int tile = it.x;
+	const int o0 = (tile % ft.nt[0]) * ft.T - ft.HL[0];
+	const int o1 = ((tile / ft.nt[0]) % ft.nt[1]) * ft.T - ft.HL[1];
+	const int o2 = (tile / (ft.nt[0] * ft.nt[1])) * ft.T - ft.HL[2];
+	const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+	const int Ni = (int)g.N;
+
+	// region loads: 8 lanes per row of E points (two points per lane),
+	// rows (coil, z, y)
+	const int lx2 = 2 * (threadIdx.x & 7);
+	const int gx = o0 + lx2;
+	const bool xin = lx2 < E && gx >= 0 && gx < n0;
+	constexpr int NROW = CCH * E * E;
+	constexpr int RSTEP = FW_THREADS / 8;
+
+	float4 kc[FW_SPT];
+	bool valid[FW_SPT];
+
+#pragma unroll
+	for (int q = 0; q < FW_SPT; q++) {
+		const int j = it.y + threadIdx.x + FW_THREADS * q;
+		valid[q] = j < it.z;
+		kc[q] = valid[q] ? fsk[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+	}
+
+	for (int c0 = 0; c0 < g.C; c0 += CCH) {
+
+		const int ncc = min(CCH, g.C - c0);
+		float2 acc[FW_SPT][CCH];
+
+#pragma unroll
+		for (int q = 0; q < FW_SPT; q++)
+#pragma unroll
+			for (int cl = 0; cl < CCH; cl++)
+				acc[q][cl] = make_float2(0.f, 0.f);
+
+		for (int s = 0; s < g.S; s++) {
+
+			const float2* gs = grid + ((long)s * g.C + c0) * g.N;
+
+			// the previous region has been consumed (and the table is in)
+			__syncthreads();
+
+			for (int r = threadIdx.x >> 3; r < NROW; r += RSTEP) {
+
+				const int cl = r / (E * E);
+				const int zy = r - cl * (E * E);
+				const int z = zy / E;
+				const int gy = o1 + zy - z * E;
+				const int gz = o2 + z;
+				float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+
+				// 32-bit offsets within the chunk (the plan bounds CCH N);
+				// x pairs start at even points (even origin and row length)
+				if (xin && cl < ncc && gy >= 0 && gy < n1 && gz >= 0 && gz < n2)
+					v = *(const float4*)(gs + cl * Ni + gx + n0 * (gy + n1 * gz));
+
+				if (lx2 < E) {
+					float2* d = reg + (zy * E + lx2) * CCH + cl;
+					d[0] = make_float2(v.x, v.y);
+					d[CCH] = make_float2(v.z, v.w);
+				}
+			}
+
+			__syncthreads();
+
+			const int v0 = g.sbit[s][0], v1 = g.sbit[s][1], v2 = g.sbit[s][2];
+
+#pragma unroll
+			for (int q = 0; q < FW_SPT; q++) {
+
+				if (!valid[q])
+					continue;
+
+				Taps1<4> t0, t1, t2;
+				taps_raw<4>(g, 0, v0, kc[q].x, table, t0);
+				taps_raw<4>(g, 1, v1, kc[q].y, table, t1);
+				taps_raw<4>(g, 2, v2, kc[q].z, table, t2);
+
+				// local tap indices (-1: no tap); taps outside the staged
+				// region cannot occur for a sample of this tile
+				int l0[4], l1[4], l2[4];
+				bool a0 = false, a1 = false, a2 = false;
+
+#pragma unroll
+				for (int j = 0; j < 4; j++) {
+					int w0 = t0.start + j, w1 = t1.start + j, w2 = t2.start + j;
+					int q0 = w0 - o0, q1 = w1 - o1, q2 = w2 - o2;
+					l0[j] = (j < t0.cnt && w0 >= 0 && w0 < n0 && q0 >= 0 && q0 < E) ? q0 : -1;
+					l1[j] = (j < t1.cnt && w1 >= 0 && w1 < n1 && q1 >= 0 && q1 < E) ? q1 : -1;
+					l2[j] = (j < t2.cnt && w2 >= 0 && w2 < n2 && q2 >= 0 && q2 < E) ? q2 : -1;
+					a0 = a0 || (l0[j] >= 0);
+					a1 = a1 || (l1[j] >= 0);
+					a2 = a2 || (l2[j] >= 0);
+				}
+
+				if (!(a0 && a1 && a2))
+					continue;
+
+				float2 val[CCH];
+#pragma unroll
+				for (int cl = 0; cl < CCH; cl++)
+					val[cl] = make_float2(0.f, 0.f);
+
+#pragma unroll
+				for (int iz = 0; iz < 4; iz++) {
+
+					if (l2[iz] < 0)
+						continue;
+
+					float dz = 1.f * t2.w[iz];
+
+#pragma unroll
+					for (int iy = 0; iy < 4; iy++) {
+
+						if (l1[iy] < 0)
+							continue;
+
+						float dy = dz * t1.w[iy];
+						const float2* row = reg + (l2[iz] * E + l1[iy]) * E * CCH;
+
+#pragma unroll
+						for (int ix = 0; ix < 4; ix++) {
+
+							if (l0[ix] < 0)
+								continue;
+
+							float dd = dy * t0.w[ix];
+							const float4* pp = (const float4*)(row + l0[ix] * CCH);
+							const float4 x01 = pp[0];
+							const float4 x23 = pp[1];
+							val[0].x += x01.x * dd;
+							val[0].y += x01.y * dd;
+							val[1].x += x01.z * dd;
+							val[1].y += x01.w * dd;
+							val[2].x += x23.x * dd;
+							val[2].y += x23.y * dd;
+							val[3].x += x23.z * dd;
+							val[3].y += x23.w * dd;
+						}
+					}
+				}
+
+#pragma unroll
+				for (int cl = 0; cl < CCH; cl++)
+					acc[q][cl] = cadd(acc[q][cl], val[cl]);
+			}
+		}
+
+#pragma unroll
+		for (int q = 0; q < FW_SPT; q++) {
+
+			if (!valid[q])
+				continue;
+
+			const int m = fsm[it.y + threadIdx.x + FW_THREADS * q];
+			const float2 w = (nullptr != weights) ? weights[m] : make_float2(1.f, 0.f);
+
+#pragma unroll
+			for (int cl = 0; cl < CCH; cl++)
+				if (cl < ncc)
+					ksp[(long)(c0 + cl) * g.M + m] = (nullptr != weights) ? cmul(acc[q][cl], w) : acc[q][cl];
+		}
+	}
+}
+
 void grid_forward(const GridPlan& gp, const float* kb, float2* ksp, const float2* grid, const float2* weights,
 		  hipStream_t st)
 {
@@ -1261,6 +1557,19 @@ void grid_forward(const GridPlan& gp, const float* kb, float2* ksp, const float2
 		return;
 
 	ProfScope prof("grid_forward", st);
+
+	if (gp.ft.on) {
+
+		const FwdTiling& ft = gp.ft;
+		const size_t lds = FW_TABLE_BYTES + (size_t)FW_CC * ft.Etot * sizeof(float2);
+		BAM_CHECK(4 == taps_bound(g) && !g.periodic && lds <= 64 * 1024 && FW_E == ft.E[0] &&
+			  FW_E == ft.E[1] && FW_E == ft.E[2], "grid_forward: bad tiling");
+		hipLaunchKernelGGL((k_grid_fwd_tile<FW_CC, FW_E>), dim3(ft.nitems), dim3(FW_THREADS), lds, st, g, ft,
+				   kb, gp.fwork.ptr, gp.fsm.ptr, gp.fsk.ptr, grid, weights, ksp);
+		BAM_HIP(hipGetLastError());
+		return;
+	}
+
 	const int tp = taps_bound(g);
 	// 4 coils per thread: the launch walks the samples chunk by chunk, so the
 	// grid planes in use at a time are those of 4 coils (measured on a 128^3,

@@ -67,6 +67,21 @@ struct GridTiling {
 	int fmlds;          // fftmod tables staged in LDS by the adjoint (they fit)
 };
 
+// Tiling of the forward gather of 3-D grids: samples are grouped by the
+// tile of T^3 grid points that holds their shift-0 position (their home
+// tile); a work-group stages the tile and its halo (HL below, E - T - HL
+// above, the reach of the kernel) of one sub-grid and a coil chunk in LDS.
+struct FwdTiling {
+
+	bool on = false;
+	int T;
+	int nt[3];
+	int HL[3];
+	int E[3];
+	int Etot;
+	int nitems;
+};
+
 struct GridPlan {
 
 	GridGeom g;
@@ -85,8 +100,17 @@ struct GridPlan {
 	DevBuf<int> queue;       // adjoint work queue head
 	int ncu = 1;             // compute units of the device
 
+	FwdTiling ft;            // forward gather tiling (3-D)
+	DevBuf<int> fsm;         // forward order -> original sample index
+	DevBuf<float4> fsk;      // coordinates in forward order
+	DevBuf<int4> fwork;      // forward work items (tile, first, end, 0), largest first
+
 	// traj: host, M x 3 floats (real parts)
 	void build(const GridGeom& geom, const float* traj);
+
+private:
+	void build_forward_tiles(const GridGeom& g, const std::vector<int>& hsm, const std::vector<float4>& hsk,
+				 const std::vector<int>& cell);
 };
 
 // Separable per-dimension tables for the element-wise factors of the NUFFT:
