@@ -146,6 +146,41 @@ template <> struct Dft<8> {
 	}
 };
 
+// radix 16: two radix-8 DFTs of the even and odd elements and the w16^k
+// combination
+template <> struct Dft<16> {
+	__device__ static void run(float2* v, float sgn) {
+		float2 e[8], o[8];
+#pragma unroll
+		for (int k = 0; k < 8; k++) {
+			e[k] = v[2 * k];
+			o[k] = v[2 * k + 1];
+		}
+		Dft<8>::run(e, sgn);
+		Dft<8>::run(o, sgn);
+		const float c = 0.92387953251128675613f;  // cos(pi / 8)
+		const float s = 0.38268343236508977173f;  // sin(pi / 8)
+		const float r = 0.70710678118654752440f;
+		// w16^k = cos(2 pi k / 16) + sgn i sin(2 pi k / 16), k = 1 .. 7
+		const float2 w[8] = { make_float2(1.f, 0.f),  make_float2(c, sgn * s),  make_float2(r, sgn * r),
+				      make_float2(s, sgn * c),   make_float2(0.f, sgn),    make_float2(-s, sgn * c),
+				      make_float2(-r, sgn * r), make_float2(-c, sgn * s) };
+		v[0] = cadd(e[0], o[0]);
+		v[8] = csub(e[0], o[0]);
+		const float2 o4 = mul_i(o[4], sgn);
+		v[4] = cadd(e[4], o4);
+		v[12] = csub(e[4], o4);
+#pragma unroll
+		for (int k = 1; k < 8; k++) {
+			if (4 == k)
+				continue;
+			const float2 t = cmul(o[k], w[k]);
+			v[k] = cadd(e[k], t);
+			v[k + 8] = csub(e[k], t);
+		}
+	}
+};
+
 // Generic odd prime radix (7, 11, 13): direct DFT with table twiddles.
 template <int R> struct DftGeneric {
 	__device__ static void run(float2* v, float sgn, const float2* tw, int n) {
@@ -171,7 +206,7 @@ template <int R> struct DftGeneric {
 
 template <int R> __device__ inline void dft_any(float2* v, float sgn, const float2* tw, int n)
 {
-	if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8)
+	if constexpr (R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16)
 		Dft<R>::run(v, sgn);
 	else
 		DftGeneric<R>::run(v, sgn, tw, n);
@@ -383,19 +418,34 @@ constexpr int ct_next_radix(int m)
 	return (0 == m % 8) ? 8 : ((0 == m % 4) ? 4 : ((0 == m % 2) ? 2 : ((0 == m % 3) ? 3 : ((0 == m % 5) ? 5 : m))));
 }
 
-constexpr int ct_radix(int n, int idx)
+#ifndef BAM_SLICE_R16
+#define BAM_SLICE_R16 1
+#endif
+#ifndef BAM_AXIS_R16
+#define BAM_AXIS_R16 0
+#endif
+constexpr bool SLICE_R16 = BAM_SLICE_R16;
+constexpr bool AXIS_R16 = BAM_AXIS_R16;
+
+// radix 16 first where it divides (fewer passes through LDS)
+constexpr int ct_next_radix16(int m)
+{
+	return (0 == m % 16) ? 16 : ct_next_radix(m);
+}
+
+constexpr int ct_radix(int n, int idx, bool r16 = false)
 {
 	int m = n;
 	for (int i = 0; i < idx; i++)
-		m /= ct_next_radix(m);
-	return ct_next_radix(m);
+		m /= r16 ? ct_next_radix16(m) : ct_next_radix(m);
+	return r16 ? ct_next_radix16(m) : ct_next_radix(m);
 }
 
-constexpr int ct_num_radices(int n)
+constexpr int ct_num_radices(int n, bool r16 = false)
 {
 	int m = n, c = 0;
 	while (m > 1) {
-		m /= ct_next_radix(m);
+		m /= r16 ? ct_next_radix16(m) : ct_next_radix(m);
 		c++;
 	}
 	return c;
@@ -481,8 +531,8 @@ __device__ inline void ct_pass(float2* lds, const float2* tw, float sgn)
 template <int N, int P, int IDX, int NS>
 __device__ inline void ct_passes(float2* lds, const float2* tw, float sgn)
 {
-	if constexpr (IDX < ct_num_radices(N)) {
-		constexpr int R = ct_radix(N, IDX);
+	if constexpr (IDX < ct_num_radices(N, AXIS_R16)) {
+		constexpr int R = ct_radix(N, IDX, AXIS_R16);
 		ct_pass<N, R, NS, P>(lds, tw, sgn);
 		ct_passes<N, P, IDX + 1, NS * R>(lds, tw, sgn);
 	}
@@ -697,10 +747,22 @@ __device__ inline void slice_pass(float2* lds, const float2* tw, float sgn)
 template <int N, int P, int ES, int TS, int NTH, int IDX = 0, int NS = 1>
 __device__ inline void slice_passes(float2* lds, const float2* tw, float sgn)
 {
-	if constexpr (IDX < ct_num_radices(N)) {
-		constexpr int R = ct_radix(N, IDX);
+	if constexpr (IDX < ct_num_radices(N, SLICE_R16)) {
+		constexpr int R = ct_radix(N, IDX, SLICE_R16);
 		slice_pass<N, R, NS, P, ES, TS, NTH>(lds, tw, sgn);
 		slice_passes<N, P, ES, TS, NTH, IDX + 1, NS * R>(lds, tw, sgn);
+	}
+}
+
+// The passes of a length-N transform after its first passes have built
+// sub-transforms of length NS in place (M = N / NS still to combine).
+template <int N, int M, int NS, int P, int ES, int TS, int NTH>
+__device__ inline void slice_passes_rest(float2* lds, const float2* tw, float sgn)
+{
+	if constexpr (M > 1) {
+		constexpr int R = SLICE_R16 ? ct_next_radix16(M) : ct_next_radix(M);
+		slice_pass<N, R, NS, P, ES, TS, NTH>(lds, tw, sgn);
+		slice_passes_rest<N, M / R, NS * R, P, ES, TS, NTH>(lds, tw, sgn);
 	}
 }
 

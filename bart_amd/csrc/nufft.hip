@@ -604,6 +604,16 @@ struct SliceGeom {
 	long N;
 };
 
+// the first pass along y runs in registers when a lane's points y0 + i DY
+// (i < EPT) are exactly the inputs of one radix-EPT butterfly
+template <int NX, int NY>
+constexpr bool slice_regpass()
+{
+	constexpr int DY = slice_threads<NX>() / NX;
+	constexpr int EPT = NY / DY;
+	return 0 == NY % DY && (2 == EPT || 4 == EPT || 8 == EPT || 16 == EPT);
+}
+
 template <int NX, int NY>
 constexpr size_t slice_lds_bytes()
 {
@@ -693,21 +703,48 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 		int xs = x, ys = y0;
 		asm volatile("" : "+v"(xs), "+v"(ys));
 
-		lds_barrier();
+		if constexpr (slice_regpass<NX, NY>()) {
+
+			// the lane's points y0 + i DY are the inputs of one butterfly of
+			// the first radix-EPT pass along y: that pass runs in registers,
+			// then the rest of y and the x passes in LDS
+			float2 v[EPT];
 
 #pragma unroll
-		for (int i = 0; i < EPT; i++) {
+			for (int i = 0; i < EPT; i++)
+				v[i] = cmul(vin[i], lin_xyz(T, s, xs, ys + i * DY, z));
 
-			const int y = ys + i * DY;
+			dft_any<EPT>(v, -1.f, twy, NY);
 
-			if (0 == NY % DY || y < NY)
-				lds[y * LD + xs] = cmul(vin[i], lin_xyz(T, s, xs, y, z));
+			lds_barrier();
+
+#pragma unroll
+			for (int q = 0; q < EPT; q++)
+				lds[(ys * EPT + q) * LD + xs] = v[q];
+
+			lds_barrier();
+
+			slice_passes_rest<NY, NY / EPT, EPT, NX, LD, 1, NTH>(lds, twy, -1.f);
+			slice_passes<NX, NY, 1, LD, NTH>(lds, twx, -1.f);
+
+		} else {
+
+			lds_barrier();
+
+#pragma unroll
+			for (int i = 0; i < EPT; i++) {
+
+				const int y = ys + i * DY;
+
+				if (0 == NY % DY || y < NY)
+					lds[y * LD + xs] = cmul(vin[i], lin_xyz(T, s, xs, y, z));
+			}
+
+			lds_barrier();
+
+			slice_passes<NX, NY, 1, LD, NTH>(lds, twx, -1.f);
+			slice_passes<NY, NX, LD, 1, NTH>(lds, twy, -1.f);
 		}
-
-		lds_barrier();
-
-		slice_passes<NX, NY, 1, LD, NTH>(lds, twx, -1.f);
-		slice_passes<NY, NX, LD, 1, NTH>(lds, twy, -1.f);
 
 		float2* out = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
 
@@ -774,20 +811,45 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 		int xs = x, ys = y0;
 		asm volatile("" : "+v"(xs), "+v"(ys));
 
-		lds_barrier();
+		if constexpr (slice_regpass<NX, NY>()) {
+
+			// first radix-EPT pass along y in registers (see k_slice_fwd)
+			float2 v[EPT];
+
+#pragma unroll
+			for (int i = 0; i < EPT; i++)
+				v[i] = in[(ys + i * DY) * NX + xs];
+
+			dft_any<EPT>(v, +1.f, twy, NY);
+
+			lds_barrier();
+
+#pragma unroll
+			for (int q = 0; q < EPT; q++)
+				lds[(ys * EPT + q) * LD + xs] = v[q];
+
+			lds_barrier();
+
+			slice_passes_rest<NY, NY / EPT, EPT, NX, LD, 1, NTH>(lds, twy, +1.f);
+
+		} else {
+
+			lds_barrier();
 
 #pragma unroll 4
-		for (int i = 0; i < EPT; i++) {
+			for (int i = 0; i < EPT; i++) {
 
-			const int y = ys + i * DY;
+				const int y = ys + i * DY;
 
-			if (0 == NY % DY || y < NY)
-				lds[y * LD + xs] = in[y * NX + xs];
+				if (0 == NY % DY || y < NY)
+					lds[y * LD + xs] = in[y * NX + xs];
+			}
+
+			lds_barrier();
+
+			slice_passes<NY, NX, LD, 1, NTH>(lds, twy, +1.f);
 		}
 
-		lds_barrier();
-
-		slice_passes<NY, NX, LD, 1, NTH>(lds, twy, +1.f);
 		slice_passes<NX, NY, 1, LD, NTH>(lds, twx, +1.f);
 
 		// sum over s of conj(linphase_s) x value
