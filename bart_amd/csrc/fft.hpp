@@ -538,6 +538,118 @@ __device__ inline void ct_passes(float2* lds, const float2* tw, float sgn)
 	}
 }
 
+// the passes of a length-N transform after sub-transforms of length NS have
+// been built in place, up to (not including) sub-transforms of length NSEND
+template <int N, int P, int NS, int NSEND>
+__device__ inline void ct_passes_between(float2* lds, const float2* tw, float sgn)
+{
+	if constexpr (NS < NSEND) {
+		constexpr int R = ct_next_radix(NSEND / NS);
+		ct_pass<N, R, NS, P>(lds, tw, sgn);
+		ct_passes_between<N, P, NS * R, NSEND>(lds, tw, sgn);
+	}
+}
+
+// Forward transform, pointwise multiply, inverse transform of a column
+// block (strided axis) with the first and last passes in registers: a
+// thread's E = P N / FFT_THREADS points k0 + j N / E are the inputs of one
+// butterfly of the first radix-E pass and the outputs of one butterfly of
+// the last radix-E pass, so the forward's last pass, the multiply and the
+// inverse's first pass need no LDS round trip.
+template <int N>
+constexpr int ct_edge_radix()
+{
+	return ct_block_p(N) * N / FFT_THREADS;
+}
+
+template <int N>
+constexpr bool ct_edges_in_regs()
+{
+	constexpr int E = ct_edge_radix<N>();
+	return 0 == (ct_block_p(N) * N) % FFT_THREADS && (2 == E || 4 == E || 8 == E || 16 == E) &&
+	       0 == N % (E * E) && 0 == FFT_THREADS % ct_block_p(N);
+}
+
+template <int N, int E, class Ld, class Md, class St>
+__device__ inline void fft_mul_ifft_regs(const FftLaunch& L, const Ld& ld, const Md& md, const St& st, float2* lds,
+					 const float2* tw, long t0)
+{
+	constexpr int P = ct_block_p(N);
+	constexpr int LOGP = ct_log2(P);
+	constexpr int PN = N + 1;
+	constexpr int NE = N / E;
+
+	const int p = threadIdx.x & (P - 1);
+	const int k0 = threadIdx.x >> LOGP;
+	const bool in = t0 + p < L.total;
+	float2 v[E];
+
+	// loads; first forward pass (radix E, no twiddles)
+	{
+		const auto lb = ld.base(t0);
+#pragma unroll
+		for (int j = 0; j < E; j++)
+			v[j] = in ? ld.at(lb, p, k0 + j * NE) : make_float2(0.f, 0.f);
+	}
+
+	dft_any<E>(v, L.sign1, tw, N);
+
+#pragma unroll
+	for (int q = 0; q < E; q++)
+		lds[p * PN + k0 * E + q] = v[q];
+
+	lds_barrier();
+
+	ct_passes_between<N, P, E, NE>(lds, tw, L.sign1);
+
+	// last forward pass (radix E, sub-transforms of length N / E)
+	auto last_pass = [&](float sgn) {
+#pragma unroll
+		for (int r = 0; r < E; r++) {
+			float2 x = lds[p * PN + k0 + r * NE];
+			if (r > 0) {
+				float2 w = tw[r * k0];
+				if (sgn > 0.f)
+					w.y = -w.y;
+				x = cmul(x, w);
+			}
+			v[r] = x;
+		}
+		dft_any<E>(v, sgn, tw, N);
+	};
+
+	last_pass(L.sign1);
+
+	// pointwise multiply of the points k0 + q N / E; first inverse pass
+	if (in) {
+		const auto mb = md.base(t0);
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			v[q] = md.at(mb, p, k0 + q * NE, v[q]);
+	}
+
+	dft_any<E>(v, L.sign2, tw, N);
+
+	lds_barrier();
+
+#pragma unroll
+	for (int q = 0; q < E; q++)
+		lds[p * PN + k0 * E + q] = v[q];
+
+	lds_barrier();
+
+	ct_passes_between<N, P, E, NE>(lds, tw, L.sign2);
+
+	last_pass(L.sign2);
+
+	if (in) {
+		const auto sb = st.base(t0);
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			st.at(sb, p, k0 + q * NE, v[q]);
+	}
+}
+
 template <int N, class Ld, class Md, class St>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, Md md, St st)
 {
@@ -564,6 +676,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 
 	for (int i = threadIdx.x; i < N; i += FFT_THREADS)
 		tw[i] = L.plan.tw[i];
+
+	if constexpr (CBF && Md::active && ct_edges_in_regs<N>()) {
+		if (cb) {
+			fft_mul_ifft_regs<N, ct_edge_radix<N>()>(L, ld, md, st, lds, tw, t0);
+			return;
+		}
+	}
 
 	if (L.contig) {
 
