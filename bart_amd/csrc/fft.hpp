@@ -550,12 +550,13 @@ __device__ inline void ct_passes_between(float2* lds, const float2* tw, float sg
 	}
 }
 
-// Forward transform, pointwise multiply, inverse transform of a column
-// block (strided axis) with the first and last passes in registers: a
-// thread's E = P N / FFT_THREADS points k0 + j N / E are the inputs of one
-// butterfly of the first radix-E pass and the outputs of one butterfly of
-// the last radix-E pass, so the forward's last pass, the multiply and the
-// inverse's first pass need no LDS round trip.
+// Transform of a column block (strided axis) -- or forward transform,
+// pointwise multiply and inverse transform -- with the first and last passes
+// in registers: a thread's E = P N / FFT_THREADS points k0 + j N / E are the
+// inputs of one butterfly of the first radix-E pass and the outputs of one
+// butterfly of the last radix-E pass, so the loads, the stores and (with a
+// multiply) the forward's last pass, the multiply and the inverse's first
+// pass need no LDS round trip.
 template <int N>
 constexpr int ct_edge_radix()
 {
@@ -620,27 +621,30 @@ __device__ inline void fft_mul_ifft_regs(const FftLaunch& L, const Ld& ld, const
 
 	last_pass(L.sign1);
 
-	// pointwise multiply of the points k0 + q N / E; first inverse pass
-	if (in) {
-		const auto mb = md.base(t0);
+	if constexpr (Md::active) {
+
+		// pointwise multiply of the points k0 + q N / E; first inverse pass
+		if (in) {
+			const auto mb = md.base(t0);
+#pragma unroll
+			for (int q = 0; q < E; q++)
+				v[q] = md.at(mb, p, k0 + q * NE, v[q]);
+		}
+
+		dft_any<E>(v, L.sign2, tw, N);
+
+		lds_barrier();
+
 #pragma unroll
 		for (int q = 0; q < E; q++)
-			v[q] = md.at(mb, p, k0 + q * NE, v[q]);
+			lds[p * PN + k0 * E + q] = v[q];
+
+		lds_barrier();
+
+		ct_passes_between<N, P, E, NE>(lds, tw, L.sign2);
+
+		last_pass(L.sign2);
 	}
-
-	dft_any<E>(v, L.sign2, tw, N);
-
-	lds_barrier();
-
-#pragma unroll
-	for (int q = 0; q < E; q++)
-		lds[p * PN + k0 * E + q] = v[q];
-
-	lds_barrier();
-
-	ct_passes_between<N, P, E, NE>(lds, tw, L.sign2);
-
-	last_pass(L.sign2);
 
 	if (in) {
 		const auto sb = st.base(t0);
@@ -677,7 +681,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 	for (int i = threadIdx.x; i < N; i += FFT_THREADS)
 		tw[i] = L.plan.tw[i];
 
-	if constexpr (CBF && Md::active && ct_edges_in_regs<N>()) {
+	if constexpr (CBF && ct_edges_in_regs<N>()) {
 		if (cb) {
 			fft_mul_ifft_regs<N, ct_edge_radix<N>()>(L, ld, md, st, lds, tw, t0);
 			return;
