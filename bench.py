@@ -32,9 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # scaled by 2 and WRITE_SIZE by 1: the factors at which the same counters
 # read 1 GiB for a 1 GiB device copy.
 TRAFFIC = {
-    ("3d_radial_kooshball_128^3_16coils_8192x256", "fft_toeplitz"): 5_374_000_000,
-    ("3d_radial_kooshball_128^3_16coils_8192x256", "grid_adjoint"): 4_278_000_000,
-    ("3d_radial_kooshball_128^3_16coils_8192x256", "grid_forward"): 11_396_000_000,
+    ("3d_radial_kooshball_128^3_16coils_8192x256", "fft_toeplitz"): 5_399_000_000,
+    ("3d_radial_kooshball_128^3_16coils_8192x256", "grid_adjoint"): 4_275_000_000,
+    ("3d_radial_kooshball_128^3_16coils_8192x256", "grid_forward"): 14_404_000_000,
 }
 
 
@@ -107,6 +107,57 @@ def algorithmic_bytes(kernel, w):
         "sum_coil_groups": 8 * G * N + 8 * N,
     }
     return table.get(kernel)
+
+
+def fp64_nudft_check(w, cim, ksp, nsub):
+    """Relative L2 error of the GPU NUFFT forward (coil 0, the step's last
+    output) against the oracle's FP64 direct NUDFT on nsub samples spread
+    evenly over the trajectory."""
+    import oracle as O
+    M = w["R"] * w["S"]
+    X, Y, Z = w["img"]
+    N = X * Y * Z
+    idx = np.linspace(0, M - 1, nsub).astype(np.int64)
+    img0 = cim[:N].cpu().numpy().astype(np.complex128).reshape(Z, Y, X)
+    chunk = max(1, int(2e8 // (16 * N)))
+    ref = O.nudft_forward(w["img"], w["traj"][idx], img0, chunk=chunk)[0]
+    got = ksp[:M].cpu().numpy()[idx].astype(np.complex128)
+    # samples whose kernel lies inside the grid (non-periodic gridding, as
+    # BART's default, drops the taps beyond the grid edge; the NUDFT does not)
+    k = np.abs(np.asarray(w["traj"])[idx].real)
+    lim = np.array([X, Y, Z]) / 2.0 - 2.0
+    inner = np.all((k <= lim) | (np.array([X, Y, Z]) == 1), axis=1)
+    e_in = float(np.linalg.norm(got[inner] - ref[inner]) / np.linalg.norm(ref[inner])) if inner.any() else None
+    return dict(forward_relerr=float(np.linalg.norm(got - ref) / np.linalg.norm(ref)), samples=int(nsub), coil=0,
+                forward_relerr_interior=e_in, interior_samples=int(inner.sum()))
+
+
+def explicit_vs_toeplitz(w, x, sense_t):
+    """SENSE normal operator: explicit adjoint(forward(x)) (grid / gridH) vs
+    the Toeplitz PSF operator -- times and relative difference."""
+    import torch
+    import bart_amd as ba
+    X, Y, Z = w["img"]
+    C, R, S = w["C"], w["R"], w["S"]
+    conf = ba.NufftConf.defaults(toeplitz=False)
+    sense_e = ba.sense_nc_init(dims16(X, Y, Z, C), dims16(X, Y, Z, C), w["maps"], dims16(1, R, S, C),
+                               dims16(3, R, S), ba.traj_array(w["traj"]), conf=conf)
+    out = {}
+    res = {}
+    for name, op in (("explicit", sense_e), ("toeplitz", sense_t)):
+        o = op.normal(x)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            op.normal(x, out=o)
+        e1.record()
+        torch.cuda.synchronize()
+        out[f"{name}_normal_ms"] = round(e0.elapsed_time(e1) / 10, 4)
+        res[name] = o
+    out["toeplitz_vs_explicit_relerr"] = float((res["toeplitz"] - res["explicit"]).norm() / res["explicit"].norm())
+    sense_e.free()
+    return out
 
 
 def cpu_baseline(w, threads):
@@ -433,6 +484,17 @@ def main():
                             share_of_step=round(ms_step / (t_total / args.steps * 1e3), 3),
                             bytes_per_launch=bts)
 
+    # accuracy: FP64 direct NUDFT on a sample subset (8-coil oracle leg aside)
+    accuracy = None
+    if not args.no_cpu:
+        accuracy = fp64_nudft_check(w, cim, ksp2, 64 if Z > 1 else 2048)
+        print(f"# fp64 NUDFT check: {accuracy}", file=sys.stderr, flush=True)
+
+    # explicit grid/gridH normal operator against the Toeplitz one (2-D)
+    normal_cmp = None
+    if Z == 1 and not args.no_cpu:
+        normal_cmp = explicit_vs_toeplitz(w, b, sense)
+
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -460,6 +522,8 @@ def main():
         "cg_rel_res": relres,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp64_nudft": accuracy,
+        "normal_explicit_vs_toeplitz": normal_cmp,
         "kernels_ms_per_step": {k: round(v[0], 4) for k, v in sorted(per_step.items())},
     }
     print(json.dumps(out), flush=True)
