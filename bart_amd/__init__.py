@@ -146,6 +146,13 @@ def lib():
     L.lsqr_conjgrad_batch.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_bool,
                                       ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
+    ci = ctypes.c_int
+    L.bart_amd_conv3x3.restype = ci
+    L.bart_amd_conv3x3.argtypes = [vp, ci, ci, ci, ci, vp, vp, vp, ci, ci, vp]
+    L.bart_amd_conv3x3_bwd_data.restype = ci
+    L.bart_amd_conv3x3_bwd_data.argtypes = [vp, ci, ci, ci, ci, vp, ci, vp]
+    L.bart_amd_conv3x3_bwd_weights.restype = ci
+    L.bart_amd_conv3x3_bwd_weights.argtypes = [vp, vp, ci, ci, ci, ci, ci, vp, vp]
     L.bart_amd_set_error_mode.argtypes = [ctypes.c_int]
     L.bart_amd_last_error.restype = ctypes.c_char_p
     L.bart_amd_clear_error.restype = None
@@ -368,6 +375,52 @@ def lsqr_conjgrad_batch(ops, bs, maxiter, lam=0.0, tol=0.0, x0s=None, fallback=F
     _check()
     st = [stats[i] for i in range(n)]
     return xs, st, [trace[i, :max(0, st[i].iterations)] for i in range(n)]
+
+
+def _fptr(t):
+    """Device pointer of a contiguous float32 torch tensor (or None)."""
+    if t is None:
+        return None
+    assert _is_torch(t) and t.is_cuda and t.is_contiguous() and str(t.dtype) == "torch.float32", \
+        "need a contiguous float32 CUDA tensor"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def conv3x3(x, w, bias=None, res=None, relu=False):
+    """Warm-start network layer: [relu](conv3x3(x, w) + bias) [+ res] on the
+    matrix cores (bfloat16 operands, FP32 accumulation).  x [B, Cin, H, W],
+    w [Cout, Cin, 3, 3], float32 CUDA tensors."""
+    import torch
+    B, Cin, H, W = x.shape
+    Cout = w.shape[0]
+    y = torch.empty((B, Cout, H, W), dtype=torch.float32, device=x.device)
+    lib().bart_amd_conv3x3(_fptr(x), B, Cin, H, W, _fptr(w), _fptr(bias), _fptr(res), Cout, 1 if relu else 0,
+                           _fptr(y))
+    _check()
+    return y
+
+
+def conv3x3_bwd_data(dy, w):
+    """Gradient of conv3x3 with respect to its input for output gradient dy."""
+    import torch
+    B, Cout, H, W = dy.shape
+    Cin = w.shape[1]
+    dx = torch.empty((B, Cin, H, W), dtype=torch.float32, device=dy.device)
+    lib().bart_amd_conv3x3_bwd_data(_fptr(dy), B, Cout, H, W, _fptr(w), Cin, _fptr(dx))
+    _check()
+    return dx
+
+
+def conv3x3_bwd_weights(x, dy, bias=True):
+    """Gradients of conv3x3 with respect to w [Cout, Cin, 3, 3] and bias."""
+    import torch
+    B, Cin, H, W = x.shape
+    Cout = dy.shape[1]
+    dw = torch.empty((Cout, Cin, 3, 3), dtype=torch.float32, device=x.device)
+    db = torch.empty(Cout, dtype=torch.float32, device=x.device) if bias else None
+    lib().bart_amd_conv3x3_bwd_weights(_fptr(x), _fptr(dy), B, Cin, Cout, H, W, _fptr(dw), _fptr(db))
+    _check()
+    return dw, db
 
 
 def fft(dims, flags, x, inverse=False):

@@ -1,0 +1,394 @@
+// cnn.hip -- 3x3 convolutions of the CG warm-start network on the matrix
+// cores (gfx950): BF16 operands, FP32 accumulation.
+//
+// No counterpart in BART: the network that supplies a CG initial guess (the
+// x0 of pics -W, iter/itop.c:54-66) is new.  Layout: activations
+// [B][C][H][W] float, weights [Cout][Cin][3][3] float (k = ci 9 + 3 u + v),
+// zero padding, cross-correlation:
+//   y[b][co][i][j] = bias[co] + sum_{ci,u,v} w[co][ci][u][v] x[b][ci][i+u-1][j+v-1]
+// Every layer is a GEMM of M = output channels, N = pixels, K = Cin 9 taps
+// (forward), or M = output channels, N = Cin 9 taps, K = pixels (weight
+// gradient), on v_mfma_f32_32x32x8_bf16: lane l supplies A[l % 32][4 (l / 32)
+// + t] and B[4 (l / 32) + t][l % 32], t < 4; accumulator register r of lane
+// l holds D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  Operands are rounded to
+// bfloat16 (nearest, ties to even) when they are staged into LDS.
+#include "cnn.hpp"
+
+#include <mutex>
+
+namespace bam {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ inline unsigned short f2bf(float f)
+{
+	unsigned u = __float_as_uint(f);
+	u += 0x7FFFu + ((u >> 16) & 1u);  // nearest, ties to even (finite values)
+	return (unsigned short)(u >> 16);
+}
+
+constexpr int CV_MAXC = 64;    // channels (in and out) per layer at most
+constexpr int CV_PIX = 64;     // forward: pixels (one row segment) per work-group
+constexpr int CV_FWD_THREADS = 256;
+constexpr int CV_WG_THREADS = 512;
+constexpr int CV_WG_ROWS = 8;  // weight gradient: image rows per work-group
+constexpr int CV_WG_MAXW = 256;
+
+__host__ __device__ inline int round8(int v) { return (v + 7) / 8 * 8; }
+
+// ---------------------------------------------------------------------------
+// Forward: work-group = 64 pixels of one row x 64 output channels, 4 waves of
+// one 32 x 32 tile each.  LDS: weights [64][KP + 8], the 3 x 66 input patch of
+// every input channel, and the patch offset of every k.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __restrict__ x, int Cin, int H, int W,
+								const float* __restrict__ w,
+								const float* __restrict__ bias,
+								const float* __restrict__ res, int Cout, int relu,
+								float* __restrict__ y)
+{
+	extern __shared__ __align__(16) unsigned char cv_smem[];
+	const int K = Cin * 9;
+	const int KP = round8(K);
+	const int WP = KP + 8;
+	unsigned short* wl = (unsigned short*)cv_smem;          // [64][WP]
+	unsigned short* xl = wl + CV_MAXC * WP;                 // [Cin][3][66]
+	int* koff = (int*)(xl + round8(Cin * 3 * 66));          // [KP]
+
+	const int x0 = blockIdx.x * CV_PIX;
+	const int yy = blockIdx.y;
+	const int b = blockIdx.z;
+
+	for (int e = threadIdx.x; e < CV_MAXC * KP; e += CV_FWD_THREADS) {
+		const int co = e / KP;
+		const int k = e - co * KP;
+		wl[co * WP + k] = f2bf((co < Cout && k < K) ? w[co * K + k] : 0.f);
+	}
+
+	for (int e = threadIdx.x; e < Cin * 198; e += CV_FWD_THREADS) {
+		const int ci = e / 198;
+		const int r = e - ci * 198;
+		const int u = r / 66;
+		const int gx = x0 + r - u * 66 - 1;
+		const int gy = yy + u - 1;
+		float v = 0.f;
+		if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+			v = x[((long)(b * Cin + ci) * H + gy) * W + gx];
+		xl[e] = f2bf(v);
+	}
+
+	for (int k = threadIdx.x; k < KP; k += CV_FWD_THREADS) {
+		const int ci = k / 9;
+		const int t = k - ci * 9;
+		koff[k] = (k < K) ? ci * 198 + (t / 3) * 66 + (t % 3) : -1;
+	}
+
+	__syncthreads();
+
+	const int lane = threadIdx.x & 63;
+	const int wv = threadIdx.x >> 6;
+	const int cot = wv & 1;
+	const int pt = wv >> 1;
+	const int i = lane & 31;
+	const int hk = lane >> 5;
+	const unsigned short* wrow = wl + (cot * 32 + i) * WP;
+	const int px = pt * 32 + i;
+
+	f32x16 acc;
+#pragma unroll
+	for (int r = 0; r < 16; r++)
+		acc[r] = 0.f;
+
+	for (int k0 = 0; k0 < KP; k0 += 8) {
+
+		const int kb = k0 + 4 * hk;
+		const bf16x4 a = *(const bf16x4*)(wrow + kb);
+		bf16x4 bv;
+#pragma unroll
+		for (int t = 0; t < 4; t++) {
+			const int o = koff[kb + t];
+			bv[t] = (o >= 0) ? (short)xl[o + px] : (short)0;
+		}
+		acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bv, acc, 0, 0, 0);
+	}
+
+	const int gx = x0 + px;
+
+	if (gx < W) {
+#pragma unroll
+		for (int r = 0; r < 16; r++) {
+			const int co = cot * 32 + 8 * (r / 4) + 4 * hk + (r % 4);
+			if (co < Cout) {
+				float v = acc[r] + ((nullptr != bias) ? bias[co] : 0.f);
+				if (relu)
+					v = fmaxf(v, 0.f);
+				const long o = ((long)(b * Cout + co) * H + yy) * W + gx;
+				if (nullptr != res)
+					v += res[o];
+				y[o] = v;
+			}
+		}
+	}
+}
+
+static size_t fwd_lds(int Cin)
+{
+	const int KP = round8(Cin * 9);
+	return sizeof(unsigned short) * ((size_t)CV_MAXC * (KP + 8) + round8(Cin * 3 * 66)) + sizeof(int) * KP;
+}
+
+// w'[ci][co][u][v] = w[co][ci][2 - u][2 - v]: the weights of the transposed
+// convolution (the data gradient)
+__global__ void k_conv_flip(const float* __restrict__ w, int Cout, int Cin, float* __restrict__ wt)
+{
+	const int n = Cout * Cin * 9;
+
+	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+		const int co = e / (Cin * 9);
+		const int r = e - co * Cin * 9;
+		const int ci = r / 9;
+		const int t = r - ci * 9;
+		wt[(ci * Cout + co) * 9 + (8 - t)] = w[e];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: work-group = CV_WG_ROWS rows of one image; per row the
+// output-gradient row (M = Cout x K = W pixels) and the 3 input rows around
+// it (N = Cin 9 taps) are staged in LDS; 8 waves share the
+// ceil(Cout / 32) x ceil(Cin 9 / 32) tiles.  Each work-group writes its
+// partial [64][NP] gradient; k_conv_wgrad_sum adds the partials in order.
+// ---------------------------------------------------------------------------
+
+constexpr int CV_WG_TILES = 4;  // tiles per wave at most (2 x 14 tiles over 8 waves)
+
+__global__ __launch_bounds__(CV_WG_THREADS) void k_conv3x3_wgrad(const float* __restrict__ x,
+								  const float* __restrict__ dy, int Cin, int Cout,
+								  int H, int W, float* __restrict__ part)
+{
+	extern __shared__ __align__(16) unsigned char cv_smem[];
+	const int K = Cin * 9;
+	const int NP = (K + 31) / 32 * 32;
+	const int WX = W + 2;
+	unsigned short* dyl = (unsigned short*)cv_smem;         // [64][W]
+	unsigned short* xl = dyl + CV_MAXC * W;                 // [Cin][3][W + 2]
+	int* koff = (int*)(xl + round8(Cin * 3 * WX));          // [NP]
+
+	const int nrg = (H + CV_WG_ROWS - 1) / CV_WG_ROWS;
+	const int b = blockIdx.x / nrg;
+	const int y0 = (blockIdx.x - b * nrg) * CV_WG_ROWS;
+
+	for (int k = threadIdx.x; k < NP; k += CV_WG_THREADS) {
+		const int ci = k / 9;
+		const int t = k - ci * 9;
+		koff[k] = (k < K) ? (ci * 3 + t / 3) * WX + (t % 3) : -1;
+	}
+
+	const int lane = threadIdx.x & 63;
+	const int wv = threadIdx.x >> 6;
+	const int i = lane & 31;
+	const int hk = lane >> 5;
+	const int ncot = (Cout + 31) / 32;
+	const int nkt = NP / 32;
+	const int ntiles = ncot * nkt;
+
+	f32x16 acc[CV_WG_TILES];
+#pragma unroll
+	for (int q = 0; q < CV_WG_TILES; q++)
+#pragma unroll
+		for (int r = 0; r < 16; r++)
+			acc[q][r] = 0.f;
+
+	for (int yy = y0; yy < min(H, y0 + CV_WG_ROWS); yy++) {
+
+		__syncthreads();
+
+		for (int e = threadIdx.x; e < CV_MAXC * W; e += CV_WG_THREADS) {
+			const int co = e / W;
+			const int xx = e - co * W;
+			dyl[e] = f2bf((co < Cout) ? dy[((long)(b * Cout + co) * H + yy) * W + xx] : 0.f);
+		}
+
+		for (int e = threadIdx.x; e < Cin * 3 * WX; e += CV_WG_THREADS) {
+			const int cu = e / WX;
+			const int ci = cu / 3;
+			const int u = cu - ci * 3;
+			const int gx = e - cu * WX - 1;
+			const int gy = yy + u - 1;
+			float v = 0.f;
+			if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+				v = x[((long)(b * Cin + ci) * H + gy) * W + gx];
+			xl[e] = f2bf(v);
+		}
+
+		__syncthreads();
+
+#pragma unroll
+		for (int q = 0; q < CV_WG_TILES; q++) {
+
+			const int tile = wv + 8 * q;
+
+			if (tile >= ntiles)
+				break;
+
+			const int cot = tile / nkt;
+			const int kt = tile - cot * nkt;
+			const unsigned short* arow = dyl + (cot * 32 + i) * W;
+			const int o = koff[kt * 32 + i];
+
+			for (int p0 = 0; p0 < W; p0 += 8) {
+
+				const int pb = p0 + 4 * hk;
+				const bf16x4 a = *(const bf16x4*)(arow + pb);
+				bf16x4 bv;
+#pragma unroll
+				for (int t = 0; t < 4; t++)
+					bv[t] = (o >= 0) ? (short)xl[o + pb + t] : (short)0;
+				acc[q] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bv, acc[q], 0, 0, 0);
+			}
+		}
+	}
+
+	// partial [64][NP] of this work-group
+	float* pp = part + (long)blockIdx.x * CV_MAXC * NP;
+
+#pragma unroll
+	for (int q = 0; q < CV_WG_TILES; q++) {
+
+		const int tile = wv + 8 * q;
+
+		if (tile >= ntiles)
+			break;
+
+		const int cot = tile / nkt;
+		const int kt = tile - cot * nkt;
+
+#pragma unroll
+		for (int r = 0; r < 16; r++) {
+			const int co = cot * 32 + 8 * (r / 4) + 4 * hk + (r % 4);
+			pp[co * NP + kt * 32 + i] = acc[q][r];
+		}
+	}
+}
+
+// dw[co][k] = sum over the partials in work-group order
+__global__ void k_conv_wgrad_sum(const float* __restrict__ part, int nparts, int Cout, int K, int NP,
+				 float* __restrict__ dw)
+{
+	const int n = Cout * K;
+
+	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+		const int co = e / K;
+		const int k = e - co * K;
+		float s = 0.f;
+		for (int p = 0; p < nparts; p++)
+			s += part[((long)p * CV_MAXC + co) * NP + k];
+		dw[e] = s;
+	}
+}
+
+// db[co] = sum over b, i, j of dy (one work-group per channel, fixed order)
+__global__ __launch_bounds__(256) void k_conv_bgrad(const float* __restrict__ dy, int B, int Cout, long HW,
+						     float* __restrict__ db)
+{
+	__shared__ float sh[256];
+	const int co = blockIdx.x;
+	float s = 0.f;
+
+	for (int b = 0; b < B; b++)
+		for (long e = threadIdx.x; e < HW; e += 256)
+			s += dy[((long)b * Cout + co) * HW + e];
+
+	sh[threadIdx.x] = s;
+	__syncthreads();
+
+	for (int h = 128; h > 0; h >>= 1) {
+		if (threadIdx.x < h)
+			sh[threadIdx.x] += sh[threadIdx.x + h];
+		__syncthreads();
+	}
+
+	if (0 == threadIdx.x)
+		db[co] = sh[0];
+}
+
+static size_t wgrad_lds(int Cin, int W)
+{
+	const int NP = (Cin * 9 + 31) / 32 * 32;
+	return sizeof(unsigned short) * ((size_t)CV_MAXC * W + round8(Cin * 3 * (W + 2))) + sizeof(int) * NP;
+}
+
+// work buffers of the weight gradient and the flipped weights
+struct CnnWork {
+	std::mutex mtx;
+	DevBuf<float> part;
+	DevBuf<float> wt;
+};
+
+static CnnWork& cnn_work()
+{
+	static CnnWork w;
+	return w;
+}
+
+static void check_shape(int B, int Cin, int Cout, int H, int W)
+{
+	BAM_CHECK(B >= 1 && H >= 1 && W >= 1, "conv3x3: bad shape");
+	BAM_CHECK(Cin >= 1 && Cin <= CV_MAXC && Cout >= 1 && Cout <= CV_MAXC,
+		  "conv3x3: 1 to %d channels in and out (got %d, %d)", CV_MAXC, Cin, Cout);
+}
+
+void conv3x3_fwd(const float* x, int B, int Cin, int H, int W, const float* w, const float* bias, const float* res,
+		 int Cout, bool relu, float* y, hipStream_t st)
+{
+	check_shape(B, Cin, Cout, H, W);
+	BAM_CHECK(H <= 65535 && B <= 65535, "conv3x3: too many rows or images");
+	dim3 grid((W + CV_PIX - 1) / CV_PIX, H, B);
+	hipLaunchKernelGGL(k_conv3x3_fwd, grid, dim3(CV_FWD_THREADS), fwd_lds(Cin), st, x, Cin, H, W, w, bias, res, Cout,
+			   relu ? 1 : 0, y);
+	BAM_HIP(hipGetLastError());
+}
+
+void conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, const float* w, int Cin, float* dx,
+		      hipStream_t st)
+{
+	check_shape(B, Cin, Cout, H, W);
+	CnnWork& cw = cnn_work();
+	std::lock_guard<std::mutex> lock(cw.mtx);
+	const long n = (long)Cout * Cin * 9;
+	if ((long)cw.wt.n < n)
+		cw.wt.alloc(n);
+	hipLaunchKernelGGL(k_conv_flip, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w, Cout, Cin, cw.wt.ptr);
+	BAM_HIP(hipGetLastError());
+	conv3x3_fwd(dy, B, Cout, H, W, cw.wt.ptr, nullptr, nullptr, Cin, false, dx, st);
+	BAM_HIP(hipStreamSynchronize(st));  // the flipped weights are reused by the next call
+}
+
+void conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin, int Cout, int H, int W, float* dw,
+			 float* db, hipStream_t st)
+{
+	check_shape(B, Cin, Cout, H, W);
+	BAM_CHECK(W <= CV_WG_MAXW && 0 == W % 8, "conv3x3 weight gradient: width a multiple of 8, at most %d",
+		  CV_WG_MAXW);
+	CnnWork& cw = cnn_work();
+	std::lock_guard<std::mutex> lock(cw.mtx);
+	const int NP = (Cin * 9 + 31) / 32 * 32;
+	BAM_CHECK(((Cout + 31) / 32) * (NP / 32) <= 8 * CV_WG_TILES, "conv3x3 weight gradient: too many tiles");
+	const int nparts = B * ((H + CV_WG_ROWS - 1) / CV_WG_ROWS);
+	const long np = (long)nparts * CV_MAXC * NP;
+	if ((long)cw.part.n < np)
+		cw.part.alloc(np);
+	hipLaunchKernelGGL(k_conv3x3_wgrad, dim3(nparts), dim3(CV_WG_THREADS), wgrad_lds(Cin, W), st, x, dy, Cin, Cout,
+			   H, W, cw.part.ptr);
+	const int n = Cout * Cin * 9;
+	hipLaunchKernelGGL(k_conv_wgrad_sum, dim3((n + 255) / 256), dim3(256), 0, st, cw.part.ptr, nparts, Cout,
+			   Cin * 9, NP, dw);
+	if (nullptr != db)
+		hipLaunchKernelGGL(k_conv_bgrad, dim3(Cout), dim3(256), 0, st, dy, B, Cout, (long)H * W, db);
+	BAM_HIP(hipGetLastError());
+	BAM_HIP(hipStreamSynchronize(st));  // the partials are reused by the next call
+}
+
+}  // namespace bam

@@ -4,6 +4,7 @@
 #include "../../include/bart_amd.h"
 
 #include "cg.hpp"
+#include "cnn.hpp"
 #include "nudft.hpp"
 #include "nufft_impl.hpp"
 
@@ -951,6 +952,50 @@ int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxite
 		}
 
 		return maxit;
+	}, -1);
+}
+
+// ---------------------------------------------------------------------------
+// Layers of the CG warm-start network (no counterpart in BART)
+// ---------------------------------------------------------------------------
+
+static void check_dev(const char* what, const void* p)
+{
+	BAM_CHECK(nullptr == p || on_device(p), "%s: pointers must be device memory", what);
+}
+
+int bart_amd_conv3x3(const float* x, int B, int Cin, int H, int W, const float* w, const float* bias,
+		     const float* res, int Cout, int relu, float* y)
+{
+	return guarded([&]() -> int {
+		BAM_CHECK(nullptr != x && nullptr != w && nullptr != y, "bart_amd_conv3x3: NULL argument");
+		for (const void* p : { (const void*)x, (const void*)w, (const void*)bias, (const void*)res, (const void*)y })
+			check_dev("bart_amd_conv3x3", p);
+		conv3x3_fwd(x, B, Cin, H, W, w, bias, res, Cout, 0 != relu, y, default_stream());
+		return 0;
+	}, -1);
+}
+
+int bart_amd_conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, const float* w, int Cin, float* dx)
+{
+	return guarded([&]() -> int {
+		BAM_CHECK(nullptr != dy && nullptr != w && nullptr != dx, "bart_amd_conv3x3_bwd_data: NULL argument");
+		for (const void* p : { (const void*)dy, (const void*)w, (const void*)dx })
+			check_dev("bart_amd_conv3x3_bwd_data", p);
+		conv3x3_bwd_data(dy, B, Cout, H, W, w, Cin, dx, default_stream());
+		return 0;
+	}, -1);
+}
+
+int bart_amd_conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin, int Cout, int H, int W, float* dw,
+				 float* db)
+{
+	return guarded([&]() -> int {
+		BAM_CHECK(nullptr != x && nullptr != dy && nullptr != dw, "bart_amd_conv3x3_bwd_weights: NULL argument");
+		for (const void* p : { (const void*)x, (const void*)dy, (const void*)dw, (const void*)db })
+			check_dev("bart_amd_conv3x3_bwd_weights", p);
+		conv3x3_bwd_weights(x, dy, B, Cin, Cout, H, W, dw, db, default_stream());
+		return 0;
 	}, -1);
 }
 

@@ -158,6 +158,24 @@ int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxite
 			BAM_COMPLEX* const x[], const BAM_COMPLEX* const b[], bool warmstart, bool warm_fallback,
 			float* resid_trace, struct cg_stats_s stats[]);
 
+/* Layers of the CG warm-start network: no counterpart in BART (the network
+ * that proposes x0 for pics -W, iter/itop.c:54-66, is new).  3x3
+ * convolution, zero padding, cross-correlation, on the matrix cores with
+ * bfloat16 operands (rounded to nearest even) and FP32 accumulation:
+ *   y[b][co][i][j] = bias[co] + sum_{ci,u,v} w[co][ci][u][v] x[b][ci][i+u-1][j+v-1]
+ * Device pointers only; activations [B][C][H][W], weights [Cout][Cin][3][3],
+ * float; 1 to 64 channels.  Launched on the null stream, not synchronised.
+ * Return 0, or -1 on error. */
+/* y = [relu](conv(x, w) + bias) [+ res]; bias and res may be NULL */
+int bart_amd_conv3x3(const float* x, int B, int Cin, int H, int W, const float* w, const float* bias,
+		     const float* res, int Cout, int relu, float* y);
+/* dx = gradient with respect to x for the output gradient dy */
+int bart_amd_conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, const float* w, int Cin, float* dx);
+/* dw, db = gradients with respect to w and bias (db may be NULL); W a
+ * multiple of 8, at most 256 */
+int bart_amd_conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin, int Cout, int H, int W, float* dw,
+				 float* db);
+
 /* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
 enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };
 void bart_amd_set_error_mode(int mode);
