@@ -14,7 +14,10 @@
 // bfloat16 (nearest, ties to even) when they are staged into LDS.
 #include "cnn.hpp"
 
+#include <cmath>
 #include <mutex>
+#include <random>
+#include <vector>
 
 namespace bam {
 
@@ -389,6 +392,179 @@ void conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin, int Co
 		hipLaunchKernelGGL(k_conv_bgrad, dim3(Cout), dim3(256), 0, st, dy, B, Cout, (long)H * W, db);
 	BAM_HIP(hipGetLastError());
 	BAM_HIP(hipStreamSynchronize(st));  // the partials are reused by the next call
+}
+
+}  // namespace bam
+
+// ---------------------------------------------------------------------------
+// The warm-start network: 6 convolution layers 2 -> C -> C -> C -> C -> C -> 2
+// with ReLU after the first five and the input added to the output
+// (residual), trained on mean squared error with Adam.
+// ---------------------------------------------------------------------------
+
+namespace bam {
+
+// g[i] = 0 where the layer's output a[i] was clipped by the ReLU
+__global__ void k_relu_bwd(float* __restrict__ g, const float* __restrict__ a, long n)
+{
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+		if (!(a[i] > 0.f))
+			g[i] = 0.f;
+}
+
+// d = 2 (out - t) / n; per-work-group partial sums of (out - t)^2
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out, const float* __restrict__ t, long n,
+					      float* __restrict__ d, double* __restrict__ part)
+{
+	__shared__ double sh[256];
+	double s = 0.;
+
+	for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+		const float e = out[i] - t[i];
+		d[i] = 2.f * e / (float)n;
+		s += (double)e * (double)e;
+	}
+
+	sh[threadIdx.x] = s;
+	__syncthreads();
+
+	for (int h = 128; h > 0; h >>= 1) {
+		if (threadIdx.x < h)
+			sh[threadIdx.x] += sh[threadIdx.x + h];
+		__syncthreads();
+	}
+
+	if (0 == threadIdx.x)
+		part[blockIdx.x] = sh[0];
+}
+
+// Adam (Kingma & Ba, ICLR 2015) with bias correction
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+		       float* __restrict__ v, long n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+		const float gi = g[i];
+		const float mi = b1 * m[i] + (1.f - b1) * gi;
+		const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+		m[i] = mi;
+		v[i] = vi;
+		p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+	}
+}
+
+constexpr int WN_LAYERS = 6;
+constexpr int WN_MSE_BLOCKS = 256;
+
+WarmNet::WarmNet(int channels, unsigned long seed)
+{
+	BAM_CHECK(channels >= 1 && channels <= CV_MAXC, "warm-start network: 1 to %d channels", CV_MAXC);
+	ch[0] = 2;
+	for (int l = 1; l < WN_LAYERS; l++)
+		ch[l] = channels;
+	ch[WN_LAYERS] = 2;
+
+	long n = 0;
+	for (int l = 0; l < WN_LAYERS; l++) {
+		woff[l] = n;
+		n += (long)ch[l + 1] * ch[l] * 9;
+		boff[l] = n;
+		n += ch[l + 1];
+	}
+	nparams = n;
+
+	// He-normal weights (std sqrt(2 / fan-in)); the last layer x 0.1 so that
+	// the untrained network is close to the identity; zero biases
+	std::vector<float> h(n, 0.f);
+	std::mt19937_64 rng(seed);
+	std::normal_distribution<double> nd(0., 1.);
+	for (int l = 0; l < WN_LAYERS; l++) {
+		const double sd = sqrt(2. / (9. * ch[l])) * ((WN_LAYERS - 1 == l) ? 0.1 : 1.);
+		for (long i = 0; i < (long)ch[l + 1] * ch[l] * 9; i++)
+			h[woff[l] + i] = (float)(sd * nd(rng));
+	}
+
+	params.alloc(n);
+	grads.alloc(n);
+	m.alloc(n);
+	v.alloc(n);
+	BAM_HIP(hipMemcpy(params.ptr, h.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+	BAM_HIP(hipMemset(m.ptr, 0, sizeof(float) * n));
+	BAM_HIP(hipMemset(v.ptr, 0, sizeof(float) * n));
+	lpart.alloc(WN_MSE_BLOCKS);
+}
+
+void WarmNet::ensure(int B, int H, int W)
+{
+	const long hid = (long)B * ch[1] * H * W;
+	if ((long)act.n < (WN_LAYERS - 1) * hid)
+		act.alloc((WN_LAYERS - 1) * hid);
+	if ((long)g0.n < hid) {
+		g0.alloc(hid);
+		g1.alloc(hid);
+	}
+	const long io = (long)B * 2 * H * W;
+	if ((long)out.n < io)
+		out.alloc(io);
+}
+
+// a_l: input of layer l (a_0 = x)
+const float* WarmNet::a(int l, const float* x, int B, int H, int W) const
+{
+	return (0 == l) ? x : act.ptr + (long)(l - 1) * B * ch[1] * H * W;
+}
+
+void WarmNet::forward(const float* x, int B, int H, int W, float* y, hipStream_t st)
+{
+	ensure(B, H, W);
+
+	for (int l = 0; l < WN_LAYERS; l++) {
+		const bool last = WN_LAYERS - 1 == l;
+		float* dst = last ? y : (float*)a(l + 1, x, B, H, W);
+		conv3x3_fwd(a(l, x, B, H, W), B, ch[l], H, W, params.ptr + woff[l], params.ptr + boff[l],
+			    last ? x : nullptr, ch[l + 1], !last, dst, st);
+	}
+}
+
+double WarmNet::train_step(const float* x, const float* t, int B, int H, int W, float lr, hipStream_t st)
+{
+	ensure(B, H, W);
+	forward(x, B, H, W, out.ptr, st);
+
+	const long n = (long)B * 2 * H * W;
+	hipLaunchKernelGGL(k_mse, dim3(WN_MSE_BLOCKS), dim3(256), 0, st, out.ptr, t, n, g0.ptr, lpart.ptr);
+	BAM_HIP(hipGetLastError());
+
+	float* g = g0.ptr;      // gradient at the output of layer l
+	float* gn = g1.ptr;
+
+	for (int l = WN_LAYERS - 1; l >= 0; l--) {
+
+		const float* al = a(l, x, B, H, W);
+		conv3x3_bwd_weights(al, g, B, ch[l], ch[l + 1], H, W, grads.ptr + woff[l], grads.ptr + boff[l], st);
+
+		if (l > 0) {
+			conv3x3_bwd_data(g, B, ch[l + 1], H, W, params.ptr + woff[l], ch[l], gn, st);
+			const long na = (long)B * ch[l] * H * W;
+			hipLaunchKernelGGL(k_relu_bwd, dim3(1024), dim3(256), 0, st, gn, al, na);
+			BAM_HIP(hipGetLastError());
+			std::swap(g, gn);
+		}
+	}
+
+	step++;
+	const float b1 = 0.9f, b2 = 0.999f;
+	const float c1 = 1.f - (float)pow(b1, step), c2 = 1.f - (float)pow(b2, step);
+	hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, st, params.ptr, grads.ptr, m.ptr, v.ptr, nparams, lr, b1, b2,
+			   1e-8f, c1, c2);
+	BAM_HIP(hipGetLastError());
+
+	std::vector<double> hp(WN_MSE_BLOCKS);
+	BAM_HIP(hipMemcpyAsync(hp.data(), lpart.ptr, sizeof(double) * WN_MSE_BLOCKS, hipMemcpyDeviceToHost, st));
+	BAM_HIP(hipStreamSynchronize(st));
+	double s = 0.;
+	for (double v : hp)
+		s += v;
+	return s / (double)n;
 }
 
 }  // namespace bam

@@ -15,4 +15,25 @@ void conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, const floa
 void conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin, int Cout, int H, int W, float* dw,
 			 float* db, hipStream_t st);
 
+// The warm-start network (2 -> C x 5 -> 2 channels, ReLU, residual), its
+// parameters packed [W0 b0 W1 b1 ...] in one buffer, Adam state beside them.
+struct WarmNet {
+
+	int ch[7];
+	long woff[6], boff[6];
+	long nparams = 0;
+	int step = 0;
+	DevBuf<float> params, grads, m, v;
+	DevBuf<float> act, g0, g1, out;  // layer outputs 1..5, gradient ping-pong, output
+	DevBuf<double> lpart;
+
+	WarmNet(int channels, unsigned long seed);
+	void ensure(int B, int H, int W);
+	const float* a(int l, const float* x, int B, int H, int W) const;
+	// y = x + net(x); x, y [B][2][H][W] device
+	void forward(const float* x, int B, int H, int W, float* y, hipStream_t st);
+	// one Adam step on mean((x + net(x) - t)^2); returns the loss before the step
+	double train_step(const float* x, const float* t, int B, int H, int W, float lr, hipStream_t st);
+};
+
 }  // namespace bam

@@ -999,4 +999,65 @@ int bart_amd_conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin
 	}, -1);
 }
 
+struct bart_amd_warmnet* bart_amd_warmnet_create(int channels, unsigned long seed)
+{
+	return guarded([&]() -> struct bart_amd_warmnet* {
+		return (struct bart_amd_warmnet*)new WarmNet(channels, seed);
+	}, nullptr);
+}
+
+void bart_amd_warmnet_free(struct bart_amd_warmnet* net)
+{
+	guarded_void([&]() {
+		BAM_HIP(hipStreamSynchronize(default_stream()));
+		delete (WarmNet*)net;
+	});
+}
+
+long bart_amd_warmnet_num_params(const struct bart_amd_warmnet* net)
+{
+	return guarded([&]() -> long {
+		BAM_CHECK(nullptr != net, "bart_amd_warmnet_num_params: NULL network");
+		return ((const WarmNet*)net)->nparams;
+	}, -1l);
+}
+
+int bart_amd_warmnet_params(struct bart_amd_warmnet* _net, float* host, int set)
+{
+	return guarded([&]() -> int {
+		WarmNet* net = (WarmNet*)_net;
+		BAM_CHECK(nullptr != net && nullptr != host, "bart_amd_warmnet_params: NULL argument");
+		BAM_CHECK(set >= 0 && set <= 2, "bart_amd_warmnet_params: mode 0, 1 or 2");
+		BAM_HIP(hipStreamSynchronize(default_stream()));
+		if (1 == set)
+			BAM_HIP(hipMemcpy(net->params.ptr, host, sizeof(float) * net->nparams, hipMemcpyHostToDevice));
+		else
+			BAM_HIP(hipMemcpy(host, (0 == set) ? net->params.ptr : net->grads.ptr, sizeof(float) * net->nparams,
+					  hipMemcpyDeviceToHost));
+		return 0;
+	}, -1);
+}
+
+int bart_amd_warmnet_forward(struct bart_amd_warmnet* net, const float* x, int B, int H, int W, float* y)
+{
+	return guarded([&]() -> int {
+		BAM_CHECK(nullptr != net && nullptr != x && nullptr != y, "bart_amd_warmnet_forward: NULL argument");
+		check_dev("bart_amd_warmnet_forward", x);
+		check_dev("bart_amd_warmnet_forward", y);
+		((WarmNet*)net)->forward(x, B, H, W, y, default_stream());
+		return 0;
+	}, -1);
+}
+
+double bart_amd_warmnet_train_step(struct bart_amd_warmnet* net, const float* x, const float* target, int B, int H,
+				   int W, float lr)
+{
+	return guarded([&]() -> double {
+		BAM_CHECK(nullptr != net && nullptr != x && nullptr != target, "bart_amd_warmnet_train_step: NULL argument");
+		check_dev("bart_amd_warmnet_train_step", x);
+		check_dev("bart_amd_warmnet_train_step", target);
+		return ((WarmNet*)net)->train_step(x, target, B, H, W, lr, default_stream());
+	}, -1.);
+}
+
 }  // extern "C"

@@ -176,6 +176,24 @@ int bart_amd_conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, co
 int bart_amd_conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin, int Cout, int H, int W, float* dw,
 				 float* db);
 
+/* The warm-start network (no counterpart in BART): six of the layers above,
+ * 2 -> channels (x5) -> 2, ReLU after the first five, the input added to the
+ * output; trained on the mean squared error with Adam (lr given, betas 0.9
+ * and 0.999, eps 1e-8).  x, y, target: device [B][2][H][W] (real and
+ * imaginary parts of a complex image).  Parameters are packed
+ * [W0 b0 W1 b1 ... W5 b5]; creation draws He-normal weights from the seed. */
+struct bart_amd_warmnet;
+struct bart_amd_warmnet* bart_amd_warmnet_create(int channels, unsigned long seed);
+void bart_amd_warmnet_free(struct bart_amd_warmnet* net);
+long bart_amd_warmnet_num_params(const struct bart_amd_warmnet* net);
+/* host buffer of num_params floats: set = 0 reads the parameters, 1 writes
+ * them, 2 reads the gradients of the last training step */
+int bart_amd_warmnet_params(struct bart_amd_warmnet* net, float* host, int set);
+int bart_amd_warmnet_forward(struct bart_amd_warmnet* net, const float* x, int B, int H, int W, float* y);
+/* one Adam step; returns the loss of x before the step, or -1 on error */
+double bart_amd_warmnet_train_step(struct bart_amd_warmnet* net, const float* x, const float* target, int B, int H,
+				   int W, float lr);
+
 /* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
 enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };
 void bart_amd_set_error_mode(int mode);

@@ -87,3 +87,65 @@ def bf16(a):
     u = a.view(np.uint32).astype(np.uint64)
     u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def net_channels(channels):
+    return [2] + [channels] * 5 + [2]
+
+
+def net_unpack(params, channels):
+    """Packed [W0 b0 W1 b1 ... W5 b5] -> [(W, b), ...]."""
+    ch = net_channels(channels)
+    p = np.asarray(params, np.float64)
+    out, o = [], 0
+    for l in range(6):
+        nw = ch[l + 1] * ch[l] * 9
+        out.append((p[o:o + nw].reshape(ch[l + 1], ch[l], 3, 3), p[o + nw:o + nw + ch[l + 1]]))
+        o += nw + ch[l + 1]
+    return out
+
+
+def _ident(a):
+    return np.asarray(a, np.float64)
+
+
+def net_forward(params, x, channels, rounding=True):
+    """x + net(x) for the warm-start network (6 convolutions, ReLU after the
+    first five).  rounding: operands of every convolution rounded to
+    bfloat16, as the matrix-core kernels do.  Returns (out, layer inputs)."""
+    rd = bf16 if rounding else _ident
+    x = np.asarray(x, np.float64)
+    acts = [x]
+    a = x
+    layers = net_unpack(params, channels)
+    for l, (W, b) in enumerate(layers):
+        z = conv3x3(rd(a), rd(W), b)
+        if l == len(layers) - 1:
+            return z + x, acts
+        a = np.maximum(z, 0.0)
+        if rounding:
+            a = a.astype(np.float32).astype(np.float64)  # the kernels store layer outputs in float32
+        acts.append(a)
+
+
+def net_grads(params, x, t, channels, rounding=True):
+    """Gradient of mean((x + net(x) - t)^2) with respect to the packed
+    parameters, and the loss, by backpropagation."""
+    rd = bf16 if rounding else _ident
+    out, acts = net_forward(params, x, channels, rounding)
+    n = out.size
+    g = 2.0 * (out - np.asarray(t, np.float64)) / n
+    if rounding:
+        g = g.astype(np.float32).astype(np.float64)
+    layers = net_unpack(params, channels)
+    grads = [None] * 6
+    for l in range(5, -1, -1):
+        W, _ = layers[l]
+        dW = conv3x3_bwd_weights(rd(acts[l]), rd(g))[0]
+        grads[l] = (dW, g.sum(axis=(0, 2, 3)))
+        if l > 0:
+            g = conv3x3_bwd_data(rd(g), rd(W)) * (acts[l] > 0)
+            if rounding:
+                g = g.astype(np.float32).astype(np.float64)
+    flat = np.concatenate([np.concatenate([dW.ravel(), db]) for dW, db in grads])
+    return flat, float(np.mean((out - t) ** 2))

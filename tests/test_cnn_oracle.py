@@ -47,3 +47,21 @@ def test_bf16_rounding():
     v = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -2.5, 3.0e38], np.float32)
     r = R.bf16(v)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2.0 ** -6 and r[3] == -2.5
+
+
+def test_net_grads_finite_differences():
+    # backpropagation of the oracle network against central differences
+    rng = np.random.default_rng(5)
+    ch = 3
+    sizes = R.net_channels(ch)
+    n = sum(sizes[l + 1] * sizes[l] * 9 + sizes[l + 1] for l in range(6))
+    p = rng.standard_normal(n) * 0.3
+    x = rng.standard_normal((2, 2, 5, 4))
+    t = rng.standard_normal((2, 2, 5, 4))
+    g, loss = R.net_grads(p, x, t, ch, rounding=False)
+    h = 1e-6
+    for i in rng.choice(n, 12, replace=False):
+        pp = p.copy(); pp[i] += h
+        pm = p.copy(); pm[i] -= h
+        fd = (R.net_grads(pp, x, t, ch, False)[1] - R.net_grads(pm, x, t, ch, False)[1]) / (2 * h)
+        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])

@@ -1,0 +1,76 @@
+"""GPU: the warm-start network (bart_amd_warmnet_*) against the FP64 oracle
+network of oracle/cnn.py evaluated with the same bfloat16 rounding points:
+forward output, loss and every parameter gradient of one training step
+(backpropagation through the matrix-core kernels), and that training lowers
+the loss."""
+import numpy as np
+import pytest
+
+from oracle import cnn as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to("cuda")
+
+
+def relerr(a, ref):
+    a = np.asarray(a, np.float64)
+    return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+
+
+def test_warmnet_forward_loss_and_gradients():
+    from bart_amd.warmstart import WarmNet
+    ch = 8
+    net = WarmNet(ch, seed=3)
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((2, 2, 8, 16)).astype(np.float32)
+    t = rng.standard_normal((2, 2, 8, 16)).astype(np.float32)
+    y = net.forward(dev(x)).cpu().numpy()
+    ref, _ = R.net_forward(p, x, ch)
+    assert relerr(y, ref) < 1e-4
+    g_ref, loss_ref = R.net_grads(p, x, t, ch)
+    loss = net.train_step(dev(x), dev(t), 1e-3)
+    assert abs(loss - loss_ref) < 1e-4 * loss_ref
+    g = net.get_grads()
+    print(f"forward rel err {relerr(y, ref):.1e}, gradient rel err {relerr(g, g_ref):.1e}")
+    assert relerr(g, g_ref) < 1e-3
+    # unrounded FP64 gradients: the bfloat16 operands show at the 1e-2 level
+    g64, _ = R.net_grads(p, x, t, ch, rounding=False)
+    assert relerr(g, g64) < 5e-2
+    # the first Adam step moves every parameter with a non-zero gradient by lr
+    d = net.get_params() - p
+    nz = np.abs(g) > 1e-6
+    assert np.allclose(np.abs(d[nz]), 1e-3, rtol=2e-2)
+    net.free()
+
+
+def test_warmnet_params_roundtrip():
+    from bart_amd.warmstart import WarmNet
+    a = WarmNet(16, seed=1)
+    b = WarmNet(16, seed=2)
+    pa = a.get_params()
+    assert not np.array_equal(pa, b.get_params())
+    b.set_params(pa)
+    x = dev(np.random.default_rng(0).standard_normal((1, 2, 12, 64)))
+    assert np.array_equal(a.forward(x).cpu().numpy(), b.forward(x).cpu().numpy())
+
+
+def test_warmnet_training_lowers_loss():
+    # denoising of smooth random fields: the loss falls well below its start
+    from bart_amd.warmstart import WarmNet
+    rng = np.random.default_rng(9)
+    n = 32
+    yy, xx = np.mgrid[0:n, 0:n] / n
+    clean = np.stack([np.stack([np.sin(2 * np.pi * (rng.random() * 2 * xx + rng.random() * 2 * yy + rng.random()))
+                                for _ in range(2)]) for _ in range(8)]).astype(np.float32)
+    noisy = clean + 0.3 * rng.standard_normal(clean.shape).astype(np.float32)
+    net = WarmNet(16, seed=0)
+    xd, td = dev(noisy), dev(clean)
+    losses = [net.train_step(xd, td, 2e-3) for _ in range(150)]
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.5 * losses[0]
