@@ -16,8 +16,9 @@
 //   * reductions are two-stage and deterministic (per-work-group FP64
 //     partials, then a fixed-order sum per problem);
 //   * one iteration -- normal operators and vector updates -- is captured
-//     once into a hipGraph and launched maxiter times; after a problem stops,
-//     its kernels see the device stop flag and return immediately.
+//     once into a hipGraph and launched up to maxiter times, in chunks
+//     whose stop flags the host reads one chunk behind; after a problem
+//     stops, its kernels see the device stop flag and return immediately.
 #include "cg.hpp"
 #include "prof.hpp"
 
@@ -28,6 +29,7 @@ namespace bam {
 
 constexpr int CG_THREADS = 256;
 constexpr int CG_BLOCKS = 512;   // work-groups per problem at most
+constexpr int CG_CHUNK = 8;      // iterations launched between stop-flag reads
 
 __device__ inline double block_sum(double v, double* sh)
 {
@@ -348,6 +350,9 @@ CgBatch::CgBatch(const std::vector<long>& nfloats, hipStream_t s) : B((int)nfloa
 	BAM_HIP(hipMemcpy(seg.ptr, seg_h.data(), sizeof(CgSeg) * B, hipMemcpyHostToDevice));
 	BAM_HIP(hipMemcpy(bprob.ptr, bp.data(), sizeof(int) * nblk, hipMemcpyHostToDevice));
 	BAM_HIP(hipHostMalloc((void**)&host_st, sizeof(CgState) * B, hipHostMallocDefault));
+	BAM_HIP(hipHostMalloc((void**)&probe, 2 * sizeof(CgState) * B, hipHostMallocDefault));
+	BAM_HIP(hipEventCreateWithFlags(&probe_ev[0], hipEventDisableTiming));
+	BAM_HIP(hipEventCreateWithFlags(&probe_ev[1], hipEventDisableTiming));
 }
 
 CgBatch::~CgBatch()
@@ -355,6 +360,11 @@ CgBatch::~CgBatch()
 	drop_graph();
 	if (nullptr != host_st)
 		(void)hipHostFree(host_st);
+	if (nullptr != probe)
+		(void)hipHostFree(probe);
+	for (hipEvent_t e : probe_ev)
+		if (nullptr != e)
+			(void)hipEventDestroy(e);
 }
 
 void CgBatch::drop_graph()
@@ -488,8 +498,44 @@ void CgBatch::iterate(const BatchNormalFn& normal, float lambda, int maxiter, bo
 		graph_lambda = lambda;
 	}
 
-	for (int i = 0; i < maxiter; i++)
-		BAM_HIP(hipGraphLaunch(gexec, stream));
+	// Launch in chunks; after each chunk the stop flags are copied to the
+	// host, and the copy of the chunk before is read while the current one
+	// runs, so the device never waits for the host.  Once every problem has
+	// stopped no more iterations are launched (at most one chunk of no-op
+	// iterations follows the last one that did work).
+	int launched = 0, chunk = 0;
+
+	while (launched < maxiter) {
+
+		const int nl = std::min(CG_CHUNK, maxiter - launched);
+
+		for (int i = 0; i < nl; i++)
+			BAM_HIP(hipGraphLaunch(gexec, stream));
+
+		launched += nl;
+
+		if (launched >= maxiter)
+			break;
+
+		const int slot = chunk & 1;
+		BAM_HIP(hipMemcpyAsync(probe + (size_t)slot * B, st.ptr, sizeof(CgState) * B, hipMemcpyDeviceToHost, stream));
+		BAM_HIP(hipEventRecord(probe_ev[slot], stream));
+
+		if (chunk > 0) {
+
+			BAM_HIP(hipEventSynchronize(probe_ev[1 - slot]));
+			const CgState* ps = probe + (size_t)(1 - slot) * B;
+			bool all = true;
+
+			for (int q = 0; q < B && all; q++)
+				all = 0 != ps[q].done;
+
+			if (all)
+				break;
+		}
+
+		chunk++;
+	}
 }
 
 void CgBatch::finish(float2* const* x_out)

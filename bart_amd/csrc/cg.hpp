@@ -71,6 +71,8 @@ struct CgBatch {
 	DevBuf<int> mask;    // problems to (re)initialise
 	DevBuf<CgState> st;
 	CgState* host_st = nullptr;
+	CgState* probe = nullptr;            // stop flags read between chunks (2 x B)
+	hipEvent_t probe_ev[2] = { nullptr, nullptr };
 	int trace_len = 0;   // per problem
 	std::vector<char> fell_back;
 

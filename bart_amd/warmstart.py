@@ -105,3 +105,123 @@ def from_channels(t, n):
     """[1, 2, n, n] float32 -> complex [n*n] device image."""
     import torch
     return torch.view_as_complex(t[0].permute(1, 2, 0).contiguous()).reshape(n * n).contiguous()
+
+
+# ---------------------------------------------------------------------------
+# Training pairs, training, and CG from the network's guess
+# ---------------------------------------------------------------------------
+
+def _dims16(*d):
+    return list(d) + [1] * (DIMS - len(d))
+
+
+def make_problem(seed, n=256, C=8, spokes=(32, 96), noise=1e-3, lam=0.01):
+    """A seeded 2-D radial CG-SENSE problem: ellipse phantom, Gaussian coil
+    maps, golden-angle radial trajectory with a seeded spoke count (readout
+    2n), k-space with complex Gaussian noise.  Returns (operator, b = A^H y,
+    ground-truth image) with device vectors."""
+    import torch
+    from . import inputs as I
+    from . import sense_nc_init, traj_array
+    rng = np.random.default_rng(seed)
+    S = int(rng.integers(spokes[0], spokes[1] + 1))
+    R = 2 * n
+    traj = I.radial_traj(R, S)
+    img = I.ellipse_phantom(n, seed=seed)
+    maps = I.gaussian_coils((1, n, n), C, seed=seed)
+    op = sense_nc_init(_dims16(n, n, 1, C), _dims16(n, n, 1, C), maps, _dims16(1, R, S, C), _dims16(3, R, S),
+                       traj_array(traj))
+    x = torch.from_numpy(np.ascontiguousarray(img).reshape(-1)).to("cuda")
+    y = op.forward(x)
+    g = torch.Generator(device="cpu").manual_seed(int(seed))
+    e = (torch.randn(y.numel(), generator=g) + 1j * torch.randn(y.numel(), generator=g)).to(torch.complex64)
+    y = (y + (noise * float(y.abs().max()) / np.sqrt(2.0)) * e.to("cuda")).contiguous()
+    b = op.adjoint(y)
+    return op, b, x
+
+
+def first_iterate(op, b, lam):
+    """x1 = alpha b, the first CG iterate from zero: the multiple of b that
+    the normal equations fit best; the network's input (scale-normalised)."""
+    from . import lsqr_conjgrad
+    x1, _, _ = lsqr_conjgrad(op, b, 1, lam, 0.0)
+    return x1
+
+
+def net_input(x1, n):
+    """Network input of the first iterate and its scale s = max |x1|."""
+    s = float(x1.abs().max())
+    s = s if s > 0 else 1.0
+    return to_channels(x1 / s, n), s
+
+
+def training_pairs(seeds, n=256, C=8, lam=0.01, maxiter=150, tol=1e-6, chunk=32):
+    """(input, target) pairs: network input from x1, target the CG-SENSE
+    solution to relative residual tol (at most maxiter iterations) scaled by
+    the same s; problems are solved chunk by chunk with lsqr_conjgrad_batch."""
+    import torch
+    from . import lsqr_conjgrad_batch
+    xs_in, xs_t, its = [], [], []
+    for i in range(0, len(seeds), chunk):
+        probs = [make_problem(s, n, C, lam=lam) for s in seeds[i:i + chunk]]
+        ops = [p[0] for p in probs]
+        bs = [p[1] for p in probs]
+        sol, st, _ = lsqr_conjgrad_batch(ops, bs, maxiter, lam, tol)
+        for (op, b, _), xsol, sti in zip(probs, sol, st):
+            inp, s = net_input(first_iterate(op, b, lam), n)
+            xs_in.append(inp)
+            xs_t.append(to_channels(xsol / s, n))
+            its.append(sti.iterations)
+        for op in ops:
+            op.free()
+        torch.cuda.synchronize()
+    return torch.cat(xs_in), torch.cat(xs_t), its
+
+
+def train(net, inputs, targets, epochs=20, batch=8, lr=1e-3, seed=0, log=None):
+    """Adam on mean squared error over shuffled mini-batches; the learning
+    rate falls linearly to lr / 10 over the epochs.  Returns the mean loss
+    of every epoch."""
+    rng = np.random.default_rng(seed)
+    N = inputs.shape[0]
+    hist = []
+    for e in range(epochs):
+        perm = rng.permutation(N)
+        lr_e = lr * (1.0 - 0.9 * e / max(1, epochs - 1))
+        tot, cnt = 0.0, 0
+        for i in range(0, N - batch + 1, batch):
+            idx = perm[i:i + batch]
+            import torch
+            ii = torch.from_numpy(idx).to(inputs.device)
+            tot += net.train_step(inputs[ii].contiguous(), targets[ii].contiguous(), lr_e)
+            cnt += 1
+        hist.append(tot / max(1, cnt))
+        if log is not None:
+            log(f"epoch {e}: loss {hist[-1]:.3e}")
+    return hist
+
+
+def solve_warm_and_cold(net, op, b, n, lam=0.01, tol=1e-4, maxiter=300):
+    """CG-SENSE to relative residual tol from zero and from the network's
+    guess (zero start as fallback when the guess's residual exceeds ||b||).
+    Returns dict with iterations and device times (ms) of both; the warm
+    time includes the first iterate and the network."""
+    import torch
+    from . import lsqr_conjgrad
+    # the operator's iteration graph is captured by its first solve: capture
+    # it before either timed solve
+    lsqr_conjgrad(op, b, 1, lam, 0.0)
+    xc, stc, _ = lsqr_conjgrad(op, b, maxiter, lam, tol)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    x1 = first_iterate(op, b, lam)
+    inp, s = net_input(x1, n)
+    x0 = (from_channels(net.forward(inp), n) * s).contiguous()
+    e1.record()
+    torch.cuda.synchronize()
+    t_guess = e0.elapsed_time(e1)
+    xw, stw, _ = lsqr_conjgrad(op, b, maxiter, lam, tol, x0=x0, fallback=True)
+    diff = float((xw - xc).norm() / xc.norm())
+    return dict(cold_it=stc.iterations, warm_it=stw.iterations, cold_ms=stc.time_ms,
+                warm_ms=stw.time_ms + t_guess, guess_ms=t_guess, fell_back=bool(stw.fell_back),
+                warm_vs_cold=diff, cold_status=stc.status, warm_status=stw.status)

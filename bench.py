@@ -160,6 +160,29 @@ def explicit_vs_toeplitz(w, x, sense_t):
     return out
 
 
+def cpu_cg_baseline(w, threads, y):
+    """Oracle CG-SENSE solve (Toeplitz, the step's iteration count, lambda)
+    on the host: PSF + A^H y + iterations, repeated for about 10 s."""
+    import oracle as O
+    O.lib().orc_set_num_threads(threads)
+    C = w["C"]
+    M = w["R"] * w["S"]
+    yh = y.cpu().numpy().reshape(C, M)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        ref = O.Nufft(w["img"], w["traj"], C, toeplitz=True)
+        b = ref.sense_adjoint(w["maps"], yh)
+        ref.cgsense(w["maps"], b, w["lam"], w["iters"], 0.0)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= 10.0 or reps >= 50:
+            break
+    X, Y, Z = w["img"]
+    return dict(value=round(reps / dt, 4), unit="solves/s", cores=threads, kind="port",
+                sample=f"oracle PSF + A^H y + {w['iters']} Toeplitz CG iterations, {C} coils, {X}x{Y}x{Z}, "
+                       f"{reps} solves ({dt:.1f} s)")
+
+
 def cpu_baseline(w, threads):
     """Oracle NUFFT adj+fwd on a bounded sample: 8 coils of the same
     trajectory and image grid, repeated until about 10 s of CPU time."""
@@ -323,12 +346,120 @@ def run_batched(args, L, report):
     print(json.dumps(out), flush=True)
 
 
+def run_warmstart(args):
+    """Learned warm start (2-D radial, 256^2, 8 coils, 32-96 spokes): training
+    pairs from this solver (CG to relative residual 1e-6), training of the
+    6-layer network, then CG-SENSE to relative residual 1e-4 on held-out
+    seeds from zero and from the network's guess.  value = warm solves/s."""
+    import torch
+    from bart_amd import warmstart as WS
+
+    n, lam = 256, 0.01
+    train_seeds = list(range(args.ws_train))
+    eval_seeds = list(range(100000, 100000 + args.ws_eval))
+    t0 = time.perf_counter()
+    inp, tgt, its = WS.training_pairs(train_seeds, n=n, lam=lam)
+    t_data = time.perf_counter() - t0
+    print(f"# {len(train_seeds)} training pairs in {t_data:.1f} s (CG iterations to 1e-6: median "
+          f"{int(np.median(its))}, max {max(its)})", file=sys.stderr, flush=True)
+    net = WS.WarmNet(48, seed=0)
+    t0 = time.perf_counter()
+    hist = WS.train(net, inp, tgt, epochs=args.ws_epochs, batch=8, lr=1e-3,
+                    log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+
+    # network forward on one 256^2 image: 2 9 (2 48 + 4 48 48 + 48 2) n^2 flops
+    flops = 2 * 9 * (2 * 48 + 4 * 48 * 48 + 48 * 2) * n * n
+    xb = inp[:8].contiguous()
+    net.forward(xb)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        net.forward(xb)
+    e1.record()
+    torch.cuda.synchronize()
+    t_fwd = e0.elapsed_time(e1) / 5 / 8
+    del inp, tgt
+
+    res = []
+    for s in eval_seeds:
+        op, b, _ = WS.make_problem(s, n=n, lam=lam)
+        res.append(WS.solve_warm_and_cold(net, op, b, n, lam=lam))
+        op.free()
+    cold_ms = sum(r["cold_ms"] for r in res)
+    warm_ms = sum(r["warm_ms"] for r in res)
+    print(f"# eval: cold {np.mean([r['cold_it'] for r in res]):.1f} it, warm {np.mean([r['warm_it'] for r in res]):.1f} it, "
+          f"fallbacks {sum(r['fell_back'] for r in res)}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        from bart_amd import inputs as I
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        O.lib().orc_set_num_threads(threads)
+        done, t0 = 0, time.perf_counter()
+        for s in eval_seeds:
+            op, b, _ = WS.make_problem(s, n=n, lam=lam)
+            rng = np.random.default_rng(s)
+            S = int(rng.integers(32, 97))
+            maps = I.gaussian_coils((1, n, n), 8, seed=s)
+            ref = O.Nufft((n, n, 1), I.radial_traj(2 * n, S), 8, toeplitz=True)
+            ref.cgsense(maps, b.cpu().numpy().reshape(1, n, n), lam, 300, 1e-4)
+            op.free()
+            done += 1
+            if time.perf_counter() - t0 >= 10.0:
+                break
+        dt = time.perf_counter() - t0
+        cpu = dict(value=round(done / dt, 4), unit="solves/s", cores=threads, kind="port",
+                   sample=f"oracle PSF + Toeplitz CG-SENSE to rel-res 1e-4 from zero on the first {done} held-out "
+                          f"problems ({dt:.1f} s)")
+
+    ach = flops / (t_fwd * 1e-3) / 1e12
+    out = {
+        "metric": "NUFFT adj+fwd Msamples/s; CG-SENSE solves/s to rel-res 1e-4, warm vs cold",
+        "value": round(len(res) / (warm_ms * 1e-3), 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(res),
+        "warmup": 0,
+        "ms_per_step": round(warm_ms / len(res), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 CG (fp64 reductions); bf16 network with fp32 accumulation",
+        "data": "synthetic",
+        "config": {"workload": "learned_warm_start_2d_radial_256^2_8coils_32-96_spokes", "train_pairs": len(train_seeds),
+                   "eval_problems": len(res), "epochs": args.ws_epochs, "network": "6 x conv3x3, 48 channels, residual",
+                   "tolerance": 1e-4, "lambda": lam},
+        "cold_solves_per_s": round(len(res) / (cold_ms * 1e-3), 3),
+        "time_to_tolerance_cold_over_warm": round(cold_ms / warm_ms, 3),
+        "iterations_cold_mean": float(np.mean([r["cold_it"] for r in res])),
+        "iterations_warm_mean": float(np.mean([r["warm_it"] for r in res])),
+        "fallbacks": int(sum(r["fell_back"] for r in res)),
+        "warm_vs_cold_solution_relerr_max": float(max(r["warm_vs_cold"] for r in res)),
+        "guess_ms_mean": round(float(np.mean([r["guess_ms"] for r in res])), 3),
+        "training_data_s": round(t_data, 2),
+        "training_s": round(t_train, 2),
+        "train_loss_first_last": [hist[0], hist[-1]],
+        "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(ach / 2500.0, 5), "traffic": None, "kernel": "network forward (6 conv3x3 layers)",
+                     "flops_per_image": flops, "ms_per_image": round(t_fwd, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="kooshball", choices=["kooshball", "radial2d", "spiral", "batched"])
+    ap.add_argument("--config", default="kooshball", choices=["kooshball", "radial2d", "spiral", "batched", "warmstart"])
+    ap.add_argument("--ws-train", type=int, default=1024, help="warmstart: training pairs")
+    ap.add_argument("--ws-eval", type=int, default=32, help="warmstart: held-out problems")
+    ap.add_argument("--ws-epochs", type=int, default=40, help="warmstart: training epochs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--pmc-pass", action="store_true",
@@ -354,6 +485,10 @@ def main():
 
     if args.config == "batched":
         run_batched(args, L, None)
+        return
+
+    if args.config == "warmstart":
+        run_warmstart(args)
         return
 
     t_gen = time.perf_counter()
@@ -496,9 +631,12 @@ def main():
         normal_cmp = explicit_vs_toeplitz(w, b, sense)
 
     cpu = None
+    cpu_cg = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = cpu_baseline(w, threads)
+        if Z == 1:
+            cpu_cg = cpu_cg_baseline(w, threads, y)
 
     out = {
         "metric": "NUFFT adj+fwd Msamples/s; CG-SENSE solves/s to rel-res 1e-4, warm vs cold",
@@ -522,6 +660,8 @@ def main():
         "cg_rel_res": relres,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_cg_sense": cpu_cg,
+        "cg_sense_gpu_over_cpu": (round(1.0 / t_cg / cpu_cg["value"], 1) if cpu_cg else None),
         "fp64_nudft": accuracy,
         "normal_explicit_vs_toeplitz": normal_cmp,
         "kernels_ms_per_step": {k: round(v[0], 4) for k, v in sorted(per_step.items())},

@@ -74,3 +74,25 @@ def test_warmnet_training_lowers_loss():
     losses = [net.train_step(xd, td, 2e-3) for _ in range(150)]
     print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     assert losses[-1] < 0.5 * losses[0]
+
+
+def test_warm_start_pipeline_and_fallback():
+    # training pairs from the solver, a few training steps, CG to 1e-4 from
+    # the guess and from zero; a network scaled far off falls back to zero
+    from bart_amd import warmstart as WS
+    n = 64
+    inp, tgt, its = WS.training_pairs(list(range(8)), n=n, C=4, maxiter=60, chunk=4)
+    assert inp.shape == (8, 2, n, n) and tgt.shape == (8, 2, n, n)
+    net = WS.WarmNet(8, seed=0)
+    hist = WS.train(net, inp, tgt, epochs=3, batch=4, lr=1e-3)
+    assert np.isfinite(hist).all()
+    op, b, _ = WS.make_problem(1000, n=n, C=4)
+    r = WS.solve_warm_and_cold(net, op, b, n)
+    assert r["cold_status"] == 2 and r["warm_status"] == 2
+    # both meet ||r|| <= 1e-4 ||b||; the solutions agree to the problem's conditioning
+    assert r["warm_vs_cold"] < 0.1
+    p = net.get_params()
+    net.set_params(p * 50.0)
+    r2 = WS.solve_warm_and_cold(net, op, b, n)
+    assert r2["fell_back"] and r2["warm_it"] == r2["cold_it"]
+    op.free()
