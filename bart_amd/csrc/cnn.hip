@@ -42,50 +42,59 @@ __host__ __device__ inline int round8(int v) { return (v + 7) / 8 * 8; }
 
 // ---------------------------------------------------------------------------
 // Forward: work-group = 64 pixels of one row x 64 output channels, 4 waves of
-// one 32 x 32 tile each.  LDS: weights [64][KP + 8], the 3 x 66 input patch of
-// every input channel, and the patch offset of every k.
+// one 32 x 32 tile each.  K is ordered (tap, channel) with the channels
+// padded to CP (a multiple of 8): the weights are first converted to
+// bfloat16 in that order ([64][9 CP], k_conv_wprep) and read from there
+// (cached), and the 3 x 66 input patch is staged in LDS channel-fastest, so
+// that a lane's 4 consecutive k of one matrix-core step -- 4 channels of one
+// tap -- are one 8-byte read for A and one LDS read for B.
 // ---------------------------------------------------------------------------
 
+__host__ __device__ inline int chan_pad(int c) { return (c + 7) / 8 * 8; }
+
+// wb[co][t CP + ci] = bf16(w[co][ci][t]), zero for padded co / ci
+__global__ void k_conv_wprep(const float* __restrict__ w, int Cin, int Cout, int CP,
+			     unsigned short* __restrict__ wb)
+{
+	const int KP = 9 * CP;
+	const int n = CV_MAXC * KP;
+
+	for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+		const int co = e / KP;
+		const int k = e - co * KP;
+		const int t = k / CP;
+		const int ci = k - t * CP;
+		wb[e] = f2bf((co < Cout && ci < Cin) ? w[(co * Cin + ci) * 9 + t] : 0.f);
+	}
+}
+
 __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __restrict__ x, int Cin, int H, int W,
-								const float* __restrict__ w,
+								const unsigned short* __restrict__ wb,
 								const float* __restrict__ bias,
 								const float* __restrict__ res, int Cout, int relu,
 								float* __restrict__ y)
 {
 	extern __shared__ __align__(16) unsigned char cv_smem[];
-	const int K = Cin * 9;
-	const int KP = round8(K);
-	const int WP = KP + 8;
-	unsigned short* wl = (unsigned short*)cv_smem;          // [64][WP]
-	unsigned short* xl = wl + CV_MAXC * WP;                 // [Cin][3][66]
-	int* koff = (int*)(xl + round8(Cin * 3 * 66));          // [KP]
+	const int CP = chan_pad(Cin);
+	const int KP = 9 * CP;
+	unsigned short* xl = (unsigned short*)cv_smem;          // [3][66][CP]
 
 	const int x0 = blockIdx.x * CV_PIX;
 	const int yy = blockIdx.y;
 	const int b = blockIdx.z;
 
-	for (int e = threadIdx.x; e < CV_MAXC * KP; e += CV_FWD_THREADS) {
-		const int co = e / KP;
-		const int k = e - co * KP;
-		wl[co * WP + k] = f2bf((co < Cout && k < K) ? w[co * K + k] : 0.f);
-	}
-
-	for (int e = threadIdx.x; e < Cin * 198; e += CV_FWD_THREADS) {
+	// input patch, channel-fastest
+	for (int e = threadIdx.x; e < CP * 198; e += CV_FWD_THREADS) {
 		const int ci = e / 198;
 		const int r = e - ci * 198;
 		const int u = r / 66;
-		const int gx = x0 + r - u * 66 - 1;
+		const int xx = r - u * 66;
+		const int gx = x0 + xx - 1;
 		const int gy = yy + u - 1;
 		float v = 0.f;
-		if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+		if (ci < Cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
 			v = x[((long)(b * Cin + ci) * H + gy) * W + gx];
-		xl[e] = f2bf(v);
-	}
-
-	for (int k = threadIdx.x; k < KP; k += CV_FWD_THREADS) {
-		const int ci = k / 9;
-		const int t = k - ci * 9;
-		koff[k] = (k < K) ? ci * 198 + (t / 3) * 66 + (t % 3) : -1;
+		xl[(u * 66 + xx) * CP + ci] = f2bf(v);
 	}
 
 	__syncthreads();
@@ -96,7 +105,8 @@ __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __r
 	const int pt = wv >> 1;
 	const int i = lane & 31;
 	const int hk = lane >> 5;
-	const unsigned short* wrow = wl + (cot * 32 + i) * WP;
+	// A fragments straight from the prepared weights (a few KiB, cached)
+	const unsigned short* wrow = wb + (cot * 32 + i) * KP + 4 * hk;
 	const int px = pt * 32 + i;
 
 	f32x16 acc;
@@ -104,17 +114,18 @@ __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __r
 	for (int r = 0; r < 16; r++)
 		acc[r] = 0.f;
 
-	for (int k0 = 0; k0 < KP; k0 += 8) {
+	for (int t = 0; t < 9; t++) {
 
-		const int kb = k0 + 4 * hk;
-		const bf16x4 a = *(const bf16x4*)(wrow + kb);
-		bf16x4 bv;
-#pragma unroll
-		for (int t = 0; t < 4; t++) {
-			const int o = koff[kb + t];
-			bv[t] = (o >= 0) ? (short)xl[o + px] : (short)0;
+		const int u = t / 3;
+		const int v = t - 3 * u;
+		const unsigned short* xrow = xl + (u * 66 + px + v) * CP + 4 * hk;
+		const unsigned short* wt = wrow + t * CP;
+
+		for (int c0 = 0; c0 < CP; c0 += 8) {
+			const bf16x4 a = *(const bf16x4*)(wt + c0);
+			const bf16x4 bv = *(const bf16x4*)(xrow + c0);
+			acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bv, acc, 0, 0, 0);
 		}
-		acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bv, acc, 0, 0, 0);
 	}
 
 	const int gx = x0 + px;
@@ -138,8 +149,7 @@ __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __r
 
 static size_t fwd_lds(int Cin)
 {
-	const int KP = round8(Cin * 9);
-	return sizeof(unsigned short) * ((size_t)CV_MAXC * (KP + 8) + round8(Cin * 3 * 66)) + sizeof(int) * KP;
+	return sizeof(unsigned short) * (size_t)198 * chan_pad(Cin);
 }
 
 // w'[ci][co][u][v] = w[co][ci][2 - u][2 - v]: the weights of the transposed
@@ -328,6 +338,7 @@ struct CnnWork {
 	std::mutex mtx;
 	DevBuf<float> part;
 	DevBuf<float> wt;
+	DevBuf<unsigned short> wb;
 };
 
 static CnnWork& cnn_work()
@@ -343,15 +354,31 @@ static void check_shape(int B, int Cin, int Cout, int H, int W)
 		  "conv3x3: 1 to %d channels in and out (got %d, %d)", CV_MAXC, Cin, Cout);
 }
 
-void conv3x3_fwd(const float* x, int B, int Cin, int H, int W, const float* w, const float* bias, const float* res,
-		 int Cout, bool relu, float* y, hipStream_t st)
+// the bfloat16 weights of the forward kernel, prepared per call
+static void fwd_launch(const float* x, int B, int Cin, int H, int W, const float* w, const float* bias,
+		       const float* res, int Cout, bool relu, float* y, hipStream_t st, DevBuf<unsigned short>& wb)
 {
 	check_shape(B, Cin, Cout, H, W);
 	BAM_CHECK(H <= 65535 && B <= 65535, "conv3x3: too many rows or images");
+	const int CP = chan_pad(Cin);
+	const long nw = (long)CV_MAXC * 9 * CP;
+	if ((long)wb.n < nw)
+		wb.alloc(nw);
+	hipLaunchKernelGGL(k_conv_wprep, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, w, Cin, Cout, CP, wb.ptr);
 	dim3 grid((W + CV_PIX - 1) / CV_PIX, H, B);
-	hipLaunchKernelGGL(k_conv3x3_fwd, grid, dim3(CV_FWD_THREADS), fwd_lds(Cin), st, x, Cin, H, W, w, bias, res, Cout,
-			   relu ? 1 : 0, y);
+	hipLaunchKernelGGL(k_conv3x3_fwd, grid, dim3(CV_FWD_THREADS), fwd_lds(Cin), st, x, Cin, H, W, wb.ptr, bias, res,
+			   Cout, relu ? 1 : 0, y);
 	BAM_HIP(hipGetLastError());
+}
+
+void conv3x3_fwd(const float* x, int B, int Cin, int H, int W, const float* w, const float* bias, const float* res,
+		 int Cout, bool relu, float* y, hipStream_t st)
+{
+	// the prepared weights are rewritten by the next call: calls are ordered
+	// on the one stream every layer call uses
+	CnnWork& cw = cnn_work();
+	std::lock_guard<std::mutex> lock(cw.mtx);
+	fwd_launch(x, B, Cin, H, W, w, bias, res, Cout, relu, y, st, cw.wb);
 }
 
 void conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, const float* w, int Cin, float* dx,
@@ -365,7 +392,7 @@ void conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, const floa
 		cw.wt.alloc(n);
 	hipLaunchKernelGGL(k_conv_flip, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w, Cout, Cin, cw.wt.ptr);
 	BAM_HIP(hipGetLastError());
-	conv3x3_fwd(dy, B, Cout, H, W, cw.wt.ptr, nullptr, nullptr, Cin, false, dx, st);
+	fwd_launch(dy, B, Cout, H, W, cw.wt.ptr, nullptr, nullptr, Cin, false, dx, st, cw.wb);
 	BAM_HIP(hipStreamSynchronize(st));  // the flipped weights are reused by the next call
 }
 
