@@ -889,6 +889,73 @@ __device__ inline void slice_passes_rest(float2* lds, const float2* tw, float sg
 	}
 }
 
+// the passes from sub-transforms of length NS up to (not including) length
+// NSEND, radices as slice_passes chooses them
+template <int N, int NS, int NSEND, int P, int ES, int TS, int NTH>
+__device__ inline void slice_passes_range(float2* lds, const float2* tw, float sgn)
+{
+	if constexpr (NS < NSEND) {
+		constexpr int R = SLICE_R16 ? ct_next_radix16(N / NS) : ct_next_radix(N / NS);
+		slice_pass<N, R, NS, P, ES, TS, NTH>(lds, tw, sgn);
+		slice_passes_range<N, NS * R, NSEND, P, ES, TS, NTH>(lds, tw, sgn);
+	}
+}
+
+// radix of the last pass of slice_passes for length n
+constexpr int slice_last_radix(int n)
+{
+	int m = n, r = n;
+	while (m > 1) {
+		r = SLICE_R16 ? ct_next_radix16(m) : ct_next_radix(m);
+		m /= r;
+	}
+	return r;
+}
+
+// The last (radix R) pass of P length-N transforms stored as rows (element
+// stride 1, row pitch LD) with the lanes along the butterfly index j of a
+// row: butterfly (p, j) reads j + r N / R and yields the outputs x = j +
+// q N / R of row p, handed to f(m, q, p, x, value) (m: the thread's
+// butterfly count) instead of being written back.
+template <int N, int R, int P, int LD, int NTH, class F>
+__device__ inline void slice_last_pass_rows(const float2* lds, const float2* tw, float sgn, F&& f)
+{
+	constexpr int NS = N / R;
+	constexpr int NB = P * NS;
+	constexpr int PER = (NB + NTH - 1) / NTH;
+
+#pragma unroll
+	for (int m = 0; m < PER; m++) {
+
+		const int b = threadIdx.x + m * NTH;
+
+		if (0 != NB % NTH && b >= NB)
+			break;
+
+		const int p = b / NS;
+		const int j = b - p * NS;
+		float2 v[R];
+
+#pragma unroll
+		for (int r = 0; r < R; r++) {
+			float2 x = lds[p * LD + j + r * NS];
+			if (r > 0) {
+				float2 w = tw[r * j];
+				if (sgn > 0.f)
+					w.y = -w.y;
+				x = cmul(x, w);
+			}
+			v[r] = x;
+		}
+
+		dft_any<R>(v, sgn, tw, N);
+
+#pragma unroll
+		for (int q = 0; q < R; q++)
+			f(m, q, p, j + q * NS, v[q]);
+	}
+}
+
 // lengths with a compile-time kernel
 inline bool ct_supported(int n)
 {

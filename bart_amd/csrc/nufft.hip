@@ -725,7 +725,16 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 			lds_barrier();
 
 			slice_passes_rest<NY, NY / EPT, EPT, NX, LD, 1, NTH>(lds, twy, -1.f);
-			slice_passes<NX, NY, 1, LD, NTH>(lds, twx, -1.f);
+
+			// x passes; the last one stores its outputs straight to the grid
+			// (16 lanes per row: 128-byte runs)
+			constexpr int RLX = slice_last_radix(NX);
+			slice_passes_range<NX, 1, NX / RLX, NY, 1, LD, NTH>(lds, twx, -1.f);
+			float2* out = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
+			slice_last_pass_rows<NX, RLX, NY, LD, NTH>(lds, twx, -1.f, [&](int, int, int p, int xo, float2 v) {
+				out[p * NX + xo] = v;
+			});
+			continue;
 
 		} else {
 
