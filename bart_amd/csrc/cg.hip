@@ -465,6 +465,104 @@ void CgBatch::enqueue_iteration(const BatchNormalFn& normal, float lambda)
 	BAM_HIP(hipGetLastError());
 }
 
+void CgBatch::capture(const BatchNormalFn& normal, float lambda)
+{
+	if (have_graph && graph_lambda == lambda)
+		return;
+
+	drop_graph();
+	BAM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+	prof_set_capturing(true);
+
+	try {
+		enqueue_iteration(normal, lambda);
+	} catch (...) {
+		prof_set_capturing(false);
+		(void)hipStreamEndCapture(stream, &graph);
+		throw;
+	}
+
+	prof_set_capturing(false);
+	BAM_HIP(hipStreamEndCapture(stream, &graph));
+	BAM_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+	have_graph = true;
+	graph_lambda = lambda;
+}
+
+void CgBatch::launch(int n)
+{
+	for (int i = 0; i < n; i++)
+		BAM_HIP(hipGraphLaunch(gexec, stream));
+}
+
+void CgBatch::post_probe(int slot)
+{
+	BAM_HIP(hipMemcpyAsync(probe + (size_t)slot * B, st.ptr, sizeof(CgState) * B, hipMemcpyDeviceToHost, stream));
+	BAM_HIP(hipEventRecord(probe_ev[slot], stream));
+}
+
+bool CgBatch::probe_all_done(int slot)
+{
+	BAM_HIP(hipEventSynchronize(probe_ev[slot]));
+	const CgState* ps = probe + (size_t)slot * B;
+
+	for (int q = 0; q < B; q++)
+		if (0 == ps[q].done)
+			return false;
+
+	return true;
+}
+
+// Launch in chunks, round-robin over the batches (one stream each); after a
+// batch's chunk its stop flags are copied to the host, and the copy of the
+// chunk before is read while the current one runs, so no device waits for
+// the host.  A batch whose problems have all stopped gets no more launches
+// (at most one chunk of no-op iterations follows its last useful one).
+void iterate_batches(const std::vector<CgBatch*>& bs, const std::vector<const BatchNormalFn*>& fns, float lambda,
+		     int maxiter)
+{
+	const int K = (int)bs.size();
+
+	if (maxiter <= 0 || 0 == K)
+		return;
+
+	for (int k = 0; k < K; k++)
+		bs[k]->capture(*fns[k], lambda);
+
+	std::vector<int> launched(K, 0);
+	std::vector<char> active(K, 1);
+	int left = K;
+
+	for (int chunk = 0; left > 0; chunk++) {
+
+		const int slot = chunk & 1;
+
+		for (int k = 0; k < K; k++) {
+
+			if (!active[k])
+				continue;
+
+			const int nl = std::min(CG_CHUNK, maxiter - launched[k]);
+			bs[k]->launch(nl);
+			launched[k] += nl;
+
+			if (launched[k] >= maxiter) {
+				active[k] = 0;
+				left--;
+			} else {
+				bs[k]->post_probe(slot);
+			}
+		}
+
+		if (chunk > 0)
+			for (int k = 0; k < K; k++)
+				if (active[k] && bs[k]->probe_all_done(1 - slot)) {
+					active[k] = 0;
+					left--;
+				}
+	}
+}
+
 void CgBatch::iterate(const BatchNormalFn& normal, float lambda, int maxiter, bool use_graph)
 {
 	if (maxiter <= 0)
@@ -477,65 +575,7 @@ void CgBatch::iterate(const BatchNormalFn& normal, float lambda, int maxiter, bo
 		return;
 	}
 
-	if (!have_graph || graph_lambda != lambda) {
-
-		drop_graph();
-		BAM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-		prof_set_capturing(true);
-
-		try {
-			enqueue_iteration(normal, lambda);
-		} catch (...) {
-			prof_set_capturing(false);
-			(void)hipStreamEndCapture(stream, &graph);
-			throw;
-		}
-
-		prof_set_capturing(false);
-		BAM_HIP(hipStreamEndCapture(stream, &graph));
-		BAM_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-		have_graph = true;
-		graph_lambda = lambda;
-	}
-
-	// Launch in chunks; after each chunk the stop flags are copied to the
-	// host, and the copy of the chunk before is read while the current one
-	// runs, so the device never waits for the host.  Once every problem has
-	// stopped no more iterations are launched (at most one chunk of no-op
-	// iterations follows the last one that did work).
-	int launched = 0, chunk = 0;
-
-	while (launched < maxiter) {
-
-		const int nl = std::min(CG_CHUNK, maxiter - launched);
-
-		for (int i = 0; i < nl; i++)
-			BAM_HIP(hipGraphLaunch(gexec, stream));
-
-		launched += nl;
-
-		if (launched >= maxiter)
-			break;
-
-		const int slot = chunk & 1;
-		BAM_HIP(hipMemcpyAsync(probe + (size_t)slot * B, st.ptr, sizeof(CgState) * B, hipMemcpyDeviceToHost, stream));
-		BAM_HIP(hipEventRecord(probe_ev[slot], stream));
-
-		if (chunk > 0) {
-
-			BAM_HIP(hipEventSynchronize(probe_ev[1 - slot]));
-			const CgState* ps = probe + (size_t)(1 - slot) * B;
-			bool all = true;
-
-			for (int q = 0; q < B && all; q++)
-				all = 0 != ps[q].done;
-
-			if (all)
-				break;
-		}
-
-		chunk++;
-	}
+	iterate_batches({ this }, { &normal }, lambda, maxiter);
 }
 
 void CgBatch::finish(float2* const* x_out)

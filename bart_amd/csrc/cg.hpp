@@ -99,9 +99,16 @@ struct CgBatch {
 	void start(const BatchNormalFn& normal, float lambda, float tol, int maxiter, const float2* const* x0,
 		   const float2* const* b_in, bool warm, bool fallback);
 
-	// Enqueues maxiter iterations: one iteration is captured once as a hipGraph
-	// (per lambda) and launched maxiter times; stopped problems skip their work.
+	// Enqueues up to maxiter iterations: one iteration is captured once as a
+	// hipGraph (per lambda) and launched in chunks until maxiter or until
+	// every problem has stopped; stopped problems skip their work.
 	void iterate(const BatchNormalFn& normal, float lambda, int maxiter, bool use_graph);
+
+	// pieces of iterate_batches
+	void capture(const BatchNormalFn& normal, float lambda);
+	void launch(int n);
+	void post_probe(int slot);
+	bool probe_all_done(int slot);
 
 	// Enqueues x_i -> x_out[i] and the state read-back.
 	void finish(float2* const* x_out);
@@ -113,6 +120,11 @@ private:
 	void enqueue_iteration(const BatchNormalFn& normal, float lambda);
 	void init_kernels(float lambda, float tol, int maxiter, bool cold, bool masked);
 };
+
+// Iterations of several batches, each on its own stream, launched
+// round-robin so that the streams run concurrently (see CgBatch::iterate).
+void iterate_batches(const std::vector<CgBatch*>& bs, const std::vector<const BatchNormalFn*>& fns, float lambda,
+		     int maxiter);
 
 // Single problem with its solver state kept across calls (graph reuse).
 struct CgSolver {

@@ -906,8 +906,15 @@ int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxite
 			bt[k]->start(fn[k], l2lambda, tol, maxiter, x0.data(), bb.data(), warmstart, warm_fallback);
 		}
 
-		for (int k = 0; k < K; k++)
-			bt[k]->iterate(fn[k], l2lambda, maxiter, true);
+		{
+			std::vector<CgBatch*> bp;
+			std::vector<const BatchNormalFn*> fp;
+			for (int k = 0; k < K; k++) {
+				bp.push_back(bt[k].get());
+				fp.push_back(&fn[k]);
+			}
+			iterate_batches(bp, fp, l2lambda, maxiter);
+		}
 
 		for (int k = 0; k < K; k++) {
 
