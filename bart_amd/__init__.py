@@ -153,6 +153,18 @@ def lib():
     L.bart_amd_conv3x3_bwd_data.argtypes = [vp, ci, ci, ci, ci, vp, ci, vp]
     L.bart_amd_conv3x3_bwd_weights.restype = ci
     L.bart_amd_conv3x3_bwd_weights.argtypes = [vp, vp, ci, ci, ci, ci, ci, vp, vp]
+    L.bart_amd_warmnet_create.restype = vp
+    L.bart_amd_warmnet_create.argtypes = [ci, ctypes.c_ulong]
+    L.bart_amd_warmnet_free.restype = None
+    L.bart_amd_warmnet_free.argtypes = [vp]
+    L.bart_amd_warmnet_num_params.restype = ctypes.c_long
+    L.bart_amd_warmnet_num_params.argtypes = [vp]
+    L.bart_amd_warmnet_params.restype = ci
+    L.bart_amd_warmnet_params.argtypes = [vp, vp, ci]
+    L.bart_amd_warmnet_forward.restype = ci
+    L.bart_amd_warmnet_forward.argtypes = [vp, vp, ci, ci, ci, vp]
+    L.bart_amd_warmnet_train_step.restype = ctypes.c_double
+    L.bart_amd_warmnet_train_step.argtypes = [vp, vp, vp, ci, ci, ci, ctypes.c_float]
     L.bart_amd_set_error_mode.argtypes = [ctypes.c_int]
     L.bart_amd_last_error.restype = ctypes.c_char_p
     L.bart_amd_clear_error.restype = None

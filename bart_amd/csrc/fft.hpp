@@ -16,6 +16,7 @@
 #pragma once
 
 #include <type_traits>
+#include <utility>
 
 #include "common.hpp"
 #include "prof.hpp"
@@ -654,6 +655,75 @@ __device__ inline void fft_mul_ifft_regs(const FftLaunch& L, const Ld& ld, const
 	}
 }
 
+// store functors that take single elements: put(t, k, value)
+template <class F, class = void>
+struct has_put : std::false_type {
+};
+template <class F>
+struct has_put<F, std::void_t<decltype(std::declval<const F&>().put(0l, 0, float2{}))>> : std::true_type {
+};
+
+// Contiguous axis (transforms along rows): a thread takes the E points
+// k0 + j N / E of one transform -- coalesced loads along k0 -- runs the
+// first radix-E pass on them, the middle passes in LDS, and, when the store
+// takes single elements, the last radix-E pass into registers and stores
+// from there.
+template <int N, int E, class Ld, class St>
+__device__ inline void fft_contig_regs(const FftLaunch& L, const Ld& ld, const St& st, float2* lds, const float2* tw,
+				       long t0)
+{
+	constexpr int P = ct_block_p(N);
+	constexpr int PN = N + 1;
+	constexpr int NE = N / E;
+
+	const int p = threadIdx.x / NE;
+	const int k0 = threadIdx.x - p * NE;
+	const long t = t0 + p;
+	const bool in = t < L.total;
+	float2 v[E];
+
+#pragma unroll
+	for (int j = 0; j < E; j++)
+		v[j] = in ? ld(t, k0 + j * NE) : make_float2(0.f, 0.f);
+
+	dft_any<E>(v, L.sign1, tw, N);
+
+#pragma unroll
+	for (int q = 0; q < E; q++)
+		lds[p * PN + k0 * E + q] = v[q];
+
+	lds_barrier();
+
+	if constexpr (has_put<St>::value) {
+
+		ct_passes_between<N, P, E, NE>(lds, tw, L.sign1);
+
+#pragma unroll
+		for (int r = 0; r < E; r++) {
+			float2 x = lds[p * PN + k0 + r * NE];
+			if (r > 0) {
+				float2 w = tw[r * k0];
+				if (L.sign1 > 0.f)
+					w.y = -w.y;
+				x = cmul(x, w);
+			}
+			v[r] = x;
+		}
+
+		dft_any<E>(v, L.sign1, tw, N);
+
+		if (in)
+#pragma unroll
+			for (int q = 0; q < E; q++)
+				st.put(t, k0 + q * NE, v[q]);
+
+	} else {
+
+		ct_passes_between<N, P, E, N>(lds, tw, L.sign1);
+		st.store(lds, L, t0);
+	}
+}
+
 template <int N, class Ld, class Md, class St>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, Md md, St st)
 {
@@ -684,6 +754,13 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 	if constexpr (CBF && ct_edges_in_regs<N>()) {
 		if (cb) {
 			fft_mul_ifft_regs<N, ct_edge_radix<N>()>(L, ld, md, st, lds, tw, t0);
+			return;
+		}
+	}
+
+	if constexpr (!Md::active && ct_edges_in_regs<N>()) {
+		if (L.contig) {
+			fft_contig_regs<N, ct_edge_radix<N>()>(L, ld, st, lds, tw, t0);
 			return;
 		}
 	}

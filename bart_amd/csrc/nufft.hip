@@ -343,18 +343,21 @@ struct StGridS {
 	float2* grid;
 	bool fm;
 
+	__device__ void put(long t, int k, float2 v) const
+	{
+		uint32_t s, o, c;
+		g.split(t, s, o, c);
+		if (fm) {
+			int px[3];
+			g.va.coords(0, o, k, px);
+			v = cmul(v, fm_val(T, px));
+		}
+		grid[g.grid_offset(s, o, c, k)] = v;
+	}
+
 	__device__ void store(const float2* lds, const FftLaunch& L, long t0) const
 	{
-		store_each(lds, L, t0, [&](long t, int k, float2 v) {
-			uint32_t s, o, c;
-			g.split(t, s, o, c);
-			if (fm) {
-				int px[3];
-				g.va.coords(0, o, k, px);
-				v = cmul(v, fm_val(T, px));
-			}
-			grid[g.grid_offset(s, o, c, k)] = v;
-		});
+		store_each(lds, L, t0, [&](long t, int k, float2 v) { put(t, k, v); });
 	}
 };
 

@@ -17,24 +17,7 @@ DIMS = 16
 
 
 def _lib():
-    L = lib()
-    if not getattr(L, "_warmnet_ready", False):
-        vp = ctypes.c_void_p
-        ci = ctypes.c_int
-        L.bart_amd_warmnet_create.restype = vp
-        L.bart_amd_warmnet_create.argtypes = [ci, ctypes.c_ulong]
-        L.bart_amd_warmnet_free.restype = None
-        L.bart_amd_warmnet_free.argtypes = [vp]
-        L.bart_amd_warmnet_num_params.restype = ctypes.c_long
-        L.bart_amd_warmnet_num_params.argtypes = [vp]
-        L.bart_amd_warmnet_params.restype = ci
-        L.bart_amd_warmnet_params.argtypes = [vp, vp, ci]
-        L.bart_amd_warmnet_forward.restype = ci
-        L.bart_amd_warmnet_forward.argtypes = [vp, vp, ci, ci, ci, vp]
-        L.bart_amd_warmnet_train_step.restype = ctypes.c_double
-        L.bart_amd_warmnet_train_step.argtypes = [vp, vp, vp, ci, ci, ci, ctypes.c_float]
-        L._warmnet_ready = True
-    return L
+    return lib()
 
 
 class WarmNet:
@@ -182,6 +165,7 @@ def train(net, inputs, targets, epochs=20, batch=8, lr=1e-3, seed=0, log=None):
     """Adam on mean squared error over shuffled mini-batches; the learning
     rate falls linearly to lr / 10 over the epochs.  Returns the mean loss
     of every epoch."""
+    import torch
     rng = np.random.default_rng(seed)
     N = inputs.shape[0]
     hist = []
@@ -191,7 +175,6 @@ def train(net, inputs, targets, epochs=20, batch=8, lr=1e-3, seed=0, log=None):
         tot, cnt = 0.0, 0
         for i in range(0, N - batch + 1, batch):
             idx = perm[i:i + batch]
-            import torch
             ii = torch.from_numpy(idx).to(inputs.device)
             tot += net.train_step(inputs[ii].contiguous(), targets[ii].contiguous(), lr_e)
             cnt += 1
