@@ -165,6 +165,8 @@ def lib():
     L.bart_amd_warmnet_forward.argtypes = [vp, vp, ci, ci, ci, vp]
     L.bart_amd_warmnet_train_step.restype = ctypes.c_double
     L.bart_amd_warmnet_train_step.argtypes = [vp, vp, vp, ci, ci, ci, ctypes.c_float]
+    L.bart_amd_warmnet_train_step_grad.restype = ci
+    L.bart_amd_warmnet_train_step_grad.argtypes = [vp, vp, vp, ci, ci, ci, ctypes.c_float]
     L.bart_amd_set_error_mode.argtypes = [ctypes.c_int]
     L.bart_amd_last_error.restype = ctypes.c_char_p
     L.bart_amd_clear_error.restype = None

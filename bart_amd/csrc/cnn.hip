@@ -552,15 +552,10 @@ void WarmNet::forward(const float* x, int B, int H, int W, float* y, hipStream_t
 	}
 }
 
-double WarmNet::train_step(const float* x, const float* t, int B, int H, int W, float lr, hipStream_t st)
+// Backpropagation of the output gradient in g0 through the layer outputs of
+// the last forward (input x), then one Adam step.
+void WarmNet::backward_update(const float* x, int B, int H, int W, float lr, hipStream_t st)
 {
-	ensure(B, H, W);
-	forward(x, B, H, W, out.ptr, st);
-
-	const long n = (long)B * 2 * H * W;
-	hipLaunchKernelGGL(k_mse, dim3(WN_MSE_BLOCKS), dim3(256), 0, st, out.ptr, t, n, g0.ptr, lpart.ptr);
-	BAM_HIP(hipGetLastError());
-
 	float* g = g0.ptr;      // gradient at the output of layer l
 	float* gn = g1.ptr;
 
@@ -584,6 +579,18 @@ double WarmNet::train_step(const float* x, const float* t, int B, int H, int W, 
 	hipLaunchKernelGGL(k_adam, dim3(256), dim3(256), 0, st, params.ptr, grads.ptr, m.ptr, v.ptr, nparams, lr, b1, b2,
 			   1e-8f, c1, c2);
 	BAM_HIP(hipGetLastError());
+}
+
+double WarmNet::train_step(const float* x, const float* t, int B, int H, int W, float lr, hipStream_t st)
+{
+	ensure(B, H, W);
+	forward(x, B, H, W, out.ptr, st);
+
+	const long n = (long)B * 2 * H * W;
+	hipLaunchKernelGGL(k_mse, dim3(WN_MSE_BLOCKS), dim3(256), 0, st, out.ptr, t, n, g0.ptr, lpart.ptr);
+	BAM_HIP(hipGetLastError());
+
+	backward_update(x, B, H, W, lr, st);
 
 	std::vector<double> hp(WN_MSE_BLOCKS);
 	BAM_HIP(hipMemcpyAsync(hp.data(), lpart.ptr, sizeof(double) * WN_MSE_BLOCKS, hipMemcpyDeviceToHost, st));
@@ -592,6 +599,15 @@ double WarmNet::train_step(const float* x, const float* t, int B, int H, int W, 
 	for (double v : hp)
 		s += v;
 	return s / (double)n;
+}
+
+void WarmNet::train_step_grad(const float* x, const float* dout, int B, int H, int W, float lr, hipStream_t st)
+{
+	ensure(B, H, W);
+	const long n = (long)B * 2 * H * W;
+	BAM_HIP(hipMemcpyAsync(g0.ptr, dout, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+	backward_update(x, B, H, W, lr, st);
+	BAM_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace bam

@@ -34,6 +34,10 @@ struct WarmNet {
 	void forward(const float* x, int B, int H, int W, float* y, hipStream_t st);
 	// one Adam step on mean((x + net(x) - t)^2); returns the loss before the step
 	double train_step(const float* x, const float* t, int B, int H, int W, float lr, hipStream_t st);
+	// one Adam step for a loss whose gradient with respect to the output of
+	// the last forward(x) is dout (the layer outputs of that forward are used)
+	void train_step_grad(const float* x, const float* dout, int B, int H, int W, float lr, hipStream_t st);
+	void backward_update(const float* x, int B, int H, int W, float lr, hipStream_t st);
 };
 
 }  // namespace bam

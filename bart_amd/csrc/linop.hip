@@ -1067,4 +1067,17 @@ double bart_amd_warmnet_train_step(struct bart_amd_warmnet* net, const float* x,
 	}, -1.);
 }
 
+int bart_amd_warmnet_train_step_grad(struct bart_amd_warmnet* net, const float* x, const float* dout, int B, int H,
+				     int W, float lr)
+{
+	return guarded([&]() -> int {
+		BAM_CHECK(nullptr != net && nullptr != x && nullptr != dout,
+			  "bart_amd_warmnet_train_step_grad: NULL argument");
+		check_dev("bart_amd_warmnet_train_step_grad", x);
+		check_dev("bart_amd_warmnet_train_step_grad", dout);
+		((WarmNet*)net)->train_step_grad(x, dout, B, H, W, lr, default_stream());
+		return 0;
+	}, -1);
+}
+
 }  // extern "C"

@@ -71,6 +71,12 @@ class WarmNet:
         _check()
         return y
 
+    def train_step_grad(self, x, dout, lr):
+        """Adam step for the output gradient dout of the last forward(x)."""
+        B, _, H, W = x.shape
+        _lib().bart_amd_warmnet_train_step_grad(self.h, _fptr(x), _fptr(dout), B, H, W, lr)
+        _check()
+
     def train_step(self, x, target, lr):
         B, _, H, W = x.shape
         loss = _lib().bart_amd_warmnet_train_step(self.h, _fptr(x), _fptr(target), B, H, W, lr)
@@ -208,3 +214,46 @@ def solve_warm_and_cold(net, op, b, n, lam=0.01, tol=1e-4, maxiter=300):
     return dict(cold_it=stc.iterations, warm_it=stw.iterations, cold_ms=stc.time_ms,
                 warm_ms=stw.time_ms + t_guess, guess_ms=t_guess, fell_back=bool(stw.fell_back),
                 warm_vs_cold=diff, cold_status=stc.status, warm_status=stw.status)
+
+
+def residual_problems(seeds, n=256, C=8, lam=0.01):
+    """Problems kept alive for residual training: (operator, b, network
+    input, its scale s, ||b||^2)."""
+    out = []
+    for sd in seeds:
+        op, b, _ = make_problem(sd, n, C, lam=lam)
+        inp, s = net_input(first_iterate(op, b, lam), n)
+        out.append((op, b, inp, s, float(b.norm()) ** 2))
+    return out
+
+
+def train_residual(net, probs, n, epochs=10, batch=8, lr=2e-4, lam=0.01, seed=0, log=None):
+    """Adam on the mean over a batch of ||b - (A^H A + lam I) x0||^2 / ||b||^2,
+    x0 = s (network output): the quantity CG's stop test measures.  The
+    output gradient -2 s (A^H A + lam I) r / ||b||^2 takes two Toeplitz normal
+    operators per image.  Returns the mean loss of every epoch."""
+    import torch
+    rng = np.random.default_rng(seed)
+    hist = []
+    for e in range(epochs):
+        perm = rng.permutation(len(probs))
+        tot, cnt = 0.0, 0
+        for i in range(0, len(probs) - batch + 1, batch):
+            idx = perm[i:i + batch]
+            x = torch.cat([probs[j][2] for j in idx]).contiguous()
+            y = net.forward(x)
+            douts = []
+            for q, j in enumerate(idx):
+                op, b, _, s, bn2 = probs[j]
+                x0 = (from_channels(y[q:q + 1], n) * s).contiguous()
+                r = (b - op.normal(x0) - lam * x0).contiguous()
+                g = op.normal(r) + lam * r
+                tot += float(r.norm()) ** 2 / bn2
+                douts.append(to_channels((-2.0 * s / bn2 / len(idx)) * g, n))
+            net.train_step_grad(x, torch.cat(douts).contiguous(), lr)
+            cnt += len(idx)
+        hist.append(tot / max(1, cnt))
+        if log is not None:
+            log(f"residual epoch {e}: {hist[-1]:.3e}")
+    return hist
+

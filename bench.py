@@ -366,6 +366,15 @@ def run_warmstart(args):
     t0 = time.perf_counter()
     hist = WS.train(net, inp, tgt, epochs=args.ws_epochs, batch=8, lr=1e-3,
                     log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+    hist_r = []
+    if args.ws_finetune > 0:
+        # fine-tuning on the normal-equation residual of the guess
+        probs = WS.residual_problems(train_seeds[:args.ws_finetune], n=n, lam=lam)
+        hist_r = WS.train_residual(net, probs, n, epochs=args.ws_ft_epochs, lr=2e-4, lam=lam,
+                                   log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        for p in probs:
+            p[0].free()
+        del probs
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
 
@@ -443,6 +452,8 @@ def run_warmstart(args):
         "training_data_s": round(t_data, 2),
         "training_s": round(t_train, 2),
         "train_loss_first_last": [hist[0], hist[-1]],
+        "residual_finetune": {"problems": args.ws_finetune, "epochs": args.ws_ft_epochs,
+                              "loss_first_last": [hist_r[0], hist_r[-1]] if hist_r else None},
         "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(ach / 2500.0, 5), "traffic": None, "kernel": "network forward (6 conv3x3 layers)",
                      "flops_per_image": flops, "ms_per_image": round(t_fwd, 4)},
@@ -460,6 +471,9 @@ def main():
     ap.add_argument("--ws-train", type=int, default=1024, help="warmstart: training pairs")
     ap.add_argument("--ws-eval", type=int, default=32, help="warmstart: held-out problems")
     ap.add_argument("--ws-epochs", type=int, default=40, help="warmstart: training epochs")
+    ap.add_argument("--ws-finetune", type=int, default=0,
+                    help="warmstart: problems for fine-tuning on the normal-equation residual (0: none)")
+    ap.add_argument("--ws-ft-epochs", type=int, default=10, help="warmstart: residual fine-tuning epochs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--pmc-pass", action="store_true",

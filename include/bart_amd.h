@@ -194,6 +194,13 @@ int bart_amd_warmnet_forward(struct bart_amd_warmnet* net, const float* x, int B
 double bart_amd_warmnet_train_step(struct bart_amd_warmnet* net, const float* x, const float* target, int B, int H,
 				   int W, float lr);
 
+/* one Adam step for a loss whose gradient with respect to the output y of
+ * the last bart_amd_warmnet_forward(net, x, ...) is dout ([B][2][H][W]),
+ * e.g. the gradient of the normal-equation residual of the guess; x must be
+ * that forward's input */
+int bart_amd_warmnet_train_step_grad(struct bart_amd_warmnet* net, const float* x, const float* dout, int B, int H,
+				     int W, float lr);
+
 /* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
 enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };
 void bart_amd_set_error_mode(int mode);

@@ -96,3 +96,19 @@ def test_warm_start_pipeline_and_fallback():
     r2 = WS.solve_warm_and_cold(net, op, b, n)
     assert r2["fell_back"] and r2["warm_it"] == r2["cold_it"]
     op.free()
+
+
+def test_train_step_with_given_output_gradient():
+    # the MSE step and the step fed the MSE's output gradient by hand agree
+    from bart_amd.warmstart import WarmNet
+    rng = np.random.default_rng(12)
+    x = dev(rng.standard_normal((2, 2, 8, 64)))
+    t = dev(rng.standard_normal((2, 2, 8, 64)))
+    a = WarmNet(8, seed=5)
+    b = WarmNet(8, seed=5)
+    a.train_step(x, t, 1e-3)
+    y = b.forward(x)
+    dout = (2.0 * (y - t) / y.numel()).contiguous()
+    b.train_step_grad(x, dout, 1e-3)
+    assert np.allclose(a.get_params(), b.get_params(), rtol=0, atol=1e-7)
+    assert np.allclose(a.get_grads(), b.get_grads(), rtol=1e-6, atol=1e-12)
