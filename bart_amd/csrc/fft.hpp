@@ -419,14 +419,11 @@ constexpr int ct_next_radix(int m)
 	return (0 == m % 8) ? 8 : ((0 == m % 4) ? 4 : ((0 == m % 2) ? 2 : ((0 == m % 3) ? 3 : ((0 == m % 5) ? 5 : m))));
 }
 
-#ifndef BAM_SLICE_R16
-#define BAM_SLICE_R16 1
-#endif
-#ifndef BAM_AXIS_R16
-#define BAM_AXIS_R16 0
-#endif
-constexpr bool SLICE_R16 = BAM_SLICE_R16;
-constexpr bool AXIS_R16 = BAM_AXIS_R16;
+// radix 16 first in the slice kernels (two passes for 128 instead of
+// three); the axis kernels keep radix 8 (radix 16 there: z-pass 4 % faster,
+// 2-D passes slower)
+constexpr bool SLICE_R16 = true;
+constexpr bool AXIS_R16 = false;
 
 // radix 16 first where it divides (fewer passes through LDS)
 constexpr int ct_next_radix16(int m)

@@ -145,3 +145,24 @@ def test_batch_device_vectors(problems):
     for i, p in enumerate(ps):
         x1, _, _ = ba.lsqr_conjgrad(p["op"], p["b"], 8, 0.01, 0.0)
         assert same(xs[i].cpu().numpy(), x1)
+
+
+def test_batch_mixed_normal_operators(problems):
+    # a SENSE operator without Toeplitz PSF (normal = adjoint(forward(x)),
+    # linops/linop.c:489-525) beside Toeplitz ones: each problem still follows
+    # its single solve bit for bit
+    n, C, R, S = 48, 3, 96, 27
+    traj = I.radial_traj(R, S)
+    maps = I.gaussian_coils((1, n, n), C, seed=77)
+    conf = ba.NufftConf.defaults(toeplitz=False)
+    op = ba.sense_nc_init(dims16(n, n, 1, C), dims16(n, n, 1, C), maps, dims16(1, R, S, C), dims16(3, R, S),
+                          ba.traj_array(traj), conf=conf)
+    y = I.add_noise(op.forward(I.ellipse_phantom(n, seed=77)).reshape(C, R * S), 1e-3, seed=77)
+    b = op.adjoint(y)
+    ops = [problems[0]["op"], op, problems[2]["op"]]
+    bs = [problems[0]["b"], b, problems[2]["b"]]
+    xs, st, tr = ba.lsqr_conjgrad_batch(ops, bs, 15, 0.01, 1e-4)
+    for i, (o, bb) in enumerate(zip(ops, bs)):
+        x1, s1, t1 = ba.lsqr_conjgrad(o, bb, 15, 0.01, 1e-4)
+        assert st[i].iterations == s1.iterations and same(xs[i], x1) and same(tr[i], t1), i
+    op.free()
