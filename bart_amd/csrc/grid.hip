@@ -408,10 +408,17 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 	ta = choose_tiling(g);
 
 	sm.alloc(std::max<long>(g.M, 1));
+	si.alloc(std::max<long>(g.M, 1));
 	sk.alloc(std::max<long>(g.M, 1));
 	cstart.alloc(ncells + 1);
 
 	BAM_HIP(hipMemcpy(sm.ptr, hsm.data(), sizeof(int) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
+	{
+		std::vector<int> hsi(std::max<long>(g.M, 1), 0);
+		for (long i = 0; i < g.M; i++)
+			hsi[hsm[i]] = (int)i;
+		BAM_HIP(hipMemcpy(si.ptr, hsi.data(), sizeof(int) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
+	}
 	BAM_HIP(hipMemcpy(sk.ptr, hsk.data(), sizeof(float4) * std::max<long>(g.M, 1), hipMemcpyHostToDevice));
 
 	BAM_HIP(hipMemcpy(cstart.ptr, count.data(), sizeof(int) * (ncells + 1), hipMemcpyHostToDevice));
@@ -498,31 +505,36 @@ static int taps_bound(const GridGeom& g)
 
 // ---------------------------------------------------------------------------
 // Adjoint input permutation: kspS[(ch M + i) cc + cl] = ksp[(ch cc + cl) M + m(i)] conj(w(m))
-// (the weights of noncart/nufft.c:1474-1479 are applied here)
+// (the weights of noncart/nufft.c:1474-1479 are applied here), written
+// from the original sample order with si = the inverse of m(i)
 // ---------------------------------------------------------------------------
 
-__global__ void k_permute_ksp(long M, int C, int cc, const int* __restrict__ sm, const float2* __restrict__ ksp,
-			      const float2* __restrict__ weights, float2* __restrict__ kspS)
+// Thread per original sample m and coil chunk: the coil values are read
+// along m (coalesced) and the cc values of the chunk are written as one
+// contiguous run at the sample's sorted position si[m].
+__global__ void k_permute_ksp_inv(long M, int C, int cc, const int* __restrict__ si, const float2* __restrict__ ksp,
+				  const float2* __restrict__ weights, float2* __restrict__ kspS)
 {
 	const int nch = (C + cc - 1) / cc;
 	const long tot = (long)nch * M;
 
 	for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < tot; j += (long)gridDim.x * blockDim.x) {
 
-		long ch = j / M;
-		long i = j - ch * M;
-		int m = sm[i];
-		float2 w = (nullptr != weights) ? weights[m] : make_float2(1.f, 0.f);
+		const long ch = j / M;
+		const long m = j - ch * M;
+		const long i = si[m];
+		const float2 w = (nullptr != weights) ? weights[m] : make_float2(1.f, 0.f);
+		float2* out = kspS + (ch * M + i) * cc;
 
 		for (int cl = 0; cl < cc; cl++) {
 
-			int c = (int)ch * cc + cl;
+			const int c = (int)ch * cc + cl;
 			float2 v = make_float2(0.f, 0.f);
 
 			if (c < C)
 				v = (nullptr != weights) ? cmulc(ksp[(long)c * M + m], w) : ksp[(long)c * M + m];
 
-			kspS[(ch * M + i) * cc + cl] = v;
+			out[cl] = v;
 		}
 	}
 }
@@ -1170,7 +1182,7 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 		long tot = (long)nch * g.M;
 		unsigned blocks = (unsigned)std::max<long>(1, std::min<long>((tot + 255) / 256, 16384));
 		if (tot > 0)
-			hipLaunchKernelGGL(k_permute_ksp, dim3(blocks), dim3(256), 0, st, g.M, g.C, t.cc, gp.sm.ptr, ksp,
+			hipLaunchKernelGGL(k_permute_ksp_inv, dim3(blocks), dim3(256), 0, st, g.M, g.C, t.cc, gp.si.ptr, ksp,
 					   weights, gp.kspS.ptr);
 		BAM_HIP(hipGetLastError());
 	}

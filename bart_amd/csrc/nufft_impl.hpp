@@ -86,6 +86,7 @@ struct GridPlan {
 
 	GridGeom g;
 	DevBuf<int> sm;          // sorted sample -> original sample index
+	DevBuf<int> si;          // original sample -> sorted position
 	DevBuf<float4> sk;       // sorted coordinates (kx, ky, kz) + wrap code
 	DevBuf<int> cstart;      // cell -> first sorted sample (ncells + 1)
 	GridTiling ta;           // adjoint tiling
