@@ -3,13 +3,15 @@
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
 leg may import this package; the product path in ``bart_amd`` never does.
 
-Two oracles live here:
+Three oracles live here:
 
 * ``liboracle.so`` (``bart_oracle.c``): a C/OpenMP restatement of BART's CPU
   NUFFT (noncart/nufft.c, noncart/grid.c), Toeplitz PSF and CG
   (iter/italgos.c:497-580).  Wrapped by :class:`Nufft` below.
 * :func:`nudft_forward` / :func:`nudft_adjoint`: FP64 direct NUDFT with BART's
   sign and centring conventions (noncart/nudft.c:17-190).
+* ``oracle/cnn.py``: FP64 statement of the warm-start network's layers and
+  of the whole network's backpropagation (no counterpart in BART).
 """
 import ctypes
 import os
