@@ -210,7 +210,7 @@ static void host_cell_span(const GridGeom& g, int d, int clo, int chi, int sp[4]
 // the samples in home-tile order (cell order within a tile) and work items of
 // at most FW_MAXS samples of one tile.
 constexpr int FW_T = 8;          // tile edge, grid points
-constexpr int FW_SPT = 2;        // samples per thread
+constexpr int FW_SPT = 4;        // samples per thread
 constexpr int FW_THREADS = 512;
 constexpr int FW_MAXS = FW_SPT * FW_THREADS;
 constexpr int FW_CC = 4;         // coils per staged region
