@@ -577,16 +577,25 @@ struct StReduceS {
 // LDS holds the slice row-major with a row pitch of NX + 1 (bank spread of
 // the column transforms) and the two twiddle tables.
 //
+// Sub-grids s and s + S/2 differ only in the z shift (the z decomposition
+// bit is the top bit of s), and the z factor of linphase, lz_s(z), is one
+// scalar per slice.  The x-y transform is linear, so
+//   FFT_xy(v linxy_s lz_s(z)) = lz_s(z) FFT_xy(v linxy_s)
+// and one transform serves both sub-grids of a pair; likewise the inverse
+// transforms the combination conj(lz_s) g_s + conj(lz_s') g_s' once.
+//
 // k_slice_fwd: work-group (z, c).  The image-side values of slice z (coil
 // image c, or image x map c; x rolloff when roll) are read once into
-// registers; for every sub-grid s they are multiplied by linphase_s,
-// transformed along x and y and stored to grid[s][c][z].
+// registers; for every pair (s, s + S/2) they are multiplied by the x-y
+// factor linxy_s, transformed along x and y, and stored to grid[s][c][z]
+// times lz_s(z) and to grid[s + S/2][c][z] times lz_{s+S/2}(z).
 //
-// k_slice_inv: work-group (z, c).  For every s, grid[s][c][z] is
-// inverse-transformed along y and x and conj(linphase_s) x value is summed
-// over s in order; the sum is scaled by the rolloff (when roll) and, with
-// maps, multiplied by conj(map_c) into partial image c (the coils are summed
-// in order by sum_groups), else stored as coil image c.  No atomics.
+// k_slice_inv: work-group (z, c).  For every pair, conj(lz_s) grid[s][c][z]
+// + conj(lz_{s+S/2}) grid[s + S/2][c][z] is inverse-transformed along y and
+// x, and conj(linxy_s) x value is summed over the pairs in order; the sum is
+// scaled by the rolloff (when roll) and, with maps, multiplied by
+// conj(map_c) into partial image c (the coils are summed in order by
+// sum_groups), else stored as coil image c.  No atomics.
 // ---------------------------------------------------------------------------
 
 // threads of a slice work-group: a multiple of NX (every lane keeps its x)
@@ -628,13 +637,11 @@ struct SliceLane {
 	int x, y0, dy;
 };
 
-__device__ inline float2 lin_xyz(const SepTables& T, int s, int x, int y, int z)
+// x-y factor of sub-grid s's linphase, lin_x lin_y, from its table
+__device__ inline float2 lin_xy(const SepTables& T, int s, int x, int y)
 {
-	// same product order as lin_val: ((x) y) z; the x-y product from its
-	// table (3-D slice kernels)
 	const int bx = (s >> T.bitpos[0]) & 1, by = (s >> T.bitpos[1]) & 1;
-	const float2 v = T.linxy[bx + 2 * by][x + T.n[0] * y];
-	return cmul(v, T.lin[2][(s >> T.bitpos[2]) & 1][z]);
+	return T.linxy[bx + 2 * by][x + T.n[0] * y];
 }
 
 __device__ inline float roll_xyz(const SepTables& T, int x, int y, int z)
@@ -699,7 +706,12 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 		}
 	}
 
-	for (int s = 0; s < sg.S; s++) {
+	// z factors of the two sub-grids of a pair; sub-grid s + SH is s shifted in z
+	const float2 lz0 = T.lin[2][0][z], lz1 = T.lin[2][1][z];
+	const int SH = sg.S / 2;
+	const long pair_off = (long)sg.C * SH * sg.N;
+
+	for (int s = 0; s < SH; s++) {
 
 		// per-iteration copies of the lane coordinates (keeps the element
 		// addresses out of registers across the loop)
@@ -715,7 +727,7 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 
 #pragma unroll
 			for (int i = 0; i < EPT; i++)
-				v[i] = cmul(vin[i], lin_xyz(T, s, xs, ys + i * DY, z));
+				v[i] = cmul(vin[i], lin_xy(T, s, xs, ys + i * DY));
 
 			dft_any<EPT>(v, -1.f, twy, NY);
 
@@ -735,7 +747,8 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 			slice_passes_range<NX, 1, NX / RLX, NY, 1, LD, NTH>(lds, twx, -1.f);
 			float2* out = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
 			slice_last_pass_rows<NX, RLX, NY, LD, NTH>(lds, twx, -1.f, [&](int, int, int p, int xo, float2 v) {
-				out[p * NX + xo] = v;
+				out[p * NX + xo] = cmul(v, lz0);
+				out[pair_off + p * NX + xo] = cmul(v, lz1);
 			});
 			continue;
 
@@ -749,7 +762,7 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 				const int y = ys + i * DY;
 
 				if (0 == NY % DY || y < NY)
-					lds[y * LD + xs] = cmul(vin[i], lin_xyz(T, s, xs, y, z));
+					lds[y * LD + xs] = cmul(vin[i], lin_xy(T, s, xs, y));
 			}
 
 			lds_barrier();
@@ -769,8 +782,11 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 
 			const int y = ys + i * DY;
 
-			if (0 == NY % DY || y < NY)
-				out[y * NX + xs] = lds[y * LD + xs];
+			if (0 == NY % DY || y < NY) {
+				const float2 l = lds[y * LD + xs];
+				out[y * NX + xs] = cmul(l, lz0);
+				out[pair_off + y * NX + xs] = cmul(l, lz1);
+			}
 		}
 	}
 }
@@ -814,7 +830,12 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 	for (int i = 0; i < EPT; i++)
 		acc[i] = make_float2(0.f, 0.f);
 
-	for (int s = 0; s < sg.S; s++) {
+	// z factors of the two sub-grids of a pair; sub-grid s + SH is s shifted in z
+	const float2 lz0 = T.lin[2][0][z], lz1 = T.lin[2][1][z];
+	const int SH = sg.S / 2;
+	const long pair_off = (long)sg.C * SH * sg.N;
+
+	for (int s = 0; s < SH; s++) {
 
 		const float2* in = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
 
@@ -829,8 +850,10 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 			float2 v[EPT];
 
 #pragma unroll
-			for (int i = 0; i < EPT; i++)
-				v[i] = in[(ys + i * DY) * NX + xs];
+			for (int i = 0; i < EPT; i++) {
+				const int e = (ys + i * DY) * NX + xs;
+				v[i] = cadd(cmulc(in[e], lz0), cmulc(in[pair_off + e], lz1));
+			}
 
 			dft_any<EPT>(v, +1.f, twy, NY);
 
@@ -854,7 +877,7 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 				const int y = ys + i * DY;
 
 				if (0 == NY % DY || y < NY)
-					lds[y * LD + xs] = in[y * NX + xs];
+					lds[y * LD + xs] = cadd(cmulc(in[y * NX + xs], lz0), cmulc(in[pair_off + y * NX + xs], lz1));
 			}
 
 			lds_barrier();
@@ -864,7 +887,7 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 
 		slice_passes<NX, NY, 1, LD, NTH>(lds, twx, +1.f);
 
-		// sum over s of conj(linphase_s) x value
+		// sum over the pairs of conj(linxy_s) x value
 		xs = x;
 		ys = y0;
 		asm volatile("" : "+v"(xs), "+v"(ys));
@@ -875,7 +898,7 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 			const int y = ys + i * DY;
 
 			if (0 == NY % DY || y < NY)
-				acc[i] = cadd(acc[i], cmulc(lds[y * LD + xs], lin_xyz(T, s, xs, y, z)));
+				acc[i] = cadd(acc[i], cmulc(lds[y * LD + xs], lin_xy(T, s, xs, y)));
 		}
 	}
 
@@ -915,12 +938,21 @@ static bool slice_supported(const int n[3])
 	}
 }
 
+// the slice kernels pair sub-grid s with s + S/2: the z decomposition bit
+// must be the top bit of s
+static void check_slice_pairs(const Nufft& p)
+{
+	if (8 != p.S || 0 != p.tabs.bitpos[0] || 1 != p.tabs.bitpos[1] || 2 != p.tabs.bitpos[2])
+		fail("slice FFT: needs the 3-D decomposition into 8 sub-grids (S = %d)", p.S);
+}
+
 static void launch_slice_fwd(const Nufft& p, const float2* src, const float2* sens, bool roll, const int* skip,
 			     const char* name)
 {
 	ProfScope prof(name, p.stream);
 	SliceGeom sg{ p.n[2], p.S, p.C, p.N };
 	dim3 blocks(p.n[2], p.C);
+	check_slice_pairs(p);
 
 #define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_fwd<NN, NN>), blocks, dim3(slice_threads<NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, src, sens, roll, p.grid.ptr, p.axis[0].tw, p.axis[1].tw,   				   skip);                                                                       		break
 
@@ -943,6 +975,7 @@ static void launch_slice_inv(const Nufft& p, float2* dst, const float2* sens, bo
 	ProfScope prof(name, p.stream);
 	SliceGeom sg{ p.n[2], p.S, p.C, p.N };
 	dim3 blocks(p.n[2], p.C);
+	check_slice_pairs(p);
 
 #define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_inv<NN, NN>), blocks, dim3(slice_threads_inv<NN, NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, p.grid.ptr, sens, roll, dst, p.axis[0].tw,                 				   p.axis[1].tw, skip);                                                         		break
 
