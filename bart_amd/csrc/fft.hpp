@@ -869,6 +869,127 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 	st.store(lds, L, t0);
 }
 
+// Lengths whose pair kernel exists (k_fft_axis_pair needs the register-edge
+// geometry of the column-block path).
+inline bool fft_pair_len_ok(int n)
+{
+	return 64 == n || 128 == n || 256 == n || 512 == n;
+}
+
+// Strided-axis transform, pointwise multiply and inverse transform for a
+// pair of inputs that differ only by a factor along the axis: the column h
+// (held in registers) is multiplied by zf(b, k), transformed, multiplied by
+// the pair member's factor table at(b, ...), inverse-transformed, and
+// conj(zf(b, k)) x result is summed over b = 0, 1 into the store.  One read
+// and one write of the column serve both members.  Md provides base(t0),
+// at(base, b, p, k, x), zf(b, k) and cb_ok(P); the host launches it only
+// where the column-block geometry holds (cb_ok for every functor, strided).
+template <int N, class Ld, class Md, class St>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_pair(FftLaunch L, Ld ld, Md md, St st)
+{
+	static_assert(ct_edges_in_regs<N>(), "pair kernel needs the register-edge geometry");
+	constexpr int P = ct_block_p(N);
+	constexpr int LOGP = ct_log2(P);
+	constexpr int PN = N + 1;
+	constexpr int E = ct_edge_radix<N>();
+	constexpr int NE = N / E;
+
+	extern __shared__ float2 lds[];
+	float2* tw = lds + P * PN;
+
+	if (nullptr != L.skip && 0 != *L.skip)
+		return;
+
+	const long t0 = (long)blockIdx.x * P;
+
+	for (int i = threadIdx.x; i < N; i += FFT_THREADS)
+		tw[i] = L.plan.tw[i];
+
+	const int p = threadIdx.x & (P - 1);
+	const int k0 = threadIdx.x >> LOGP;
+	const bool in = t0 + p < L.total;
+	float2 h[E], acc[E], v[E];
+
+	{
+		const auto lb = ld.base(t0);
+#pragma unroll
+		for (int j = 0; j < E; j++) {
+			h[j] = in ? ld.at(lb, p, k0 + j * NE) : make_float2(0.f, 0.f);
+			acc[j] = make_float2(0.f, 0.f);
+		}
+	}
+
+	const auto mb = md.base(t0);
+
+	// last radix-E pass of a transform (sub-transforms of length N / E) from
+	// LDS into registers
+	auto last_pass = [&](float sgn) {
+#pragma unroll
+		for (int r = 0; r < E; r++) {
+			float2 x = lds[p * PN + k0 + r * NE];
+			if (r > 0) {
+				float2 w = tw[r * k0];
+				if (sgn > 0.f)
+					w.y = -w.y;
+				x = cmul(x, w);
+			}
+			v[r] = x;
+		}
+		dft_any<E>(v, sgn, tw, N);
+	};
+
+	for (int b = 0; b < 2; b++) {
+
+		// factor along the axis; first forward pass in registers
+#pragma unroll
+		for (int j = 0; j < E; j++)
+			v[j] = cmul(h[j], md.zf(b, k0 + j * NE));
+
+		dft_any<E>(v, L.sign1, tw, N);
+
+		if (b > 0)
+			lds_barrier();  // the previous member's last pass has read LDS
+
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			lds[p * PN + k0 * E + q] = v[q];
+
+		lds_barrier();
+		ct_passes_between<N, P, E, NE>(lds, tw, L.sign1);
+		last_pass(L.sign1);
+
+		// pointwise multiply; first inverse pass
+		if (in) {
+#pragma unroll
+			for (int q = 0; q < E; q++)
+				v[q] = md.at(mb, b, p, k0 + q * NE, v[q]);
+		}
+
+		dft_any<E>(v, L.sign2, tw, N);
+
+		lds_barrier();
+
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			lds[p * PN + k0 * E + q] = v[q];
+
+		lds_barrier();
+		ct_passes_between<N, P, E, NE>(lds, tw, L.sign2);
+		last_pass(L.sign2);
+
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			acc[q] = cadd(acc[q], cmulc(v[q], md.zf(b, k0 + q * NE)));
+	}
+
+	if (in) {
+		const auto sb = st.base(t0);
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			st.at(sb, p, k0 + q * NE, acc[q]);
+	}
+}
+
 // ---------------------------------------------------------------------------
 // Strided multi-transform passes in LDS for whole 2-D slices: P transforms
 // of length N, element j of transform p at lds[p TS + j ES].  Butterflies
@@ -1138,6 +1259,32 @@ void launch_fft_axis(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, 
 		default: hipLaunchKernelGGL((k_fft_axis<32, false, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break;
 		}
 	}
+
+	BAM_HIP(hipGetLastError());
+}
+
+template <class Ld, class Md, class St>
+void launch_fft_axis_pair(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, const Md& md, const St& st)
+{
+	BAM_CHECK(cfg.fixed && !cfg.L.contig && fft_pair_len_ok(cfg.L.plan.n),
+		  "pair transform: unsupported geometry (n = %d)", cfg.L.plan.n);
+
+	if (0 == cfg.blocks)
+		return;
+
+	ProfScope prof(cfg.name, stream);
+	const dim3 gb(cfg.blocks), tb(FFT_THREADS);
+
+#define BAM_FFT_N(NN) case NN: hipLaunchKernelGGL((k_fft_axis_pair<NN, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break
+	switch (cfg.L.plan.n) {
+		BAM_FFT_N(64);
+		BAM_FFT_N(128);
+		BAM_FFT_N(256);
+		BAM_FFT_N(512);
+	default:
+		fail("pair transform: no kernel for n = %d", cfg.L.plan.n);
+	}
+#undef BAM_FFT_N
 
 	BAM_HIP(hipGetLastError());
 }

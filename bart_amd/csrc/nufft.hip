@@ -495,6 +495,34 @@ struct MdPsf {
 	}
 };
 
+// Toeplitz z-pass over pairs of sub-grids s, s + S/2 (the same x-y
+// transform H, z factors lz_0 and lz_1): the stack holds the volumes
+// v = c + C s, s < S/2, the PSF all S sub-grids
+struct MdPsfPair {
+	VolAxis g;
+	int C;
+	long half;  // PSF offset of sub-grid s + S/2 from sub-grid s: (S/2) N
+	const float2* psf;
+	const float2* lz0;
+	const float2* lz1;
+
+	__device__ bool cb_ok(int P) const { return 0 == g.inner % (uint32_t)P; }
+
+	__device__ ColBase base(long t0) const
+	{
+		ColBase b = col_base(g, t0);
+		b.off = (long)(b.v / (uint32_t)C) * g.N + g.offset_iov(b.i0, b.o, 0, 0);
+		return b;
+	}
+
+	__device__ float2 at(const ColBase& b, int m, int p, int k, float2 x) const
+	{
+		return cmul(x, psf[b.off + m * half + p + (long)g.inner * k]);
+	}
+
+	__device__ float2 zf(int m, int k) const { return (0 == m) ? lz0[k] : lz1[k]; }
+};
+
 // PSF multiply in the S-grouped geometry (single active axis)
 struct MdPsfS {
 	static constexpr bool active = true;
@@ -655,7 +683,8 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 								    const float2* __restrict__ sens, bool roll,
 								    float2* __restrict__ grid,
 								    const float2* __restrict__ twx_g,
-								    const float2* __restrict__ twy_g, const int* skip)
+								    const float2* __restrict__ twy_g, const int* skip,
+								    bool zpair)
 {
 	constexpr int NTH = slice_threads<NX>();
 	constexpr int DY = NTH / NX;
@@ -747,8 +776,12 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 			slice_passes_range<NX, 1, NX / RLX, NY, 1, LD, NTH>(lds, twx, -1.f);
 			float2* out = grid + ((long)c + (long)sg.C * s) * sg.N + zoff;
 			slice_last_pass_rows<NX, RLX, NY, LD, NTH>(lds, twx, -1.f, [&](int, int, int p, int xo, float2 v) {
-				out[p * NX + xo] = cmul(v, lz0);
-				out[pair_off + p * NX + xo] = cmul(v, lz1);
+				if (zpair) {
+					out[p * NX + xo] = v;
+				} else {
+					out[p * NX + xo] = cmul(v, lz0);
+					out[pair_off + p * NX + xo] = cmul(v, lz1);
+				}
 			});
 			continue;
 
@@ -784,8 +817,12 @@ __global__ __launch_bounds__(slice_threads<NX>()) void k_slice_fwd(SliceGeom sg,
 
 			if (0 == NY % DY || y < NY) {
 				const float2 l = lds[y * LD + xs];
-				out[y * NX + xs] = cmul(l, lz0);
-				out[pair_off + y * NX + xs] = cmul(l, lz1);
+				if (zpair) {
+					out[y * NX + xs] = l;
+				} else {
+					out[y * NX + xs] = cmul(l, lz0);
+					out[pair_off + y * NX + xs] = cmul(l, lz1);
+				}
 			}
 		}
 	}
@@ -797,7 +834,8 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 								    const float2* __restrict__ sens, bool roll,
 								    float2* __restrict__ dst,
 								    const float2* __restrict__ twx_g,
-								    const float2* __restrict__ twy_g, const int* skip)
+								    const float2* __restrict__ twy_g, const int* skip,
+								    bool zpair)
 {
 	constexpr int NTH = slice_threads_inv<NX, NY>();
 	constexpr int DY = NTH / NX;
@@ -852,7 +890,7 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 #pragma unroll
 			for (int i = 0; i < EPT; i++) {
 				const int e = (ys + i * DY) * NX + xs;
-				v[i] = cadd(cmulc(in[e], lz0), cmulc(in[pair_off + e], lz1));
+				v[i] = zpair ? in[e] : cadd(cmulc(in[e], lz0), cmulc(in[pair_off + e], lz1));
 			}
 
 			dft_any<EPT>(v, +1.f, twy, NY);
@@ -877,7 +915,9 @@ __global__ __launch_bounds__((slice_threads_inv<NX, NY>())) void k_slice_inv(Sli
 				const int y = ys + i * DY;
 
 				if (0 == NY % DY || y < NY)
-					lds[y * LD + xs] = cadd(cmulc(in[y * NX + xs], lz0), cmulc(in[pair_off + y * NX + xs], lz1));
+					lds[y * LD + xs] = zpair ? in[y * NX + xs]
+								 : cadd(cmulc(in[y * NX + xs], lz0),
+									cmulc(in[pair_off + y * NX + xs], lz1));
 			}
 
 			lds_barrier();
@@ -946,15 +986,17 @@ static void check_slice_pairs(const Nufft& p)
 		fail("slice FFT: needs the 3-D decomposition into 8 sub-grids (S = %d)", p.S);
 }
 
+// zpair: store only the x-y transform H of each pair (volumes s < S/2),
+// without the z factors, for the pair z-pass of the Toeplitz operator
 static void launch_slice_fwd(const Nufft& p, const float2* src, const float2* sens, bool roll, const int* skip,
-			     const char* name)
+			     const char* name, bool zpair = false)
 {
 	ProfScope prof(name, p.stream);
 	SliceGeom sg{ p.n[2], p.S, p.C, p.N };
 	dim3 blocks(p.n[2], p.C);
 	check_slice_pairs(p);
 
-#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_fwd<NN, NN>), blocks, dim3(slice_threads<NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, src, sens, roll, p.grid.ptr, p.axis[0].tw, p.axis[1].tw,   				   skip);                                                                       		break
+#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_fwd<NN, NN>), blocks, dim3(slice_threads<NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, src, sens, roll, p.grid.ptr, p.axis[0].tw, p.axis[1].tw,   				   skip, zpair);                                                                       		break
 
 	switch (p.n[0]) {
 		BAM_SLICE(32);
@@ -969,15 +1011,17 @@ static void launch_slice_fwd(const Nufft& p, const float2* src, const float2* se
 	BAM_HIP(hipGetLastError());
 }
 
+// zpair: the volumes s < S/2 already hold conj(lz_s) g_s + conj(lz_s') g_s'
+// (pair z-pass output)
 static void launch_slice_inv(const Nufft& p, float2* dst, const float2* sens, bool roll, const int* skip,
-			     const char* name)
+			     const char* name, bool zpair = false)
 {
 	ProfScope prof(name, p.stream);
 	SliceGeom sg{ p.n[2], p.S, p.C, p.N };
 	dim3 blocks(p.n[2], p.C);
 	check_slice_pairs(p);
 
-#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_inv<NN, NN>), blocks, dim3(slice_threads_inv<NN, NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, p.grid.ptr, sens, roll, dst, p.axis[0].tw,                 				   p.axis[1].tw, skip);                                                         		break
+#define BAM_SLICE(NN)                                                                                           	case NN:                                                                                                		hipLaunchKernelGGL((k_slice_inv<NN, NN>), blocks, dim3(slice_threads_inv<NN, NN>()), (slice_lds_bytes<NN, NN>()), 				   p.stream, sg, p.tabs, p.grid.ptr, sens, roll, dst, p.axis[0].tw,                 				   p.axis[1].tw, skip, zpair);                                                         		break
 
 	switch (p.n[0]) {
 		BAM_SLICE(32);
@@ -1101,6 +1145,11 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	// slices fit in LDS
 	const char* sl = getenv("BART_AMD_SLICE");
 	slice = (3 == nact) && slice_supported(n) && !(nullptr != sl && 0 == strcmp(sl, "off"));
+
+	// BART_AMD_PAIRZ=off: Toeplitz z-pass over all S sub-grids instead of
+	// over the pairs that share an x-y transform
+	const char* pz = getenv("BART_AMD_PAIRZ");
+	pair_zpass = slice && fft_pair_len_ok(n[2]) && !(nullptr != pz && 0 == strcmp(pz, "off"));
 
 	has_weights = (nullptr != weights_host);
 
@@ -1268,23 +1317,39 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 
 	if (slice) {
 
-		launch_slice_fwd(*this, src, sens, false, skip, "fft_toeplitz_slice_fwd");
-
-		VolAxis va = VolAxis::make(n, 2, (long)S * C);
+		// sub-grids s and s + S/2 share the x-y transform H: the slice
+		// kernels store and read H once per pair, and the z-pass applies both
+		// members' z factors and PSFs to it (pair_z)
+		VolAxis va = VolAxis::make(n, 2, (long)(S / 2) * C);
 		FftLaunchCfg cfg = fft_config_skip(skip, axis[2], va.transforms(), va.contig(), 1, -1.f, +1.f);
+		const bool pair_z = pair_zpass && cfg.fixed && !cfg.L.contig && 0 == va.inner % (uint32_t)cfg.L.P;
+
+		if (!pair_z) {
+			va = VolAxis::make(n, 2, (long)S * C);
+			cfg = fft_config_skip(skip, axis[2], va.transforms(), va.contig(), 1, -1.f, +1.f);
+		}
+
+		launch_slice_fwd(*this, src, sens, false, skip, "fft_toeplitz_slice_fwd", pair_z);
+
 		// the C coil volumes of a sub-grid share its PSF: launch order
 		// column group -> volume, so that the PSF columns come from cache
 		va.group_volumes(cfg.L.P);
 		LdGrid ld{ va, tabs, grid.ptr, false };
 		StGrid st{ va, tabs, grid.ptr, false };
-		MdPsf md{ va, C, psf.ptr };
-		launch_fft_axis(cfg, stream, ld, md, st);
+
+		if (pair_z) {
+			MdPsfPair md{ va, C, (long)(S / 2) * N, psf.ptr, tabs.lin[2][0], tabs.lin[2][1] };
+			launch_fft_axis_pair(cfg, stream, ld, md, st);
+		} else {
+			MdPsf md{ va, C, psf.ptr };
+			launch_fft_axis(cfg, stream, ld, md, st);
+		}
 
 		// with maps: one partial image per coil, summed in coil order
 		const int G = C;
 		float2* out = (nullptr != sens && G > 1) ? part.ptr : dst;
 
-		launch_slice_inv(*this, out, sens, false, skip, "fft_toeplitz_slice_inv");
+		launch_slice_inv(*this, out, sens, false, skip, "fft_toeplitz_slice_inv", pair_z);
 
 		if (out != dst)
 			sum_groups(dst, part.ptr, N, G, stream, skip);

@@ -161,6 +161,7 @@ struct Nufft {
 	DevBuf<float2> part;   // per-coil-group partial images [G][N]
 	AxisPlan axis[3];
 	bool slice = false;    // 3-D with x-y slices transformed in LDS
+	bool pair_zpass = false;  // Toeplitz z-pass over sub-grid pairs (slice path)
 
 	hipStream_t stream = 0;
 

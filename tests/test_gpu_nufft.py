@@ -266,7 +266,11 @@ def test_nufft_adjoint_dot_test():
 
 @pytest.mark.parametrize("img,R,S,C,weighted", [((64, 64, 1), 128, 21, 2, False), ((48, 48, 1), 96, 13, 3, True),
                                                 ((20, 20, 20), 40, 48, 2, False),
-                                                ((32, 32, 32), 64, 150, 2, True)])
+                                                ((32, 32, 32), 64, 150, 2, True),
+                                                # z lengths with the pair z-pass (sub-grids s, s + 4
+                                                # share one x-y transform)
+                                                ((32, 32, 64), 64, 120, 2, True),
+                                                ((64, 64, 64), 128, 100, 2, False)])
 def test_psf_and_toeplitz_normal(img, R, S, C, weighted):
     X, Y, Z = img
     traj = I.radial_traj(R, S) if Z == 1 else I.phyllotaxis_kooshball(spokes=S, readout=R)
@@ -278,6 +282,24 @@ def test_psf_and_toeplitz_normal(img, R, S, C, weighted):
     x = crandn((C, Z, Y, X), 15)
     y = op.normal(x).reshape(C, Z, Y, X)
     assert relerr(y, ref.normal(x)) < 1e-5
+
+
+def test_toeplitz_pair_zpass_vs_all_subgrids(monkeypatch):
+    # the Toeplitz normal with the pair z-pass (one x-y transform per pair of
+    # sub-grids that differ only in the z shift) against the z-pass over all
+    # 8 sub-grids, and both against the oracle
+    img, R, S, C = (64, 64, 64), 128, 80, 3
+    traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
+    a = make_nufft(img, C, traj, R, S, toeplitz=True)
+    monkeypatch.setenv("BART_AMD_PAIRZ", "off")
+    b = make_nufft(img, C, traj, R, S, toeplitz=True)
+    ref = O.Nufft(img, traj, C, toeplitz=True)
+    x = crandn((C, 64, 64, 64), 17)
+    ya, yb = a.normal(x).reshape(C, 64, 64, 64), b.normal(x).reshape(C, 64, 64, 64)
+    yr = ref.normal(x)
+    print(f"pair vs all {relerr(ya, yb):.2e}, pair vs oracle {relerr(ya, yr):.2e}, all vs oracle {relerr(yb, yr):.2e}")
+    assert relerr(ya, yb) < 1e-5
+    assert relerr(ya, yr) < 1e-5 and relerr(yb, yr) < 1e-5
 
 
 def test_psf_update_roundtrip():
@@ -325,10 +347,11 @@ def test_sense_forward_adjoint_normal():
     assert relerr(nn, ref.sense_normal(maps, x)) < 1e-5
 
 
-@pytest.mark.parametrize("C", [4, 6])
-def test_sense_3d_slice_path(C):
-    # slice kernels with the coil sum of S^H over coil groups
-    n, R, S = 32, 64, 120
+@pytest.mark.parametrize("n,C", [(32, 4), (32, 6), (64, 3)])
+def test_sense_3d_slice_path(n, C):
+    # slice kernels with the coil sum of S^H over coil groups (64^3: the
+    # Toeplitz normal with the pair z-pass)
+    R, S = 2 * n, 120
     op, ref, maps, img, traj = make_sense(n, C, R, S, Z=n, seed=25)
     k = op.forward(img).reshape(C, -1)
     assert relerr(k, ref.sense_forward(maps, img)) < 1e-5
