@@ -876,6 +876,24 @@ inline bool fft_pair_len_ok(int n)
 	return 64 == n || 128 == n || 256 == n || 512 == n;
 }
 
+// columns per block of the pair kernel: 2048 elements up to n = 128 (for
+// n = 128, 16 columns of 4 points per thread need 63 registers and allow
+// 8 waves per SIMD, against 112 and 4 for 32 columns: 0.94 against 1.05 ms
+// per kooshball z-pass)
+constexpr int pair_block_p(int n)
+{
+	return (n <= 128) ? 2048 / n : ct_block_p(n);
+}
+
+// the register-edge geometry for P columns of length N
+template <int N, int P>
+constexpr bool edges_in_regs_p()
+{
+	constexpr int E = P * N / FFT_THREADS;
+	return 0 == (P * N) % FFT_THREADS && (2 == E || 4 == E || 8 == E || 16 == E) && 0 == N % (E * E) &&
+	       0 == FFT_THREADS % P;
+}
+
 // Strided-axis transform, pointwise multiply and inverse transform for a
 // pair of inputs that differ only by a factor along the axis: the column h
 // (held in registers) is multiplied by zf(b, k), transformed, multiplied by
@@ -887,11 +905,11 @@ inline bool fft_pair_len_ok(int n)
 template <int N, class Ld, class Md, class St>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_pair(FftLaunch L, Ld ld, Md md, St st)
 {
-	static_assert(ct_edges_in_regs<N>(), "pair kernel needs the register-edge geometry");
-	constexpr int P = ct_block_p(N);
+	constexpr int P = pair_block_p(N);
+	static_assert(edges_in_regs_p<N, P>(), "pair kernel needs the register-edge geometry");
 	constexpr int LOGP = ct_log2(P);
 	constexpr int PN = N + 1;
-	constexpr int E = ct_edge_radix<N>();
+	constexpr int E = P * N / FFT_THREADS;
 	constexpr int NE = N / E;
 
 	extern __shared__ float2 lds[];
@@ -1204,6 +1222,20 @@ struct FftLaunchCfg {
 
 FftLaunchCfg fft_config(const AxisPlan& plan, long total, bool contig, int P_min, float sign1, float sign2);
 
+// the launch geometry of k_fft_axis_pair (its own columns per block)
+inline void fft_config_pair(FftLaunchCfg& c)
+{
+	const int n = c.L.plan.n;
+	const int P = pair_block_p(n);
+	int logP = 0;
+	while ((1 << logP) < P)
+		logP++;
+	c.L.P = P;
+	c.L.logP = logP;
+	c.blocks = (unsigned)((c.L.total + P - 1) / P);
+	c.lds_bytes = (size_t)(P * (n + 1) + n) * sizeof(float2);
+}
+
 template <class Ld, class Md, class St>
 void launch_fft_axis(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, const Md& md, const St& st)
 {
@@ -1266,8 +1298,9 @@ void launch_fft_axis(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, 
 template <class Ld, class Md, class St>
 void launch_fft_axis_pair(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, const Md& md, const St& st)
 {
-	BAM_CHECK(cfg.fixed && !cfg.L.contig && fft_pair_len_ok(cfg.L.plan.n),
-		  "pair transform: unsupported geometry (n = %d)", cfg.L.plan.n);
+	BAM_CHECK(cfg.fixed && !cfg.L.contig && fft_pair_len_ok(cfg.L.plan.n) &&
+			  cfg.L.P == pair_block_p(cfg.L.plan.n),
+		  "pair transform: unsupported geometry (n = %d, P = %d)", cfg.L.plan.n, cfg.L.P);
 
 	if (0 == cfg.blocks)
 		return;

@@ -1322,6 +1322,8 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 		// members' z factors and PSFs to it (pair_z)
 		VolAxis va = VolAxis::make(n, 2, (long)(S / 2) * C);
 		FftLaunchCfg cfg = fft_config_skip(skip, axis[2], va.transforms(), va.contig(), 1, -1.f, +1.f);
+		if (cfg.fixed)
+			fft_config_pair(cfg);
 		const bool pair_z = pair_zpass && cfg.fixed && !cfg.L.contig && 0 == va.inner % (uint32_t)cfg.L.P;
 
 		if (!pair_z) {
