@@ -1008,6 +1008,108 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_pair(FftLaunch L, Ld l
 	}
 }
 
+// Strided-axis transform for a pair of volumes that differ only by a
+// factor along the axis, without a pointwise multiply.  Split (forward): the
+// column h of volume v is multiplied by zf_b(k) and transformed, for b = 0, 1,
+// into volumes v and v + pair_off.  Merge (adjoint): the columns of volumes
+// v and v + pair_off are transformed and conj(zf_b(k)) x result is summed
+// over b into volume v.  Same launch geometry as k_fft_axis_pair.
+template <int N, bool MERGE, class Ld, class St>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_pair_io(FftLaunch L, Ld ld, St st, const float2* zf0,
+								  const float2* zf1, long pair_off)
+{
+	constexpr int P = pair_block_p(N);
+	static_assert(edges_in_regs_p<N, P>(), "pair kernel needs the register-edge geometry");
+	constexpr int LOGP = ct_log2(P);
+	constexpr int PN = N + 1;
+	constexpr int E = P * N / FFT_THREADS;
+	constexpr int NE = N / E;
+
+	extern __shared__ float2 lds[];
+	float2* tw = lds + P * PN;
+
+	if (nullptr != L.skip && 0 != *L.skip)
+		return;
+
+	const long t0 = (long)blockIdx.x * P;
+
+	for (int i = threadIdx.x; i < N; i += FFT_THREADS)
+		tw[i] = L.plan.tw[i];
+
+	const int p = threadIdx.x & (P - 1);
+	const int k0 = threadIdx.x >> LOGP;
+	const bool in = t0 + p < L.total;
+	float2 h[E], v[E];  // split: the input column; merge: the sum
+
+	const auto lb = ld.base(t0);
+	const auto sb = st.base(t0);
+
+#pragma unroll
+	for (int j = 0; j < E; j++)
+		h[j] = (!MERGE && in) ? ld.at(lb, p, k0 + j * NE) : make_float2(0.f, 0.f);
+
+	for (int b = 0; b < 2; b++) {
+
+		const float2* zf = (0 == b) ? zf0 : zf1;
+
+		if constexpr (MERGE) {
+			auto lbb = lb;
+			lbb.off += b * pair_off;
+#pragma unroll
+			for (int j = 0; j < E; j++)
+				v[j] = in ? ld.at(lbb, p, k0 + j * NE) : make_float2(0.f, 0.f);
+		} else {
+#pragma unroll
+			for (int j = 0; j < E; j++)
+				v[j] = cmul(h[j], zf[k0 + j * NE]);
+		}
+
+		dft_any<E>(v, L.sign1, tw, N);
+
+		if (b > 0)
+			lds_barrier();  // the previous member's last pass has read LDS
+
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			lds[p * PN + k0 * E + q] = v[q];
+
+		lds_barrier();
+		ct_passes_between<N, P, E, NE>(lds, tw, L.sign1);
+
+		// last radix-E pass into registers
+#pragma unroll
+		for (int r = 0; r < E; r++) {
+			float2 x = lds[p * PN + k0 + r * NE];
+			if (r > 0) {
+				float2 w = tw[r * k0];
+				if (L.sign1 > 0.f)
+					w.y = -w.y;
+				x = cmul(x, w);
+			}
+			v[r] = x;
+		}
+		dft_any<E>(v, L.sign1, tw, N);
+
+		if constexpr (MERGE) {
+#pragma unroll
+			for (int q = 0; q < E; q++)
+				h[q] = cadd(h[q], cmulc(v[q], zf[k0 + q * NE]));
+		} else if (in) {
+			auto sbb = sb;
+			sbb.off += b * pair_off;
+#pragma unroll
+			for (int q = 0; q < E; q++)
+				st.at(sbb, p, k0 + q * NE, v[q]);
+		}
+	}
+
+	if (MERGE && in) {
+#pragma unroll
+		for (int q = 0; q < E; q++)
+			st.at(sb, p, k0 + q * NE, h[q]);
+	}
+}
+
 // ---------------------------------------------------------------------------
 // Strided multi-transform passes in LDS for whole 2-D slices: P transforms
 // of length N, element j of transform p at lds[p TS + j ES].  Butterflies
@@ -1309,6 +1411,34 @@ void launch_fft_axis_pair(const FftLaunchCfg& cfg, hipStream_t stream, const Ld&
 	const dim3 gb(cfg.blocks), tb(FFT_THREADS);
 
 #define BAM_FFT_N(NN) case NN: hipLaunchKernelGGL((k_fft_axis_pair<NN, Ld, Md, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, md, st); break
+	switch (cfg.L.plan.n) {
+		BAM_FFT_N(64);
+		BAM_FFT_N(128);
+		BAM_FFT_N(256);
+		BAM_FFT_N(512);
+	default:
+		fail("pair transform: no kernel for n = %d", cfg.L.plan.n);
+	}
+#undef BAM_FFT_N
+
+	BAM_HIP(hipGetLastError());
+}
+
+template <bool MERGE, class Ld, class St>
+void launch_fft_axis_pair_io(const FftLaunchCfg& cfg, hipStream_t stream, const Ld& ld, const St& st,
+			     const float2* zf0, const float2* zf1, long pair_off)
+{
+	BAM_CHECK(cfg.fixed && !cfg.L.contig && fft_pair_len_ok(cfg.L.plan.n) &&
+			  cfg.L.P == pair_block_p(cfg.L.plan.n),
+		  "pair transform: unsupported geometry (n = %d, P = %d)", cfg.L.plan.n, cfg.L.P);
+
+	if (0 == cfg.blocks)
+		return;
+
+	ProfScope prof(cfg.name, stream);
+	const dim3 gb(cfg.blocks), tb(FFT_THREADS);
+
+#define BAM_FFT_N(NN) case NN: hipLaunchKernelGGL((k_fft_axis_pair_io<NN, MERGE, Ld, St>), gb, tb, cfg.lds_bytes, stream, cfg.L, ld, st, zf0, zf1, pair_off); break
 	switch (cfg.L.plan.n) {
 		BAM_FFT_N(64);
 		BAM_FFT_N(128);

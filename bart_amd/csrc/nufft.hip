@@ -1213,22 +1213,66 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 // Forward / adjoint / normal
 // ---------------------------------------------------------------------------
 
-// z axis of the sub-grid stack (slice path)
-static void z_pass(const Nufft& p, float sign, bool conj_fm, bool fm, const char* name)
+// z axis of the sub-grid stack (slice path).  pair: the stack is read
+// (forward) or written (adjoint) as the S/2 pair volumes of the slice
+// kernels' zpair mode -- forward: volume s holds H, the x-y transform shared
+// by sub-grids s and s + S/2, and both are written; adjoint: sub-grids s and
+// s + S/2 are read and conj(lz_s) Z_s + conj(lz_{s+S/2}) Z_{s+S/2} is
+// written to volume s.  Returns whether the pair form was used.
+static bool z_pass(const Nufft& p, float sign, bool conj_fm, bool fm, const char* name)
 {
+	if (p.pair_zpass) {
+		VolAxis va = VolAxis::make(p.n, 2, (long)(p.S / 2) * p.C);
+		FftLaunchCfg cfg = fft_config(p.axis[2], va.transforms(), va.contig(), 1, sign, sign);
+
+		if (cfg.fixed) {
+			fft_config_pair(cfg);
+
+			if (!cfg.L.contig && 0 == va.inner % (uint32_t)cfg.L.P) {
+				cfg.name = name;
+				LdGrid ld{ va, p.tabs, p.grid.ptr, conj_fm };
+				StGrid st{ va, p.tabs, p.grid.ptr, fm };
+				const long pair_off = (long)(p.S / 2) * p.C * p.N;
+				if (sign < 0.f)
+					launch_fft_axis_pair_io<false>(cfg, p.stream, ld, st, p.tabs.lin[2][0], p.tabs.lin[2][1],
+								       pair_off);
+				else
+					launch_fft_axis_pair_io<true>(cfg, p.stream, ld, st, p.tabs.lin[2][0], p.tabs.lin[2][1],
+								      pair_off);
+				return true;
+			}
+		}
+	}
+
 	VolAxis va = VolAxis::make(p.n, 2, (long)p.S * p.C);
 	FftLaunchCfg cfg = fft_config(p.axis[2], va.transforms(), va.contig(), 1, sign, sign);
 	cfg.name = name;
 	LdGrid ld{ va, p.tabs, p.grid.ptr, conj_fm };
 	StGrid st{ va, p.tabs, p.grid.ptr, fm };
 	launch_fft_axis(cfg, p.stream, ld, NoMid{}, st);
+	return false;
+}
+
+// whether z_pass takes the pair form (same conditions)
+static bool z_pass_pairs(const Nufft& p)
+{
+	if (!p.pair_zpass)
+		return false;
+	VolAxis va = VolAxis::make(p.n, 2, (long)(p.S / 2) * p.C);
+	FftLaunchCfg cfg = fft_config(p.axis[2], va.transforms(), va.contig(), 1, -1.f, -1.f);
+	if (!cfg.fixed)
+		return false;
+	fft_config_pair(cfg);
+	return !cfg.L.contig && 0 == va.inner % (uint32_t)cfg.L.P;
 }
 
 void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
 {
 	if (slice) {
-		launch_slice_fwd(*this, src, sens, true, nullptr, "fft_fwd_slice");
-		z_pass(*this, -1.f, false, true, "fft_fwd_axis");
+		const bool pz = z_pass_pairs(*this);
+		launch_slice_fwd(*this, src, sens, true, nullptr, "fft_fwd_slice", pz);
+		const bool used = z_pass(*this, -1.f, false, true, "fft_fwd_axis");
+		BAM_CHECK(used == pz, "z-pass form mismatch");
 		grid_forward(gp, kb.ptr, ksp, grid.ptr, has_weights ? weights.ptr : nullptr, stream);
 		return;
 	}
@@ -1266,13 +1310,13 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 
 	if (slice) {
 
-		z_pass(*this, +1.f, false, false, "fft_adj_axis");
+		const bool pz = z_pass(*this, +1.f, false, false, "fft_adj_axis");
 
 		// with maps: one partial image per coil, summed in coil order
 		const int G = C;
 		float2* out = (nullptr != sens && G > 1) ? part.ptr : dst;
 
-		launch_slice_inv(*this, out, sens, true, nullptr, "fft_adj_slice");
+		launch_slice_inv(*this, out, sens, true, nullptr, "fft_adj_slice", pz);
 
 		if (out != dst)
 			sum_groups(dst, part.ptr, N, G, stream, nullptr);

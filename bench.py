@@ -95,14 +95,17 @@ def algorithmic_bytes(kernel, w):
         "grid_adjoint": 8 * C * M + 4 * d * M + grid,
         # sub-grid stack + coordinates in, k-space out
         "grid_forward": grid + 4 * d * M + 8 * C * M,
-        # one axis of the sub-grid stack: read + write
-        "fft_fwd_axis": 2 * grid,
-        "fft_adj_axis": 2 * grid,
+        # one axis of the sub-grid stack: read + write (z-pass with pairs:
+        # the pair stack on one side)
+        "fft_fwd_axis": tgrid + grid,
+        "fft_adj_axis": grid + tgrid,
         # coil images in, sub-grid stack out / back
         "fft_fwd_first": 8 * C * N + grid,
         "fft_adj_last": grid + 8 * C * N,
-        "fft_fwd_slice": 8 * C * N + grid,
-        "fft_adj_slice": grid + 8 * C * N,
+        # (3-D slice path with pairs: the slice kernels store / read one
+        # x-y transform per pair, the z-pass writes / reads all S sub-grids)
+        "fft_fwd_slice": 8 * C * N + tgrid,
+        "fft_adj_slice": tgrid + 8 * C * N,
         # SENSE normal operator: image + maps in, stack out
         "fft_toeplitz_slice_fwd": 8 * N + 8 * C * N + tgrid,
         # z axis forward, x PSF, inverse: stack read + write, PSF read

@@ -284,6 +284,29 @@ def test_psf_and_toeplitz_normal(img, R, S, C, weighted):
     assert relerr(y, ref.normal(x)) < 1e-5
 
 
+def test_nufft_pair_zpass_vs_all_subgrids(monkeypatch):
+    # forward / adjoint NUFFT with the pair z-pass (the slice kernels store
+    # and read one x-y transform per pair of sub-grids that differ only in
+    # the z shift) against the z-pass over all 8 sub-grids and the oracle
+    img, R, S, C = (32, 32, 64), 64, 90, 2
+    X, Y, Z = img
+    traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
+    w = (0.5 + np.random.default_rng(5).random(R * S)).astype(np.complex64)
+    a = make_nufft(img, C, traj, R, S, weights=w, toeplitz=False)
+    monkeypatch.setenv("BART_AMD_PAIRZ", "off")
+    b = make_nufft(img, C, traj, R, S, weights=w, toeplitz=False)
+    ref = O.Nufft(img, traj, C, weights=w, toeplitz=False)
+    cim = crandn((C, Z, Y, X), 18)
+    ksp = crandn((C, R * S), 19)
+    fa, fb = a.forward(cim).reshape(C, -1), b.forward(cim).reshape(C, -1)
+    aa, ab = a.adjoint(ksp).reshape(C, Z, Y, X), b.adjoint(ksp).reshape(C, Z, Y, X)
+    fr, ar = ref.forward(cim), ref.adjoint(ksp)
+    print(f"forward pair vs all {relerr(fa, fb):.2e}, vs oracle {relerr(fa, fr):.2e}; "
+          f"adjoint pair vs all {relerr(aa, ab):.2e}, vs oracle {relerr(aa, ar):.2e}")
+    assert relerr(fa, fb) < 1e-5 and relerr(aa, ab) < 1e-5
+    assert relerr(fa, fr) < 1e-5 and relerr(aa, ar) < 1e-5
+
+
 def test_toeplitz_pair_zpass_vs_all_subgrids(monkeypatch):
     # the Toeplitz normal with the pair z-pass (one x-y transform per pair of
     # sub-grids that differ only in the z shift) against the z-pass over all
