@@ -284,11 +284,13 @@ def test_psf_and_toeplitz_normal(img, R, S, C, weighted):
     assert relerr(y, ref.normal(x)) < 1e-5
 
 
-def test_nufft_pair_zpass_vs_all_subgrids(monkeypatch):
+@pytest.mark.parametrize("Z", [64, 128, 256, 512])
+def test_nufft_pair_zpass_vs_all_subgrids(monkeypatch, Z):
     # forward / adjoint NUFFT with the pair z-pass (the slice kernels store
     # and read one x-y transform per pair of sub-grids that differ only in
-    # the z shift) against the z-pass over all 8 sub-grids and the oracle
-    img, R, S, C = (32, 32, 64), 64, 90, 2
+    # the z shift) against the z-pass over all 8 sub-grids and the oracle;
+    # every z length with a pair kernel
+    img, R, S, C = (32, 32, Z), 64, 90, 2
     X, Y, Z = img
     traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
     w = (0.5 + np.random.default_rng(5).random(R * S)).astype(np.complex64)
@@ -305,6 +307,12 @@ def test_nufft_pair_zpass_vs_all_subgrids(monkeypatch):
           f"adjoint pair vs all {relerr(aa, ab):.2e}, vs oracle {relerr(aa, ar):.2e}")
     assert relerr(fa, fb) < 1e-5 and relerr(aa, ab) < 1e-5
     assert relerr(fa, fr) < 1e-5 and relerr(aa, ar) < 1e-5
+    # the Toeplitz normal of the same geometry (pair z-pass with the PSF)
+    monkeypatch.delenv("BART_AMD_PAIRZ")
+    t = make_nufft(img, C, traj, R, S, weights=w, toeplitz=True)
+    reft = O.Nufft(img, traj, C, weights=w, toeplitz=True)
+    assert relerr(t.normal(cim).reshape(C, Z, Y, X), reft.normal(cim)) < 1e-5
+    t.free()
 
 
 def test_toeplitz_pair_zpass_vs_all_subgrids(monkeypatch):
