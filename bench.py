@@ -490,8 +490,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(sharding the problems or coil groups over GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import torch
