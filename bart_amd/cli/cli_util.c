@@ -94,7 +94,7 @@ int parse_int(const char* s, const char* opt)
 
 void unsupported_option(const char* tool, const char* opt)
 {
-	cli_error("%s: option %s is not supported by this build (GPU NUFFT / CG-SENSE path only)", tool, opt);
+	cli_error("%s: option %s is not supported (GPU NUFFT / CG-SENSE path only)", tool, opt);
 }
 
 void estimate_im_dims(const long traj_dims[DIMS], const complex float* traj, long dims[3])

@@ -130,7 +130,7 @@ int main_pics(int argc, char* argv[])
 	}
 
 	if (NULL == traj_file)
-		cli_error("pics: this build reconstructs non-Cartesian data only (-t)");
+		cli_error("pics: only non-Cartesian data (-t) is supported");
 
 	if (maxiter < 0)
 		cli_error("pics: -i needs a non-negative iteration count");

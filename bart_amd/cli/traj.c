@@ -128,7 +128,7 @@ int main_traj(int argc, char* argv[])
 	}
 
 	if (radial && Z > 1 && !d3d)
-		cli_error("traj: -z with radial trajectories is not supported by this build");
+		cli_error("traj: -z with radial trajectories is not supported");
 
 	long dims[DIMS];
 
