@@ -119,17 +119,19 @@ __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
-// Work-group barrier that orders LDS only: waits for this wave's LDS
-// operations (lgkmcnt(0)) but not for its vector memory operations, so that
-// global stores (and loads whose results are not yet needed) stay in flight
-// across it; __syncthreads() waits for them as part of its fence.  For
-// barriers that only separate LDS writes from LDS reads.
+// Work-group barrier that orders LDS only: a release fence and an acquire
+// fence of work-group scope restricted to the "local" (LDS) address space
+// around s_barrier.  The compiler orders every LDS access of the thread
+// against them and emits s_waitcnt lgkmcnt(0) before the barrier, but no
+// wait for vector memory operations, so that global stores (and loads whose
+// results are not yet needed) stay in flight across it; __syncthreads()
+// waits for those too.  For barriers that only separate LDS writes from LDS
+// reads.
 __device__ inline void lds_barrier()
 {
-	__builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-	__atomic_signal_fence(__ATOMIC_SEQ_CST);
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
 	__builtin_amdgcn_s_barrier();
-	__atomic_signal_fence(__ATOMIC_SEQ_CST);
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Streams: the library launches on one stream per handle; default = 0.

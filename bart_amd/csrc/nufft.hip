@@ -313,26 +313,16 @@ struct LdImgLin {
 	}
 };
 
-// Plain grid loads in S-grouped order (optionally x conj(fftmod))
+// Plain grid loads in S-grouped order
 struct LdGridS {
 	GeomS g;
-	SepTables T;
 	const float2* grid;
-	bool conj_fm;
 
 	__device__ float2 operator()(long t, int k) const
 	{
 		uint32_t s, o, c;
 		g.split(t, s, o, c);
-		float2 v = grid[g.grid_offset(s, o, c, k)];
-
-		if (conj_fm) {
-			int px[3];
-			g.va.coords(0, o, k, px);
-			v = cmulc(v, fm_val(T, px));
-		}
-
-		return v;
+		return grid[g.grid_offset(s, o, c, k)];
 	}
 };
 
@@ -384,40 +374,19 @@ __device__ inline ColBase col_base(const VolAxis& g, long t0)
 struct LdGrid {
 	static constexpr bool colblock = true;
 	VolAxis g;
-	SepTables T;
 	const float2* grid;
-	bool conj_fm;
 
 	__device__ float2 operator()(long t, int k) const
 	{
 		uint32_t i, o, v;
 		g.split(t, i, o, v);
-		float2 x = grid[g.offset_iov(i, o, v, k)];
-
-		if (conj_fm) {
-			int px[3];
-			g.coords(i, o, k, px);
-			x = cmulc(x, fm_val(T, px));
-		}
-
-		return x;
+		return grid[g.offset_iov(i, o, v, k)];
 	}
 
 	__device__ bool cb_ok(int P) const { return 0 == g.inner % (uint32_t)P; }
 	__device__ ColBase base(long t0) const { return col_base(g, t0); }
 
-	__device__ float2 at(const ColBase& b, int p, int k) const
-	{
-		float2 x = grid[b.off + p + (long)g.inner * k];
-
-		if (conj_fm) {
-			int px[3];
-			g.coords(b.i0 + p, b.o, k, px);
-			x = cmulc(x, fm_val(T, px));
-		}
-
-		return x;
-	}
+	__device__ float2 at(const ColBase& b, int p, int k) const { return grid[b.off + p + (long)g.inner * k]; }
 };
 
 struct StGrid {
@@ -1127,7 +1096,11 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	M = M_;
 	width = width_;
 	os = os_;
-	beta = calc_beta(os, width);
+	// BART's KB table is built by the first kb_init call, which
+	// rolloff_correction makes with beta passed as float (noncart/nufft.c:898,
+	// noncart/grid.c:537-540); the table, I0(beta) and the rolloff all use
+	// the float-rounded beta
+	beta = (double)(float)calc_beta(os, width);
 	periodic = periodic_;
 	toeplitz = toeplitz_;
 
@@ -1219,7 +1192,7 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 // by sub-grids s and s + S/2, and both are written; adjoint: sub-grids s and
 // s + S/2 are read and conj(lz_s) Z_s + conj(lz_{s+S/2}) Z_{s+S/2} is
 // written to volume s.  Returns whether the pair form was used.
-static bool z_pass(const Nufft& p, float sign, bool conj_fm, bool fm, const char* name)
+static bool z_pass(const Nufft& p, float sign, bool fm, const char* name)
 {
 	if (p.pair_zpass) {
 		VolAxis va = VolAxis::make(p.n, 2, (long)(p.S / 2) * p.C);
@@ -1230,7 +1203,7 @@ static bool z_pass(const Nufft& p, float sign, bool conj_fm, bool fm, const char
 
 			if (!cfg.L.contig && 0 == va.inner % (uint32_t)cfg.L.P) {
 				cfg.name = name;
-				LdGrid ld{ va, p.tabs, p.grid.ptr, conj_fm };
+				LdGrid ld{ va, p.grid.ptr };
 				StGrid st{ va, p.tabs, p.grid.ptr, fm };
 				const long pair_off = (long)(p.S / 2) * p.C * p.N;
 				if (sign < 0.f)
@@ -1247,7 +1220,7 @@ static bool z_pass(const Nufft& p, float sign, bool conj_fm, bool fm, const char
 	VolAxis va = VolAxis::make(p.n, 2, (long)p.S * p.C);
 	FftLaunchCfg cfg = fft_config(p.axis[2], va.transforms(), va.contig(), 1, sign, sign);
 	cfg.name = name;
-	LdGrid ld{ va, p.tabs, p.grid.ptr, conj_fm };
+	LdGrid ld{ va, p.grid.ptr };
 	StGrid st{ va, p.tabs, p.grid.ptr, fm };
 	launch_fft_axis(cfg, p.stream, ld, NoMid{}, st);
 	return false;
@@ -1271,7 +1244,7 @@ void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
 	if (slice) {
 		const bool pz = z_pass_pairs(*this);
 		launch_slice_fwd(*this, src, sens, true, nullptr, "fft_fwd_slice", pz);
-		const bool used = z_pass(*this, -1.f, false, true, "fft_fwd_axis");
+		const bool used = z_pass(*this, -1.f, true, "fft_fwd_axis");
 		BAM_CHECK(used == pz, "z-pass form mismatch");
 		grid_forward(gp, kb.ptr, ksp, grid.ptr, has_weights ? weights.ptr : nullptr, stream);
 		return;
@@ -1295,7 +1268,7 @@ void Nufft::forward(float2* ksp, const float2* src, const float2* sens)
 		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
 		FftLaunchCfg cfg = fft_config(axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
 		cfg.name = "fft_fwd_axis";
-		LdGrid ld{ va, tabs, grid.ptr, false };
+		LdGrid ld{ va, grid.ptr };
 		StGrid st{ va, tabs, grid.ptr, j == nact - 1 };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 	}
@@ -1310,7 +1283,7 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 
 	if (slice) {
 
-		const bool pz = z_pass(*this, +1.f, false, false, "fft_adj_axis");
+		const bool pz = z_pass(*this, +1.f, false, "fft_adj_axis");
 
 		// with maps: one partial image per coil, summed in coil order
 		const int G = C;
@@ -1329,7 +1302,7 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
 		FftLaunchCfg cfg = fft_config(axis[j], va.transforms(), va.contig(), 1, +1.f, +1.f);
 		cfg.name = "fft_adj_axis";
-		LdGrid ld{ va, tabs, grid.ptr, false };
+		LdGrid ld{ va, grid.ptr };
 		StGrid st{ va, tabs, grid.ptr, false };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 	}
@@ -1339,7 +1312,7 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 	GeomS gs = GeomS::make(n, act[0], S, C, cc);
 	FftLaunchCfg cfg = fft_config(axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
 	cfg.name = "fft_adj_last";
-	LdGridS ld{ gs, tabs, grid.ptr, false };
+	LdGridS ld{ gs, grid.ptr };
 	float2* out = dst;
 
 	if (nullptr != sens && G > 1) {
@@ -1380,7 +1353,7 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 		// the C coil volumes of a sub-grid share its PSF: launch order
 		// column group -> volume, so that the PSF columns come from cache
 		va.group_volumes(cfg.L.P);
-		LdGrid ld{ va, tabs, grid.ptr, false };
+		LdGrid ld{ va, grid.ptr };
 		StGrid st{ va, tabs, grid.ptr, false };
 
 		if (pair_z) {
@@ -1437,7 +1410,7 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 	for (int j = 1; j < nact; j++) {
 
 		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
-		LdGrid ld{ va, tabs, grid.ptr, false };
+		LdGrid ld{ va, grid.ptr };
 		StGrid st{ va, tabs, grid.ptr, false };
 
 		if (j == nact - 1) {
@@ -1454,13 +1427,13 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 
 		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
 		FftLaunchCfg cfg = fft_config_skip(skip, axis[j], va.transforms(), va.contig(), 1, +1.f, +1.f);
-		LdGrid ld{ va, tabs, grid.ptr, false };
+		LdGrid ld{ va, grid.ptr };
 		StGrid st{ va, tabs, grid.ptr, false };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 	}
 
 	FftLaunchCfg cfg = fft_config_skip(skip, axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
-	LdGridS ld{ gs, tabs, grid.ptr, false };
+	LdGridS ld{ gs, grid.ptr };
 	StReduceS st{ gs, tabs, out, sens, false };
 	launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 

@@ -58,6 +58,8 @@ def lib():
         L.orc_nufft_get_psf.argtypes = [vp, vp]
         L.orc_nufft_set_psf.argtypes = [vp, vp]
         L.orc_nufft_get_linphase.argtypes = [vp, vp]
+        L.orc_nufft_beta.restype = ctypes.c_double
+        L.orc_nufft_beta.argtypes = [vp]
         L.orc_nufft_subgrids.restype = ctypes.c_int
         L.orc_nufft_subgrids.argtypes = [vp]
         L.orc_nufft_compute_psf.argtypes = [c_long3, ctypes.c_long, vp, vp, vp]
@@ -234,6 +236,36 @@ class Nufft:
         lib().orc_sense_normal(self.h, _ptr(maps), _ptr(out), _ptr(img))
         return out
 
+    @property
+    def beta(self):
+        """Kaiser-Bessel beta of the operator's table and rolloff."""
+        return lib().orc_nufft_beta(self.h)
+
+    def sense_normal_fp64(self, maps, img):
+        """S^H T S in complex128: the Toeplitz normal operator
+        (noncart/nufft.c:1546-1625, sum_s conj(L_s) IFFT(P_s FFT(L_s x)) per
+        coil, then the coil sum of linops/fmac.c) evaluated in double
+        precision on this operator's own tables (linear phases, PSF) and the
+        maps, each taken as exact.  The reference for the FP32 rounding of
+        CG: an FP32 solver's deviation from CG on this operator is the
+        rounding it accumulates."""
+        import scipy.fft as sf
+        assert self.toeplitz
+        if not hasattr(self, "_lph64"):
+            self._lph64 = self.linphase().astype(np.complex128)
+            self._psf64 = self.psf().astype(np.complex128)
+        m = np.asarray(maps).astype(np.complex128).reshape(self.cim_shape)
+        x = np.asarray(img).astype(np.complex128).reshape(self.dims[::-1])
+        axes = tuple(a for a in (1, 2, 3) if self.cim_shape[a] > 1)
+        cim = m * x[None]
+        acc = np.zeros(self.cim_shape, np.complex128)
+        for s in range(self.S):
+            g = sf.fftn(cim * self._lph64[s][None], axes=axes, workers=-1)
+            g *= self._psf64[s][None]
+            g = sf.ifftn(g, axes=axes, workers=-1) * self.N
+            acc += g * np.conj(self._lph64[s])[None]
+        return np.sum(np.conj(m) * acc, axis=0)
+
     def cgsense(self, maps, b, lam, maxiter, tol=0.0, x0=None):
         """CG on (A^H A + lam I) x = b as BART's conjgrad.  Returns
         (x, iterations, residual trace, final residual norm)."""
@@ -245,6 +277,41 @@ class Nufft:
         it = lib().orc_cgsense(self.h, _ptr(maps), lam, maxiter, tol, _ptr(x), _ptr(b),
                                _ptr(trace), _ptr(fin))
         return x, it, trace[:it], float(fin[0])
+
+
+def cg_fp64(normal, b, lam, maxiter, tol=0.0, x0=None):
+    """CG on (A^H A + lam I) x = b in complex128, step by step as BART's
+    conjgrad (iter/italgos.c:497-580): the inner products of the float
+    vectors are Re<u, v> (num/vecops.c:89-124), stop when
+    ||r||^2 <= (tol ||b||)^2 (iter/iter2.c:227-270) or p^H A p = 0.
+    normal: function image -> A^H A image (complex128).  Returns (x, trace
+    of ||r|| after each iteration)."""
+    b = np.asarray(b, np.complex128)
+    shape = b.shape
+    b = b.ravel()
+    x = np.zeros_like(b) if x0 is None else np.asarray(x0, np.complex128).ravel().copy()
+    eps2 = (tol * np.linalg.norm(b)) ** 2
+    r = b - (normal(x.reshape(shape)).ravel() + lam * x)
+    p = r.copy()
+    rsold = float(np.vdot(r, r).real)
+    trace = []
+    if rsold == 0.0:
+        return x.reshape(shape), np.array(trace)
+    for _ in range(maxiter):
+        Ap = normal(p.reshape(shape)).ravel() + lam * p
+        pAp = float(np.vdot(p, Ap).real)
+        if pAp == 0.0:
+            break
+        alpha = rsold / pAp
+        x += alpha * p
+        r -= alpha * Ap
+        rsnew = float(np.vdot(r, r).real)
+        trace.append(np.sqrt(rsnew))
+        if rsnew <= eps2:
+            break
+        p = r + (rsnew / rsold) * p
+        rsold = rsnew
+    return x.reshape(shape), np.array(trace)
 
 
 # --------------------------------------------------------------------------

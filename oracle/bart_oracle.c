@@ -219,53 +219,96 @@ static void sample_pos(const struct orc_grid_conf* conf, const long gdims[3], co
  * noncart/grid.c:202-253 and grid_point :469-490.  The loop nest is
  * z (outer), y, x (inner) with the weight accumulated in that order
  * (grid_point_gen recursion :443-464).  Zero samples (all coils) are
- * skipped as in :241-250.  grid is [C][gdims] and accumulated into. */
+ * skipped as in :241-250.  grid is [C][gdims] and accumulated into.
+ *
+ * BART runs the sample loop in parallel and adds with atomics
+ * (grid.c:482-485), so the order of the additions into one grid point, and
+ * with it the rounding, varies from run to run.  The oracle fixes the order
+ * to that of the serial loop: the grid is cut into slabs along its last
+ * dimension of size > 1, a thread owns a slab, walks all samples in order
+ * and adds only the taps that fall into its slab.  Every grid point then
+ * receives its contributions in sample order, whatever the thread count. */
 void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, const cf* traj,
 		const long gdims[3], int C, cf* grid, long gstride_c, const cf* ksp, long kstride_c)
 {
-	#pragma omp parallel for schedule(static)
-	for (long m = 0; m < M; m++) {
+	int D = 2;
 
-		bool skip = true;
+	while (D > 0 && 1 == gdims[D])
+		D--;
 
-		for (int c = 0; c < C; c++)
-			skip = skip && (0. == ksp[m + c * kstride_c]);
+	long nslab = (gdims[D] < 64) ? gdims[D] : 64;
 
-		if (skip)
-			continue;
+	#pragma omp parallel for schedule(dynamic, 1)
+	for (long sl = 0; sl < nslab; sl++) {
 
-		float pos[3];
-		sample_pos(conf, gdims, traj + 3 * m, pos);
+		long lo = sl * gdims[D] / nslab;
+		long hi = (sl + 1) * gdims[D] / nslab;
 
-		struct taps1d t[3];
+		for (long m = 0; m < M; m++) {
 
-		if (!taps_1d(&t[0], pos[0], gdims[0], conf->periodic, conf->width, table)
-		    || !taps_1d(&t[1], pos[1], gdims[1], conf->periodic, conf->width, table)
-		    || !taps_1d(&t[2], pos[2], gdims[2], conf->periodic, conf->width, table))
-			continue;
+			bool skip = true;
 
-		for (int iz = 0; iz < t[2].n; iz++) {
+			for (int c = 0; c < C; c++)
+				skip = skip && (0. == ksp[m + c * kstride_c]);
 
-			float dz = 1.f * t[2].w[iz];
+			if (skip)
+				continue;
 
-			for (int iy = 0; iy < t[1].n; iy++) {
+			float pos[3];
+			sample_pos(conf, gdims, traj + 3 * m, pos);
 
-				float dy = dz * t[1].w[iy];
+			struct taps1d t[3];
 
-				for (int ix = 0; ix < t[0].n; ix++) {
+			if (!taps_1d(&t[D], pos[D], gdims[D], conf->periodic, conf->width, table))
+				continue;
 
-					float d = dy * t[0].w[ix];
-					long ind = t[0].idx[ix] + gdims[0] * (t[1].idx[iy] + gdims[1] * t[2].idx[iz]);
+			bool mine = false;
 
-					for (int c = 0; c < C; c++) {
+			for (int i = 0; i < t[D].n; i++)
+				mine = mine || (lo <= t[D].idx[i] && t[D].idx[i] < hi);
 
-						cf v = ksp[m + c * kstride_c];
-						float* dst = (float*)&grid[ind + c * gstride_c];
+			if (!mine)
+				continue;
 
-						#pragma omp atomic
-						dst[0] += crealf(v) * d;
-						#pragma omp atomic
-						dst[1] += cimagf(v) * d;
+			bool ok = true;
+
+			for (int j = 0; j < 3; j++)
+				if (j != D)
+					ok = ok && taps_1d(&t[j], pos[j], gdims[j], conf->periodic, conf->width, table);
+
+			if (!ok)
+				continue;
+
+			for (int iz = 0; iz < t[2].n; iz++) {
+
+				if (2 == D && (t[2].idx[iz] < lo || t[2].idx[iz] >= hi))
+					continue;
+
+				float dz = 1.f * t[2].w[iz];
+
+				for (int iy = 0; iy < t[1].n; iy++) {
+
+					if (1 == D && (t[1].idx[iy] < lo || t[1].idx[iy] >= hi))
+						continue;
+
+					float dy = dz * t[1].w[iy];
+
+					for (int ix = 0; ix < t[0].n; ix++) {
+
+						if (0 == D && (t[0].idx[ix] < lo || t[0].idx[ix] >= hi))
+							continue;
+
+						float d = dy * t[0].w[ix];
+						long ind = t[0].idx[ix] + gdims[0] * (t[1].idx[iy] + gdims[1] * t[2].idx[iz]);
+
+						for (int c = 0; c < C; c++) {
+
+							cf v = ksp[m + c * kstride_c];
+							float* dst = (float*)&grid[ind + c * gstride_c];
+
+							dst[0] += crealf(v) * d;
+							dst[1] += cimagf(v) * d;
+						}
 					}
 				}
 			}
@@ -691,7 +734,12 @@ orc_nufft_t* orc_nufft_create(const long dims[3], long M, const cf* traj, const 
 	nf->S = decomp_count(dims);
 	nf->width = width;
 	nf->os = os;
-	nf->beta = orc_calc_beta(os, width);
+	/* BART builds the KB table on the first kb_init call, which comes from
+	 * rolloff_correction with beta passed as float (nufft.c:898,
+	 * grid.c:537-540); later calls with the double beta pass its 1e-6
+	 * check (grid.c:58-59).  So table, I0(beta) and rolloff all use the
+	 * float-rounded beta. */
+	nf->beta = (double)(float)orc_calc_beta(os, width);
 	nf->periodic = periodic;
 	nf->toeplitz = toeplitz;
 
@@ -1002,6 +1050,11 @@ void orc_nufft_get_linphase(const orc_nufft_t* nf, cf* lph)
 int orc_nufft_subgrids(const orc_nufft_t* nf)
 {
 	return nf->S;
+}
+
+double orc_nufft_beta(const orc_nufft_t* nf)
+{
+	return nf->beta;
 }
 
 

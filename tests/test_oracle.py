@@ -213,3 +213,101 @@ def test_cg_zero_iterations_returns_warm_start():
     b = crandn((1, n, n), 12)
     x, it, _, _ = nf.cgsense(maps, b, 0.01, 0, 0.0, x0=x0)
     assert it == 0 and np.array_equal(x, x0)
+
+
+def test_adjoint_gridding_is_deterministic_over_threads():
+    # every grid point receives its contributions in sample order whatever
+    # the thread count, so the adjoint is bitwise reproducible (a dense 2-D
+    # centre and a 3-D trajectory)
+    L = O.lib()
+    saved = L.orc_num_threads()
+    try:
+        for dims, traj in [((64, 64, 1), I.radial_traj(128, 201)),
+                           ((16, 16, 16), I.phyllotaxis_kooshball(spokes=200, readout=32))]:
+            ksp = crandn((3, traj.shape[0]), 40)
+            outs = []
+            for t in (1, 3, 8):
+                L.orc_set_num_threads(t)
+                outs.append(O.Nufft(dims, traj, 3, toeplitz=False).adjoint(ksp))
+            assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    finally:
+        L.orc_set_num_threads(saved)
+
+
+def test_operator_beta_is_float_rounded():
+    # BART's first kb_init call comes from rolloff_correction, whose beta
+    # argument is a float (noncart/nufft.c:898, noncart/grid.c:537-540)
+    nf = O.Nufft((16, 16, 1), I.radial_traj(32, 5), 1, toeplitz=False)
+    assert nf.beta == float(np.float32(O.calc_beta(2.0, 6.0)))
+    assert nf.beta != O.calc_beta(2.0, 6.0)
+
+
+def test_sense_operators():
+    # A = NUFFT S with S: img -> maps * img (linops/fmac.c:63-86), A^H with
+    # the conjugate maps summed over coils; dot test <A x, y> = <x, A^H y>;
+    # the Toeplitz S^H T S against the exact A^H A from the FP64 NUDFT
+    n, C = 32, 3
+    traj = I.radial_traj(64, 23)
+    maps = I.gaussian_coils((1, n, n), C, seed=6)
+    nf = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    x = crandn((1, n, n), 41)
+    y = crandn((C, nf.M), 42)
+    assert relerr(nf.sense_forward(maps, x), nf.forward(maps * x)) < 1e-6
+    ah = np.sum(np.conj(maps.astype(np.complex128)) * nf.adjoint(y), axis=0)
+    assert relerr(nf.sense_adjoint(maps, y), ah) < 1e-6
+    s1 = np.vdot(y.astype(np.complex128), nf.sense_forward(maps, x))
+    s2 = np.vdot(nf.sense_adjoint(maps, y).astype(np.complex128), x)
+    assert abs(s1 - s2) / abs(s1) < 1e-6
+    m64 = maps.astype(np.complex128)
+    exact = np.sum(np.conj(m64) * O.nudft_adjoint((n, n, 1), traj, O.nudft_forward((n, n, 1), traj, m64 * x)), axis=0)
+    assert relerr(nf.sense_normal(maps, x), exact) < 1e-5
+
+
+def test_sense_normal_fp64():
+    # the complex128 Toeplitz normal operator on the oracle's own tables is
+    # the FP32 one without its rounding, and the exact A^H A up to the KB
+    # approximation; it is linear and Hermitian
+    n, C = 32, 3
+    traj = I.radial_traj(64, 23)
+    maps = I.gaussian_coils((1, n, n), C, seed=7)
+    nf = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    x = crandn((1, n, n), 43)
+    z = crandn((1, n, n), 44)
+    a64 = nf.sense_normal_fp64(maps, x)
+    assert relerr(nf.sense_normal(maps, x), a64) < 1e-6
+    m64 = maps.astype(np.complex128)
+    exact = np.sum(np.conj(m64) * O.nudft_adjoint((n, n, 1), traj, O.nudft_forward((n, n, 1), traj, m64 * x)), axis=0)
+    assert relerr(a64, exact) < 1e-5
+    lhs = np.vdot(z, a64)
+    rhs = np.vdot(nf.sense_normal_fp64(maps, z), x)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-6
+    assert relerr(nf.sense_normal_fp64(maps, 2 * x.astype(np.complex128) + z), 2 * a64 + nf.sense_normal_fp64(maps, z)) < 1e-10
+
+
+def test_cg_fp64_matches_direct_solve_and_oracle():
+    # complex128 CG (the FP32 CG's reference) converges to the dense solution
+    # and follows the FP32 oracle CG over the first iterations
+    n, C = 16, 3
+    traj = I.radial_traj(32, 15)
+    maps = I.gaussian_coils((1, n, n), C, seed=2)
+    img = I.ellipse_phantom(n, seed=2)
+    nf = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    b = nf.sense_adjoint(maps, nf.sense_forward(maps, img))
+    lam = 0.05
+    N = n * n
+    A = np.zeros((N, N), np.complex128)
+    for j in range(N):
+        e = np.zeros((1, n, n), np.complex128)
+        e.flat[j] = 1
+        A[:, j] = nf.sense_normal_fp64(maps, e).ravel()
+    xs = np.linalg.solve(A + lam * np.eye(N), b.ravel().astype(np.complex128))
+    normal = lambda v: nf.sense_normal_fp64(maps, v)
+    x, tr = O.cg_fp64(normal, b, lam, 300, 1e-10)
+    assert len(tr) < 300 and relerr(x.ravel(), xs) < 1e-8
+    x5, tr5 = O.cg_fp64(normal, b, lam, 5)
+    xo, it, tro, _ = nf.cgsense(maps, b, lam, 5, 0.0)
+    assert it == 5 and relerr(xo, x5) < 1e-5
+    assert np.max(np.abs(tro / tr5 - 1)) < 1e-4
+    # stop rule: ||r|| <= tol ||b||
+    x6, tr6 = O.cg_fp64(normal, b, lam, 300, 1e-4)
+    assert tr6[-1] <= 1e-4 * np.linalg.norm(b) and (len(tr6) == 1 or tr6[-2] > 1e-4 * np.linalg.norm(b))
