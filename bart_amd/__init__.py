@@ -136,6 +136,19 @@ def lib():
     L.nufft_get_psf.argtypes = [vp, ctypes.c_int, lp, vp]
     L.nufft_update_psf.restype = None
     L.nufft_update_psf.argtypes = [vp, ctypes.c_int, lp, vp]
+    L.nufft_get_psf2.restype = None
+    L.nufft_get_psf2.argtypes = [vp, ctypes.c_int, lp, lp, vp]
+    L.nufft_update_psf2.restype = None
+    L.nufft_update_psf2.argtypes = [vp, ctypes.c_int, lp, lp, vp]
+    L.nufft_create_normal.restype = vp
+    L.nufft_create_normal.argtypes = [ctypes.c_int, lp, ctypes.c_int, lp, vp, ctypes.c_bool, NufftConf]
+    L.nufft_update_traj.restype = None
+    L.nufft_update_traj.argtypes = [vp, ctypes.c_int, lp, vp, lp, vp, lp, vp]
+    L.compute_psf2.restype = vp
+    L.compute_psf2.argtypes = [ctypes.c_int, lp, ctypes.c_ulong, lp, vp, lp, vp, lp, vp, ctypes.c_bool,
+                               ctypes.c_bool, ctypes.c_bool]
+    L.bart_amd_linop_set_stream.restype = ctypes.c_int
+    L.bart_amd_linop_set_stream.argtypes = [vp, vp]
     for f in (L.fft, L.ifft):
         f.restype = None
         f.argtypes = [ctypes.c_int, lp, ctypes.c_ulong, vp, vp]
@@ -224,6 +237,18 @@ def _host_c64(a):
     return a
 
 
+def _torch_stream(*arrays):
+    """torch's current stream for the first CUDA tensor among arrays (0 =
+    the default stream, on which the library's own blocking stream is
+    ordered implicitly), so that the library runs after the work that wrote
+    its inputs and before the work that reads its outputs."""
+    for a in arrays:
+        if a is not None and _is_torch(a):
+            import torch
+            return int(torch.cuda.current_stream(a.device).cuda_stream)
+    return 0
+
+
 class Linop:
     """A BART linop_s handle with its output (odims) and input (idims) dims."""
 
@@ -247,9 +272,22 @@ class Linop:
         except Exception:
             pass
 
+    def set_stream(self, stream):
+        """Launch this handle's applies on a hipStream_t (int handle; 0: its
+        own stream)."""
+        lib().bart_amd_linop_set_stream(self.h, ctypes.c_void_p(int(stream)))
+        _check()
+
+    def _stream_of(self, *arrays):
+        st = _torch_stream(*arrays)
+        if st != getattr(self, "_stream", 0):
+            self.set_stream(st)
+            self._stream = st
+
     def forward(self, x, out=None):
         x = _host_c64(x)
         out = _like(x, self.odims) if out is None else out
+        self._stream_of(x, out)
         lib().linop_forward(self.h, DIMS, _dims(self.odims), _ptr(out), DIMS, _dims(self.idims), _ptr(x))
         _check()
         return out
@@ -257,6 +295,7 @@ class Linop:
     def adjoint(self, y, out=None):
         y = _host_c64(y)
         out = _like(y, self.idims) if out is None else out
+        self._stream_of(y, out)
         lib().linop_adjoint(self.h, DIMS, _dims(self.idims), _ptr(out), DIMS, _dims(self.odims), _ptr(y))
         _check()
         return out
@@ -264,6 +303,7 @@ class Linop:
     def normal(self, x, out=None):
         x = _host_c64(x)
         out = _like(x, self.idims) if out is None else out
+        self._stream_of(x, out)
         lib().linop_normal(self.h, DIMS, _dims(self.idims), _ptr(out), _ptr(x))
         _check()
         return out
@@ -287,6 +327,30 @@ class Linop:
         d = self.psf_dims()
         psf = _host_c64(psf)
         lib().nufft_update_psf(self.h, len(d), _dims(d, len(d)), _ptr(psf))
+        _check()
+
+    def get_psf2(self, out, strides):
+        """nufft_get_psf2: the PSF into out (complex64, host or device) at
+        the given byte strides."""
+        d = self.psf_dims()
+        lib().nufft_get_psf2(self.h, len(d), _dims(d, len(d)), _dims(strides, len(d)), _ptr(out))
+        _check()
+        return out
+
+    def update_psf2(self, psf, strides):
+        d = self.psf_dims()
+        lib().nufft_update_psf2(self.h, len(d), _dims(d, len(d)), _dims(strides, len(d)), _ptr(psf))
+        _check()
+
+    def update_traj(self, traj_dims, traj, weights=None, wgh_dims=None):
+        """nufft_update_traj: a new trajectory (and weights) of the same
+        sample count; the plan and, in Toeplitz mode, the PSF are rebuilt."""
+        traj = _host_c64(traj) if not _is_torch(traj) else traj
+        w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
+        if weights is not None and wgh_dims is None:
+            wgh_dims = [1] + list(traj_dims[1:3])
+        lib().nufft_update_traj(self.h, DIMS, _dims(traj_dims), _ptr(traj),
+                                _dims(wgh_dims) if wgh_dims is not None else None, _ptr(w), None, None)
         _check()
 
 
@@ -315,6 +379,35 @@ def nufft_create(ksp_dims, cim_dims, traj_dims, traj, weights=None, conf=None):
     w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
     h = lib().nufft_create(DIMS, _dims(ksp_dims), _dims(cim_dims), _dims(traj_dims), _ptr(traj), _ptr(w), conf)
     return Linop(h, ksp_dims, cim_dims)
+
+
+def nufft_create_normal(cim_dims, psf_dims, psf=None, conf=None):
+    """noncart/nufft.h:84-86: the Toeplitz normal operator of a PSF alone
+    (coil images -> coil images; forward and adjoint are errors)."""
+    conf = NufftConf.defaults() if conf is None else conf
+    psf = _host_c64(psf) if psf is not None and not _is_torch(psf) else psf
+    h = lib().nufft_create_normal(DIMS, _dims(cim_dims), DIMS + 1, _dims(psf_dims, DIMS + 1), _ptr(psf), False,
+                                  conf)
+    return Linop(h, [1] * DIMS, cim_dims)
+
+
+def compute_psf2(psf_dims, traj_dims, traj, weights=None, wgh_dims=None, periodic=True):
+    """noncart/nufft.h:70-73: the decomposed PSF of a trajectory
+    (psf_dims: DIMS + 1 dims, image dims and 2^d last) as a host array."""
+    traj = _host_c64(traj)
+    w = _host_c64(weights) if weights is not None else None
+    if weights is not None and wgh_dims is None:
+        wgh_dims = [1] + list(traj_dims[1:3])
+    n = int(np.prod(psf_dims))
+    p = lib().compute_psf2(DIMS, _dims(psf_dims, DIMS + 1), 7, _dims(traj_dims, DIMS + 1), _ptr(traj), None, None,
+                           _dims(wgh_dims, DIMS + 1) if wgh_dims is not None else None, _ptr(w), periodic, False,
+                           False)
+    _check()
+    if not p:
+        raise BartError("compute_psf2 failed")
+    out = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_float)), shape=(2 * n,)).copy()
+    ctypes.CDLL(None).free(ctypes.c_void_p(p))
+    return out.view(np.complex64)
 
 
 def nudft_create(ksp_dims, cim_dims, traj_dims, traj):

@@ -114,7 +114,9 @@ struct Op {
 	std::shared_ptr<Nufft> nufft;
 	std::shared_ptr<Nudft> nudft;
 	std::shared_ptr<DevBuf<float2>> sens;
-	hipStream_t stream = 0;
+	hipStream_t stream = 0;       // the handle's own stream
+	hipStream_t user_stream = 0;  // set by bart_amd_linop_set_stream (0: none)
+	bool normal_only = false;     // nufft_create_normal: no forward / adjoint
 	std::mutex& mtx() { return q->mtx; }
 	DevBuf<float2> stage_in, stage_out, kbuf;
 	std::unique_ptr<CgSolver> cg;
@@ -129,9 +131,10 @@ struct Op {
 
 	const float2* maps() const { return (OP_SENSE == kind) ? sens->ptr : nullptr; }
 
-	// device pointers, on this operator's stream
+	// device pointers, on the plan's current stream
 	void fwd(float2* d, const float2* s)
 	{
+		BAM_CHECK(!normal_only, "NuFFT with normal operator only!");
 		if (OP_NUDFT == kind)
 			nudft->forward(d, s);
 		else
@@ -140,6 +143,7 @@ struct Op {
 
 	void adj(float2* d, const float2* s)
 	{
+		BAM_CHECK(!normal_only, "NuFFT with normal operator only!");
 		if (OP_NUDFT == kind)
 			nudft->adjoint(d, s);
 		else
@@ -224,10 +228,17 @@ template <class T> static std::vector<T> to_host(const T* p, long n)
 }
 
 // Applies f(dst_dev, src_dev) with staging of host pointers and a final sync.
+// The launches go to the handle's own stream, or to the stream set with
+// bart_amd_linop_set_stream; with device pointers on a stream of the
+// caller's the call returns without waiting (the caller orders its work on
+// that stream), otherwise it returns when the result is in dst.
 template <class F>
 static void apply_staged(Op* op, float2* dst, long dn, const float2* src, long sn, F&& f)
 {
 	std::lock_guard<std::mutex> lock(op->mtx());
+
+	const hipStream_t st = (0 != op->user_stream) ? op->user_stream : op->stream;
+	op->set_plan_stream(st);
 
 	bool dd = on_device(dst);
 	bool sd = on_device(src);
@@ -238,7 +249,7 @@ static void apply_staged(Op* op, float2* dst, long dn, const float2* src, long s
 	if (!sd) {
 		if ((long)op->stage_in.n < sn)
 			op->stage_in.alloc(sn);
-		BAM_HIP(hipMemcpyAsync(op->stage_in.ptr, src, sizeof(float2) * sn, hipMemcpyHostToDevice, op->stream));
+		BAM_HIP(hipMemcpyAsync(op->stage_in.ptr, src, sizeof(float2) * sn, hipMemcpyHostToDevice, st));
 		s = op->stage_in.ptr;
 	}
 
@@ -251,9 +262,12 @@ static void apply_staged(Op* op, float2* dst, long dn, const float2* src, long s
 	f(d, s);
 
 	if (!dd)
-		BAM_HIP(hipMemcpyAsync(dst, d, sizeof(float2) * dn, hipMemcpyDeviceToHost, op->stream));
+		BAM_HIP(hipMemcpyAsync(dst, d, sizeof(float2) * dn, hipMemcpyDeviceToHost, st));
 
-	BAM_HIP(hipStreamSynchronize(op->stream));
+	if (0 == op->user_stream || !dd || !sd)
+		BAM_HIP(hipStreamSynchronize(st));
+
+	op->set_plan_stream(op->stream);
 }
 
 // Parse BART dims for a NUFFT: image dims 0-2, coils dim 3, samples dims 1-2
@@ -365,6 +379,122 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 	return op;
 }
 
+// trajectory [3][M] (host or device, complex; coordinates in the real parts)
+// -> host M x 3 floats
+static std::vector<float> traj_reals(const float2* traj, long M)
+{
+	std::vector<float2> th = to_host(traj, 3 * M);
+	std::vector<float> tr(3 * M);
+
+	for (long i = 0; i < 3 * M; i++)
+		tr[i] = th[i].x;
+
+	return tr;
+}
+
+// N-d strided copy of complex floats (BART's md_copy2, strides in bytes)
+struct StrideND {
+	int nd;
+	long dims[33];
+	long dstr[33];
+	long sstr[33];
+};
+
+__global__ void k_copy_strided(StrideND g, long total, char* dst, const char* src)
+{
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+		long r = e, od = 0, os = 0;
+		for (int i = 0; i < g.nd; i++) {
+			long q = r % g.dims[i];
+			r /= g.dims[i];
+			od += q * g.dstr[i];
+			os += q * g.sstr[i];
+		}
+		*(float2*)(dst + od) = *(const float2*)(src + os);
+	}
+}
+
+// copy between a dense device array and an array of the given byte strides
+// (host or device); to_strided: dense -> strided, else strided -> dense
+static void copy_strides(int N, const long* dims, const long* strs, float2* strided, float2* dense, bool to_strided,
+			 hipStream_t st)
+{
+	BAM_CHECK(N >= 1 && N <= 33, "strided copy: %d dims", N);
+	long total = md_size(N, dims);
+	long dstr[33];
+	long d = sizeof(float2);
+	bool contig = true;
+
+	for (int i = 0; i < N; i++) {
+		dstr[i] = d;
+		contig = contig && (1 == dims[i] || strs[i] == d);
+		d *= dims[i];
+	}
+
+	const bool dev = on_device(strided);
+
+	if (contig) {
+		if (to_strided)
+			BAM_HIP(hipMemcpyAsync(strided, dense, sizeof(float2) * total,
+					       dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+		else
+			BAM_HIP(hipMemcpyAsync(dense, strided, sizeof(float2) * total,
+					       dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+		BAM_HIP(hipStreamSynchronize(st));
+		return;
+	}
+
+	if (!dev) {
+		std::vector<float2> h(total);
+		if (!to_strided)
+			for (long e = 0; e < total; e++) {
+				long r = e, o = 0;
+				for (int i = 0; i < N; i++) {
+					o += (r % dims[i]) * strs[i];
+					r /= dims[i];
+				}
+				h[e] = *(const float2*)((const char*)strided + o);
+			}
+		if (to_strided) {
+			BAM_HIP(hipMemcpyAsync(h.data(), dense, sizeof(float2) * total, hipMemcpyDeviceToHost, st));
+			BAM_HIP(hipStreamSynchronize(st));
+			for (long e = 0; e < total; e++) {
+				long r = e, o = 0;
+				for (int i = 0; i < N; i++) {
+					o += (r % dims[i]) * strs[i];
+					r /= dims[i];
+				}
+				*(float2*)((char*)strided + o) = h[e];
+			}
+		} else {
+			BAM_HIP(hipMemcpyAsync(dense, h.data(), sizeof(float2) * total, hipMemcpyHostToDevice, st));
+			BAM_HIP(hipStreamSynchronize(st));
+		}
+		return;
+	}
+
+	StrideND g;
+	g.nd = N;
+	for (int i = 0; i < N; i++) {
+		g.dims[i] = dims[i];
+		g.dstr[i] = to_strided ? strs[i] : dstr[i];
+		g.sstr[i] = to_strided ? dstr[i] : strs[i];
+	}
+	unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 8192);
+	hipLaunchKernelGGL(k_copy_strided, dim3(std::max(1u, blocks)), dim3(256), 0, st, g, total,
+			   (char*)(to_strided ? strided : dense), (const char*)(to_strided ? dense : strided));
+	BAM_HIP(hipGetLastError());
+	BAM_HIP(hipStreamSynchronize(st));
+}
+
+// PSF dims of a NUFFT handle: image dims, 1..., and the decomposition index
+static void psf_dims_of(const Op* op, long* d)
+{
+	for (int i = 0; i < op->N; i++)
+		d[i] = (i < 3) ? op->nufft->n[i] : 1;
+	d[op->N] = op->nufft->S;
+}
+
 }  // namespace bam
 
 using namespace bam;
@@ -375,7 +505,7 @@ using namespace bam;
 
 extern "C" {
 
-const struct nufft_conf_s nufft_conf_defaults = {
+struct nufft_conf_s nufft_conf_defaults = {
 	.toeplitz = true,
 	.pcycle = false,
 	.periodic = false,
@@ -601,34 +731,48 @@ int nufft_get_psf_dims(const struct linop_s* _op, int N, long psf_dims[])
 {
 	return guarded([&]() -> int {
 		Op* op = (Op*)_op;
-		BAM_CHECK(nullptr != op && op->plan("nufft_get_psf_dims").has_psf, "nufft_get_psf_dims: operator has no PSF");
+		BAM_CHECK(nullptr != op, "nufft_get_psf_dims: NULL operator");
+		op->plan("nufft_get_psf_dims");
 
 		if (N > 0) {
 			BAM_CHECK(N == op->N + 1, "nufft_get_psf_dims: need %d dims", op->N + 1);
-			for (int i = 0; i < op->N; i++)
-				psf_dims[i] = (i < 3) ? op->nufft->n[i] : 1;
-			psf_dims[op->N] = op->nufft->S;
+			psf_dims_of(op, psf_dims);
 		}
 
 		return op->N + 1;
 	}, -1);
 }
 
-void nufft_get_psf(const struct linop_s* _op, int N, const long psf_dims[], float _Complex* psf)
+void nufft_get_psf2(const struct linop_s* _op, int N, const long psf_dims[], const long psf_strs[],
+		    float _Complex* psf)
 {
 	guarded_void([&]() {
 		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op && op->plan("nufft_get_psf").has_psf, "nufft_get_psf: operator has no PSF");
 		long want[33];
-		BAM_CHECK(N == nufft_get_psf_dims(_op, 0, nullptr), "nufft_get_psf: dims");
-		nufft_get_psf_dims(_op, N, want);
+		BAM_CHECK(N == op->N + 1, "nufft_get_psf: need %d dims", op->N + 1);
+		psf_dims_of(op, want);
 		check_dims("nufft_get_psf", N, psf_dims, N, want);
-		long n = (long)op->nufft->S * op->nufft->N;
-		BAM_HIP(hipMemcpy(psf, op->nufft->psf.ptr, sizeof(float2) * n,
-				  on_device(psf) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+		std::lock_guard<std::mutex> lock(op->mtx());
+		copy_strides(N, psf_dims, psf_strs, (float2*)psf, op->nufft->psf.ptr, true, op->stream);
 	});
 }
 
-void nufft_update_psf(const struct linop_s* _op, int ND, const long psf_dims[], const float _Complex* psf)
+void nufft_get_psf(const struct linop_s* _op, int N, const long psf_dims[], float _Complex* psf)
+{
+	long strs[33];
+	long d = sizeof(float2);
+
+	for (int i = 0; i < N && i < 33; i++) {
+		strs[i] = d;
+		d *= psf_dims[i];
+	}
+
+	nufft_get_psf2(_op, N, psf_dims, strs, psf);
+}
+
+void nufft_update_psf2(const struct linop_s* _op, int ND, const long psf_dims[], const long psf_strs[],
+		       const float _Complex* psf)
 {
 	guarded_void([&]() {
 		Op* op = (Op*)_op;
@@ -636,19 +780,188 @@ void nufft_update_psf(const struct linop_s* _op, int ND, const long psf_dims[], 
 		BAM_CHECK(op->plan("nufft_update_psf").toeplitz, "nufft_update_psf: operator is not in Toeplitz mode");
 		long want[33];
 		BAM_CHECK(ND == op->N + 1, "nufft_update_psf: dims");
-		for (int i = 0; i < op->N; i++)
-			want[i] = (i < 3) ? op->nufft->n[i] : 1;
-		want[op->N] = op->nufft->S;
+		psf_dims_of(op, want);
 		check_dims("nufft_update_psf", ND, psf_dims, ND, want);
+		std::lock_guard<std::mutex> lock(op->mtx());
 		long n = (long)op->nufft->S * op->nufft->N;
 		if ((long)op->nufft->psf.n < n)
 			op->nufft->psf.alloc(n);
-		BAM_HIP(hipMemcpy(op->nufft->psf.ptr, psf, sizeof(float2) * n,
-				  on_device(psf) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+		copy_strides(ND, psf_dims, psf_strs, (float2*)psf, op->nufft->psf.ptr, false, op->stream);
 		op->nufft->has_psf = true;
 		if (op->cg)
 			op->cg->drop_graph();
 	});
+}
+
+void nufft_update_psf(const struct linop_s* _op, int ND, const long psf_dims[], const float _Complex* psf)
+{
+	long strs[33];
+	long d = sizeof(float2);
+
+	for (int i = 0; i < ND && i < 33; i++) {
+		strs[i] = d;
+		d *= psf_dims[i];
+	}
+
+	nufft_update_psf2(_op, ND, psf_dims, strs, psf);
+}
+
+struct linop_s* nufft_create_normal(int N, const long cim_dims[], int ND, const long psf_dims[],
+				    const float _Complex* psf, bool basis, struct nufft_conf_s conf)
+{
+	return guarded([&]() -> struct linop_s* {
+		check_conf(conf, basis);
+		BAM_CHECK(conf.toeplitz, "nufft_create_normal: needs the Toeplitz mode");
+		BAM_CHECK(N >= 4 && N <= 32 && ND == N + 1, "nufft_create_normal: dims (N = %d, ND = %d)", N, ND);
+
+		for (int i = 4; i < N; i++)
+			BAM_CHECK(1 == cim_dims[i], "nufft_create_normal: dim %d must be 1", i);
+
+		int img[3];
+
+		for (int i = 0; i < 3; i++) {
+			BAM_CHECK(cim_dims[i] >= 1 && cim_dims[i] < (1l << 20), "nufft_create_normal: bad image dim");
+			img[i] = (int)cim_dims[i];
+		}
+
+		auto op = new Op();
+		op->kind = OP_NUFFT;
+		op->N = N;
+		op->normal_only = true;
+
+		// BART's normal-only linop has singleton output dims
+		for (int i = 0; i < N; i++)
+			op->odims[i] = 1;
+		memcpy(op->idims, cim_dims, sizeof(long) * N);
+		op->osize = 1;
+		op->isize = md_size(N, cim_dims);
+		op->q = std::make_shared<Queue>();
+		op->stream = op->q->stream;
+
+		try {
+			op->nufft = std::make_shared<Nufft>();
+			op->nufft->stream = op->stream;
+			op->nufft->create_geometry(img, (int)cim_dims[3], true, conf.periodic, conf.width, conf.os);
+			long want[33];
+			psf_dims_of(op, want);
+			check_dims("nufft_create_normal (psf)", ND, psf_dims, ND, want);
+			op->nufft->psf.alloc((size_t)op->nufft->S * op->nufft->N);
+		} catch (...) {
+			delete op;
+			throw;
+		}
+
+		if (nullptr != psf) {
+			long strs[33];
+			long d = sizeof(float2);
+			for (int i = 0; i < ND; i++) {
+				strs[i] = d;
+				d *= psf_dims[i];
+			}
+			try {
+				copy_strides(ND, psf_dims, strs, (float2*)psf, op->nufft->psf.ptr, false, op->stream);
+			} catch (...) {
+				delete op;
+				throw;
+			}
+			op->nufft->has_psf = true;
+		}
+
+		return (struct linop_s*)op;
+	}, nullptr);
+}
+
+void nufft_update_traj(const struct linop_s* _op, int N, const long trj_dims[], const float _Complex* traj,
+		       const long wgh_dims[], const float _Complex* weights, const long bas_dims[],
+		       const float _Complex* basis)
+{
+	(void)bas_dims;
+	guarded_void([&]() {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "nufft_update_traj: NULL operator");
+		Nufft& p = op->plan("nufft_update_traj");
+		BAM_CHECK(!op->normal_only, "nufft_update_traj: normal-only operator has no trajectory");
+		BAM_CHECK(nullptr == basis, "nufft_update_traj: subspace basis is not supported by this library");
+		BAM_CHECK(nullptr != traj, "nufft_update_traj: trajectory is NULL");
+		BAM_CHECK(N == op->N, "nufft_update_traj: dimension count %d != %d", N, op->N);
+		BAM_CHECK(3 == trj_dims[0] && trj_dims[1] == op->odims[1] && trj_dims[2] == op->odims[2],
+			  "Incompatible dimensions of k-space and trajectory!");
+
+		for (int i = 3; i < N; i++)
+			BAM_CHECK(1 == trj_dims[i], "nufft_update_traj: trajectory dim %d must be 1", i);
+
+		const long M = trj_dims[1] * trj_dims[2];
+		std::vector<float> tr = traj_reals((const float2*)traj, M);
+		std::vector<float2> w = expand_weights(N, wgh_dims, (const float2*)weights, op->odims);
+
+		std::lock_guard<std::mutex> lock(op->mtx());
+		BAM_HIP(hipStreamSynchronize(op->stream));
+		p.set_traj(M, tr.data(), w.empty() ? nullptr : w.data(), p.toeplitz);
+		if (op->cg)
+			op->cg->drop_graph();
+	});
+}
+
+float _Complex* compute_psf2(int N, const long psf_dims[], unsigned long flags, const long trj_dims[],
+			     const float _Complex* traj, const long bas_dims[], const float _Complex* basis,
+			     const long wgh_dims[], const float _Complex* weights, bool periodic, bool lowmem,
+			     bool upper_triag)
+{
+	(void)bas_dims;
+	(void)lowmem;
+	return guarded([&]() -> float _Complex* {
+		BAM_CHECK(nullptr == basis && !upper_triag, "compute_psf2: subspace basis is not supported by this library");
+		BAM_CHECK(7 == (flags & 7) && 0 == (flags & ~7ul), "compute_psf2: flags must be 7");
+		BAM_CHECK(N >= 4 && N <= 32, "compute_psf2: unsupported dimension count %d", N);
+		BAM_CHECK(nullptr != traj, "compute_psf2: trajectory is NULL");
+		BAM_CHECK(3 == trj_dims[0], "compute_psf2: trajectory dim 0 must be 3");
+
+		for (int i = 3; i <= N; i++)
+			BAM_CHECK(1 == trj_dims[i] && (i == N || 1 == psf_dims[i]), "compute_psf2: dim %d must be 1", i);
+
+		int img[3];
+		for (int i = 0; i < 3; i++)
+			img[i] = (int)psf_dims[i];
+
+		const long M = trj_dims[1] * trj_dims[2];
+		std::vector<float> tr = traj_reals((const float2*)traj, M);
+		long ksp_dims[33];
+		for (int i = 0; i <= N; i++)
+			ksp_dims[i] = (0 == i) ? 1 : trj_dims[i];
+		std::vector<float2> w = expand_weights(N, wgh_dims, (const float2*)weights, ksp_dims);
+
+		Queue q;
+		Nufft p;
+		p.stream = q.stream;
+		p.create_geometry(img, 1, true, periodic, nufft_conf_defaults.width, nufft_conf_defaults.os);
+		BAM_CHECK(psf_dims[N] == p.S, "compute_psf2: psf dim %d must be %d", N, p.S);
+		p.M = M;
+		p.compute_psf_from_traj(tr.data(), w.empty() ? nullptr : w.data(), periodic);
+
+		// the result lives where the trajectory does (BART: md_alloc_sameplace)
+		long n = (long)p.S * p.N;
+		float2* out = nullptr;
+		if (on_device(traj)) {
+			BAM_HIP(hipMalloc((void**)&out, sizeof(float2) * n));
+			BAM_HIP(hipMemcpy(out, p.psf.ptr, sizeof(float2) * n, hipMemcpyDeviceToDevice));
+		} else {
+			out = (float2*)malloc(sizeof(float2) * n);
+			BAM_CHECK(nullptr != out, "compute_psf2: out of memory");
+			BAM_HIP(hipMemcpy(out, p.psf.ptr, sizeof(float2) * n, hipMemcpyDeviceToHost));
+		}
+		return (float _Complex*)out;
+	}, nullptr);
+}
+
+int bart_amd_linop_set_stream(const struct linop_s* _op, void* stream)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "bart_amd_linop_set_stream: NULL operator");
+		std::lock_guard<std::mutex> lock(op->mtx());
+		op->user_stream = (hipStream_t)stream;
+		return 0;
+	}, -1);
 }
 
 static void fft_common(int D, const long dims[], unsigned long flags, float _Complex* dst, const float _Complex* src,
@@ -706,6 +1019,10 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		BAM_CHECK(!op->toeplitz() || op->nufft->has_psf, "lsqr_conjgrad: Toeplitz operator without PSF");
 
 		std::lock_guard<std::mutex> lock(op->mtx());
+
+		// x and b may have been written on the caller's stream
+		if (0 != op->user_stream)
+			BAM_HIP(hipStreamSynchronize(op->user_stream));
 
 		long ne = op->isize;
 

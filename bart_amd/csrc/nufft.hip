@@ -1070,6 +1070,12 @@ static FftLaunchCfg fft_config_skip(const int* skip, const AxisPlan& plan, long 
 void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const float2* weights_host,
 		   bool toeplitz_, bool periodic_, float width_, float os_, bool compute_psf)
 {
+	create_geometry(dims, C_, toeplitz_, periodic_, width_, os_);
+	set_traj(M_, traj, weights_host, compute_psf);
+}
+
+void Nufft::create_geometry(const int dims[3], int C_, bool toeplitz_, bool periodic_, float width_, float os_)
+{
 	BAM_CHECK(2.f == os_, "only oversampling 2 is supported (got %f)", os_);
 	BAM_CHECK(width_ > 0.f && width_ / 2.f <= (float)(MAX_TAPS - 1), "kernel width %f not supported", width_);
 
@@ -1093,7 +1099,7 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	BAM_CHECK(nact >= 1, "NUFFT needs at least one non-trivial image dim");
 
 	C = C_;
-	M = M_;
+	M = 0;
 	width = width_;
 	os = os_;
 	// BART's KB table is built by the first kb_init call, which
@@ -1124,6 +1130,25 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	const char* pz = getenv("BART_AMD_PAIRZ");
 	pair_zpass = slice && fft_pair_len_ok(n[2]) && !(nullptr != pz && 0 == strcmp(pz, "off"));
 
+	grid.alloc((size_t)S * C * N);
+
+	// partial images of the coil groups of S^H (allocated here so that no
+	// allocation happens inside a captured CG iteration)
+	// partial images of the S^H sum: one per coil with the slice kernels
+	const int G = slice ? C : C / coil_group(S, C, n[act[0]], true);
+
+	if (G > 1)
+		part.alloc((size_t)G * N);
+}
+
+// Trajectory, weights, gridding plan and (Toeplitz) PSF -- nufft_set_traj
+// (noncart/nufft.c:1002-1159).  Called again by nufft_update_traj: the
+// sample count stays that of the k-space dims.
+void Nufft::set_traj(long M_, const float* traj, const float2* weights_host, bool compute_psf)
+{
+	M = M_;
+	has_traj = false;
+	has_psf = false;
 	has_weights = (nullptr != weights_host);
 
 	if (has_weights) {
@@ -1167,19 +1192,10 @@ void Nufft::create(const int dims[3], int C_, long M_, const float* traj, const 
 	}
 
 	gp.build(g, traj);
-
-	grid.alloc((size_t)S * C * N);
-
-	// partial images of the coil groups of S^H (allocated here so that no
-	// allocation happens inside a captured CG iteration)
-	// partial images of the S^H sum: one per coil with the slice kernels
-	const int G = slice ? C : C / coil_group(S, C, n[act[0]], true);
-
-	if (G > 1)
-		part.alloc((size_t)G * N);
+	has_traj = true;
 
 	if (toeplitz && compute_psf)
-		compute_psf_from_traj(traj, weights_host);
+		compute_psf_from_traj(traj, weights_host, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1513,7 +1529,7 @@ struct StDecomp {
 	}
 };
 
-void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host)
+void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host, bool periodic_psf)
 {
 	int n2[3];
 
@@ -1539,7 +1555,7 @@ void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host)
 
 	Nufft p2;
 	p2.stream = stream;
-	p2.create(n2, 1, M, traj2.data(), sqw.empty() ? nullptr : sqw.data(), false, true, width, os, false);
+	p2.create(n2, 1, M, traj2.data(), sqw.empty() ? nullptr : sqw.data(), false, periodic_psf, width, os, false);
 
 	long N2 = p2.N;
 	DevBuf<float2> ones(std::max<long>(M, 1));

@@ -168,6 +168,13 @@ struct Nufft {
 	// dims: image dims; traj: host M x 3 (real parts); weights: host M or null
 	void create(const int dims[3], int C, long M, const float* traj, const float2* weights_host,
 		    bool toeplitz, bool periodic, float width, float os, bool compute_psf);
+	// geometry, tables and work buffers without a trajectory (the PSF-only
+	// normal operator of nufft_create_normal)
+	void create_geometry(const int dims[3], int C, bool toeplitz, bool periodic, float width, float os);
+	// (re)sets trajectory (host, M x 3) and weights, builds the gridding
+	// plan and, in Toeplitz mode with compute_psf, the PSF
+	void set_traj(long M, const float* traj, const float2* weights_host, bool compute_psf);
+	bool has_traj = false;
 
 	// device pointers; sens == nullptr: coil images in/out, else single image
 	// combined with coil maps
@@ -175,7 +182,9 @@ struct Nufft {
 	void adjoint(float2* cim, const float2* ksp, const float2* sens);
 	// Toeplitz; skip: optional device flag that turns the launches into no-ops
 	void normal(float2* dst, const float2* src, const float2* sens, const int* skip = nullptr);
-	void compute_psf_from_traj(const float* traj, const float2* weights_host);
+	// PSF of the trajectory (periodic_psf: periodic gridding of the 2x
+	// grid, as nufft_set_traj always uses)
+	void compute_psf_from_traj(const float* traj, const float2* weights_host, bool periodic_psf);
 };
 
 void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float2* ksp, const float2* weights,

@@ -174,3 +174,52 @@ def add_noise(ksp, rel_sigma=1e-3, seed=0):
     s = rel_sigma * float(np.max(np.abs(ksp)))
     n = rng.standard_normal(ksp.shape) + 1j * rng.standard_normal(ksp.shape)
     return (ksp + s / np.sqrt(2.0) * n).astype(np.complex64)
+
+
+# --------------------------------------------------------------------------
+# The benchmark workloads (sizes, seeds and trajectories in one place, for
+# the benchmark and the full-size parity tests)
+# --------------------------------------------------------------------------
+
+def config_radial2d():
+    """2-D golden-angle radial: 256^2 image of 10 ellipses, 8 Gaussian ring
+    coils, 201 spokes x 512 samples (2x readout oversampling), lambda 0.01,
+    30 CG iterations."""
+    n, C, spokes, R = 256, 8, 201, 512
+    return dict(name="2d_golden_radial_256^2_8coils_201x512", img=(n, n, 1), C=C, R=R, S=spokes,
+                traj=radial_traj(R, spokes), x=ellipse_phantom(n, nell=10, seed=1),
+                maps=gaussian_coils((1, n, n), C, seed=1), lam=0.01, iters=30, groups=2, seed=1)
+
+
+def config_kooshball():
+    """3-D radial kooshball: 128^3 image of 16 ellipsoids, 16 coils with 3-D
+    Gaussian maps, 8192 phyllotaxis spokes x 256 samples, lambda 0.01,
+    20 CG iterations."""
+    n, C, spokes, R = 128, 16, 8192, 256
+    return dict(name="3d_radial_kooshball_128^3_16coils_8192x256", img=(n, n, n), C=C, R=R, S=spokes,
+                traj=phyllotaxis_kooshball(spokes=spokes, readout=R), x=ellipsoid_phantom(n, nell=16, seed=2),
+                maps=gaussian_coils((n, n, n), C, seed=2), lam=0.01, iters=20, groups=16, seed=2)
+
+
+def config_spiral():
+    """2-D variable-density spiral: 320^2 image of 10 ellipses, 32 coils,
+    24 interleaves x 4096 samples with density ~ 1 / (1 + 8 |k|), lambda
+    0.01, 30 CG iterations."""
+    n, C = 320, 32
+    return dict(name="2d_vd_spiral_320^2_32coils_24x4096", img=(n, n, 1), C=C, R=4096, S=24,
+                traj=vd_spiral(interleaves=24, samples=4096, n=n), x=ellipse_phantom(n, nell=10, seed=3),
+                maps=gaussian_coils((1, n, n), C, seed=3), lam=0.01, iters=30, groups=16, seed=3)
+
+
+def batched_slices(nslices=96, seed=4):
+    """Uneven 2-D CG-SENSE problems: matrix n in U{128,192,256,320,384}, coils
+    in U{8,16,32}, golden-angle radial spokes in U[21,201] with readout 2n;
+    per slice an ellipse phantom and Gaussian coil maps of its own seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(nslices):
+        n = int(rng.choice([128, 192, 256, 320, 384]))
+        C = int(rng.choice([8, 16, 32]))
+        spokes = int(rng.integers(21, 202))
+        out.append(dict(n=n, C=C, R=2 * n, S=spokes, seed=1000 + s))
+    return out

@@ -4,15 +4,21 @@ Default workload (one GPU): 3-D radial kooshball, 128^3 image, 16 coils,
 8192 phyllotaxis spokes x 256 samples (2,097,152 samples per coil), 2x
 oversampled grid (decomposed into 8 sub-grids of 128^3), Kaiser-Bessel width 6.
 One step = adjoint NUFFT (k-space -> coil images), forward NUFFT (coil images ->
-k-space) and one CG-SENSE solve (A^H y, then up to 20 Toeplitz iterations,
-lambda = 0.01, stop at relative residual 1e-4).
+k-space) and one CG-SENSE solve (A^H y, then 20 Toeplitz iterations,
+lambda = 0.01).  Measured beside the steps in the same run: the kooshball
+CG-SENSE solve to relative residual 1e-4, and the 2-D radial workload's
+(256^2, 8 coils, 201 x 512) CG-SENSE solve to relative residual 1e-4 from
+zero and from the learned warm start.
 
 Prints one JSON line.  `value` is the NUFFT adjoint+forward throughput in
 million coil-samples per second (2 C M per adj+fwd pair / time of the pair).
 Inputs are resident in HBM before the timed region.  `roofline` is for the
 kernel with the largest share of the timed region, timed live with HIP
-events on the stream it runs on; `cpu_baseline` is the CPU oracle (a C/OpenMP
-restatement of BART's CPU algorithm) on a bounded sample of the same workload.
+events on the stream it runs on, charged with the compulsory bytes of the
+unit of work it belongs to (one Toeplitz CG iteration: 8 N (9 + C + 2^d);
+one NUFFT: 8 C M + 4 d M + 8 C N); its own input/output bytes are
+reported beside.  `cpu_baseline` is the CPU oracle (a C/OpenMP restatement
+of BART's CPU algorithm) on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -44,29 +50,35 @@ def dims16(*d):
 
 def workload(name):
     from bart_amd import inputs as I
-    if name == "kooshball":
-        n, C, spokes, R = 128, 16, 8192, 256
-        traj = I.phyllotaxis_kooshball(spokes=spokes, readout=R)
-        img = I.ellipsoid_phantom(n, nell=16, seed=2)
-        maps = I.gaussian_coils((n, n, n), C, seed=2)
-        # groups: coil groups of the SENSE normal operator's S^H sum
-        return dict(name="3d_radial_kooshball_128^3_16coils_8192x256", img=(n, n, n), C=C, R=R, S=spokes,
-                    traj=traj, x=img, maps=maps, lam=0.01, iters=20, groups=16)
-    if name == "radial2d":
-        n, C, spokes, R = 256, 8, 201, 512
-        traj = I.radial_traj(R, spokes)
-        img = I.ellipse_phantom(n, nell=10, seed=1)
-        maps = I.gaussian_coils((1, n, n), C, seed=1)
-        return dict(name="2d_golden_radial_256^2_8coils_201x512", img=(n, n, 1), C=C, R=R, S=spokes,
-                    traj=traj, x=img, maps=maps, lam=0.01, iters=30, groups=2)
-    if name == "spiral":
-        n, C = 320, 32
-        traj = I.vd_spiral(interleaves=24, samples=4096, n=n)
-        img = I.ellipse_phantom(n, nell=10, seed=3)
-        maps = I.gaussian_coils((1, n, n), C, seed=3)
-        return dict(name="2d_vd_spiral_320^2_32coils_24x4096", img=(n, n, 1), C=C, R=4096, S=24,
-                    traj=traj, x=img, maps=maps, lam=0.01, iters=30, groups=16)
-    raise ValueError(name)
+    return {"kooshball": I.config_kooshball, "radial2d": I.config_radial2d, "spiral": I.config_spiral}[name]()
+
+
+def survey_bytes(w):
+    """Compulsory HBM bytes of the units of work of the metric (each input
+    read once, each output written once; complex64 8 B, coordinates 4 B per
+    dim): one NUFFT application 8 C M + 4 d M + 8 C N, one Toeplitz CG
+    iteration 8 N (9 + C + 2^d) (image in and out, maps, PSF, and the CG
+    update reading x, r, p, Ap and writing x, r, p), one SENSE A^H y
+    8 C M + 4 d M + 8 N + 8 C N."""
+    X, Y, Z = w["img"]
+    N = X * Y * Z
+    C = w["C"]
+    M = w["R"] * w["S"]
+    d = sum(1 for v in w["img"] if v > 1)
+    return dict(nufft=8 * C * M + 4 * d * M + 8 * C * N, iteration=8 * N * (9 + C + 2 ** d),
+                sense_adjoint=8 * C * M + 4 * d * M + 8 * N + 8 * C * N)
+
+
+# the unit of work each kernel class belongs to: a kernel that runs once
+# per Toeplitz CG iteration is charged with the whole iteration's
+# compulsory bytes, a kernel of the NUFFT with the whole application's
+UNIT_OF = {
+    "fft_toeplitz": "iteration", "fft_toeplitz_slice_fwd": "iteration", "fft_toeplitz_slice_inv": "iteration",
+    "sum_coil_groups": "iteration", "fft_toeplitz_fwd": "iteration", "fft_toeplitz_inv": "iteration",
+    "grid_adjoint": "nufft", "grid_forward": "nufft", "permute_ksp": "nufft", "fft_fwd_axis": "nufft",
+    "fft_adj_axis": "nufft", "fft_fwd_first": "nufft", "fft_adj_last": "nufft", "fft_fwd_slice": "nufft",
+    "fft_adj_slice": "nufft",
+}
 
 
 def algorithmic_bytes(kernel, w):
@@ -168,27 +180,37 @@ def explicit_vs_toeplitz(w, x, sense_t):
     return out
 
 
-def cpu_cg_baseline(w, threads, y):
-    """Oracle CG-SENSE solve (Toeplitz, the step's iteration count, lambda)
-    on the host: PSF + A^H y + iterations, repeated for about 10 s."""
+def cpu_probe():
+    """Whether BART itself or FFTW (single precision) is installed on this
+    host: the CPU baseline would then be BART's own path.  Neither can ship
+    with this repository."""
+    import ctypes.util
+    import shutil
+    return dict(bart=shutil.which("bart"), fftw3f=ctypes.util.find_library("fftw3f"))
+
+
+def cpu_cg_baseline(w, threads, y, tol=1e-4, maxiter=300):
+    """Oracle CG-SENSE solve (Toeplitz, lambda) to relative residual tol on
+    the host: PSF + A^H y + iterations, repeated for about 10 s."""
     import oracle as O
     O.lib().orc_set_num_threads(threads)
     C = w["C"]
     M = w["R"] * w["S"]
     yh = y.cpu().numpy().reshape(C, M)
-    reps, t0 = 0, time.perf_counter()
+    reps, its, t0 = 0, 0, time.perf_counter()
     while True:
         ref = O.Nufft(w["img"], w["traj"], C, toeplitz=True)
         b = ref.sense_adjoint(w["maps"], yh)
-        ref.cgsense(w["maps"], b, w["lam"], w["iters"], 0.0)
+        _, it, _, _ = ref.cgsense(w["maps"], b, w["lam"], maxiter, tol)
+        its = it
         reps += 1
         dt = time.perf_counter() - t0
         if dt >= 10.0 or reps >= 50:
             break
     X, Y, Z = w["img"]
-    return dict(value=round(reps / dt, 4), unit="solves/s", cores=threads, kind="port",
-                sample=f"oracle PSF + A^H y + {w['iters']} Toeplitz CG iterations, {C} coils, {X}x{Y}x{Z}, "
-                       f"{reps} solves ({dt:.1f} s)")
+    return dict(value=round(reps / dt, 4), unit="solves/s", cores=threads, kind="port", iterations=its,
+                sample=f"oracle (BART-algorithm CPU restatement, not BART) PSF + A^H y + Toeplitz CG to relative "
+                       f"residual {tol:g} ({its} iterations), {C} coils, {X}x{Y}x{Z}, {reps} solves ({dt:.1f} s)")
 
 
 def cpu_baseline(w, threads):
@@ -218,21 +240,6 @@ def cpu_baseline(w, threads):
                        f"{X}x{Y}x{Z} grid, {reps} repetitions ({dt:.1f} s)")
 
 
-def batched_slices(nslices=96, seed=4):
-    """Uneven 2-D CG-SENSE problems: matrix n in U{128,192,256,320,384}, coils
-    in U{8,16,32}, golden-angle radial spokes in U[21,201] with readout 2n;
-    per slice an ellipse phantom and Gaussian coil maps of its own seed."""
-    from bart_amd import inputs as I
-    rng = np.random.default_rng(seed)
-    out = []
-    for s in range(nslices):
-        n = int(rng.choice([128, 192, 256, 320, 384]))
-        C = int(rng.choice([8, 16, 32]))
-        spokes = int(rng.integers(21, 202))
-        out.append(dict(n=n, C=C, R=2 * n, S=spokes, seed=1000 + s))
-    return out
-
-
 def run_batched(args, L, report):
     """96 independent CG-SENSE solves (30 iterations each, lambda 0.01): the
     right-hand sides A^H y, then one lsqr_conjgrad_batch call.  value = solves/s."""
@@ -241,7 +248,7 @@ def run_batched(args, L, report):
     from bart_amd import inputs as I
 
     dev = torch.device("cuda:0")
-    sl = batched_slices()
+    sl = I.batched_slices()
     iters, lam = 30, 0.01
     t0 = time.perf_counter()
     ops, ys, bs = [], [], []
@@ -483,6 +490,11 @@ def main():
                     help="warmstart: problems for fine-tuning on the normal-equation residual (0: none)")
     ap.add_argument("--ws-ft-epochs", type=int, default=10, help="warmstart: residual fine-tuning epochs")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-radial", action="store_true",
+                    help="kooshball: skip the 2-D radial CG-SENSE solves to tolerance")
+    ap.add_argument("--tol-solves", type=int, default=3, help="CG-SENSE solves to 1e-4 timed (median)")
+    ap.add_argument("--tol-maxiter", type=int, default=200, help="iteration cap of the solves to 1e-4")
+    ap.add_argument("--radial-solves", type=int, default=10, help="2-D radial solves to 1e-4 timed (median)")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="hardware-counter target: run the adjoint, forward and SENSE normal operator twice "
@@ -564,9 +576,11 @@ def main():
         t0 = time.perf_counter()
         nufft.adjoint(y, out=cim)
         nufft.forward(cim, out=ksp2)
+        torch.cuda.synchronize()
         t1 = time.perf_counter()
         sense.adjoint(y, out=b)
-        xs, st, tr = ba.lsqr_conjgrad(sense, b, w["iters"], w["lam"], 1e-4)
+        xs, st, tr = ba.lsqr_conjgrad(sense, b, w["iters"], w["lam"], 0.0)
+        torch.cuda.synchronize()
         t2 = time.perf_counter()
         rec.append((t1 - t0, t2 - t1, st.iterations, st.resid / max(st.bnorm, 1e-30)))
 
@@ -618,6 +632,18 @@ def main():
     relres = rec[-1][3]
     value = 2 * C * M / t_nufft / 1e6
 
+    # CG-SENSE to relative residual 1e-4 (the metric's solve), cold start
+    tol_rec = []
+    for _ in range(args.tol_solves):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sense.adjoint(y, out=b)
+        xs, st, tr = ba.lsqr_conjgrad(sense, b, args.tol_maxiter, w["lam"], 1e-4)
+        torch.cuda.synchronize()
+        tol_rec.append((time.perf_counter() - t0, st.iterations, st.status, st.resid / max(st.bnorm, 1e-30)))
+    t_tol = float(np.median([r[0] for r in tol_rec]))
+    print(f"# cg-sense to 1e-4: {tol_rec[-1][1]} it, {t_tol * 1e3:.2f} ms", file=sys.stderr, flush=True)
+
     # per step: (ms, launches, mean ms per launch) of every kernel class
     per_step = {}
     for k, (cnt, tot, mn, mx) in prof.items():
@@ -628,18 +654,37 @@ def main():
         e[0] += tot / cnt * launches
         e[1] += launches
 
+    sb = survey_bytes(w)
     roofline = None
     if per_step:
         name, (ms_step, launches, avg_ms) = max(per_step.items(), key=lambda kv: kv[1][0])
-        bts = algorithmic_bytes(name, w)
-        if bts is not None:
+        unit = UNIT_OF.get(name)
+        kb = algorithmic_bytes(name, w)
+        if unit is not None:
+            bts = sb[unit]
             ach = bts / (avg_ms * 1e-3) / 1e9
             tr = TRAFFIC.get((w["name"], name))
             roofline = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                             frac=round(ach / HBM_PEAK_GBS, 4), traffic=tr, kernel=name,
+                            bytes_per_launch=bts, bytes_unit=f"one {unit} (compulsory bytes)",
                             avg_ms=round(avg_ms, 4), launches_per_step=round(launches, 1),
                             share_of_step=round(ms_step / (t_total / args.steps * 1e3), 3),
-                            bytes_per_launch=bts)
+                            kernel_bytes_per_launch=kb,
+                            kernel_bytes_frac=(round(kb / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if kb else None),
+                            traffic_source="rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --pmc-pass "
+                                           "(profiles/)" if tr else None)
+
+    # the whole Toeplitz CG iteration and the NUFFT pair against their
+    # compulsory bytes (end to end, all kernels)
+    iter_ms = sum(v[0] for k, v in per_step.items() if UNIT_OF.get(k) == "iteration" or k.startswith("cg_")) / max(its, 1)
+    e2e = dict(nufft_adj_fwd=dict(ms=round(t_nufft * 1e3, 3), bytes=2 * sb["nufft"],
+                                  frac=round(2 * sb["nufft"] / t_nufft / 1e9 / HBM_PEAK_GBS, 4)),
+               cg_iteration=(dict(ms=round(iter_ms, 4), bytes=sb["iteration"],
+                                  frac=round(sb["iteration"] / (iter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+                             if iter_ms > 0 else None),
+               solve_to_tol=dict(ms=round(t_tol * 1e3, 3), bytes=sb["sense_adjoint"] + tol_rec[-1][1] * sb["iteration"],
+                                 frac=round((sb["sense_adjoint"] + tol_rec[-1][1] * sb["iteration"]) / t_tol / 1e9
+                                            / HBM_PEAK_GBS, 4)))
 
     # accuracy: FP64 direct NUDFT on a sample subset (8-coil oracle leg aside)
     accuracy = None
@@ -652,13 +697,22 @@ def main():
     if Z == 1 and not args.no_cpu:
         normal_cmp = explicit_vs_toeplitz(w, b, sense)
 
+    # the 2-D radial CG-SENSE solve of the metric (256^2, 8 coils, 201 x 512)
+    # to relative residual 1e-4, cold and from the learned warm start
+    radial = None
+    if args.config == "kooshball" and not args.no_radial:
+        radial = radial_solves(args, dev)
+
     cpu = None
     cpu_cg = None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = cpu_baseline(w, threads)
+        cpu["probe"] = cpu_probe()
         if Z == 1:
             cpu_cg = cpu_cg_baseline(w, threads, y)
+        elif radial is not None:
+            cpu_cg = radial.pop("cpu", None)
 
     out = {
         "metric": "NUFFT adj+fwd Msamples/s; CG-SENSE solves/s to rel-res 1e-4, warm vs cold",
@@ -674,21 +728,73 @@ def main():
         "dtype": "c64 (fp32 complex; fp64 CG reductions)",
         "data": "synthetic",
         "config": {"workload": w["name"], "coils": C, "samples_per_coil": M, "image": list(w["img"]),
-                   "oversampling": 2.0, "kb_width": 6, "cg_max_iters": w["iters"], "lambda": w["lam"]},
+                   "oversampling": 2.0, "kb_width": 6, "cg_iters": w["iters"], "lambda": w["lam"]},
         "nufft_adj_fwd_ms": round(t_nufft * 1e3, 3),
-        "cg_sense_solves_per_s_cold": round(1.0 / t_cg, 3),
-        "cg_sense_ms_per_solve_cold": round(t_cg * 1e3, 3),
-        "cg_iterations": its,
-        "cg_rel_res": relres,
+        "cg_sense_fixed_iters": dict(iterations=its, ms=round(t_cg * 1e3, 3), solves_per_s=round(1.0 / t_cg, 3),
+                                     rel_res=relres),
+        "cg_sense_to_tol_cold": dict(tol=1e-4, iterations=tol_rec[-1][1], converged=tol_rec[-1][2] == 2,
+                                     rel_res=tol_rec[-1][3], ms=round(t_tol * 1e3, 3),
+                                     solves_per_s=round(1.0 / t_tol, 3)),
+        "radial2d_cg_sense_to_tol": radial,
         "roofline": roofline,
+        "end_to_end_vs_compulsory_bytes": e2e,
         "cpu_baseline": cpu,
         "cpu_baseline_cg_sense": cpu_cg,
-        "cg_sense_gpu_over_cpu": (round(1.0 / t_cg / cpu_cg["value"], 1) if cpu_cg else None),
         "fp64_nudft": accuracy,
         "normal_explicit_vs_toeplitz": normal_cmp,
         "kernels_ms_per_step": {k: round(v[0], 4) for k, v in sorted(per_step.items())},
     }
+    if cpu_cg and radial and radial.get("cold"):
+        out["cg_sense_gpu_over_cpu_radial2d"] = round(radial["cold"]["solves_per_s"] / cpu_cg["value"], 1)
+    elif cpu_cg and Z == 1:
+        out["cg_sense_gpu_over_cpu"] = round(1.0 / t_tol / cpu_cg["value"], 1)
     print(json.dumps(out), flush=True)
+
+
+def radial_solves(args, dev):
+    """CG-SENSE on the 2-D radial workload (256^2, 8 coils, 201 x 512,
+    lambda 0.01) to relative residual 1e-4: A^H y then CG from zero, and
+    from the learned warm start when a trained network is present.  Also
+    the oracle's CPU solve of the same problem (about 10 s)."""
+    import torch
+    import bart_amd as ba
+    w = workload("radial2d")
+    X, Y, Z = w["img"]
+    C, R, S = w["C"], w["R"], w["S"]
+    M = R * S
+    N = X * Y * Z
+    sense = ba.sense_nc_init(dims16(X, Y, Z, C), dims16(X, Y, Z, C), w["maps"], dims16(1, R, S, C),
+                             dims16(3, R, S), ba.traj_array(w["traj"]))
+    x = torch.from_numpy(w["x"].reshape(-1)).to(dev)
+    y = sense.forward(x)
+    scale = float(y.abs().max())
+    g = torch.Generator(device="cpu").manual_seed(1)
+    noise = (torch.randn(M * C, generator=g) + 1j * torch.randn(M * C, generator=g)).to(torch.complex64)
+    y = (y + (1e-3 * scale / np.sqrt(2.0)) * noise.to(dev)).contiguous()
+    b = torch.empty(N, dtype=torch.complex64, device=dev)
+
+    def cold():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sense.adjoint(y, out=b)
+        xs, st, _ = ba.lsqr_conjgrad(sense, b, 300, w["lam"], 1e-4)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, st
+
+    cold()
+    rec = [cold() for _ in range(args.radial_solves)]
+    t = float(np.median([r[0] for r in rec]))
+    st = rec[-1][1]
+    out = dict(workload=w["name"], tol=1e-4,
+               cold=dict(iterations=st.iterations, converged=st.status == 2, ms=round(t * 1e3, 3),
+                         solves_per_s=round(1.0 / t, 2)),
+               warm=None)
+    print(f"# radial2d cg-sense to 1e-4 (cold): {st.iterations} it, {t * 1e3:.3f} ms", file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu"] = cpu_cg_baseline(w, threads, y)
+    sense.free()
+    return out
 
 
 if __name__ == "__main__":

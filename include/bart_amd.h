@@ -56,8 +56,8 @@ struct nufft_conf_s {
 	float os;
 };
 
-/* noncart/nufft.c:53-75 */
-extern const struct nufft_conf_s nufft_conf_defaults;
+/* noncart/nufft.h:32 (values of noncart/nufft.c:53-75) */
+extern struct nufft_conf_s nufft_conf_defaults;
 
 /* noncart/nufft.h:48-57 -- replaces nufft_create2 (noncart/nufft.c:1174-1276).
  * Supported: os = 2, decomp = true, basis = NULL, kernel width <= 14,
@@ -103,12 +103,57 @@ void linop_normal(const struct linop_s* op, int N, const long dims[], BAM_COMPLE
 /* linops/linop.h:55 -- replaces linop_free (linops/linop.c:998) */
 void linop_free(const struct linop_s* op);
 
-/* noncart/nufft.h:95-100 -- replace nufft_get_psf_dims / nufft_get_psf /
- * nufft_update_psf (noncart/nufft.c:2008-2073); PSF dims are N + 1 with the
- * decomposition index last. */
+/* noncart/nufft.h:92-99 -- replace nufft_get_psf_dims / nufft_get_psf2 /
+ * nufft_get_psf / nufft_update_psf2 / nufft_update_psf
+ * (noncart/nufft.c:2008-2073); PSF dims are N + 1 with the decomposition
+ * index last; psf_strs are byte strides (BART's md strides), and psf may be
+ * host or device memory. */
 int nufft_get_psf_dims(const struct linop_s* nufft, int N, long psf_dims[]);
+void nufft_get_psf2(const struct linop_s* nufft, int N, const long psf_dims[], const long psf_strs[],
+		    BAM_COMPLEX* psf);
 void nufft_get_psf(const struct linop_s* nufft, int N, const long psf_dims[], BAM_COMPLEX* psf);
+void nufft_update_psf2(const struct linop_s* nufft, int ND, const long psf_dims[], const long psf_strs[],
+		       const BAM_COMPLEX* psf);
 void nufft_update_psf(const struct linop_s* nufft, int ND, const long psf_dims[], const BAM_COMPLEX* psf);
+
+/* noncart/nufft.h:84-86 -- replaces nufft_create_normal
+ * (noncart/nufft.c:1285-1312): the Toeplitz normal operator of a PSF alone,
+ * on coil images cim_dims; psf (dims N + 1, may be NULL and set later with
+ * nufft_update_psf) is copied.  Its forward and adjoint are errors ("NuFFT
+ * with normal operator only!"); linop_normal applies it.  basis must be
+ * false. */
+struct linop_s* nufft_create_normal(int N, const long cim_dims[], int ND, const long psf_dims[],
+				    const BAM_COMPLEX* psf, bool basis, struct nufft_conf_s conf);
+
+/* noncart/nufft.h:88-91 -- replaces nufft_update_traj
+ * (noncart/nufft.c:2041-2054, nufft_set_traj :1002-1159): a new trajectory
+ * (and weights) of the same sample count for an existing NUFFT or SENSE
+ * operator; the gridding plan and, in Toeplitz mode, the PSF are rebuilt.
+ * basis must be NULL. */
+void nufft_update_traj(const struct linop_s* nufft, int N, const long trj_dims[], const BAM_COMPLEX* traj,
+		       const long wgh_dims[], const BAM_COMPLEX* weights, const long bas_dims[],
+		       const BAM_COMPLEX* basis);
+
+/* noncart/nufft.h:70-73 -- replaces compute_psf2 (noncart/nufft.c:727-776):
+ * the decomposed PSF of a trajectory, psf_dims = image dims, 1..., 2^d in
+ * dim N.  flags = 7, basis NULL, upper_triag false; lowmem changes nothing.
+ * The result is allocated where the trajectory lives, as BART's
+ * md_alloc_sameplace: malloc() host memory for a host trajectory (release
+ * with free()), hipMalloc() device memory for a device trajectory (release
+ * with hipFree()).  NULL on error. */
+BAM_COMPLEX* compute_psf2(int N, const long psf_dims[], unsigned long flags, const long trj_dims[],
+			  const BAM_COMPLEX* traj, const long bas_dims[], const BAM_COMPLEX* basis,
+			  const long wgh_dims[], const BAM_COMPLEX* weights, bool periodic, bool lowmem,
+			  bool upper_triag);
+
+/* The stream an operator's applies launch on (BART runs each OpenMP
+ * thread's GPU work on a stream of its own, num/gpuops.c:214-278): a
+ * hipStream_t, or NULL for the handle's own stream.  With device pointers
+ * and a stream set, linop_forward / adjoint / normal enqueue on it and
+ * return without waiting; lsqr_conjgrad waits for the stream before it
+ * reads x and b.  Handles that share a plan (sense_nc_init's fft_opp)
+ * share the lock, not the stream setting.  Returns 0, or -1 on error. */
+int bart_amd_linop_set_stream(const struct linop_s* op, void* stream);
 
 /* num/fft.h -- replace fft/ifft (num/fft.c:387-417): unnormalised FFT,
  * sign -1 (fft) / +1 (ifft) over the dims in flags (subset of dims 0-2);

@@ -132,6 +132,7 @@ void orc_rolloff(float os, float width, double beta, const long dims[3], cf* dst
 {
 	double i0beta = orc_bessel_i0(beta);
 
+	#pragma omp parallel for collapse(2) schedule(static)
 	for (long z = 0; z < dims[2]; z++)
 		for (long y = 0; y < dims[1]; y++)
 			for (long x = 0; x < dims[0]; x++)
@@ -225,9 +226,10 @@ static void sample_pos(const struct orc_grid_conf* conf, const long gdims[3], co
  * (grid.c:482-485), so the order of the additions into one grid point, and
  * with it the rounding, varies from run to run.  The oracle fixes the order
  * to that of the serial loop: the grid is cut into slabs along its last
- * dimension of size > 1, a thread owns a slab, walks all samples in order
- * and adds only the taps that fall into its slab.  Every grid point then
- * receives its contributions in sample order, whatever the thread count. */
+ * dimension of size > 1, each slab gets the list of the samples whose taps
+ * reach it (in sample order), and a thread adds a slab's samples, taps
+ * inside the slab only.  Every grid point then receives its contributions
+ * in sample order, whatever the thread count. */
 void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, const cf* traj,
 		const long gdims[3], int C, cf* grid, long gstride_c, const cf* ksp, long kstride_c)
 {
@@ -236,7 +238,66 @@ void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, cons
 	while (D > 0 && 1 == gdims[D])
 		D--;
 
-	long nslab = (gdims[D] < 64) ? gdims[D] : 64;
+	long nslab = (gdims[D] < 256) ? gdims[D] : 256;
+	int* slab_of = malloc(sizeof(int) * (size_t)gdims[D]);
+
+	for (long sl = 0; sl < nslab; sl++)
+		for (long r = sl * gdims[D] / nslab; r < (sl + 1) * gdims[D] / nslab; r++)
+			slab_of[r] = (int)sl;
+
+	/* samples per slab, then the lists in sample order */
+	long* cnt = calloc((size_t)nslab + 1, sizeof(long));
+	uint8_t* nsl = malloc((size_t)M);
+	int (*sls)[16] = malloc(sizeof(int[16]) * (size_t)M);
+
+	#pragma omp parallel for schedule(static)
+	for (long m = 0; m < M; m++) {
+
+		nsl[m] = 0;
+
+		bool skip = true;
+
+		for (int c = 0; c < C; c++)
+			skip = skip && (0. == ksp[m + c * kstride_c]);
+
+		if (skip)
+			continue;
+
+		float pos[3];
+		sample_pos(conf, gdims, traj + 3 * m, pos);
+
+		struct taps1d t;
+
+		if (!taps_1d(&t, pos[D], gdims[D], conf->periodic, conf->width, table))
+			continue;
+
+		for (int i = 0; i < t.n; i++) {
+
+			int sl = slab_of[t.idx[i]];
+			bool seen = false;
+
+			for (int j = 0; j < nsl[m]; j++)
+				seen = seen || (sls[m][j] == sl);
+
+			if (!seen)
+				sls[m][nsl[m]++] = sl;
+		}
+	}
+
+	for (long m = 0; m < M; m++)
+		for (int j = 0; j < nsl[m]; j++)
+			cnt[sls[m][j] + 1]++;
+
+	for (long sl = 0; sl < nslab; sl++)
+		cnt[sl + 1] += cnt[sl];
+
+	long* fill = malloc(sizeof(long) * (size_t)nslab);
+	memcpy(fill, cnt, sizeof(long) * (size_t)nslab);
+	long* list = malloc(sizeof(long) * (size_t)(cnt[nslab] > 0 ? cnt[nslab] : 1));
+
+	for (long m = 0; m < M; m++)
+		for (int j = 0; j < nsl[m]; j++)
+			list[fill[sls[m][j]]++] = m;
 
 	#pragma omp parallel for schedule(dynamic, 1)
 	for (long sl = 0; sl < nslab; sl++) {
@@ -244,39 +305,18 @@ void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, cons
 		long lo = sl * gdims[D] / nslab;
 		long hi = (sl + 1) * gdims[D] / nslab;
 
-		for (long m = 0; m < M; m++) {
+		for (long e = cnt[sl]; e < cnt[sl + 1]; e++) {
 
-			bool skip = true;
-
-			for (int c = 0; c < C; c++)
-				skip = skip && (0. == ksp[m + c * kstride_c]);
-
-			if (skip)
-				continue;
+			long m = list[e];
 
 			float pos[3];
 			sample_pos(conf, gdims, traj + 3 * m, pos);
 
 			struct taps1d t[3];
 
-			if (!taps_1d(&t[D], pos[D], gdims[D], conf->periodic, conf->width, table))
-				continue;
-
-			bool mine = false;
-
-			for (int i = 0; i < t[D].n; i++)
-				mine = mine || (lo <= t[D].idx[i] && t[D].idx[i] < hi);
-
-			if (!mine)
-				continue;
-
-			bool ok = true;
-
-			for (int j = 0; j < 3; j++)
-				if (j != D)
-					ok = ok && taps_1d(&t[j], pos[j], gdims[j], conf->periodic, conf->width, table);
-
-			if (!ok)
+			if (!taps_1d(&t[0], pos[0], gdims[0], conf->periodic, conf->width, table)
+			    || !taps_1d(&t[1], pos[1], gdims[1], conf->periodic, conf->width, table)
+			    || !taps_1d(&t[2], pos[2], gdims[2], conf->periodic, conf->width, table))
 				continue;
 
 			for (int iz = 0; iz < t[2].n; iz++) {
@@ -314,6 +354,13 @@ void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, cons
 			}
 		}
 	}
+
+	free(list);
+	free(fill);
+	free(sls);
+	free(nsl);
+	free(cnt);
+	free(slab_of);
 }
 
 /* Forward gridding ("gridH" in BART, grid -> k-space, gather):
@@ -578,6 +625,7 @@ static double complex fftmod_value(double phase, bool inv)
 /* fftmod table over dims[3] (num/fft.c:92-157; all dims of size > 1 set) */
 void orc_fftmod_table(const long dims[3], bool inv, cf* dst)
 {
+	#pragma omp parallel for collapse(2) schedule(static)
 	for (long z = 0; z < dims[2]; z++)
 		for (long y = 0; y < dims[1]; y++)
 			for (long x = 0; x < dims[0]; x++) {
@@ -642,6 +690,7 @@ void orc_linphase_table(const long dims[3], int s, const cf* roll, cf* dst)
 	long N = dims[0] * dims[1] * dims[2];
 	double scale = 1. / sqrt((double)N);
 
+	#pragma omp parallel for collapse(2) schedule(static)
 	for (long z = 0; z < dims[2]; z++)
 		for (long y = 0; y < dims[1]; y++)
 			for (long x = 0; x < dims[0]; x++) {

@@ -23,7 +23,8 @@ def test_header_declares_the_boundary():
     fns = header_functions()
     for f in ["nufft_create2", "nufft_create", "sense_nc_init", "linop_forward", "linop_adjoint", "linop_normal",
               "linop_free", "nufft_get_psf_dims", "nufft_get_psf", "nufft_update_psf", "fft", "ifft",
-              "lsqr_conjgrad"]:
+              "lsqr_conjgrad", "nufft_get_psf2", "nufft_update_psf2", "nufft_create_normal", "nufft_update_traj",
+              "compute_psf2", "bart_amd_linop_set_stream"]:
         assert f in fns
 
 

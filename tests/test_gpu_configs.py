@@ -1,0 +1,356 @@
+"""GPU parity at the benchmark workloads' full sizes: the HIP library through
+its C ABI against the CPU oracle (a C restatement of BART's NUFFT, Toeplitz
+and CG path) on the same seeded inputs, and against complex128 references
+where FP32 rounding has to be told apart from a wrong result.
+
+Workloads (bart_amd.inputs):
+  radial2d   256^2, 8 coils, golden-angle radial 201 x 512, lambda 0.01
+  kooshball  128^3, 16 coils, phyllotaxis kooshball 8192 x 256 (2.1 M
+             samples per coil), 2x grid 256^3
+  spiral     320^2, 32 coils, variable-density spiral 24 x 4096
+  batched    96 uneven 2-D slices (128..384, 8/16/32 coils, 21..201 spokes)
+
+Single operator applications are held to the north-star bar, relative L2
+error <= 1e-5 against the oracle.  CG results are held to a bound derived
+from the FP32 rounding of the method itself (stated at the CG tests).
+"""
+import numpy as np
+import pytest
+
+import bart_amd as ba
+import oracle as O
+from bart_amd import inputs as I
+from conftest import crandn, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def dims16(*d):
+    return list(d) + [1] * (16 - len(d))
+
+
+def nufft_op(w, C, toeplitz):
+    X, Y, Z = w["img"]
+    conf = ba.NufftConf.defaults(toeplitz=toeplitz)
+    return ba.nufft_create2(dims16(1, w["R"], w["S"], C), dims16(X, Y, Z, C), dims16(3, w["R"], w["S"]),
+                            ba.traj_array(w["traj"]), conf=conf)
+
+
+def sense_op(w, toeplitz=True):
+    X, Y, Z = w["img"]
+    C = w["C"]
+    conf = ba.NufftConf.defaults(toeplitz=toeplitz)
+    return ba.sense_nc_init(dims16(X, Y, Z, C), dims16(X, Y, Z, C), w["maps"], dims16(1, w["R"], w["S"], C),
+                            dims16(3, w["R"], w["S"]), ba.traj_array(w["traj"]), conf=conf)
+
+
+def report(name, **vals):
+    txt = ", ".join(f"{k} {v:.2e}" if isinstance(v, float) else f"{k} {v}" for k, v in vals.items())
+    print(f"[{name}] {txt}", flush=True)
+
+
+# --------------------------------------------------------------------------
+# radial2d: 256^2, 8 coils, 201 x 512
+# --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def radial():
+    w = I.config_radial2d()
+    ref = O.Nufft(w["img"], w["traj"], w["C"], toeplitz=True)
+    y = I.add_noise(ref.sense_forward(w["maps"], w["x"]), 1e-3, seed=w["seed"])
+    b = ref.sense_adjoint(w["maps"], y)
+    return w, ref, y, b
+
+
+def test_radial2d_nufft(radial):
+    w, ref, y, b = radial
+    C = w["C"]
+    n = w["img"][0]
+    cim = (w["maps"] * w["x"][None]).reshape(C, 1, n, n)
+    op = nufft_op(w, C, toeplitz=False)
+    orc = O.Nufft(w["img"], w["traj"], C, toeplitz=False)
+    ef = relerr(op.forward(cim).reshape(C, -1), orc.forward(cim))
+    ea = relerr(op.adjoint(y).reshape(C, 1, n, n), orc.adjoint(y))
+    report("radial2d nufft", forward=ef, adjoint=ea)
+    assert ef < 1e-5 and ea < 1e-5
+    op.free()
+
+
+def test_radial2d_sense_and_toeplitz(radial):
+    w, ref, y, b = radial
+    n = w["img"][0]
+    op = sense_op(w)
+    ef = relerr(op.forward(w["x"]).reshape(w["C"], -1), ref.sense_forward(w["maps"], w["x"]))
+    ea = relerr(op.adjoint(y).reshape(1, n, n), b)
+    x = crandn((1, n, n), 101)
+    en = relerr(op.normal(x).reshape(1, n, n), ref.sense_normal(w["maps"], x))
+    report("radial2d sense", forward=ef, adjoint=ea, toeplitz_normal=en)
+    assert ef < 1e-5 and ea < 1e-5 and en < 1e-5
+    op.free()
+
+
+def cg_bound_check(name, op, ref, maps, b, lam, iters, shape, fp64=True):
+    """30 (or iters) CG iterations on the GPU, in the oracle (FP32, the
+    BART-algorithm CPU path) and in complex128 on the oracle's operator.
+
+    The bound: FP32 CG is the exact method plus the rounding it accumulates;
+    the oracle's distance to the complex128 iterate measures how far that
+    rounding carries on this problem after `iters` steps.  The GPU runs the
+    same method in FP32 on an operator of the same accuracy, so its
+    distance to the complex128 iterate may be of the same size: at most
+    twice the oracle's, plus 1e-5 (the operator tolerance) for the GPU's
+    own tables and PSF.  Its relative residual (evaluated in complex128)
+    may exceed the complex128 iterate's by no more than the oracle's does,
+    with the same factor."""
+    xg, st, trg = ba.lsqr_conjgrad(op, b, iters, lam, 0.0)
+    xo, ito, tro, _ = ref.cgsense(maps, b, lam, iters, 0.0)
+    xg = xg.reshape(shape)
+    assert st.iterations == ito == iters
+    vals = dict(gpu_vs_oracle=relerr(xg, xo))
+    if fp64:
+        normal = lambda v: ref.sense_normal_fp64(maps, v)
+        x64, tr64 = O.cg_fp64(normal, b, lam, iters)
+        b64 = b.astype(np.complex128)
+
+        def rres(v):
+            v = v.astype(np.complex128)
+            return float(np.linalg.norm(b64 - normal(v) - lam * v) / np.linalg.norm(b64))
+
+        eg, eo = relerr(xg, x64), relerr(xo, x64)
+        rg, ro, r64 = rres(xg), rres(xo), rres(x64)
+        vals.update(gpu_vs_fp64=eg, oracle_vs_fp64=eo, rres_gpu=rg, rres_oracle=ro, rres_fp64=r64,
+                    trace_gpu_vs_fp64=float(np.max(np.abs(trg[:6] / tr64[:6] - 1))))
+        report(name, **vals)
+        assert eg <= 2.0 * eo + 1e-5
+        assert rg - r64 <= 2.0 * max(ro - r64, 0.0) + 1e-5 * r64 + 1e-7
+    else:
+        # without the complex128 reference: the true residuals (FP32 oracle
+        # operator) of the GPU and oracle iterates agree within 10 %
+        def rres32(v):
+            r = b.astype(np.complex128) - ref.sense_normal(maps, v) - lam * v.astype(np.complex128)
+            return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+        rg, ro = rres32(xg), rres32(xo)
+        vals.update(rres_gpu=rg, rres_oracle=ro)
+        report(name, **vals)
+        assert rg <= 1.1 * ro
+    return xg, xo, vals
+
+
+def test_radial2d_cgsense_30_iterations(radial):
+    # pics -S -w 1 -r 0.01 -i 30 -t: CG on (A^H A + lambda I) x = A^H y
+    w, ref, y, b = radial
+    n = w["img"][0]
+    op = sense_op(w)
+    xg, xo, vals = cg_bound_check("radial2d cg 30", op, ref, w["maps"], b, w["lam"], 30, (1, n, n))
+    # deterministic: a second solve gives the same bits
+    x2, _, _ = ba.lsqr_conjgrad(op, b, 30, w["lam"], 0.0)
+    assert np.array_equal(x2.reshape(1, n, n), xg)
+    op.free()
+
+
+def test_radial2d_cgsense_to_tolerance(radial):
+    # the metric's solve: CG to relative residual 1e-4 (||r|| <= 1e-4 ||b||)
+    w, ref, y, b = radial
+    n = w["img"][0]
+    op = sense_op(w)
+    xg, st, trg = ba.lsqr_conjgrad(op, b, 300, w["lam"], 1e-4)
+    xo, ito, tro, fo = ref.cgsense(w["maps"], b, w["lam"], 300, 1e-4)
+    normal = lambda v: ref.sense_normal_fp64(w["maps"], v)
+    x64, tr64 = O.cg_fp64(normal, b, w["lam"], 300, 1e-4)
+    report("radial2d cg to 1e-4", it_gpu=st.iterations, it_oracle=ito, it_fp64=len(tr64),
+           gpu_vs_oracle=relerr(xg.reshape(1, n, n), xo), gpu_vs_fp64=relerr(xg.reshape(1, n, n), x64),
+           oracle_vs_fp64=relerr(xo, x64))
+    assert st.status == 2 and st.resid <= 1e-4 * st.bnorm * 1.0001
+    # the stop iteration is where ||r_k|| first drops below 1e-4 ||b||; FP32
+    # rounding may move that crossing by an iteration
+    assert abs(st.iterations - len(tr64)) <= 1 and abs(ito - len(tr64)) <= 1
+    op.free()
+
+
+# --------------------------------------------------------------------------
+# kooshball: 128^3, 16 coils, 8192 x 256 (full trajectory and grid)
+# --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def koosh():
+    w = I.config_kooshball()
+    return w
+
+
+def test_kooshball_nufft_two_coil_share(koosh):
+    # the NUFFT on the full trajectory and grid for a 2-coil share of the
+    # coil images (one GPU's share of 16 coils over 8), against the oracle;
+    # the 16-coil operator gives the same values for those two coils
+    w = koosh
+    n = w["img"][0]
+    M = w["R"] * w["S"]
+    cim = (w["maps"][:2] * w["x"][None]).astype(np.complex64)
+    ksp = crandn((2, M), 102)
+    op = nufft_op(w, 2, toeplitz=False)
+    orc = O.Nufft(w["img"], w["traj"], 2, toeplitz=False)
+    kf = op.forward(cim).reshape(2, M)
+    ka = op.adjoint(ksp).reshape(2, n, n, n)
+    ef = relerr(kf, orc.forward(cim))
+    ea = relerr(ka, orc.adjoint(ksp))
+    op16 = nufft_op(w, 16, toeplitz=False)
+    cim16 = np.concatenate([cim, np.zeros((14, n, n, n), np.complex64)])
+    ksp16 = np.concatenate([ksp, np.zeros((14, M), np.complex64)])
+    e16f = relerr(op16.forward(cim16).reshape(16, M)[:2], kf)
+    e16a = relerr(op16.adjoint(ksp16).reshape(16, n, n, n)[:2], ka)
+    report("kooshball nufft 2 coils", forward=ef, adjoint=ea, coils16_vs_2_fwd=e16f, coils16_vs_2_adj=e16a)
+    assert ef < 1e-5 and ea < 1e-5
+    assert e16f < 1e-6 and e16a < 1e-6
+    op.free()
+    op16.free()
+
+
+def test_kooshball_psf_and_toeplitz_two_coils(koosh):
+    w = koosh
+    n = w["img"][0]
+    op = nufft_op(w, 2, toeplitz=True)
+    orc = O.Nufft(w["img"], w["traj"], 2, toeplitz=True)
+    ep = relerr(op.get_psf().reshape(orc.psf().shape), orc.psf())
+    x = crandn((2, n, n, n), 103)
+    en = relerr(op.normal(x).reshape(2, n, n, n), orc.normal(x))
+    report("kooshball psf / toeplitz 2 coils", psf=ep, normal=en)
+    assert ep < 1e-5 and en < 1e-5
+    op.free()
+
+
+def test_kooshball_sense_16_coils_and_cg(koosh):
+    # the full 16-coil SENSE model: A^H y (the CG right-hand side), the
+    # Toeplitz normal operator, and 20 CG iterations against the oracle
+    w = koosh
+    n = w["img"][0]
+    C = w["C"]
+    ref = O.Nufft(w["img"], w["traj"], C, toeplitz=True)
+    y = I.add_noise(ref.sense_forward(w["maps"], w["x"]), 1e-3, seed=w["seed"])
+    op = sense_op(w)
+    ef = relerr(op.forward(w["x"]).reshape(C, -1), ref.sense_forward(w["maps"], w["x"]))
+    b = ref.sense_adjoint(w["maps"], y)
+    ea = relerr(op.adjoint(y).reshape(n, n, n), b)
+    x = crandn((n, n, n), 104)
+    en = relerr(op.normal(x).reshape(n, n, n), ref.sense_normal(w["maps"], x))
+    report("kooshball sense 16 coils", forward=ef, adjoint=ea, toeplitz_normal=en)
+    assert ef < 1e-5 and ea < 1e-5 and en < 1e-5
+    # CG: the complex128 reference costs minutes at this size, so the bound
+    # is taken from the residual: the GPU's true residual (FP32 oracle
+    # normal operator) after 20 iterations within 10 % of the oracle CG's
+    xg, st, trg = ba.lsqr_conjgrad(op, b, w["iters"], w["lam"], 0.0)
+    xo, ito, tro, _ = ref.cgsense(w["maps"], b, w["lam"], w["iters"], 0.0)
+    xg = xg.reshape(n, n, n)
+
+    def rres(v):
+        r = b.astype(np.complex128) - ref.sense_normal(w["maps"], v) - w["lam"] * v.astype(np.complex128)
+        return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+    rg, ro = rres(xg), rres(xo)
+    report("kooshball cg 20", gpu_vs_oracle=relerr(xg, xo), rres_gpu=rg, rres_oracle=ro,
+           trace_vs_oracle=float(np.max(np.abs(trg / tro - 1))))
+    assert st.iterations == ito == w["iters"]
+    assert relerr(xg, xo) < 1e-4
+    assert rg <= 1.1 * ro
+    op.free()
+
+
+# --------------------------------------------------------------------------
+# spiral: 320^2, 32 coils, 24 x 4096
+# --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def spiral():
+    w = I.config_spiral()
+    ref = O.Nufft(w["img"], w["traj"], w["C"], toeplitz=True)
+    y = I.add_noise(ref.sense_forward(w["maps"], w["x"]), 1e-3, seed=w["seed"])
+    b = ref.sense_adjoint(w["maps"], y)
+    return w, ref, y, b
+
+
+def test_spiral_nufft(spiral):
+    w, ref, y, b = spiral
+    C = w["C"]
+    n = w["img"][0]
+    cim = (w["maps"] * w["x"][None]).reshape(C, 1, n, n)
+    op = nufft_op(w, C, toeplitz=False)
+    orc = O.Nufft(w["img"], w["traj"], C, toeplitz=False)
+    ef = relerr(op.forward(cim).reshape(C, -1), orc.forward(cim))
+    ea = relerr(op.adjoint(y).reshape(C, 1, n, n), orc.adjoint(y))
+    report("spiral nufft", forward=ef, adjoint=ea)
+    assert ef < 1e-5 and ea < 1e-5
+    op.free()
+
+
+def test_spiral_explicit_and_toeplitz_normal(spiral):
+    # SENSE normal operator as adjoint(forward(.)) (grid / gridH) and as the
+    # Toeplitz PSF operator, each against the oracle's; the two differ from
+    # each other only by the kernel approximation
+    w, ref, y, b = spiral
+    n = w["img"][0]
+    x = crandn((1, n, n), 105)
+    opt = sense_op(w, toeplitz=True)
+    ope = sense_op(w, toeplitz=False)
+    refe = O.Nufft(w["img"], w["traj"], w["C"], toeplitz=False)
+    nt = opt.normal(x).reshape(1, n, n)
+    ne = ope.normal(x).reshape(1, n, n)
+    et = relerr(nt, ref.sense_normal(w["maps"], x))
+    ee = relerr(ne, refe.sense_normal(w["maps"], x))
+    d = relerr(nt, ne)
+    ea = relerr(opt.adjoint(y).reshape(1, n, n), b)
+    report("spiral normal", toeplitz=et, explicit=ee, toeplitz_vs_explicit=d, adjoint=ea)
+    assert et < 1e-5 and ee < 1e-5 and ea < 1e-5
+    assert d < 1e-4
+    opt.free()
+    ope.free()
+
+
+def test_spiral_cgsense(spiral):
+    w, ref, y, b = spiral
+    n = w["img"][0]
+    op = sense_op(w)
+    cg_bound_check("spiral cg 10", op, ref, w["maps"], b, w["lam"], 10, (1, n, n))
+    op.free()
+
+
+# --------------------------------------------------------------------------
+# batched: 96 uneven 2-D slices in one lsqr_conjgrad_batch call
+# --------------------------------------------------------------------------
+
+def test_batched_96_slices_match_single_solves_and_oracle():
+    sl = I.batched_slices()
+    iters, lam = 30, 0.01
+    ops, bs, maps_l, trajs = [], [], [], []
+    for d in sl:
+        n, C, R, S = d["n"], d["C"], d["R"], d["S"]
+        traj = I.radial_traj(R, S)
+        maps = I.gaussian_coils((1, n, n), C, seed=d["seed"])
+        img = I.ellipse_phantom(n, seed=d["seed"])
+        op = ba.sense_nc_init(dims16(n, n, 1, C), dims16(n, n, 1, C), maps, dims16(1, R, S, C), dims16(3, R, S),
+                              ba.traj_array(traj))
+        y = I.add_noise(op.forward(img).reshape(C, -1), 1e-3, seed=d["seed"])
+        bs.append(op.adjoint(y).reshape(1, n, n))
+        ops.append(op)
+        maps_l.append(maps)
+        trajs.append(traj)
+    xs, st, tr = ba.lsqr_conjgrad_batch(ops, bs, iters, lam, 0.0)
+    for i, (op, b) in enumerate(zip(ops, bs)):
+        x1, s1, t1 = ba.lsqr_conjgrad(op, b, iters, lam, 0.0)
+        assert np.array_equal(xs[i].reshape(-1), x1.reshape(-1)), f"slice {i}"
+        assert st[i].iterations == s1.iterations == iters and np.array_equal(tr[i], t1)
+    # three slices against the oracle: the smallest, the smallest with 32
+    # coils and the smallest of matrix 384 (30 iterations with the rounding
+    # bound where the complex128 reference is affordable)
+    order = sorted(range(len(sl)), key=lambda i: sl[i]["n"] ** 2 * sl[i]["C"] * sl[i]["S"])
+    pick = [order[0], next(i for i in order if sl[i]["C"] == 32), next(i for i in order if sl[i]["n"] == 384)]
+    for i in pick:
+        d = sl[i]
+        n, C = d["n"], d["C"]
+        ref = O.Nufft((n, n, 1), trajs[i], C, toeplitz=True)
+        xn = crandn((1, n, n), 200 + i)
+        en = relerr(ops[i].normal(xn).reshape(1, n, n), ref.sense_normal(maps_l[i], xn))
+        report(f"batched slice {i} ({n}^2, {C} coils, {d['S']} spokes)", normal=en)
+        assert en < 1e-5
+        cg_bound_check(f"batched slice {i} cg 30", ops[i], ref, maps_l[i], bs[i], lam, iters, (1, n, n),
+                       fp64=n * n * C <= 256 * 256 * 16)
+    for op in ops:
+        op.free()
