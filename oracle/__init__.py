@@ -63,6 +63,8 @@ def lib():
         L.orc_nufft_subgrids.restype = ctypes.c_int
         L.orc_nufft_subgrids.argtypes = [vp]
         L.orc_nufft_compute_psf.argtypes = [c_long3, ctypes.c_long, vp, vp, vp]
+        L.orc_psf_grid_fp64.argtypes = [c_long3, ctypes.c_long, vp, vp, ctypes.c_int, vp]
+        L.orc_nufft_grid_fp64.argtypes = [vp, ctypes.c_int, vp, vp]
         L.orc_sense_forward.argtypes = [vp, vp, vp, vp]
         L.orc_sense_adjoint.argtypes = [vp, vp, vp, vp]
         L.orc_sense_normal.argtypes = [vp, vp, vp, vp]
@@ -146,6 +148,53 @@ def compute_psf(dims, traj, weights=None):
     w = _c64(weights).reshape(-1) if weights is not None else None
     out = np.zeros([S] + d[::-1], np.complex64)
     lib().orc_nufft_compute_psf(_dims3(d), tr.shape[0], _ptr(tr), _ptr(w), _ptr(out))
+    return out
+
+
+def compute_psf_fp64(dims, traj, weights=None):
+    """The PSF of :func:`compute_psf` (noncart/nufft.c:727-776 with
+    :535-586) evaluated in complex128: gridding with double products and
+    sums (``orc_psf_grid_fp64``), then conj(fftmod), unnormalised inverse
+    FFT, conj(linphase_s) summed over the sub-grids, rolloff, and the
+    centred unitary FFT and decomposition, all in complex128 on the FP32
+    tables taken as exact.  It differs from the FP32 PSF only by the FP32
+    rounding, which grows with the number of samples gridded onto one
+    point (the dense centre of radial trajectories).  Returns [S, Z, Y, X]
+    complex128."""
+    import scipy.fft as sf
+    d = list(dims) + [1] * (3 - len(dims))
+    d2 = [1 if v == 1 else 2 * v for v in d]
+    tr = _c64(traj).reshape(-1, 3)
+    w = _c64(weights).reshape(-1) if weights is not None else None
+    M = tr.shape[0]
+    S2 = 2 ** sum(1 for v in d2 if v > 1)
+    N2 = int(np.prod(d2))
+    axes = tuple(a for a in range(3) if d2[::-1][a] > 1)
+    beta = float(np.float32(calc_beta(2.0, 6.0)))
+    fm = fftmod_table(d2).astype(np.complex128)
+    acc = np.zeros(d2[::-1], np.complex128)
+    g = np.zeros(d2[::-1], np.complex128)
+    for s in range(S2):
+        lib().orc_psf_grid_fp64(_dims3(d), M, _ptr(tr), _ptr(w), s, _ptr(g))
+        v = sf.ifftn(g * np.conj(fm), axes=axes, workers=-1) * N2
+        v *= np.conj(linphase_table(d2, s).astype(np.complex128))
+        acc += v
+    del g
+    acc *= rolloff(d2, 2.0, 6.0, beta).astype(np.complex128)
+    acc *= fm
+    acc = sf.fftn(acc, axes=axes, overwrite_x=True, workers=-1)
+    acc *= fm / np.sqrt(N2)
+    # md_decompose: psf[s][j] = psft[2 j + r(s)] (r(s) bit i = shift of dim i)
+    S = 2 ** sum(1 for v in d if v > 1)
+    out = np.zeros((S,) + tuple(d[::-1]), np.complex128)
+    for s in range(S):
+        r, idx = [], s
+        for v in d:
+            f = 2 if v > 1 else 1
+            r.append(idx % f)
+            idx //= f
+        sl = tuple(slice(r[i], None, 2) if d[i] > 1 else slice(None) for i in (2, 1, 0))
+        out[s] = acc[sl]
     return out
 
 
@@ -241,12 +290,65 @@ class Nufft:
         """Kaiser-Bessel beta of the operator's table and rolloff."""
         return lib().orc_nufft_beta(self.h)
 
-    def sense_normal_fp64(self, maps, img):
+    def adjoint_fp64(self, ksp):
+        """The adjoint NUFFT (noncart/nufft.c:1462-1540) in complex128:
+        gridding with double products and sums (``orc_nufft_grid_fp64``),
+        then conj(fftmod), the unnormalised inverse FFT and conj(linphase_s)
+        summed over the sub-grids (times the rolloff of a Toeplitz
+        operator, whose linear phases do not hold it), on the FP32 tables
+        taken as exact.  It differs from :meth:`adjoint` only by the FP32
+        rounding, which grows with the number of samples gridded onto one
+        point (the dense centre of radial trajectories).  Returns
+        [C, Z, Y, X] complex128."""
+        import scipy.fft as sf
+        ksp = _c64(ksp).reshape(self.C, self.M)
+        lph = self.linphase()
+        fm = fftmod_table(self.dims).astype(np.complex128)
+        axes = tuple(a for a in (1, 2, 3) if self.cim_shape[a] > 1)
+        acc = np.zeros(self.cim_shape, np.complex128)
+        g = np.zeros(self.cim_shape, np.complex128)
+        for s in range(self.S):
+            lib().orc_nufft_grid_fp64(self.h, s, _ptr(ksp), _ptr(g))
+            v = sf.ifftn(g * np.conj(fm)[None], axes=axes, workers=-1) * self.N
+            v *= np.conj(lph[s].astype(np.complex128))[None]
+            acc += v
+        if self.toeplitz:
+            acc *= rolloff(self.dims, 2.0, 6.0, self.beta).astype(np.complex128)[None]
+        return acc
+
+    def sense_adjoint_fp64(self, maps, ksp):
+        """S^H of :meth:`adjoint_fp64`: sum over coils of conj(map) times the
+        coil image, in complex128."""
+        m = np.asarray(maps).astype(np.complex128).reshape(self.cim_shape)
+        return np.sum(np.conj(m) * self.adjoint_fp64(ksp), axis=0)
+
+    def normal_fp64(self, cim, psf=None):
+        """The Toeplitz normal operator of the coil images (no maps) in
+        complex128: per coil sum_s conj(L_s) IFFT(P_s FFT(L_s x))
+        (noncart/nufft.c:1546-1625) on this operator's linear phases taken
+        as exact, with this operator's PSF or the one given (for example
+        :func:`compute_psf_fp64`)."""
+        import scipy.fft as sf
+        assert self.toeplitz
+        lph = self.linphase().astype(np.complex128)
+        P = self.psf().astype(np.complex128) if psf is None else np.asarray(psf, np.complex128)
+        x = np.asarray(cim).astype(np.complex128).reshape(self.cim_shape)
+        axes = tuple(a for a in (1, 2, 3) if self.cim_shape[a] > 1)
+        acc = np.zeros(self.cim_shape, np.complex128)
+        for s in range(self.S):
+            g = sf.fftn(x * lph[s][None], axes=axes, workers=-1)
+            g *= P[s][None]
+            g = sf.ifftn(g, axes=axes, workers=-1, overwrite_x=True) * self.N
+            acc += g * np.conj(lph[s])[None]
+        return acc
+
+    def sense_normal_fp64(self, maps, img, psf=None):
         """S^H T S in complex128: the Toeplitz normal operator
         (noncart/nufft.c:1546-1625, sum_s conj(L_s) IFFT(P_s FFT(L_s x)) per
         coil, then the coil sum of linops/fmac.c) evaluated in double
         precision on this operator's own tables (linear phases, PSF) and the
-        maps, each taken as exact.  The reference for the FP32 rounding of
+        maps, each taken as exact (or with the PSF given, for example
+        :func:`compute_psf_fp64`).  The reference for the FP32 rounding of
         CG: an FP32 solver's deviation from CG on this operator is the
         rounding it accumulates."""
         import scipy.fft as sf
@@ -254,6 +356,7 @@ class Nufft:
         if not hasattr(self, "_lph64"):
             self._lph64 = self.linphase().astype(np.complex128)
             self._psf64 = self.psf().astype(np.complex128)
+        P = self._psf64 if psf is None else np.asarray(psf, np.complex128)
         m = np.asarray(maps).astype(np.complex128).reshape(self.cim_shape)
         x = np.asarray(img).astype(np.complex128).reshape(self.dims[::-1])
         axes = tuple(a for a in (1, 2, 3) if self.cim_shape[a] > 1)
@@ -261,7 +364,7 @@ class Nufft:
         acc = np.zeros(self.cim_shape, np.complex128)
         for s in range(self.S):
             g = sf.fftn(cim * self._lph64[s][None], axes=axes, workers=-1)
-            g *= self._psf64[s][None]
+            g *= P[s][None]
             g = sf.ifftn(g, axes=axes, workers=-1) * self.N
             acc += g * np.conj(self._lph64[s])[None]
         return np.sum(np.conj(m) * acc, axis=0)

@@ -230,8 +230,9 @@ static void sample_pos(const struct orc_grid_conf* conf, const long gdims[3], co
  * reach it (in sample order), and a thread adds a slab's samples, taps
  * inside the slab only.  Every grid point then receives its contributions
  * in sample order, whatever the thread count. */
-void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, const cf* traj,
-		const long gdims[3], int C, cf* grid, long gstride_c, const cf* ksp, long kstride_c)
+static void grid_core(const struct orc_grid_conf* conf, const float* table, long M, const cf* traj,
+		const long gdims[3], int C, cf* grid, double complex* grid64, long gstride_c,
+		const cf* ksp, long kstride_c)
 {
 	int D = 2;
 
@@ -341,6 +342,24 @@ void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, cons
 						float d = dy * t[0].w[ix];
 						long ind = t[0].idx[ix] + gdims[0] * (t[1].idx[iy] + gdims[1] * t[2].idx[iz]);
 
+						if (NULL != grid64) {
+
+							/* products and sums in double: the
+							 * reference without FP32 rounding */
+							double d64 = (double)t[2].w[iz] * (double)t[1].w[iy] * (double)t[0].w[ix];
+
+							for (int c = 0; c < C; c++) {
+
+								cf v = ksp[m + c * kstride_c];
+								double* dst = (double*)&grid64[ind + c * gstride_c];
+
+								dst[0] += (double)crealf(v) * d64;
+								dst[1] += (double)cimagf(v) * d64;
+							}
+
+							continue;
+						}
+
 						for (int c = 0; c < C; c++) {
 
 							cf v = ksp[m + c * kstride_c];
@@ -361,6 +380,12 @@ void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, cons
 	free(nsl);
 	free(cnt);
 	free(slab_of);
+}
+
+void orc_grid(const struct orc_grid_conf* conf, const float* table, long M, const cf* traj,
+		const long gdims[3], int C, cf* grid, long gstride_c, const cf* ksp, long kstride_c)
+{
+	grid_core(conf, table, M, traj, gdims, C, grid, NULL, gstride_c, ksp, kstride_c);
 }
 
 /* Forward gridding ("gridH" in BART, grid -> k-space, gather):
@@ -1079,6 +1104,82 @@ void orc_nufft_compute_psf(const long dims[3], long M, const cf* traj, const cf*
 	free(sqw);
 	free(traj2);
 	orc_nufft_free(nf2);
+}
+
+/* The gridding step of the PSF above (the adjoint NUFFT of |w|^2 on the 2x
+ * image with 2*traj, periodic, sub-grid s of its decomposition) with the
+ * kernel products and the sums in double: the FP64 reference for the
+ * rounding of the PSF's accumulation (not a BART function; the remaining
+ * steps of compute_psf2 are applied in complex128 by the caller).  grid is
+ * [dims2] complex128, zeroed here. */
+void orc_psf_grid_fp64(const long dims[3], long M, const cf* traj, const cf* weights, int s,
+		double complex* grid)
+{
+	long dims2[3];
+
+	for (int i = 0; i < 3; i++)
+		dims2[i] = (1 == dims[i]) ? 1 : 2 * dims[i];
+
+	long N2 = dims2[0] * dims2[1] * dims2[2];
+
+	cf* traj2 = malloc(sizeof(cf) * 3 * (size_t)M);
+
+	for (long i = 0; i < 3 * M; i++)
+		traj2[i] = traj[i] * 2.f;
+
+	cf* val = malloc(sizeof(cf) * (size_t)M);
+
+	for (long m = 0; m < M; m++)
+		val[m] = (NULL != weights) ? weights[m] * conjf(weights[m]) : 1.f;
+
+	float table[KB_SIZE + 1];
+	orc_kb_table((double)(float)orc_calc_beta(2.f, 6.f), table);
+
+	/* decomp_conf of the 2x operator: width 6 / 2, os 1, periodic */
+	struct orc_grid_conf g = { .os = 1., .width = 3., .periodic = true };
+
+	decomp_shift(dims2, s, g.shift);
+
+	for (int i = 0; i < 3; i++)
+		if (dims2[i] > 1)
+			g.shift[i] += (dims2[i] / 2.0 - dims2[i] / 2) / g.os;
+
+	memset(grid, 0, sizeof(double complex) * (size_t)N2);
+	grid_core(&g, table, M, traj2, dims2, 1, NULL, grid, N2, val, M);
+
+	free(val);
+	free(traj2);
+}
+
+/* The gridding step of the adjoint above for sub-grid s (k-space times
+ * conj(weights), noncart/nufft.c:1462-1540) with the kernel products and
+ * the sums in double: the FP64 reference for the rounding of the adjoint's
+ * accumulation (not a BART function; the caller applies conj(fftmod), the
+ * FFT and the linear phases in complex128).  grid is [C][N] complex128,
+ * zeroed here. */
+void orc_nufft_grid_fp64(const orc_nufft_t* nf, int s, const cf* ksp, double complex* grid)
+{
+	long M = nf->M;
+	int C = nf->C;
+	cf* wk = NULL;
+
+	if (NULL != nf->weights) {
+
+		wk = malloc(sizeof(cf) * (size_t)(M * C));
+
+		for (int c = 0; c < C; c++)
+			for (long m = 0; m < M; m++)
+				wk[m + c * M] = ksp[m + c * M] * conjf(nf->weights[m]);
+
+		ksp = wk;
+	}
+
+	memset(grid, 0, sizeof(double complex) * (size_t)C * (size_t)nf->N);
+
+	struct orc_grid_conf g = decomp_conf(nf, s);
+	grid_core(&g, nf->table, M, nf->traj, nf->dims, C, NULL, grid, nf->N, ksp, M);
+
+	free(wk);
 }
 
 void orc_nufft_get_psf(const orc_nufft_t* nf, cf* psf)

@@ -311,3 +311,36 @@ def test_cg_fp64_matches_direct_solve_and_oracle():
     # stop rule: ||r|| <= tol ||b||
     x6, tr6 = O.cg_fp64(normal, b, lam, 300, 1e-4)
     assert tr6[-1] <= 1e-4 * np.linalg.norm(b) and (len(tr6) == 1 or tr6[-2] > 1e-4 * np.linalg.norm(b))
+
+
+def test_psf_fp64_is_the_fp32_psf_without_rounding():
+    # compute_psf_fp64 is compute_psf in complex128: equal to the FP32 PSF up
+    # to rounding (2-D, 3-D, weights), and to the exact PSF up to the KB
+    # approximation, as the FP32 one
+    w = (0.3 + np.random.default_rng(1).random(7 * 32) + 0.1j).astype(np.complex64)
+    cases = [((16, 16, 1), I.radial_traj(32, 7), None), ((16, 16, 1), I.radial_traj(32, 7), w),
+             ((16, 16, 16), I.phyllotaxis_kooshball(spokes=300, readout=32), None)]
+    for dims, traj, wt in cases:
+        p64 = O.compute_psf_fp64(dims, traj, wt)
+        assert relerr(O.compute_psf(dims, traj, wt), p64) < 1e-6
+    n = 16
+    traj = I.radial_traj(32, 7)
+    m = 2 * n
+    a = O.nudft_adjoint((m, m, 1), 2 * np.real(traj).astype(np.float64), np.ones((1, traj.shape[0])))[0, 0]
+    F = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(a))) / m
+    ref = np.stack([F[r1::2, r0::2] for r1 in (0, 1) for r0 in (0, 1)])
+    assert relerr(O.compute_psf_fp64((n, n, 1), traj)[:, 0], ref) < 2e-5
+
+
+def test_normal_fp64_of_coil_images():
+    # the complex128 Toeplitz operator of coil images: the FP32 one without
+    # rounding, the exact A^H A up to the KB approximation
+    n, C = 32, 2
+    traj = I.radial_traj(64, 23)
+    nf = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    x = crandn((C, 1, n, n), 45)
+    a64 = nf.normal_fp64(x)
+    assert relerr(nf.normal(x), a64) < 1e-6
+    exact = O.nudft_adjoint((n, n, 1), traj, O.nudft_forward((n, n, 1), traj, x))
+    assert relerr(a64, exact) < 1e-5
+    assert relerr(nf.normal_fp64(x, psf=O.compute_psf_fp64((n, n, 1), traj)), a64) < 1e-6
