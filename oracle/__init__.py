@@ -382,13 +382,15 @@ class Nufft:
         return x, it, trace[:it], float(fin[0])
 
 
-def cg_fp64(normal, b, lam, maxiter, tol=0.0, x0=None):
+def cg_fp64(normal, b, lam, maxiter, tol=0.0, x0=None, snapshots=None):
     """CG on (A^H A + lam I) x = b in complex128, step by step as BART's
     conjgrad (iter/italgos.c:497-580): the inner products of the float
     vectors are Re<u, v> (num/vecops.c:89-124), stop when
     ||r||^2 <= (tol ||b||)^2 (iter/iter2.c:227-270) or p^H A p = 0.
     normal: function image -> A^H A image (complex128).  Returns (x, trace
-    of ||r|| after each iteration)."""
+    of ||r|| after each iteration); with snapshots (a dict whose keys are
+    iteration counts) the iterate after each of those iterations is stored
+    under its key (0: the start)."""
     b = np.asarray(b, np.complex128)
     shape = b.shape
     b = b.ravel()
@@ -398,6 +400,8 @@ def cg_fp64(normal, b, lam, maxiter, tol=0.0, x0=None):
     p = r.copy()
     rsold = float(np.vdot(r, r).real)
     trace = []
+    if snapshots is not None and 0 in snapshots:
+        snapshots[0] = x.reshape(shape).copy()
     if rsold == 0.0:
         return x.reshape(shape), np.array(trace)
     for _ in range(maxiter):
@@ -410,6 +414,8 @@ def cg_fp64(normal, b, lam, maxiter, tol=0.0, x0=None):
         r -= alpha * Ap
         rsnew = float(np.vdot(r, r).real)
         trace.append(np.sqrt(rsnew))
+        if snapshots is not None and len(trace) in snapshots:
+            snapshots[len(trace)] = x.reshape(shape).copy()
         if rsnew <= eps2:
             break
         p = r + (rsnew / rsold) * p
