@@ -155,6 +155,9 @@ def lib():
     L.lsqr_conjgrad.restype = ctypes.c_int
     L.lsqr_conjgrad.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, lp, vp, vp,
                                 ctypes.c_bool, ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
+    L.lsqr_conjgrad_noalloc.restype = ctypes.c_int
+    L.lsqr_conjgrad_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, lp, vp, vp,
+                                        ctypes.c_bool, ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
     L.lsqr_conjgrad_batch.restype = ctypes.c_int
     L.lsqr_conjgrad_batch.argtypes = [ctypes.c_int, ctypes.POINTER(vp), ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.c_bool,
@@ -455,6 +458,23 @@ def lsqr_conjgrad(op, b, maxiter, lam=0.0, tol=0.0, x0=None, fallback=False):
     st = CgStats()
     it = lib().lsqr_conjgrad(op.h, maxiter, lam, tol, DIMS, _dims(op.idims), _ptr(x), _ptr(b), warm, fallback,
                              trace.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
+    _check()
+    return x, st, trace[:max(0, it)]
+
+
+def lsqr_conjgrad_noalloc(op, b, maxiter, lam=0.0, tol=0.0, x0=None, fallback=False):
+    """lsqr_conjgrad in the library's static working memory (no device
+    allocation by the library): b and x0 must be CUDA tensors.  Returns
+    (x, stats, residual trace)."""
+    assert _is_torch(b) and b.is_cuda, "lsqr_conjgrad_noalloc takes device tensors"
+    warm = x0 is not None
+    x = x0.clone() if warm else _like(b, op.idims)
+    import torch
+    torch.cuda.synchronize()
+    trace = np.zeros(max(1, maxiter), np.float32)
+    st = CgStats()
+    it = lib().lsqr_conjgrad_noalloc(op.h, maxiter, lam, tol, DIMS, _dims(op.idims), _ptr(x), _ptr(b), warm,
+                                     fallback, trace.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
     _check()
     return x, st, trace[:max(0, it)]
 

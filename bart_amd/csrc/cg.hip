@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace bam {
 
@@ -602,6 +603,168 @@ CgResult CgBatch::result(int i) const
 		BAM_HIP(hipMemcpy(res.trace.data(), trace.ptr + (long)i * trace_len, sizeof(float) * res.iterations,
 				  hipMemcpyDeviceToHost));
 	}
+
+	return res;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free single solve
+// ---------------------------------------------------------------------------
+
+// Working memory of cg_solve_noalloc -- r, p and A p of one problem of at most
+// CG_NOALLOC_ELEMS complex elements, the reduction partials, the state, the
+// segment tables and the residual trace -- in one array of this library.
+// Every GPU the library runs on holds its own copy (48 MiB + 64 KiB) until the
+// process ends; every allocation-free solve on that GPU shares it, so such
+// calls that overlap on one GPU race.
+constexpr size_t NA_VEC = (size_t)CG_NOALLOC_ELEMS * sizeof(float2);
+constexpr size_t NA_PART = NA_VEC * 3;                                 // double[CG_BLOCKS]
+constexpr size_t NA_PART_B = NA_PART + CG_BLOCKS * sizeof(double);     // double[CG_BLOCKS]
+constexpr size_t NA_ST = NA_PART_B + CG_BLOCKS * sizeof(double);       // CgState
+constexpr size_t NA_SEG = NA_ST + 256;                                 // CgSeg
+constexpr size_t NA_BPROB = NA_SEG + 256;                              // int[CG_BLOCKS]
+constexpr size_t NA_TRACE = NA_BPROB + CG_BLOCKS * sizeof(int);        // float[CG_NOALLOC_TRACE]
+constexpr size_t NA_BYTES = NA_VEC * 3 + 65536;
+static_assert(NA_TRACE + CG_NOALLOC_TRACE * sizeof(float) <= NA_BYTES, "allocation-free CG layout");
+
+static __device__ __align__(256) char scratch[NA_BYTES];
+
+// device address of this GPU's copy of scratch (the host-side name is a
+// stand-in, not a device address), looked up once per device
+static std::mutex scratch_mtx;
+static void* scratch_dev[64] = {};
+
+static char* scratch_base()
+{
+	int dev = 0;
+	BAM_HIP(hipGetDevice(&dev));
+	BAM_CHECK(dev >= 0 && dev < 64, "cg: device index %d out of range", dev);
+	std::lock_guard<std::mutex> lock(scratch_mtx);
+
+	if (nullptr == scratch_dev[dev]) {
+
+		void* p = nullptr;
+		BAM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+		BAM_CHECK(nullptr != p, "cg: no device address for the solver's working memory");
+		scratch_dev[dev] = p;
+	}
+
+	return (char*)scratch_dev[dev];
+}
+
+// the segment table of one problem of len floats over nblk work-groups
+__global__ void k_cg_single_tables(CgSeg* seg, int* bprob, long len, int nblk)
+{
+	if (0 == threadIdx.x && 0 == blockIdx.x) {
+		seg->off = 0;
+		seg->len = len;
+		seg->blk0 = 0;
+		seg->nblk = nblk;
+	}
+
+	for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+		bprob[i] = 0;
+}
+
+CgResult cg_solve_noalloc(const NormalFn& normal, float lambda, int maxiter, float tol, float2* x,
+			  const float2* b, long nfloats, bool warm, bool fallback, hipStream_t stream,
+			  float* trace_host)
+{
+	BAM_CHECK(nfloats >= 0 && nfloats <= 2 * CG_NOALLOC_ELEMS,
+		  "lsqr_conjgrad_noalloc: image of %ld elements exceeds the maximum of %ld", nfloats / 2,
+		  CG_NOALLOC_ELEMS);
+	BAM_CHECK(maxiter <= CG_NOALLOC_TRACE, "lsqr_conjgrad_noalloc: %d iterations exceed the maximum of %d",
+		  maxiter, CG_NOALLOC_TRACE);
+
+	char* base = scratch_base();
+	float* r = (float*)base;
+	float* p = (float*)(base + NA_VEC);
+	float* Ap = (float*)(base + 2 * NA_VEC);
+	double* part = (double*)(base + NA_PART);
+	double* part_b = (double*)(base + NA_PART_B);
+	CgState* st = (CgState*)(base + NA_ST);
+	CgSeg* seg = (CgSeg*)(base + NA_SEG);
+	int* bprob = (int*)(base + NA_BPROB);
+	float* trace = (float*)(base + NA_TRACE);
+	float* xf = (float*)x;
+	const float* bf = (const float*)b;
+
+	// the blocking of a batch of one (CgBatch), so that the results equal
+	// those of CgSolver bit for bit
+	const int nblk = (int)std::min<long>(CG_BLOCKS, std::max<long>(1, (nfloats + CG_THREADS - 1) / CG_THREADS));
+
+	hipLaunchKernelGGL(k_cg_single_tables, dim3(1), dim3(CG_THREADS), 0, stream, seg, bprob, nfloats, nblk);
+
+	auto init = [&](bool cold) {
+		hipLaunchKernelGGL(k_cg_init, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, lambda, xf, bf, r, p,
+				   part, part_b, cold, (const int*)nullptr);
+		hipLaunchKernelGGL(k_cg_init_fin, dim3(1), dim3(CG_THREADS), 0, stream, seg, part, part_b, tol, maxiter,
+				   st, (const int*)nullptr);
+		BAM_HIP(hipGetLastError());
+	};
+
+	CgResult res;
+	CgState hs;
+
+	if (warm) {
+		normal((float2*)r, x, nullptr);
+		init(false);
+	} else {
+		fill(x, nfloats / 2, make_float2(0.f, 0.f), stream);
+		init(true);
+	}
+
+	if (warm && fallback) {
+
+		BAM_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+		BAM_HIP(hipStreamSynchronize(stream));
+
+		if (hs.r0norm > hs.bnorm) {
+			fill(x, nfloats / 2, make_float2(0.f, 0.f), stream);
+			init(true);
+			res.fell_back = true;
+		}
+	}
+
+	// iterations in chunks; the stop flag is read after each chunk (the
+	// kernels of a stopped problem return at once)
+	for (int done = 0; done < maxiter;) {
+
+		const int nl = std::min(CG_CHUNK, maxiter - done);
+
+		for (int i = 0; i < nl; i++) {
+			normal((float2*)Ap, (const float2*)p, &st->done);
+			hipLaunchKernelGGL(k_cg_apdot, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, lambda, p, Ap,
+					   part, (const CgState*)st);
+			hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(CG_THREADS), 0, stream, seg, part, st);
+			hipLaunchKernelGGL(k_cg_xr, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, xf, r, p, Ap, part,
+					   (const CgState*)st);
+			hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(CG_THREADS), 0, stream, seg, part, st, trace,
+					   CG_NOALLOC_TRACE);
+			hipLaunchKernelGGL(k_cg_p, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, p, r,
+					   (const CgState*)st);
+			BAM_HIP(hipGetLastError());
+		}
+
+		done += nl;
+
+		BAM_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+		BAM_HIP(hipStreamSynchronize(stream));
+
+		if (0 != hs.done)
+			break;
+	}
+
+	BAM_HIP(hipMemcpyAsync(&hs, st, sizeof(CgState), hipMemcpyDeviceToHost, stream));
+	BAM_HIP(hipStreamSynchronize(stream));
+
+	res.iterations = hs.iter;
+	res.resid = sqrtf(hs.rsnew);
+	res.bnorm = hs.bnorm;
+	res.status = hs.done;
+
+	if (nullptr != trace_host && res.iterations > 0)
+		BAM_HIP(hipMemcpy(trace_host, trace, sizeof(float) * res.iterations, hipMemcpyDeviceToHost));
 
 	return res;
 }

@@ -126,6 +126,19 @@ private:
 void iterate_batches(const std::vector<CgBatch*>& bs, const std::vector<const BatchNormalFn*>& fns, float lambda,
 		     int maxiter);
 
+// Largest problem (complex elements) and iteration count of cg_solve_noalloc
+constexpr long CG_NOALLOC_ELEMS = 1L << 21;
+constexpr int CG_NOALLOC_TRACE = 4096;
+
+// One problem solved in the library's static working memory: no device
+// memory is allocated (no graph either; the iterations are launched one by
+// one, in chunks after which the stop flag is read).  x (warm start in, the
+// solution out) and b are device pointers of nfloats floats; trace_host (may
+// be null) receives the residual norms.  Calls that overlap on one GPU race.
+CgResult cg_solve_noalloc(const NormalFn& normal, float lambda, int maxiter, float tol, float2* x,
+			  const float2* b, long nfloats, bool warm, bool fallback, hipStream_t stream,
+			  float* trace_host);
+
 // Single problem with its solver state kept across calls (graph reuse).
 struct CgSolver {
 

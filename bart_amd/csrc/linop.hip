@@ -1072,6 +1072,43 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 	}, -1);
 }
 
+int lsqr_conjgrad_noalloc(const struct linop_s* _op, int maxiter, float l2lambda, float tol, int N,
+			  const long img_dims[], float _Complex* x, const float _Complex* b, bool warmstart,
+			  bool warm_fallback, float* resid_trace, struct cg_stats_s* stats)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "lsqr_conjgrad_noalloc: NULL operator");
+		check_dims("lsqr_conjgrad_noalloc", N, img_dims, op->N, op->idims);
+		BAM_CHECK(!op->toeplitz() || op->nufft->has_psf, "lsqr_conjgrad_noalloc: Toeplitz operator without PSF");
+		BAM_CHECK(op->toeplitz() || (long)op->kbuf.n >= op->osize,
+			  "lsqr_conjgrad_noalloc: the normal operator adjoint(forward(x)) needs its k-space buffer "
+			  "(create the operator with conf.toeplitz, or call lsqr_conjgrad on it once)");
+		BAM_CHECK(on_device(x) && on_device(b), "lsqr_conjgrad_noalloc: x and b must be device pointers");
+
+		std::lock_guard<std::mutex> lock(op->mtx());
+
+		// x and b may have been written on the caller's stream
+		if (0 != op->user_stream)
+			BAM_HIP(hipStreamSynchronize(op->user_stream));
+
+		NormalFn normal = [op](float2* d, const float2* s, const int* skip) { op->nrm(d, s, skip); };
+		CgResult r = cg_solve_noalloc(normal, l2lambda, maxiter, tol, (float2*)x, (const float2*)b, 2 * op->isize,
+					      warmstart, warm_fallback, op->stream, resid_trace);
+
+		if (nullptr != stats) {
+			stats->iterations = r.iterations;
+			stats->status = r.status;
+			stats->resid = r.resid;
+			stats->bnorm = r.bnorm;
+			stats->time_ms = 0.f;
+			stats->fell_back = r.fell_back;
+		}
+
+		return r.iterations;
+	}, -1);
+}
+
 // Streams the problems of a batched solve are spread over; the device has
 // four hardware queues per process by default.
 constexpr int CG_BATCH_STREAMS = 4;

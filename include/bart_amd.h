@@ -184,6 +184,22 @@ int lsqr_conjgrad(const struct linop_s* op, int maxiter, float l2lambda, float t
 		  BAM_COMPLEX* x, const BAM_COMPLEX* b, bool warmstart, bool warm_fallback, float* resid_trace,
 		  struct cg_stats_s* stats);
 
+/* lsqr_conjgrad without any device allocation: no counterpart in BART.  For
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The same solve, with the same
+ * results bit for bit, but r, p, A p and the solver state live in a static
+ * working array of the library (one per GPU, 48 MiB + 64 KiB, held until the
+ * process ends), the iterations are launched one by one (no graph) and the
+ * stop flag is read after every 8.  Limits: at most 2^21 complex image
+ * elements (e.g. 128^3 or 1448^2) and 4096 iterations -- larger requests
+ * fail with a message; x and b must be device pointers; an operator without
+ * conf.toeplitz needs its k-space buffer from an earlier lsqr_conjgrad call.
+ * Every call on a GPU shares that GPU's working array: calls that overlap on
+ * one GPU race. */
+int lsqr_conjgrad_noalloc(const struct linop_s* op, int maxiter, float l2lambda, float tol, int N,
+			  const long img_dims[], BAM_COMPLEX* x, const BAM_COMPLEX* b, bool warmstart,
+			  bool warm_fallback, float* resid_trace, struct cg_stats_s* stats);
+
 /* Batched CG: nprob independent problems (A_i^H A_i + l2lambda I) x_i = b_i,
  * each with its own operator ops[i] (image size, coils and trajectory may
  * differ), solved together on the device.  Replaces BART's tool-level loop
