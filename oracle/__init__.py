@@ -382,6 +382,34 @@ class Nufft:
         return x, it, trace[:it], float(fin[0])
 
 
+def precond_strang(nf):
+    """Strang's circulant preconditioner of the Toeplitz operator nf
+    (noncart/precond.c:46-71), complex128 on nf's FP32 PSF and linear phases:
+    pre = |fftuc(sum_s ifftuc(psf_s) conj(linphase_s))| + 1e-3 over the
+    image dims.  Returns (pre [Z, Y, X] real, apply) with apply(x) =
+    IFFT(FFT(x) / pre) per coil, unnormalised FFTs (precond.c:92-106)."""
+    assert nf.toeplitz
+    d = nf.dims
+    axes = tuple(a for a in range(-3, 0) if d[::-1][a] > 1)
+    fm = fftmod_table(d).astype(np.complex128)
+    N = float(np.prod(d))
+    psf = nf.psf().astype(np.complex128)
+    lph = nf.linphase().astype(np.complex128)
+    acc = np.zeros(d[::-1], np.complex128)
+    for s in range(nf.S):
+        t = np.fft.ifftn(np.conj(fm) * psf[s], axes=axes) * N     # unnormalised inverse FFT
+        acc += (np.conj(fm) * t / np.sqrt(N)) * np.conj(lph[s])
+    pre = np.abs(fm * np.fft.fftn(fm * acc, axes=axes) / np.sqrt(N)) + 1e-3
+
+    def apply(x):
+        x = np.asarray(x, np.complex128)
+        ax = tuple(range(x.ndim - 3, x.ndim))
+        ax = tuple(a for a in ax if x.shape[a] > 1)
+        return np.fft.ifftn(np.fft.fftn(x, axes=ax) / pre, axes=ax) * N
+
+    return pre, apply
+
+
 def cg_fp64(normal, b, lam, maxiter, tol=0.0, x0=None, snapshots=None):
     """CG on (A^H A + lam I) x = b in complex128, step by step as BART's
     conjgrad (iter/italgos.c:497-580): the inner products of the float

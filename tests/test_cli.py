@@ -45,6 +45,37 @@ def test_traj_kooshball_points_on_sphere(tmp_path):
     assert np.isclose(z[0], 1.0, atol=1e-6) and np.all(np.diff(z) < 0) and z[-1] > 0
 
 
+def test_traj_invariants_of_the_reference_tests(tmp_path):
+    # tests/traj.mk test-traj-over: -x128 -r scaled by 0.5 equals -x64 -o2 -r
+    run_bart(["traj", "-x", 128, "-y", 71, "-r", "a"], tmp_path)
+    run_bart(["traj", "-x", 64, "-y", 71, "-o", 2.0, "-r", "b"], tmp_path)
+    _, a = read_raw(str(tmp_path / "a"))
+    _, b = read_raw(str(tmp_path / "b"))
+    assert np.linalg.norm(0.5 * a - b) <= 1e-7 * np.linalg.norm(b)
+    # test-traj-3D: the first sample of every kooshball spoke lies at radius 63.5
+    run_bart(["traj", "-3", "-x", 128, "-y", 128, "-r", "k"], tmp_path)
+    _, k = read_raw(str(tmp_path / "k"))
+    k = k.real.reshape(128, 128, 3)
+    r0 = np.linalg.norm(k[:, 0, :].astype(np.float64), axis=-1)
+    assert np.linalg.norm(r0 - 63.5) <= 1e-7 * np.linalg.norm(np.full(128, 63.5))
+
+
+def test_traj_golden_angle_increment(tmp_path):
+    # -G: consecutive spokes turn by the golden angle pi / phi (phi the golden
+    # ratio, 111.246 degrees; modulo pi, as a spoke through the centre has no
+    # direction), -H by the half-circle golden angle; -r alone: pi / spokes
+    phi = (1 + np.sqrt(5.0)) / 2
+    for flags, inc in ((["-G"], np.pi / phi), ([], np.pi / 21)):
+        run_bart(["traj", "-r", *flags, "-x", 64, "-y", 21, "g"], tmp_path)
+        _, t = read_raw(str(tmp_path / "g"))
+        k = t.real.reshape(21, 64, 3).astype(np.float64)
+        ang = np.arctan2(k[:, -1, 1], k[:, -1, 0])
+        # (the turning direction is BART's: negative)
+        d = np.mod(np.diff(ang) + inc, np.pi)
+        d = np.minimum(d, np.pi - d)
+        assert np.max(d) < 1e-5, (flags, np.max(d))
+
+
 def test_cfl_and_ra_roundtrip(tmp_path):
     rng = np.random.default_rng(0)
     a = (rng.standard_normal((5, 4, 3)) + 1j * rng.standard_normal((5, 4, 3))).astype(np.complex64)

@@ -344,3 +344,28 @@ def test_normal_fp64_of_coil_images():
     exact = O.nudft_adjoint((n, n, 1), traj, O.nudft_forward((n, n, 1), traj, x))
     assert relerr(a64, exact) < 1e-5
     assert relerr(nf.normal_fp64(x, psf=O.compute_psf_fp64((n, n, 1), traj)), a64) < 1e-6
+
+
+def test_strang_preconditioner_restatement():
+    # noncart/precond.c:46-106.  For full Cartesian sampling the Toeplitz
+    # operator is the identity and the preconditioner flat; it is Hermitian
+    # and positive; and it is Strang's circulant approximation: pre - 1e-3
+    # is the magnitude of the (uncentred, as it is applied) DFT of the
+    # circulant whose kernel is the Toeplitz kernel, the centre column of the
+    # exact A^H A from the FP64 NUDFT, / sqrt(N) -- to the KB approximation
+    n = 32
+    nf = O.Nufft((n, n, 1), I.cartesian_traj(n, n), 1, toeplitz=True)
+    pre, apply = O.precond_strang(nf)
+    assert np.std(pre) / np.mean(pre) < 1e-5
+    for spokes in (21, 64):
+        traj = I.radial_traj(2 * n, spokes)
+        nf = O.Nufft((n, n, 1), traj, 1, toeplitz=True)
+        pre, apply = O.precond_strang(nf)
+        x, y = crandn((1, 1, n, n), 51), crandn((1, 1, n, n), 52)
+        assert abs(np.vdot(apply(x), y) - np.vdot(x, apply(y))) < 1e-8 * abs(np.vdot(x, apply(y)))
+        assert np.vdot(x, apply(x)).real > 0
+        e = np.zeros((1, 1, n, n), np.complex128)
+        e[0, 0, n // 2, n // 2] = 1
+        col = O.nudft_adjoint((n, n, 1), traj, O.nudft_forward((n, n, 1), traj, e))[0, 0]
+        exact = np.abs(np.fft.fft2(np.fft.ifftshift(col))) / n
+        assert relerr(pre[0] - 1e-3, exact) < 2e-5
