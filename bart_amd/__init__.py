@@ -155,6 +155,25 @@ def lib():
     L.lsqr_conjgrad.restype = ctypes.c_int
     L.lsqr_conjgrad.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, lp, vp, vp,
                                 ctypes.c_bool, ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
+    L.bart_amd_warmnet_input.restype = ctypes.c_int
+    L.bart_amd_warmnet_input.argtypes = [vp, ctypes.c_float, ctypes.c_int, lp, vp, vp, ctypes.POINTER(ctypes.c_float)]
+    L.bart_amd_warmnet_pack.restype = ctypes.c_int
+    L.bart_amd_warmnet_pack.argtypes = [ctypes.c_int, lp, vp, ctypes.c_float, vp]
+    L.bart_amd_warmnet_guess.restype = ctypes.c_int
+    L.bart_amd_warmnet_guess.argtypes = [vp, vp, ctypes.c_float, ctypes.c_int, lp, vp, vp]
+    L.nufft_precond_create.restype = vp
+    L.nufft_precond_create.argtypes = [vp]
+    L.operator_free.restype = None
+    L.operator_free.argtypes = [vp]
+    L.bart_amd_precond_apply.restype = ctypes.c_int
+    L.bart_amd_precond_apply.argtypes = [vp, vp, vp]
+    L.lsqr_conjgrad_precond.restype = ctypes.c_int
+    L.lsqr_conjgrad_precond.argtypes = [vp, vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, lp, vp,
+                                        vp, ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
+    L.bart_amd_tuning_report.restype = ctypes.c_int
+    L.bart_amd_tuning_report.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.bart_amd_cg_monitor.restype = ctypes.c_int
+    L.bart_amd_cg_monitor.argtypes = [vp, vp, vp]
     L.lsqr_conjgrad_noalloc.restype = ctypes.c_int
     L.lsqr_conjgrad_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int, lp, vp, vp,
                                         ctypes.c_bool, ctypes.c_bool, vp, ctypes.POINTER(CgStats)]
@@ -460,6 +479,71 @@ def lsqr_conjgrad(op, b, maxiter, lam=0.0, tol=0.0, x0=None, fallback=False):
                              trace.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
     _check()
     return x, st, trace[:max(0, it)]
+
+
+class Precond:
+    """Strang's circulant preconditioner of a Toeplitz operator
+    (nufft_precond_create)."""
+
+    def __init__(self, op):
+        self.op = op
+        self.h = lib().nufft_precond_create(op.h)
+        _check()
+
+    def apply(self, x):
+        x = _host_c64(x)
+        out = _like(x, self.op.idims)
+        lib().bart_amd_precond_apply(self.h, _ptr(out), _ptr(x))
+        _check()
+        return out
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().operator_free(self.h)
+            self.h = None
+
+
+def lsqr_conjgrad_precond(op, pre, b, maxiter, lam=0.0, tol=0.0):
+    """CG on P (A^H A + lam I) x = P b (nufft -i -c).  Returns (x, stats,
+    residual trace)."""
+    b = _host_c64(b)
+    x = _like(b, op.idims)
+    trace = np.zeros(max(1, maxiter), np.float32)
+    st = CgStats()
+    it = lib().lsqr_conjgrad_precond(op.h, pre.h, maxiter, lam, tol, DIMS, _dims(op.idims), _ptr(x), _ptr(b), False,
+                                     trace.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st))
+    _check()
+    return x, st, trace[:max(0, it)]
+
+
+def tuning_report():
+    """The autotuner's choices in this process (list of dicts: shape class,
+    coils per work-group, work-groups per compute unit, ms, candidates)."""
+    import json
+    n = lib().bart_amd_tuning_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 16)
+    lib().bart_amd_tuning_report(buf, n + 16)
+    _check()
+    return json.loads(buf.value.decode())
+
+
+class CgMonitor:
+    """Truth-image monitor of op's CG solves (pics -T): after each
+    lsqr_conjgrad on op, errors()[k] = ||truth - x_k|| / ||truth|| for the
+    iterate x_k that iteration k started from."""
+
+    def __init__(self, op, truth, maxiter):
+        self.op = op
+        self.err = np.zeros(max(1, maxiter), np.float32)
+        lib().bart_amd_cg_monitor(op.h, _ptr(_host_c64(truth)), self.err.ctypes.data_as(ctypes.c_void_p))
+        _check()
+
+    def errors(self, n):
+        return self.err[:n].copy()
+
+    def close(self):
+        lib().bart_amd_cg_monitor(self.op.h, None, None)
+        _check()
 
 
 def lsqr_conjgrad_noalloc(op, b, maxiter, lam=0.0, tol=0.0, x0=None, fallback=False):

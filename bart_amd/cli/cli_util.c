@@ -5,6 +5,7 @@
  * them.
  */
 #include "tools.h"
+#include "../../include/bart_amd.h"
 #include "cfl.h"
 
 #include <complex.h>
@@ -95,6 +96,47 @@ int parse_int(const char* s, const char* opt)
 void unsupported_option(const char* tool, const char* opt)
 {
 	cli_error("%s: option %s is not supported (GPU NUFFT / CG-SENSE path only)", tool, opt);
+}
+
+void parse_nufft_conf(const char* arg, struct nufft_conf_s* conf)
+{
+	char buf[512];
+	snprintf(buf, sizeof(buf), "%s", arg);
+
+	for (char* tok = strtok(buf, ","); NULL != tok; tok = strtok(NULL, ",")) {
+
+		char* val = strchr(tok, '=');
+
+		if (NULL != val)
+			*val++ = '\0';
+
+		if (!strcmp(tok, "r") || !strcmp(tok, "no-toeplitz"))
+			conf->toeplitz = false;
+		else if (!strcmp(tok, "P") || !strcmp(tok, "periodic"))
+			conf->periodic = true;
+		else if (!strcmp(tok, "1"))
+			conf->decomp = false;
+		else if (!strcmp(tok, "lowmem"))
+			conf->lowmem = true;
+		else if (!strcmp(tok, "zero-mem"))
+			conf->zero_overhead = true;
+		else if (!strcmp(tok, "no-precomp"))
+			conf->precomp = false;
+		else if ((!strcmp(tok, "o") || !strcmp(tok, "oversampling")) && NULL != val)
+			conf->os = parse_float(val, "--nufft-conf oversampling");
+		else if ((!strcmp(tok, "w") || !strcmp(tok, "width")) && NULL != val)
+			conf->width = parse_float(val, "--nufft-conf width");
+		else if (!strcmp(tok, "real-psf"))
+			conf->real = true;
+		else if (!strcmp(tok, "compress-psf"))
+			conf->compress_psf = true;
+		else if (!strcmp(tok, "decomposed-psf"))
+			conf->decomposed_psf = true;
+		else if (!strcmp(tok, "upper-triag-psf"))
+			conf->upper_triag = true;
+		else
+			cli_error("Sub-option could not be parsed: %s", tok);
+	}
 }
 
 void estimate_im_dims(const long traj_dims[DIMS], const complex float* traj, long dims[3])

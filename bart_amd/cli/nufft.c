@@ -10,8 +10,11 @@
  * normal operator unless -r, 50 iterations by default as
  * iter_conjgrad_defaults, iter/iter.c:51-60).
  * -s: the direct NUDFT instead of the NUFFT (noncart/nudft.c).
- * -g is accepted (every path runs on the GPU).  -c (preconditioner), -1
- * (non-decomposed grid), -B (basis), -F/-T (B0) are rejected with an error.
+ * -c: Strang's circulant preconditioner for -i (noncart/precond.c).
+ * --nufft-conf: BART's sub-options.  -g is accepted (every path runs on
+ * the GPU).  -1 (the image is the oversampled grid itself, gridded without
+ * further oversampling), -B (basis) and -F/-T (B0) are rejected with an
+ * error.
  */
 #include "tools.h"
 #include "cfl.h"
@@ -36,6 +39,7 @@ int main_nufft(int argc, char* argv[])
 {
 	bool adjoint = false;
 	bool inverse = false;
+	bool precond = false;
 	bool dft = false;
 	bool precomp = true;
 	long coilim_vec[3] = { 0, 0, 0 };
@@ -79,13 +83,13 @@ int main_nufft(int argc, char* argv[])
 		case O_LOWMEM: conf.lowmem = true; break;
 		case O_ZEROMEM: conf.zero_overhead = true; break;
 		case O_NOPRECOMP: precomp = false; break;
-		case 'c': unsupported_option("nufft", "-c"); break;
+		case 'c': precond = true; break;
 		case 's': dft = true; break;
 		case '1': unsupported_option("nufft", "-1"); break;
 		case 'B': unsupported_option("nufft", "-B"); break;
 		case 'F': unsupported_option("nufft", "-F"); break;
 		case 'T': unsupported_option("nufft", "-T"); break;
-		case O_NUFFTCONF: unsupported_option("nufft", "--nufft-conf"); break;
+		case O_NUFFTCONF: parse_nufft_conf(optarg, &conf); break;
 		case 'h': usage(); return 0;
 		default: usage(); return 1;
 		}
@@ -188,7 +192,21 @@ int main_nufft(int argc, char* argv[])
 			linop_adjoint(op, DIMS, coilim_dims, b, DIMS, ksp_dims, ksp);
 
 			struct cg_stats_s st;
-			lsqr_conjgrad(op, maxiter, lambda, 0.f, DIMS, coilim_dims, img, b, false, false, NULL, &st);
+
+			// -c: Strang's circulant preconditioner with the Toeplitz
+			// operator (src/nufft.c:250-263)
+			if (precond && conf.toeplitz && !dft) {
+
+				const struct operator_s* pre = nufft_precond_create(op);
+				lsqr_conjgrad_precond(op, pre, maxiter, lambda, 0.f, DIMS, coilim_dims, img, b, false, NULL,
+						      &st);
+				operator_free(pre);
+
+			} else {
+
+				lsqr_conjgrad(op, maxiter, lambda, 0.f, DIMS, coilim_dims, img, b, false, false, NULL, &st);
+			}
+
 			debug_printf(3, "CG: %d iterations, |r| / |b| = %e, %.3f ms\n", st.iterations,
 				     st.bnorm > 0.f ? st.resid / st.bnorm : 0.f, st.time_ms);
 			free(b);

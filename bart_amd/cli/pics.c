@@ -41,6 +41,7 @@ int main_pics(int argc, char* argv[])
 	const char* traj_file = NULL;
 	const char* pat_file = NULL;
 	const char* start_file = NULL;
+	const char* truth_file = NULL;
 	const char* psf_ofile = NULL;
 	const char* psf_ifile = NULL;
 	float lambda = -1.f;
@@ -50,7 +51,7 @@ int main_pics(int argc, char* argv[])
 	float tol = 0.f;
 	struct nufft_conf_s conf = nufft_conf_defaults;
 
-	enum { O_GPUGRID = 256, O_NOTOEP, O_PSFEX, O_PSFIM, O_TOL, O_LOWMEM, O_UNSUP };
+	enum { O_GPUGRID = 256, O_NOTOEP, O_PSFEX, O_PSFIM, O_TOL, O_LOWMEM, O_NUFFTCONF, O_UNSUP };
 
 	static const struct option lopts[] = {
 		{ "gpu-gridding", no_argument, NULL, O_GPUGRID },
@@ -67,7 +68,7 @@ int main_pics(int argc, char* argv[])
 		{ "wavelet", required_argument, NULL, O_UNSUP },
 		{ "shared-img-dims", required_argument, NULL, O_UNSUP },
 		{ "motion-field", required_argument, NULL, O_UNSUP },
-		{ "nufft-conf", required_argument, NULL, O_UNSUP },
+		{ "nufft-conf", required_argument, NULL, O_NUFFTCONF },
 		{ NULL, 0, NULL, 0 },
 	};
 
@@ -102,15 +103,14 @@ int main_pics(int argc, char* argv[])
 		case O_PSFEX: psf_ofile = optarg; break;
 		case O_PSFIM: psf_ifile = optarg; break;
 		case O_TOL: tol = parse_float(optarg, "--tol"); break;
+		case O_NUFFTCONF: parse_nufft_conf(optarg, &conf); break;
 
 		// options that do not change a CG reconstruction (step size,
 		// wavelet/LLR/ADMM/IST settings, truth image for monitoring)
 		case 's': case 'n': case 'N': case 'b': case 'e': case 'H':
 		case 'D': case 'F': case 'J': case 'u': case 'C': case 'o':
 			break;
-		case 'T':
-			debug_printf(1, "pics: -T (truth image) is ignored\n");
-			break;
+		case 'T': truth_file = optarg; break;
 
 		case O_UNSUP: unsupported_option("pics", lopts[li].name); break;
 		case 'h': usage(); return 0;
@@ -306,6 +306,24 @@ int main_pics(int argc, char* argv[])
 	linop_adjoint(forward_op, DIMS, img_dims, b, DIMS, ksp_dims, kspace);
 
 	float* trace = calloc((size_t)(maxiter > 0 ? maxiter : 1), sizeof(float));
+	float* errs = NULL;
+
+	if (NULL != truth_file) {
+
+		// the monitor compares the iterates with the truth image as given,
+		// without the data scaling (src/pics.c:548-575)
+		long td[DIMS];
+		complex float* truth = cfl_load(truth_file, DIMS, td);
+
+		for (int i = 0; i < DIMS; i++)
+			if (td[i] != img_dims[i])
+				cli_error("pics: truth image has the wrong dimensions");
+
+		errs = calloc((size_t)(maxiter > 0 ? maxiter : 1), sizeof(float));
+		bart_amd_cg_monitor(forward_op, truth, errs);
+		free(truth);
+	}
+
 	struct cg_stats_s st;
 	// lsqr's lambda (cclambda) and the l2 penalty both add a multiple of the
 	// identity to the normal operator (iter/lsqr.c:52-60, iter/italgos.c:497)
@@ -313,6 +331,11 @@ int main_pics(int argc, char* argv[])
 
 	for (int i = 0; i < st.iterations; i++)
 		debug_printf(5, "#CG %3d: |r| = %e\n", i + 1, trace[i]);
+
+	// iter/monitor.c:68 (no objective in CG: -1)
+	if (NULL != errs)
+		for (int i = 0; i < st.iterations; i++)
+			debug_printf(2, "[Iter %04d] Objective: %f, Error: %f\n", i + 1, -1., (double)errs[i]);
 
 	debug_printf(3, "CG: %d iterations, |r| / |b| = %e, %.3f ms on the GPU\n", st.iterations,
 		     st.bnorm > 0.f ? st.resid / st.bnorm : 0.f, st.time_ms);
@@ -330,6 +353,7 @@ int main_pics(int argc, char* argv[])
 	linop_free(forward_op);
 
 	free(trace);
+	free(errs);
 	free(b);
 	free(image);
 	free(pattern);

@@ -26,6 +26,12 @@ float parse_float(const char* s, const char* opt);
 int parse_int(const char* s, const char* opt);
 void unsupported_option(const char* tool, const char* opt);
 
+/* --nufft-conf sub-options (noncart/nufft.c:100-113, parsed as
+ * misc/opts.c:1227-1300 does: comma-separated short or long names, with
+ * =value for the two that take one) */
+struct nufft_conf_s;
+void parse_nufft_conf(const char* arg, struct nufft_conf_s* conf);
+
 /* misc/mri2.c:231-264 */
 void estimate_im_dims(const long traj_dims[DIMS], const _Complex float* traj, long dims[3]);
 

@@ -65,35 +65,51 @@ __device__ inline double sum_partials(const double* part, int n, double* sh)
 	return block_sum(v, sh);
 }
 
-// the element range of this work-group's problem: first element and stride
+// the element range of work-group b of its problem: first element and stride
 struct SegIt {
 
 	long i0, step, len, off;
 };
 
-__device__ inline SegIt seg_it(const CgSeg& s)
+__device__ inline SegIt seg_it(const CgSeg& s, int b)
 {
 	SegIt it;
-	it.i0 = (long)(blockIdx.x - s.blk0) * blockDim.x + threadIdx.x;
+	it.i0 = (long)(b - s.blk0) * blockDim.x + threadIdx.x;
 	it.step = (long)s.nblk * blockDim.x;
 	it.len = s.len;
 	it.off = s.off;
 	return it;
 }
 
-// Ap += lambda p; partial sums of p . Ap (float products, FP64 accumulation)
+// the sum of a problem's partials in fixed order, in every thread of the
+// work-group (each work-group of a vector launch reduces them itself, so
+// that no separate single-work-group launch is needed and no atomics)
+__device__ inline double sum_partials_all(const double* part, int n, double* sh, double* bc)
+{
+	double v = sum_partials(part, n, sh);
+
+	if (0 == threadIdx.x)
+		*bc = v;
+
+	__syncthreads();
+	return *bc;
+}
+
+// Ap += lambda p; partial sums of p . Ap (float products, FP64 accumulation).
+// Work-group b = blk_base + blockIdx.x.
 __global__ __launch_bounds__(CG_THREADS) void k_cg_apdot(const int* __restrict__ bprob, const CgSeg* __restrict__ seg,
 							 float lambda, const float* __restrict__ p,
 							 float* __restrict__ Ap, double* __restrict__ part,
-							 const CgState* st)
+							 const CgState* st, int blk_base)
 {
 	__shared__ double sh[4];
-	const int q = bprob[blockIdx.x];
+	const int b = blk_base + (int)blockIdx.x;
+	const int q = bprob[b];
 
 	if (0 != st[q].done)
 		return;
 
-	const SegIt it = seg_it(seg[q]);
+	const SegIt it = seg_it(seg[q], b);
 	p += it.off;
 	Ap += it.off;
 	double acc = 0.;
@@ -109,114 +125,235 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_apdot(const int* __restrict__
 	double s = block_sum(acc, sh);
 
 	if (0 == threadIdx.x)
-		part[blockIdx.x] = s;
+		part[b] = s;
 }
 
-// one work-group per problem
-__global__ __launch_bounds__(CG_THREADS) void k_cg_alpha(const CgSeg* __restrict__ seg,
-							 const double* __restrict__ part, CgState* st)
+// The same step fused into the sum of the coil-group partial images that
+// ends a normal operator: Ap = sum_g grp[g] + lambda p (groups added in
+// order, as k_sum_groups), partial sums of p . Ap in the layout of
+// k_cg_apdot.
+__global__ __launch_bounds__(CG_THREADS) void k_cg_apdot_groups(CgApFuse f, const float* __restrict__ grp,
+								long gstride, int G)
 {
 	__shared__ double sh[4];
-	const int q = blockIdx.x;
 
-	if (0 != st[q].done)
+	if (nullptr != f.skip && 0 != *f.skip)
 		return;
 
-	double s = sum_partials(part + seg[q].blk0, seg[q].nblk, sh);
+	const long step = (long)f.nblk * blockDim.x;
+	double acc = 0.;
 
-	if (0 == threadIdx.x) {
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < f.len; i += step) {
 
-		float pAp = (float)s;
+		float a = grp[i];
 
-		if (0.f == pAp) {
-			st[q].done = 3;     // breakdown: x unchanged (italgos.c:544-545)
-		} else {
-			st[q].alpha = st[q].rsold / pAp;
-		}
+		for (int g = 1; g < G; g++)
+			a = a + grp[(long)g * gstride + i];
+
+		float pv = f.p[i];
+		float v = a + f.lambda * pv;
+		f.Ap[i] = v;
+		acc += (double)(pv * v);
 	}
+
+	double s = block_sum(acc, sh);
+
+	if (0 == threadIdx.x)
+		f.part[f.blk0 + blockIdx.x] = s;
 }
 
-// x += alpha p; r -= alpha Ap; partial sums of r . r
+// The same step for every problem of a batch whose normal operator ran
+// grouped (items[q].grp != null), in the layout of k_cg_apdot.
+__global__ __launch_bounds__(CG_THREADS) void k_cg_apdot_sum_batch(const int* __restrict__ bprob,
+								   const CgSeg* __restrict__ seg,
+								   const CgGrpItem* __restrict__ items, float lambda,
+								   const float* __restrict__ p, float* __restrict__ Ap,
+								   double* __restrict__ part, const CgState* st)
+{
+	__shared__ double sh[4];
+	const int b = (int)blockIdx.x;
+	const int q = bprob[b];
+	const CgGrpItem it = items[q];
+
+	if (nullptr == it.grp || 0 != st[q].done)
+		return;
+
+	const SegIt si = seg_it(seg[q], b);
+	p += si.off;
+	Ap += si.off;
+	double acc = 0.;
+
+	for (long i = si.i0; i < si.len; i += si.step) {
+
+		float a = it.grp[i];
+
+		for (int g = 1; g < it.G; g++)
+			a = a + it.grp[(long)g * it.gstride + i];
+
+		float pv = p[i];
+		float v = a + lambda * pv;
+		Ap[i] = v;
+		acc += (double)(pv * v);
+	}
+
+	double s = block_sum(acc, sh);
+
+	if (0 == threadIdx.x)
+		part[b] = s;
+}
+
+void cg_apdot_groups(const CgApFuse& f, const float2* grp, long N, int G, hipStream_t st)
+{
+	ProfScope prof("cg_apdot_groups", st);
+	hipLaunchKernelGGL(k_cg_apdot_groups, dim3(f.nblk), dim3(CG_THREADS), 0, st, f, (const float*)grp, 2 * N, G);
+	BAM_HIP(hipGetLastError());
+}
+
+// alpha = rsnew / p.Ap (every work-group reduces the p.Ap partials itself;
+// the first one records alpha, copies rsnew to rsold for the beta step and
+// flags a breakdown, pAp = 0, italgos.c:544-545); x += alpha p;
+// r -= alpha Ap; partial sums of r . r
 __global__ __launch_bounds__(CG_THREADS) void k_cg_xr(const int* __restrict__ bprob, const CgSeg* __restrict__ seg,
 						      float* __restrict__ x, float* __restrict__ r,
 						      const float* __restrict__ p, const float* __restrict__ Ap,
-						      double* __restrict__ part, const CgState* st)
+						      const double* __restrict__ part_ap, double* __restrict__ part_rr,
+						      CgState* st, CgMon mon)
 {
 	__shared__ double sh[4];
+	__shared__ double bc;
+	__shared__ float s_alpha;
 	const int q = bprob[blockIdx.x];
 
 	if (0 != st[q].done)
 		return;
 
-	const float alpha = st[q].alpha;
-	const SegIt it = seg_it(seg[q]);
+	const CgSeg sg = seg[q];
+	const double s = sum_partials_all(part_ap + sg.blk0, sg.nblk, sh, &bc);
+	const float pAp = (float)s;
+
+	if (0.f == pAp) {
+		if (0 == threadIdx.x && (int)blockIdx.x == sg.blk0)
+			st[q].done = 3;     // x unchanged
+		return;
+	}
+
+	if (0 == threadIdx.x) {
+		const float rs = st[q].rsnew;
+		s_alpha = rs / pAp;
+		if ((int)blockIdx.x == sg.blk0) {
+			st[q].alpha = s_alpha;
+			st[q].rsold = rs;
+		}
+	}
+
+	__syncthreads();
+	const float alpha = s_alpha;
+	const SegIt it = seg_it(sg, (int)blockIdx.x);
 	x += it.off;
 	r += it.off;
 	p += it.off;
 	Ap += it.off;
 	double acc = 0.;
 
-	for (long i = it.i0; i < it.len; i += it.step) {
+	if (nullptr != mon.truth && 0.f != mon.tinv[q]) {
 
-		x[i] += alpha * p[i];
-		float rv = r[i] + (-alpha) * Ap[i];
-		r[i] = rv;
-		acc += (double)(rv * rv);
+		// the monitor sees the iterate this iteration starts from
+		const float* t = mon.truth + it.off;
+		double acce = 0.;
+
+		for (long i = it.i0; i < it.len; i += it.step) {
+
+			const float xo = x[i];
+			const float d = t[i] - xo;
+			acce += (double)(d * d);
+			x[i] = xo + alpha * p[i];
+			float rv = r[i] + (-alpha) * Ap[i];
+			r[i] = rv;
+			acc += (double)(rv * rv);
+		}
+
+		__syncthreads();
+		double se = block_sum(acce, sh);
+
+		if (0 == threadIdx.x)
+			mon.part[blockIdx.x] = se;
+
+	} else {
+
+		for (long i = it.i0; i < it.len; i += it.step) {
+
+			x[i] += alpha * p[i];
+			float rv = r[i] + (-alpha) * Ap[i];
+			r[i] = rv;
+			acc += (double)(rv * rv);
+		}
 	}
 
-	double s = block_sum(acc, sh);
+	__syncthreads();
+	double s2 = block_sum(acc, sh);
 
 	if (0 == threadIdx.x)
-		part[blockIdx.x] = s;
+		part_rr[blockIdx.x] = s2;
 }
 
-// one work-group per problem; trace: [B][tstride] residual norms
-__global__ __launch_bounds__(CG_THREADS) void k_cg_beta(const CgSeg* __restrict__ seg,
-							const double* __restrict__ part, CgState* st,
-							float* __restrict__ trace, int tstride)
-{
-	__shared__ double sh[4];
-	const int q = blockIdx.x;
-
-	if (0 != st[q].done)
-		return;
-
-	double s = sum_partials(part + seg[q].blk0, seg[q].nblk, sh);
-
-	if (0 == threadIdx.x) {
-
-		CgState* sq = st + q;
-		double nr = sqrt(s);
-		float rsnew = (float)(nr * nr);
-		int it = sq->iter;
-
-		sq->beta = rsnew / sq->rsold;
-		sq->rsold = rsnew;
-		sq->rsnew = rsnew;
-		sq->iter = it + 1;
-
-		if (it < tstride)
-			trace[(long)q * tstride + it] = sqrtf(rsnew);
-
-		if (rsnew <= sq->eps2)
-			sq->done = 2;  // converged
-		else if (sq->iter >= sq->maxiter)
-			sq->done = 1;  // iteration budget used
-	}
-}
-
-// p = beta p + r
+// beta = rsnew / rsold (every work-group reduces the r.r partials itself;
+// the first one stores rsnew, the iteration count, the trace entry
+// ([B][tstride] residual norms) and the stop flag); p = beta p + r
 __global__ __launch_bounds__(CG_THREADS) void k_cg_p(const int* __restrict__ bprob, const CgSeg* __restrict__ seg,
 						     float* __restrict__ p, const float* __restrict__ r,
-						     const CgState* st)
+						     const double* __restrict__ part_rr, CgState* st,
+						     float* __restrict__ trace, int tstride, CgMon mon)
 {
+	__shared__ double sh[4];
+	__shared__ double bc;
+	__shared__ float s_beta;
 	const int q = bprob[blockIdx.x];
 
 	if (0 != st[q].done)
 		return;
 
-	const float beta = st[q].beta;
-	const SegIt it = seg_it(seg[q]);
+	const CgSeg sg = seg[q];
+
+	if ((int)blockIdx.x == sg.blk0 && nullptr != mon.truth && 0.f != mon.tinv[q]) {
+
+		const double se = sum_partials(mon.part + sg.blk0, sg.nblk, sh);
+
+		if (0 == threadIdx.x && st[q].iter < tstride)
+			mon.err[(long)q * tstride + st[q].iter] = (float)(sqrt(se) * (double)mon.tinv[q]);
+
+		__syncthreads();
+	}
+
+	const double s = sum_partials_all(part_rr + sg.blk0, sg.nblk, sh, &bc);
+
+	if (0 == threadIdx.x) {
+
+		const double nr = sqrt(s);
+		const float rsnew = (float)(nr * nr);
+		s_beta = rsnew / st[q].rsold;
+
+		if ((int)blockIdx.x == sg.blk0) {
+
+			CgState* sq = st + q;
+			const int itn = sq->iter;
+
+			sq->beta = s_beta;
+			sq->rsnew = rsnew;
+			sq->iter = itn + 1;
+
+			if (itn < tstride)
+				trace[(long)q * tstride + itn] = sqrtf(rsnew);
+
+			if (rsnew <= sq->eps2)
+				sq->done = 2;  // converged
+			else if (sq->iter >= sq->maxiter)
+				sq->done = 1;  // iteration budget used
+		}
+	}
+
+	__syncthreads();
+	const float beta = s_beta;
+	const SegIt it = seg_it(sg, (int)blockIdx.x);
 	p += it.off;
 	r += it.off;
 
@@ -238,7 +375,7 @@ __global__ __launch_bounds__(CG_THREADS) void k_cg_init(const int* __restrict__ 
 	if (nullptr != mask && 0 == mask[q])
 		return;
 
-	const SegIt it = seg_it(seg[q]);
+	const SegIt it = seg_it(seg[q], (int)blockIdx.x);
 	x += it.off;
 	b += it.off;
 	r += it.off;
@@ -340,6 +477,7 @@ CgBatch::CgBatch(const std::vector<long>& nfloats, hipStream_t s) : B((int)nfloa
 	Ap.alloc(total);
 	part.alloc(nblk);
 	part_b.alloc(nblk);
+	part2.alloc(nblk);
 	seg.alloc(B);
 	bprob.alloc(nblk);
 	mask.alloc(B);
@@ -366,6 +504,42 @@ CgBatch::~CgBatch()
 	for (hipEvent_t e : probe_ev)
 		if (nullptr != e)
 			(void)hipEventDestroy(e);
+}
+
+CgMon CgBatch::mon() const
+{
+	if (!monitored)
+		return CgMon{};
+	return CgMon{ mon_truth.ptr, mon_tinv.ptr, mon_part.ptr, mon_err.ptr };
+}
+
+void CgBatch::set_monitor(const float* truth, double tnorm)
+{
+	BAM_CHECK(1 == B, "cg monitor: single problems only");
+	drop_graph();  // the iteration's kernels take the monitor's pointers
+
+	if (nullptr == truth) {
+		monitored = false;
+		return;
+	}
+
+	const long n = seg_h[0].len;
+	if ((long)mon_truth.n < n)
+		mon_truth.alloc(n);
+	if ((long)mon_part.n < nblk)
+		mon_part.alloc(nblk);
+	if (mon_tinv.n < 1)
+		mon_tinv.alloc(1);
+	BAM_HIP(hipMemcpy(mon_truth.ptr, truth, sizeof(float) * n, hipMemcpyDeviceToDevice));
+	const float tinv = (tnorm > 0.) ? (float)(1. / tnorm) : 0.f;
+	BAM_HIP(hipMemcpy(mon_tinv.ptr, &tinv, sizeof(float), hipMemcpyHostToDevice));
+	monitored = true;
+}
+
+void CgBatch::monitor_errors(float* host, int n) const
+{
+	if (monitored && n > 0)
+		BAM_HIP(hipMemcpy(host, mon_err.ptr, sizeof(float) * n, hipMemcpyDeviceToHost));
 }
 
 void CgBatch::drop_graph()
@@ -401,6 +575,11 @@ void CgBatch::start(const BatchNormalFn& normal, float lambda, float tol, int ma
 		drop_graph();
 	}
 
+	if (monitored && (long)mon_err.n < (long)B * trace_len) {
+		mon_err.alloc((size_t)B * trace_len);
+		drop_graph();
+	}
+
 	for (int i = 0; i < B; i++)
 		BAM_HIP(hipMemcpyAsync(bs(i), b_in[i], sizeof(float) * seg_h[i].len, hipMemcpyDeviceToDevice, stream));
 
@@ -408,7 +587,7 @@ void CgBatch::start(const BatchNormalFn& normal, float lambda, float tol, int ma
 
 		for (int i = 0; i < B; i++) {
 			BAM_HIP(hipMemcpyAsync(xs(i), x0[i], sizeof(float) * seg_h[i].len, hipMemcpyDeviceToDevice, stream));
-			normal(i, (float2*)rs(i), (const float2*)xs(i), nullptr);
+			normal(i, (float2*)rs(i), (const float2*)xs(i), nullptr, nullptr);
 		}
 
 	} else {
@@ -451,18 +630,49 @@ void CgBatch::start(const BatchNormalFn& normal, float lambda, float tol, int ma
 
 void CgBatch::enqueue_iteration(const BatchNormalFn& normal, float lambda)
 {
-	for (int i = 0; i < B; i++)
-		normal(i, (float2*)(Ap.ptr + seg_h[i].off), (const float2*)(p.ptr + seg_h[i].off), &st.ptr[i].done);
+	// A p, and where the operator ends with a coil-group sum, the step
+	// A p += lambda p with its p . Ap partials fused into that sum
+	std::vector<char> fused(B, 0);
+	int nfused = 0;
+
+	// the problems in groups: their passes, then one launch of the step
+	if (nullptr != hooks && hooks_ready) {
+		hooks->launch(*this);
+		hipLaunchKernelGGL(k_cg_apdot_sum_batch, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr,
+				   grp.ptr, lambda, p.ptr, Ap.ptr, part.ptr, st.ptr);
+		for (int i = 0; i < B; i++)
+			if (nullptr != grp_h[i].grp) {
+				fused[i] = 1;
+				nfused++;
+			}
+	}
+
+	for (int i = 0; i < B; i++) {
+		if (fused[i])
+			continue;
+		const CgSeg& sg = seg_h[i];
+		CgApFuse f{ lambda, p.ptr + sg.off, Ap.ptr + sg.off, part.ptr, sg.blk0, sg.nblk, sg.len, &st.ptr[i].done };
+		fused[i] = normal(i, (float2*)(Ap.ptr + sg.off), (const float2*)(p.ptr + sg.off), &st.ptr[i].done, &f);
+		nfused += fused[i] ? 1 : 0;
+	}
 
 	ProfScope prof("cg_vector_ops", stream);
-	hipLaunchKernelGGL(k_cg_apdot, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, lambda, p.ptr, Ap.ptr,
-			   part.ptr, st.ptr);
-	hipLaunchKernelGGL(k_cg_alpha, dim3(B), dim3(CG_THREADS), 0, stream, seg.ptr, part.ptr, st.ptr);
+
+	if (0 == nfused) {
+		hipLaunchKernelGGL(k_cg_apdot, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, lambda, p.ptr,
+				   Ap.ptr, part.ptr, st.ptr, 0);
+	} else {
+		for (int i = 0; i < B; i++)
+			if (!fused[i])
+				hipLaunchKernelGGL(k_cg_apdot, dim3(seg_h[i].nblk), dim3(CG_THREADS), 0, stream, bprob.ptr,
+						   seg.ptr, lambda, p.ptr, Ap.ptr, part.ptr, st.ptr, seg_h[i].blk0);
+	}
+
+	const CgMon m = mon();
 	hipLaunchKernelGGL(k_cg_xr, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, x.ptr, r.ptr, p.ptr,
-			   Ap.ptr, part.ptr, st.ptr);
-	hipLaunchKernelGGL(k_cg_beta, dim3(B), dim3(CG_THREADS), 0, stream, seg.ptr, part.ptr, st.ptr, trace.ptr,
-			   trace_len);
-	hipLaunchKernelGGL(k_cg_p, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, p.ptr, r.ptr, st.ptr);
+			   Ap.ptr, part.ptr, part2.ptr, st.ptr, m);
+	hipLaunchKernelGGL(k_cg_p, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob.ptr, seg.ptr, p.ptr, r.ptr, part2.ptr,
+			   st.ptr, trace.ptr, trace_len, m);
 	BAM_HIP(hipGetLastError());
 }
 
@@ -472,6 +682,15 @@ void CgBatch::capture(const BatchNormalFn& normal, float lambda)
 		return;
 
 	drop_graph();
+
+	// tables of the grouped normal operators (device memory: before the capture)
+	if (nullptr != hooks && !hooks_ready) {
+		grp_h.assign(B, CgGrpItem{ nullptr, 0, 0 });
+		hooks->prepare(*this, grp_h);
+		grp.alloc(B);
+		BAM_HIP(hipMemcpy(grp.ptr, grp_h.data(), sizeof(CgGrpItem) * B, hipMemcpyHostToDevice));
+		hooks_ready = true;
+	}
 	BAM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
 	prof_set_capturing(true);
 
@@ -623,7 +842,8 @@ constexpr size_t NA_PART_B = NA_PART + CG_BLOCKS * sizeof(double);     // double
 constexpr size_t NA_ST = NA_PART_B + CG_BLOCKS * sizeof(double);       // CgState
 constexpr size_t NA_SEG = NA_ST + 256;                                 // CgSeg
 constexpr size_t NA_BPROB = NA_SEG + 256;                              // int[CG_BLOCKS]
-constexpr size_t NA_TRACE = NA_BPROB + CG_BLOCKS * sizeof(int);        // float[CG_NOALLOC_TRACE]
+constexpr size_t NA_PART2 = NA_BPROB + CG_BLOCKS * sizeof(int);        // double[CG_BLOCKS]
+constexpr size_t NA_TRACE = NA_PART2 + CG_BLOCKS * sizeof(double);     // float[CG_NOALLOC_TRACE]
 constexpr size_t NA_BYTES = NA_VEC * 3 + 65536;
 static_assert(NA_TRACE + CG_NOALLOC_TRACE * sizeof(float) <= NA_BYTES, "allocation-free CG layout");
 
@@ -685,6 +905,7 @@ CgResult cg_solve_noalloc(const NormalFn& normal, float lambda, int maxiter, flo
 	CgState* st = (CgState*)(base + NA_ST);
 	CgSeg* seg = (CgSeg*)(base + NA_SEG);
 	int* bprob = (int*)(base + NA_BPROB);
+	double* part2 = (double*)(base + NA_PART2);
 	float* trace = (float*)(base + NA_TRACE);
 	float* xf = (float*)x;
 	const float* bf = (const float*)b;
@@ -707,7 +928,7 @@ CgResult cg_solve_noalloc(const NormalFn& normal, float lambda, int maxiter, flo
 	CgState hs;
 
 	if (warm) {
-		normal((float2*)r, x, nullptr);
+		normal((float2*)r, x, nullptr, nullptr);
 		init(false);
 	} else {
 		fill(x, nfloats / 2, make_float2(0.f, 0.f), stream);
@@ -733,16 +954,14 @@ CgResult cg_solve_noalloc(const NormalFn& normal, float lambda, int maxiter, flo
 		const int nl = std::min(CG_CHUNK, maxiter - done);
 
 		for (int i = 0; i < nl; i++) {
-			normal((float2*)Ap, (const float2*)p, &st->done);
-			hipLaunchKernelGGL(k_cg_apdot, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, lambda, p, Ap,
-					   part, (const CgState*)st);
-			hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(CG_THREADS), 0, stream, seg, part, st);
+			CgApFuse f{ lambda, p, Ap, part, 0, nblk, nfloats, &st->done };
+			if (!normal((float2*)Ap, (const float2*)p, &st->done, &f))
+				hipLaunchKernelGGL(k_cg_apdot, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, lambda, p,
+						   Ap, part, (const CgState*)st, 0);
 			hipLaunchKernelGGL(k_cg_xr, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, xf, r, p, Ap, part,
-					   (const CgState*)st);
-			hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(CG_THREADS), 0, stream, seg, part, st, trace,
-					   CG_NOALLOC_TRACE);
-			hipLaunchKernelGGL(k_cg_p, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, p, r,
-					   (const CgState*)st);
+					   part2, st, CgMon{});
+			hipLaunchKernelGGL(k_cg_p, dim3(nblk), dim3(CG_THREADS), 0, stream, bprob, seg, p, r, part2, st, trace,
+					   CG_NOALLOC_TRACE, CgMon{});
 			BAM_HIP(hipGetLastError());
 		}
 
@@ -772,7 +991,9 @@ CgResult cg_solve_noalloc(const NormalFn& normal, float lambda, int maxiter, flo
 CgResult CgSolver::solve(const NormalFn& normal, float lambda, int maxiter, float tol, float2* x_io,
 			 const float2* b_in, bool warm, bool fallback, bool use_graph)
 {
-	BatchNormalFn bn = [&normal](int, float2* d, const float2* s, const int* skip) { normal(d, s, skip); };
+	BatchNormalFn bn = [&normal](int, float2* d, const float2* s, const int* skip, const CgApFuse* f) {
+		return normal(d, s, skip, f);
+	};
 
 	hipEvent_t e0, e1;
 	BAM_HIP(hipEventCreate(&e0));

@@ -44,13 +44,71 @@ struct CgSeg {
 	int nblk;
 };
 
-// normal(i, dst, src, skip): enqueue dst = A_i^H A_i src for problem i on
-// the batch's stream; skip is a device flag (may be null) that turns the
-// launches into no-ops once problem i has stopped.
-using BatchNormalFn = std::function<void(int, float2*, const float2*, const int*)>;
+// The CG step that follows A^H A p -- Ap = A^H A p + lambda p and the
+// partial sums of p . Ap -- for a normal operator whose last kernel can take
+// it (the sum of coil-group partial images): the problem's p, its A p (the
+// normal operator's dst), the solver's partials array and the work-groups
+// [blk0, blk0 + nblk) of its vector launches, len floats.
+struct CgApFuse {
 
-// normal(dst, src, skip) of a single problem
-using NormalFn = std::function<void(float2*, const float2*, const int*)>;
+	float lambda;
+	const float* p;
+	float* Ap;
+	double* part;
+	int blk0;
+	int nblk;
+	long len;
+	const int* skip;
+};
+
+// f.Ap = sum_g grp[g N + i] (groups added in order) + lambda f.p, with the
+// p . Ap partials; grp holds G partial images of N complex elements
+void cg_apdot_groups(const CgApFuse& f, const float2* grp, long N, int G, hipStream_t st);
+
+// Monitor of the iterates (iter/monitor.c, pics -T): ||truth - x_k|| /
+// ||truth|| of the iterate x_k that iteration k starts from, as BART's
+// conjgrad calls its monitor (iter/italgos.c:534).  truth is packed like x
+// (null: nothing is monitored); tinv[q] = 1 / ||truth_q|| (0: problem q is
+// not monitored); part: partial sums per work-group; err: [B][tstride].
+struct CgMon {
+
+	const float* truth;
+	const float* tinv;
+	double* part;
+	float* err;
+};
+
+// normal(i, dst, src, skip, fuse): enqueue dst = A_i^H A_i src for problem i
+// on the batch's stream; skip is a device flag (may be null) that turns the
+// launches into no-ops once problem i has stopped.  With fuse (may be null)
+// the operator may finish with the CG step instead (dst = A^H A src +
+// lambda src and the partials): it returns whether it did.
+using BatchNormalFn = std::function<bool(int, float2*, const float2*, const int*, const CgApFuse*)>;
+
+// normal(dst, src, skip, fuse) of a single problem
+using NormalFn = std::function<bool(float2*, const float2*, const int*, const CgApFuse*)>;
+
+// Partial images a grouped normal operator leaves for problem q: the CG step
+// sums them (G of them, gstride floats apart; grp = A p itself when G = 1)
+// and adds lambda p; grp = null: the problem is not in a group.
+struct CgGrpItem {
+
+	const float* grp;
+	long gstride;
+	int G;
+};
+
+struct CgBatch;
+
+// Normal operators of many problems applied together: prepare(bt) builds
+// their tables before the iteration is captured and fills items (one per
+// problem); launch(bt) enqueues the grouped passes on bt.stream.  Problems
+// with items[q].grp == null go through the per-problem normal function.
+struct CgGroupHooks {
+
+	std::function<void(CgBatch&, std::vector<CgGrpItem>&)> prepare;
+	std::function<void(CgBatch&)> launch;
+};
 
 // Independent problems (A_i^H A_i + lambda I) x_i = b_i solved in lockstep:
 // the vectors of all problems are packed into one buffer each, every vector
@@ -65,7 +123,12 @@ struct CgBatch {
 	hipStream_t stream = 0;
 	std::vector<CgSeg> seg_h;
 	DevBuf<float> x, b, r, p, Ap, trace;
-	DevBuf<double> part, part_b;
+	DevBuf<double> part, part_b, part2;  // p.Ap (and ||r0||^2), ||b||^2, r.r
+	// monitor (single problem): truth image, 1 / ||truth||, partials, errors
+	DevBuf<float> mon_truth, mon_tinv, mon_err;
+	DevBuf<double> mon_part;
+	bool monitored = false;
+	CgMon mon() const;
 	DevBuf<CgSeg> seg;
 	DevBuf<int> bprob;   // work-group -> problem
 	DevBuf<int> mask;    // problems to (re)initialise
@@ -75,6 +138,12 @@ struct CgBatch {
 	hipEvent_t probe_ev[2] = { nullptr, nullptr };
 	int trace_len = 0;   // per problem
 	std::vector<char> fell_back;
+
+	// grouped normal operators (null: every problem by the normal function)
+	const CgGroupHooks* hooks = nullptr;
+	bool hooks_ready = false;
+	std::vector<CgGrpItem> grp_h;
+	DevBuf<CgGrpItem> grp;
 
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t gexec = nullptr;
@@ -92,6 +161,12 @@ struct CgBatch {
 	float* rs(int i) const { return r.ptr + seg_h[i].off; }
 
 	void drop_graph();
+
+	// truth (device, n floats of problem 0; null: off) for the monitor of
+	// the next solves
+	void set_monitor(const float* truth, double tnorm);
+	// errors of the last solve (host, iterations entries)
+	void monitor_errors(float* host, int n) const;
 
 	// Enqueues the start: b_i copied in, x_i = warm start or zero, r = b - (A^H A
 	// + lambda I) x, p = r.  With warm && fallback, waits for the device and
@@ -151,6 +226,9 @@ struct CgSolver {
 	// exceeds ||b||.
 	CgResult solve(const NormalFn& normal, float lambda, int maxiter, float tol, float2* x_io, const float2* b,
 		       bool warm, bool fallback, bool use_graph);
+
+	// with a truth image (device; null: off), the monitor of the next solves
+	void set_monitor(const float2* truth, double tnorm) { bt.set_monitor((const float*)truth, tnorm); }
 
 	void drop_graph() { bt.drop_graph(); }
 };

@@ -721,8 +721,11 @@ __device__ inline void fft_contig_regs(const FftLaunch& L, const Ld& ld, const S
 	}
 }
 
+// The transforms [t0, t0 + P) of one launch (the body of k_fft_axis_n and of
+// its batched form k_fft_axis_n_batch).
 template <int N, class Ld, class Md, class St>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, Md md, St st)
+__device__ inline void fft_axis_block(const FftLaunch& L, const Ld& ld, const Md& md, const St& st, long t0,
+				      float2* lds)
 {
 	constexpr int P = ct_block_p(N);
 	constexpr int TOT = P * N;
@@ -730,13 +733,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 	constexpr int PN = N + 1;  // transform pitch: strided accesses (p fastest) spread over the banks
 	static_assert((1 << LOGP) == P, "transforms per block must be a power of two");
 
-	extern __shared__ float2 lds[];
 	float2* tw = lds + P * PN;
-
-	if (nullptr != L.skip && 0 != *L.skip)
-		return;
-
-	const long t0 = (long)blockIdx.x * P;
 
 	// strided axis with column-block functors: per-block base addresses
 	constexpr bool CBF = colblock_of<Ld>::value && colblock_of<Md>::value && colblock_of<St>::value;
@@ -867,6 +864,42 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, 
 	}
 
 	st.store(lds, L, t0);
+}
+
+template <int N, class Ld, class Md, class St>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n(FftLaunch L, Ld ld, Md md, St st)
+{
+	extern __shared__ float2 lds[];
+
+	if (nullptr != L.skip && 0 != *L.skip)
+		return;
+
+	fft_axis_block<N>(L, ld, md, st, (long)blockIdx.x * ct_block_p(N), lds);
+}
+
+// One pass of many problems of the same transform length in one launch:
+// items[q] holds problem q's launch and functors, btab[b] = (q, block of
+// problem q) for work-group b.
+template <class Ld, class Md, class St>
+struct FftItem {
+	FftLaunch L;
+	Ld ld;
+	Md md;
+	St st;
+};
+
+template <int N, class Ld, class Md, class St>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_axis_n_batch(const FftItem<Ld, Md, St>* __restrict__ items,
+								  const int2* __restrict__ btab)
+{
+	extern __shared__ float2 lds[];
+	const int2 e = btab[blockIdx.x];
+	const FftItem<Ld, Md, St>& it = items[e.x];
+
+	if (nullptr != it.L.skip && 0 != *it.L.skip)
+		return;
+
+	fft_axis_block<N>(it.L, it.ld, it.md, it.st, (long)e.y * ct_block_p(N), lds);
 }
 
 // Lengths whose pair kernel exists (k_fft_axis_pair needs the register-edge
@@ -1336,6 +1369,39 @@ inline void fft_config_pair(FftLaunchCfg& c)
 	c.L.logP = logP;
 	c.blocks = (unsigned)((c.L.total + P - 1) / P);
 	c.lds_bytes = (size_t)(P * (n + 1) + n) * sizeof(float2);
+}
+
+// k_fft_axis_n_batch over nblocks work-groups for transforms of length n
+// (a compile-time length); items and btab device arrays
+template <class Ld, class Md, class St>
+void launch_fft_axis_batch(int n, const FftItem<Ld, Md, St>* items, const int2* btab, int nblocks, size_t lds_bytes,
+			   hipStream_t stream, const char* name)
+{
+	if (nblocks <= 0)
+		return;
+
+	ProfScope prof(name, stream);
+	const dim3 gb((unsigned)nblocks), tb(FFT_THREADS);
+
+#define BAM_FFT_N(NN) case NN: hipLaunchKernelGGL((k_fft_axis_n_batch<NN, Ld, Md, St>), gb, tb, lds_bytes, stream, items, btab); break
+	switch (n) {
+		BAM_FFT_N(16);
+		BAM_FFT_N(32);
+		BAM_FFT_N(64);
+		BAM_FFT_N(128);
+		BAM_FFT_N(192);
+		BAM_FFT_N(256);
+		BAM_FFT_N(320);
+		BAM_FFT_N(384);
+		BAM_FFT_N(512);
+		BAM_FFT_N(640);
+		BAM_FFT_N(768);
+	default:
+		fail("batched FFT: no kernel for n = %d", n);
+	}
+#undef BAM_FFT_N
+
+	BAM_HIP(hipGetLastError());
 }
 
 template <class Ld, class Md, class St>

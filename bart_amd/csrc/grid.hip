@@ -19,7 +19,14 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include <sys/stat.h>
 
 namespace bam {
 
@@ -423,8 +430,13 @@ void GridPlan::build(const GridGeom& geom, const float* traj)
 
 	BAM_HIP(hipMemcpy(cstart.ptr, count.data(), sizeof(int) * (ncells + 1), hipMemcpyHostToDevice));
 
-	int nch = (g.C + ta.cc - 1) / ta.cc;
-	kspS.alloc((size_t)nch * ta.cc * std::max<long>(g.M, 1));
+	// room for every coils-per-work-group value the autotuner may pick
+	// (4 .. the default): the coils rounded up to each
+	cc_max = ta.cc;
+	long kch = 0;
+	for (int c = 4; c <= ta.cc; c *= 2)
+		kch = std::max<long>(kch, (long)((g.C + c - 1) / c) * c);
+	kspS.alloc((size_t)kch * std::max<long>(g.M, 1));
 
 	// adjoint work list: the visits of a tile (its cells and their ring) in
 	// chunks of at most MAXV; tiles with more than one chunk are split
@@ -1193,7 +1205,7 @@ void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float
 	const int nthr = 64 * t.nw;
 	const int nch = (g.C + t.cc - 1) / t.cc;
 	const int nitems = gp.nwork * nch;
-	const unsigned nwg = (unsigned)std::max(1, std::min(nitems, 2 * gp.ncu));
+	const unsigned nwg = (unsigned)std::max(1, std::min(nitems, gp.wpc * gp.ncu));
 	size_t lds = SMEM_DATA + sizeof(float4) * ADJ_MAXV + sizeof(float2) * ADJ_BATCH * t.cc +
 		     sizeof(int) * ADJ_MAXV + sizeof(float) * ADJ_WROW * ADJ_BATCH + sizeof(unsigned) * 6 * ADJ_BATCH +
 		     sizeof(int) * 4;
@@ -1598,6 +1610,142 @@ void grid_forward(const GridPlan& gp, const float* kb, float2* ksp, const float2
 				   weights, ksp);
 
 	BAM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner of the adjoint gridding launch
+// ---------------------------------------------------------------------------
+
+struct TuneEntry {
+	std::string key;
+	int cc, wpc;
+	float ms;
+	int tried;
+};
+
+static std::mutex tune_mtx;
+static std::vector<TuneEntry> tune_table;  // this process's choices (and the file's)
+static bool tune_loaded = false;
+
+static std::string tune_file()
+{
+	const char* f = getenv("BART_AMD_TUNE_FILE");
+	if (nullptr != f)
+		return f;
+	const char* h = getenv("HOME");
+	return std::string(nullptr != h ? h : "/tmp") + "/.cache/bart_amd_tune.txt";
+}
+
+static void tune_load()
+{
+	if (tune_loaded)
+		return;
+	tune_loaded = true;
+	FILE* fp = fopen(tune_file().c_str(), "r");
+	if (nullptr == fp)
+		return;
+	char key[256];
+	int cc, wpc;
+	float ms;
+	while (4 == fscanf(fp, "%255s %d %d %f", key, &cc, &wpc, &ms))
+		tune_table.push_back(TuneEntry{ key, cc, wpc, ms, 0 });
+	fclose(fp);
+}
+
+static void tune_store(const TuneEntry& e)
+{
+	std::string f = tune_file();
+	std::string dir = f.substr(0, f.find_last_of('/'));
+	if (!dir.empty())
+		(void)mkdir(dir.c_str(), 0755);
+	FILE* fp = fopen(f.c_str(), "a");
+	if (nullptr == fp)
+		return;  // not writable: the choice holds for this process only
+	fprintf(fp, "%s %d %d %.4f\n", e.key.c_str(), e.cc, e.wpc, e.ms);
+	fclose(fp);
+}
+
+void tune_grid_adjoint(GridPlan& gp, const float* kb, float2* grid, hipStream_t st)
+{
+	const GridGeom& g = gp.g;
+
+	static const bool on = [] {
+		const char* e = getenv("BART_AMD_AUTOTUNE");
+		return !(nullptr != e && 0 == strcmp(e, "off"));
+	}();
+
+	if (!on || 0 == g.C || 0 == g.M || 0 == gp.nwork)
+		return;
+
+	char kbuf[256];
+	snprintf(kbuf, sizeof(kbuf), "adj:%dx%dx%d:S%d:C%d:M%ld:ccmax%d", g.n[0], g.n[1], g.n[2], g.S, g.C, g.M,
+		 gp.cc_max);
+	const std::string key = kbuf;
+
+	{
+		std::lock_guard<std::mutex> lock(tune_mtx);
+		tune_load();
+		for (const TuneEntry& e : tune_table)
+			if (e.key == key && e.cc >= 4 && e.cc <= gp.cc_max && e.wpc >= 1 && e.wpc <= 4) {
+				gp.ta.cc = e.cc;
+				gp.wpc = e.wpc;
+				return;
+			}
+	}
+
+	// time every candidate on the plan with a zero k-space (two launches,
+	// the second timed; the output is the same for all of them)
+	DevBuf<float2> ksp((size_t)g.C * g.M);
+	BAM_HIP(hipMemsetAsync(ksp.ptr, 0, sizeof(float2) * g.C * g.M, st));
+	hipEvent_t e0, e1;
+	BAM_HIP(hipEventCreate(&e0));
+	BAM_HIP(hipEventCreate(&e1));
+	const int cc0 = gp.ta.cc, wpc0 = gp.wpc;
+	TuneEntry best{ key, cc0, wpc0, 1e30f, 0 };
+
+	for (int cc = 4; cc <= gp.cc_max; cc *= 2) {
+		for (int wpc = 1; wpc <= 4; wpc *= 2) {
+			gp.ta.cc = cc;
+			gp.wpc = wpc;
+			grid_adjoint(gp, kb, grid, ksp.ptr, nullptr, st);
+			BAM_HIP(hipEventRecord(e0, st));
+			grid_adjoint(gp, kb, grid, ksp.ptr, nullptr, st);
+			BAM_HIP(hipEventRecord(e1, st));
+			BAM_HIP(hipEventSynchronize(e1));
+			float ms = 0.f;
+			BAM_HIP(hipEventElapsedTime(&ms, e0, e1));
+			best.tried++;
+			if (ms < best.ms) {
+				best.ms = ms;
+				best.cc = cc;
+				best.wpc = wpc;
+			}
+		}
+	}
+
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	gp.ta.cc = best.cc;
+	gp.wpc = best.wpc;
+
+	std::lock_guard<std::mutex> lock(tune_mtx);
+	tune_table.push_back(best);
+	tune_store(best);
+}
+
+std::string tuning_report()
+{
+	std::lock_guard<std::mutex> lock(tune_mtx);
+	std::string out = "[";
+	for (size_t i = 0; i < tune_table.size(); i++) {
+		const TuneEntry& e = tune_table[i];
+		char b[512];
+		snprintf(b, sizeof(b), "%s{\"shape\": \"%s\", \"coils_per_workgroup\": %d, \"workgroups_per_cu\": %d, "
+			 "\"ms\": %.4f, \"candidates_timed\": %d}", i ? ", " : "", e.key.c_str(), e.cc, e.wpc, e.ms,
+			 e.tried);
+		out += b;
+	}
+	return out + "]";
 }
 
 }  // namespace bam

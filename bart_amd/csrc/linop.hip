@@ -117,9 +117,19 @@ struct Op {
 	hipStream_t stream = 0;       // the handle's own stream
 	hipStream_t user_stream = 0;  // set by bart_amd_linop_set_stream (0: none)
 	bool normal_only = false;     // nufft_create_normal: no forward / adjoint
+	// low-memory modes (lowmem, zero_overhead): the plan holds a chunk of
+	// cchunk coils (its sub-grid stack S cchunk N instead of S C N) and is
+	// applied chunk by chunk; ctotal coils in all
+	int ctotal = 0, cchunk = 0;
+	DevBuf<float2> lm_part;       // per-chunk partial images of S^H (SENSE)
 	std::mutex& mtx() { return q->mtx; }
 	DevBuf<float2> stage_in, stage_out, kbuf;
 	std::unique_ptr<CgSolver> cg;
+	float* mon_err = nullptr;     // host array of bart_amd_cg_monitor (null: off)
+	// warm-start guess: A^H A b, partial sums, scalars, network channels in / out
+	DevBuf<float2> g_ab, g_b, g_x;
+	DevBuf<double> g_part;
+	DevBuf<float> g_sc, g_in, g_out;
 
 	~Op()
 	{
@@ -131,37 +141,88 @@ struct Op {
 
 	const float2* maps() const { return (OP_SENSE == kind) ? sens->ptr : nullptr; }
 
+	int nchunks() const { return (cchunk > 0) ? ctotal / cchunk : 1; }
+
+	// chunk k's coil images, maps and k-space
+	long cim_off(int k) const { return (long)k * cchunk * nufft->N; }
+	long ksp_off(int k) const { return (long)k * cchunk * nufft->M; }
+
+	// S^H over the chunks: partial image k of chunk k, summed in chunk order
+	// (allocated with the SENSE operator, so that no allocation happens
+	// inside a captured CG iteration)
+	float2* chunk_part(int k)
+	{
+		BAM_CHECK((long)lm_part.n >= (long)nchunks() * nufft->N, "low-memory SENSE: no partial images");
+		return lm_part.ptr + (long)k * nufft->N;
+	}
+
 	// device pointers, on the plan's current stream
 	void fwd(float2* d, const float2* s)
 	{
 		BAM_CHECK(!normal_only, "NuFFT with normal operator only!");
-		if (OP_NUDFT == kind)
+		if (OP_NUDFT == kind) {
 			nudft->forward(d, s);
-		else
-			nufft->forward(d, s, maps());
+			return;
+		}
+		for (int k = 0; k < nchunks(); k++) {
+			const float2* m = maps();
+			nufft->forward(d + ksp_off(k), m ? s : s + cim_off(k), m ? m + cim_off(k) : nullptr);
+		}
 	}
 
 	void adj(float2* d, const float2* s)
 	{
 		BAM_CHECK(!normal_only, "NuFFT with normal operator only!");
-		if (OP_NUDFT == kind)
+		if (OP_NUDFT == kind) {
 			nudft->adjoint(d, s);
-		else
-			nufft->adjoint(d, s, maps());
+			return;
+		}
+		const float2* m = maps();
+		if (1 == nchunks() || nullptr == m) {
+			for (int k = 0; k < nchunks(); k++)
+				nufft->adjoint(d + (m ? 0 : cim_off(k)), s + ksp_off(k), m);
+			return;
+		}
+		for (int k = 0; k < nchunks(); k++)
+			nufft->adjoint(chunk_part(k), s + ksp_off(k), m + cim_off(k));
+		sum_images(d, lm_part.ptr, nufft->N, nchunks(), nufft->stream, nullptr);
 	}
 
 	bool toeplitz() const { return (OP_NUDFT != kind) && nufft->toeplitz; }
 
 	// A^H A: Toeplitz when available, else adjoint(forward(x)) through kbuf
 	// (linops/linop.c:489-525); kbuf must be allocated (prepare_normal)
-	void nrm(float2* d, const float2* s, const int* skip)
+	// (with fuse: the CG step after it may be fused into the operator's last
+	// kernel; returns whether it was)
+	bool nrm(float2* d, const float2* s, const int* skip, const CgApFuse* fuse = nullptr)
 	{
-		if (toeplitz()) {
-			nufft->normal(d, s, maps(), skip);
-		} else {
+		if (!toeplitz()) {
 			fwd(kbuf.ptr, s);
 			adj(d, kbuf.ptr);
+			return false;
 		}
+
+		if (1 == nchunks())
+			return nufft->normal(d, s, maps(), skip, fuse);
+
+		const float2* m = maps();
+
+		if (nullptr == m) {
+			for (int k = 0; k < nchunks(); k++)
+				nufft->normal(d + cim_off(k), s + cim_off(k), nullptr, skip);
+			return false;
+		}
+
+		for (int k = 0; k < nchunks(); k++)
+			nufft->normal(chunk_part(k), s, m + cim_off(k), skip);
+
+		if (nullptr != fuse) {
+			cg_apdot_groups(*fuse, lm_part.ptr, nufft->N, nchunks(), nufft->stream);
+			return true;
+		}
+
+		sum_images(d, lm_part.ptr, nufft->N, nchunks(), nufft->stream, skip);
+		return false;
 	}
 
 	void prepare_normal()
@@ -186,7 +247,7 @@ struct Op {
 	{
 		if (OP_NUDFT == kind)
 			return (double)isize * (double)osize;
-		return (double)nufft->S * nufft->C * (double)nufft->N;
+		return (double)nufft->S * nufft->C * nchunks() * (double)nufft->N;
 	}
 
 	Nufft& plan(const char* what)
@@ -310,8 +371,8 @@ static void check_conf(const nufft_conf_s& conf, bool has_basis)
 	BAM_CHECK(2.f == conf.os, "nufft: only oversampling 2 is supported (chained NUFFT not implemented)");
 	BAM_CHECK(conf.decomp, "nufft: the non-decomposed oversampled grid (-1) is not supported");
 	BAM_CHECK(!conf.pcycle, "nufft: phase cycling is not supported");
-	BAM_CHECK(!conf.real && !conf.compress_psf && !conf.upper_triag,
-		  "nufft: real / compressed / upper-triangular PSF storage is not supported");
+	BAM_CHECK(!conf.compress_psf && !conf.upper_triag,
+		  "nufft: compressed / upper-triangular PSF storage is not supported");
 	BAM_CHECK(0 == conf.cfft, "nufft: cfft flags are not supported");
 	BAM_CHECK(7 == (conf.flags & 7), "nufft: flags must include dims 0-2");
 }
@@ -348,6 +409,7 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 {
 	NufftDims nd = parse_nufft_dims(N, ksp_dims, cim_dims, traj_dims);
 
+
 	std::vector<float2> th = to_host(traj, 3 * nd.M);
 	std::vector<float> tr(3 * nd.M);
 
@@ -366,10 +428,28 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 	op->q = std::make_shared<Queue>();
 	op->stream = op->q->stream;
 
+	// low-memory modes (noncart/nufft.c:1629-2003 keeps one sub-grid at a
+	// time, C N grid points): here coil chunks whose stack S c N is at most
+	// C N for lowmem, and single coils (S N) for zero_overhead; the largest
+	// chunk size that divides C
+	op->ctotal = nd.C;
+	op->cchunk = nd.C;
+
+	if (conf.lowmem || conf.zero_overhead) {
+		int S = 1;
+		for (int i = 0; i < 3; i++)
+			S *= (nd.img[i] > 1) ? 2 : 1;
+		int target = conf.zero_overhead ? 1 : std::max(1, nd.C / S);
+		while (nd.C % target)
+			target--;
+		op->cchunk = target;
+	}
+
 	try {
 		op->nufft = std::make_shared<Nufft>();
 		op->nufft->stream = op->stream;
-		op->nufft->create(nd.img, nd.C, nd.M, tr.data(), w.empty() ? nullptr : w.data(), conf.toeplitz,
+		op->nufft->real_psf = conf.real;
+		op->nufft->create(nd.img, op->cchunk, nd.M, tr.data(), w.empty() ? nullptr : w.data(), conf.toeplitz,
 				  conf.periodic, conf.width, conf.os, conf.toeplitz && !conf.nopsf);
 	} catch (...) {
 		delete op;
@@ -640,6 +720,15 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		memcpy(op->idims, img_dims, sizeof(long) * N);
 		op->isize = md_size(N, img_dims);
 
+		if (op->nchunks() > 1) {
+			try {
+				op->lm_part.alloc((size_t)op->nchunks() * op->nufft->N);
+			} catch (...) {
+				delete op;
+				throw;
+			}
+		}
+
 		long nmap = md_size(4, map_dims);
 
 		try {
@@ -668,6 +757,8 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 			nop->q = op->q;
 			nop->stream = op->stream;
 			nop->nufft = op->nufft;
+			nop->ctotal = op->ctotal;
+			nop->cchunk = op->cchunk;
 			*fft_opp = (const struct linop_s*)nop;
 		}
 
@@ -787,6 +878,7 @@ void nufft_update_psf2(const struct linop_s* _op, int ND, const long psf_dims[],
 		if ((long)op->nufft->psf.n < n)
 			op->nufft->psf.alloc(n);
 		copy_strides(ND, psf_dims, psf_strs, (float2*)psf, op->nufft->psf.ptr, false, op->stream);
+		op->nufft->apply_real_psf();
 		op->nufft->has_psf = true;
 		if (op->cg)
 			op->cg->drop_graph();
@@ -1032,7 +1124,9 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		// the k-space buffer of an explicit normal operator exists before
 		// the iterations are captured
 		op->prepare_normal();
-		NormalFn normal = [op](float2* d, const float2* s, const int* skip) { op->nrm(d, s, skip); };
+		NormalFn normal = [op](float2* d, const float2* s, const int* skip, const CgApFuse* f) {
+			return op->nrm(d, s, skip, f);
+		};
 
 		bool xd = on_device(x), bd = on_device(b);
 		float2* xdev = (float2*)x;
@@ -1059,6 +1153,9 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		if (nullptr != resid_trace && r.iterations > 0)
 			memcpy(resid_trace, r.trace.data(), sizeof(float) * r.iterations);
 
+		if (nullptr != op->mon_err)
+			op->cg->bt.monitor_errors(op->mon_err, r.iterations);
+
 		if (nullptr != stats) {
 			stats->iterations = r.iterations;
 			stats->status = r.status;
@@ -1069,6 +1166,36 @@ int lsqr_conjgrad(const struct linop_s* _op, int maxiter, float l2lambda, float 
 		}
 
 		return r.iterations;
+	}, -1);
+}
+
+int bart_amd_cg_monitor(const struct linop_s* _op, const float _Complex* truth, float* err)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op, "bart_amd_cg_monitor: NULL operator");
+		std::lock_guard<std::mutex> lock(op->mtx());
+		const long ne = op->isize;
+
+		if (!op->cg)
+			op->cg.reset(new CgSolver(2 * ne, op->stream));
+
+		if (nullptr == truth) {
+			op->cg->set_monitor(nullptr, 0.);
+			op->mon_err = nullptr;
+			return 0;
+		}
+
+		BAM_CHECK(nullptr != err, "bart_amd_cg_monitor: NULL error array");
+		std::vector<float2> h = to_host((const float2*)truth, ne);
+		double s = 0.;
+		for (const float2& v : h)
+			s += (double)v.x * v.x + (double)v.y * v.y;
+		DevBuf<float2> d(ne);
+		BAM_HIP(hipMemcpy(d.ptr, h.data(), sizeof(float2) * ne, hipMemcpyHostToDevice));
+		op->cg->set_monitor(d.ptr, sqrt(s));
+		op->mon_err = err;
+		return 0;
 	}, -1);
 }
 
@@ -1092,7 +1219,9 @@ int lsqr_conjgrad_noalloc(const struct linop_s* _op, int maxiter, float l2lambda
 		if (0 != op->user_stream)
 			BAM_HIP(hipStreamSynchronize(op->user_stream));
 
-		NormalFn normal = [op](float2* d, const float2* s, const int* skip) { op->nrm(d, s, skip); };
+		NormalFn normal = [op](float2* d, const float2* s, const int* skip, const CgApFuse* f) {
+			return op->nrm(d, s, skip, f);
+		};
 		CgResult r = cg_solve_noalloc(normal, l2lambda, maxiter, tol, (float2*)x, (const float2*)b, 2 * op->isize,
 					      warmstart, warm_fallback, op->stream, resid_trace);
 
@@ -1198,8 +1327,22 @@ int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxite
 			}
 		}
 
+		// When every problem is a 2-D Toeplitz SENSE operator, one stream
+		// whose iteration runs the normal operators of all problems grouped
+		// by image size, one launch per pass and size
+		// (BART_AMD_BATCH_GROUPS=off: the problems spread over streams instead)
+		static const bool groups_on = [] {
+			const char* e = getenv("BART_AMD_BATCH_GROUPS");
+			return !(nullptr != e && 0 == strcmp(e, "off"));
+		}();
+		bool grouped = groups_on && nprob > 1;
+
+		for (int i = 0; i < nprob && grouped; i++)
+			grouped = OP_SENSE == op[i]->kind && 1 == op[i]->nchunks() &&
+				  normal_groupable(*op[i]->nufft, op[i]->maps());
+
 		// longest-processing-time assignment of the problems to the streams
-		const int K = std::min(nprob, CG_BATCH_STREAMS);
+		const int K = grouped ? 1 : std::min(nprob, CG_BATCH_STREAMS);
 		std::vector<int> order(nprob);
 		for (int i = 0; i < nprob; i++)
 			order[i] = i;
@@ -1243,12 +1386,50 @@ int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxite
 			bt[k].reset(new CgBatch(nf, s));
 		}
 
+		// the grouped normal operators of a single batch (grouped)
+		std::shared_ptr<NormalGroupSet> gset;
+		CgGroupHooks hooks;
+
+		if (grouped) {
+
+			const std::vector<int>& mem = members[0];
+
+			hooks.prepare = [&op, &mem, &gset](CgBatch& b, std::vector<CgGrpItem>& items) {
+				std::vector<const Nufft*> plans;
+				std::vector<const float2*> sens, src;
+				std::vector<float2*> dst;
+				std::vector<const int*> skip;
+
+				for (size_t j = 0; j < mem.size(); j++) {
+					const Op* o = op[mem[j]];
+					plans.push_back(o->nufft.get());
+					sens.push_back(o->maps());
+					src.push_back((const float2*)(b.p.ptr + b.seg_h[j].off));
+					dst.push_back((float2*)(b.Ap.ptr + b.seg_h[j].off));
+					skip.push_back(&b.st.ptr[j].done);
+				}
+
+				std::vector<int> ng;
+				gset = normal_groups_build(plans, sens, src, dst, skip, ng);
+
+				for (size_t j = 0; j < mem.size(); j++) {
+					auto pr = normal_group_partials(*plans[j], dst[j]);
+					items[j] = CgGrpItem{ (const float*)pr.first, 2 * plans[j]->N, pr.second };
+				}
+			};
+
+			hooks.launch = [&gset](CgBatch& b) { normal_groups_launch(*gset, b.stream); };
+			bt[0]->hooks = &hooks;
+		}
+
 		std::vector<BatchNormalFn> fn(K);
 
 		for (int k = 0; k < K; k++) {
 
 			const std::vector<int>& mem = members[k];
-			fn[k] = [&op, &mem](int j, float2* d, const float2* s, const int* skip) { op[mem[j]]->nrm(d, s, skip); };
+			fn[k] = [&op, &mem](int j, float2* d, const float2* s, const int* skip, const CgApFuse* f) {
+				return op[mem[j]]->nrm(d, s, skip, f);
+			};
 
 			std::vector<const float2*> x0, bb;
 			for (int i : mem) {
@@ -1430,6 +1611,233 @@ int bart_amd_warmnet_train_step_grad(struct bart_amd_warmnet* net, const float* 
 		check_dev("bart_amd_warmnet_train_step_grad", x);
 		check_dev("bart_amd_warmnet_train_step_grad", dout);
 		((WarmNet*)net)->train_step_grad(x, dout, B, H, W, lr, default_stream());
+		return 0;
+	}, -1);
+}
+
+// first CG iterate from zero, its scale and the network channels of x1 / s
+// in op->g_in (b: host or device); returns the device pointer of b used
+static const float2* warm_input(Op* op, float l2lambda, int N, const long img_dims[], const float _Complex* b,
+				const char* what)
+{
+	check_dims(what, N, img_dims, op->N, op->idims);
+	BAM_CHECK(OP_NUDFT != op->kind && 1 == op->nufft->n[2] && op->nufft->n[0] > 1 && op->nufft->n[1] > 1,
+		  "%s: 2-D images only", what);
+	BAM_CHECK(!op->toeplitz() || op->nufft->has_psf, "%s: Toeplitz operator without PSF", what);
+	const long npix = op->isize;
+
+	if ((long)op->g_ab.n < npix) {
+		op->g_ab.alloc(npix);
+		op->g_in.alloc(2 * npix);
+		op->g_out.alloc(2 * npix);
+	}
+	if (op->g_part.n < (size_t)GUESS_PARTS) {
+		op->g_part.alloc(GUESS_PARTS);
+		op->g_sc.alloc(4);
+	}
+
+	const float2* bd = (const float2*)b;
+
+	if (!on_device(b)) {
+		if ((long)op->g_b.n < npix)
+			op->g_b.alloc(npix);
+		BAM_HIP(hipMemcpyAsync(op->g_b.ptr, b, sizeof(float2) * npix, hipMemcpyHostToDevice, op->stream));
+		bd = op->g_b.ptr;
+	}
+
+	op->prepare_normal();
+	op->nrm(op->g_ab.ptr, bd, nullptr);
+	guess_scalars(bd, op->g_ab.ptr, l2lambda, npix, op->g_part.ptr, op->g_sc.ptr, op->stream);
+	guess_pack(bd, op->g_sc.ptr, 1.f, npix, true, op->g_in.ptr, op->stream);
+	return bd;
+}
+
+int bart_amd_warmnet_input(const struct linop_s* _op, float l2lambda, int N, const long img_dims[],
+			   const float _Complex* b, float* inp, float* scale)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op && nullptr != b && nullptr != inp, "bart_amd_warmnet_input: NULL argument");
+		check_dev("bart_amd_warmnet_input", inp);
+		std::lock_guard<std::mutex> lock(op->mtx());
+		if (0 != op->user_stream)
+			BAM_HIP(hipStreamSynchronize(op->user_stream));
+		warm_input(op, l2lambda, N, img_dims, b, "bart_amd_warmnet_input");
+		BAM_HIP(hipMemcpyAsync(inp, op->g_in.ptr, sizeof(float) * 2 * op->isize, hipMemcpyDeviceToDevice,
+				       op->stream));
+		float sc[3];
+		BAM_HIP(hipMemcpyAsync(sc, op->g_sc.ptr, sizeof(float) * 3, hipMemcpyDeviceToHost, op->stream));
+		BAM_HIP(hipStreamSynchronize(op->stream));
+		if (nullptr != scale)
+			*scale = sc[1];
+		return 0;
+	}, -1);
+}
+
+int bart_amd_warmnet_pack(int N, const long img_dims[], const float _Complex* img, float scale, float* out)
+{
+	return guarded([&]() -> int {
+		BAM_CHECK(nullptr != img && nullptr != out && scale > 0.f, "bart_amd_warmnet_pack: bad argument");
+		check_dev("bart_amd_warmnet_pack", img);
+		check_dev("bart_amd_warmnet_pack", out);
+		const long npix = md_size(N, img_dims);
+		guess_pack((const float2*)img, nullptr, 1.f / scale, npix, false, out, default_stream());
+		BAM_HIP(hipStreamSynchronize(default_stream()));
+		return 0;
+	}, -1);
+}
+
+int bart_amd_warmnet_guess(struct bart_amd_warmnet* _net, const struct linop_s* _op, float l2lambda, int N,
+			   const long img_dims[], const float _Complex* b, float _Complex* x0)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		WarmNet* net = (WarmNet*)_net;
+		BAM_CHECK(nullptr != op && nullptr != net && nullptr != b && nullptr != x0,
+			  "bart_amd_warmnet_guess: NULL argument");
+		std::lock_guard<std::mutex> lock(op->mtx());
+		if (0 != op->user_stream)
+			BAM_HIP(hipStreamSynchronize(op->user_stream));
+		warm_input(op, l2lambda, N, img_dims, b, "bart_amd_warmnet_guess");
+		net->forward(op->g_in.ptr, 1, op->nufft->n[1], op->nufft->n[0], op->g_out.ptr, op->stream);
+
+		const long npix = op->isize;
+		float2* xd = (float2*)x0;
+		const bool xdev = on_device(x0);
+
+		if (!xdev) {
+			if ((long)op->g_x.n < npix)
+				op->g_x.alloc(npix);
+			xd = op->g_x.ptr;
+		}
+
+		guess_unpack(op->g_out.ptr, op->g_sc.ptr, npix, xd, op->stream);
+
+		if (!xdev)
+			BAM_HIP(hipMemcpyAsync(x0, xd, sizeof(float2) * npix, hipMemcpyDeviceToHost, op->stream));
+
+		BAM_HIP(hipStreamSynchronize(op->stream));
+		return 0;
+	}, -1);
+}
+
+int bart_amd_tuning_report(char* buf, int n)
+{
+	return guarded([&]() -> int {
+		const std::string r = tuning_report();
+		if (nullptr != buf && n > 0) {
+			strncpy(buf, r.c_str(), (size_t)n - 1);
+			buf[n - 1] = 0;
+		}
+		return (int)r.size();
+	}, -1);
+}
+
+// Strang's circulant preconditioner of a Toeplitz operator (opaque to C)
+struct Precond {
+	Op* op;
+	DevBuf<float2> pre;
+	int ncoil;
+};
+
+const struct operator_s* nufft_precond_create(const struct linop_s* _op)
+{
+	return guarded([&]() -> const struct operator_s* {
+		Op* op = (Op*)_op;
+		BAM_CHECK(nullptr != op && OP_NUDFT != op->kind, "nufft_precond_create: not a NUFFT operator");
+		BAM_CHECK(op->toeplitz(), "nufft_precond_create: the operator must be in Toeplitz mode");
+		BAM_CHECK(1 == op->nchunks(), "nufft_precond_create: not with the low-memory modes");
+		std::lock_guard<std::mutex> lock(op->mtx());
+		auto p = new Precond{ op, {}, (int)(op->isize / op->nufft->N) };
+		try {
+			op->nufft->compute_precond(p->pre);
+		} catch (...) {
+			delete p;
+			throw;
+		}
+		return (const struct operator_s*)p;
+	}, nullptr);
+}
+
+void operator_free(const struct operator_s* _p)
+{
+	guarded_void([&]() {
+		const Precond* p = (const Precond*)_p;
+		if (nullptr != p) {
+			BAM_HIP(hipStreamSynchronize(p->op->stream));
+			delete p;
+		}
+	});
+}
+
+int lsqr_conjgrad_precond(const struct linop_s* _op, const struct operator_s* _pre, int maxiter, float l2lambda,
+			  float tol, int N, const long img_dims[], float _Complex* x, const float _Complex* b,
+			  bool warmstart, float* resid_trace, struct cg_stats_s* stats)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		const Precond* pc = (const Precond*)_pre;
+		BAM_CHECK(nullptr != op && nullptr != pc && pc->op == op,
+			  "lsqr_conjgrad_precond: the preconditioner belongs to another operator");
+		check_dims("lsqr_conjgrad_precond", N, img_dims, op->N, op->idims);
+		std::lock_guard<std::mutex> lock(op->mtx());
+		if (0 != op->user_stream)
+			BAM_HIP(hipStreamSynchronize(op->user_stream));
+		op->set_plan_stream(op->stream);
+
+		const long ne = op->isize;
+		// its own solver: the preconditioned iteration is another graph
+		CgSolver cg(2 * ne, op->stream);
+
+		DevBuf<float2> xs(ne), bs(ne), bp(ne);
+		BAM_HIP(hipMemcpyAsync(bs.ptr, b, sizeof(float2) * ne, hipMemcpyDefault, op->stream));
+		if (warmstart)
+			BAM_HIP(hipMemcpyAsync(xs.ptr, x, sizeof(float2) * ne, hipMemcpyDefault, op->stream));
+
+		// CG on P (A^H A + lambda I) x = P b (iter/lsqr.c:126-142: the
+		// preconditioner chained after the normal equations and the adjoint;
+		// CG's own lambda 0)
+		const float lam = l2lambda;
+		op->nufft->apply_precond(pc->pre.ptr, bp.ptr, bs.ptr, pc->ncoil);
+		NormalFn normal = [op, pc, lam, ne](float2* d, const float2* s, const int* skip, const CgApFuse*) {
+			op->nrm(d, s, skip);
+			axpy(d, s, lam, ne, op->nufft->stream, skip);
+			op->nufft->apply_precond(pc->pre.ptr, d, d, pc->ncoil);
+			return false;
+		};
+
+		CgResult r = cg.solve(normal, 0.f, maxiter, tol, xs.ptr, bp.ptr, warmstart, false, true);
+		BAM_HIP(hipMemcpy(x, xs.ptr, sizeof(float2) * ne, hipMemcpyDefault));
+
+		if (nullptr != resid_trace && r.iterations > 0)
+			memcpy(resid_trace, r.trace.data(), sizeof(float) * r.iterations);
+
+		if (nullptr != stats) {
+			stats->iterations = r.iterations;
+			stats->status = r.status;
+			stats->resid = r.resid;
+			stats->bnorm = r.bnorm;
+			stats->time_ms = r.ms;
+			stats->fell_back = false;
+		}
+
+		return r.iterations;
+	}, -1);
+}
+
+int bart_amd_precond_apply(const struct operator_s* _pre, float _Complex* dst, const float _Complex* src)
+{
+	return guarded([&]() -> int {
+		const Precond* pc = (const Precond*)_pre;
+		BAM_CHECK(nullptr != pc, "bart_amd_precond_apply: NULL operator");
+		Op* op = pc->op;
+		std::lock_guard<std::mutex> lock(op->mtx());
+		const long ne = op->isize;
+		DevBuf<float2> d(ne);
+		BAM_HIP(hipMemcpyAsync(d.ptr, src, sizeof(float2) * ne, hipMemcpyDefault, op->stream));
+		op->nufft->apply_precond(pc->pre.ptr, d.ptr, d.ptr, pc->ncoil);
+		BAM_HIP(hipMemcpyAsync(dst, d.ptr, sizeof(float2) * ne, hipMemcpyDefault, op->stream));
+		BAM_HIP(hipStreamSynchronize(op->stream));
 		return 0;
 	}, -1);
 }

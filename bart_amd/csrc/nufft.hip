@@ -9,6 +9,7 @@
 // Every element-wise factor is fused into the first or last FFT pass; the
 // product with the coil maps (sense/modelnc.c, linops/fmac.c) and the sum
 // over coils of S^H are fused there as well.
+#include "cg.hpp"
 #include "nufft_impl.hpp"
 #include "prof.hpp"
 
@@ -1044,6 +1045,29 @@ static void sum_groups(float2* dst, const float2* part, long N, int G, hipStream
 	BAM_HIP(hipGetLastError());
 }
 
+__global__ void k_axpy(float2* __restrict__ d, const float2* __restrict__ x, float a, long n, const int* skip)
+{
+	if (nullptr != skip && 0 != *skip)
+		return;
+
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+		const float2 v = x[i];
+		d[i] = make_float2(d[i].x + a * v.x, d[i].y + a * v.y);
+	}
+}
+
+void axpy(float2* d, const float2* x, float a, long n, hipStream_t st, const int* skip)
+{
+	const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 8192);
+	hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(256), 0, st, d, x, a, n, skip);
+	BAM_HIP(hipGetLastError());
+}
+
+void sum_images(float2* dst, const float2* part, long N, int G, hipStream_t st, const int* skip)
+{
+	sum_groups(dst, part, N, G, st, skip);
+}
+
 void fill(float2* p, long n, float2 v, hipStream_t st, const int* skip)
 {
 	if (n <= 0)
@@ -1193,6 +1217,8 @@ void Nufft::set_traj(long M_, const float* traj, const float2* weights_host, boo
 
 	gp.build(g, traj);
 	has_traj = true;
+	if (tune)
+		tune_grid_adjoint(gp, kb.ptr, grid.ptr, stream);
 
 	if (toeplitz && compute_psf)
 		compute_psf_from_traj(traj, weights_host, true);
@@ -1344,7 +1370,22 @@ void Nufft::adjoint(float2* dst, const float2* ksp, const float2* sens)
 		sum_groups(dst, part.ptr, N, G, stream, nullptr);
 }
 
-void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int* skip)
+// the coil-group sum that ends a normal operator, or with fuse the CG step
+// fused into it
+static bool sum_groups_or_fuse(float2* dst, const float2* part, long N, int G, hipStream_t st, const int* skip,
+			       const CgApFuse* fuse)
+{
+	if (nullptr != fuse) {
+		BAM_CHECK((const float*)dst == fuse->Ap && fuse->len == 2 * N, "normal: fused CG step mismatch");
+		cg_apdot_groups(*fuse, part, N, G, st);
+		return true;
+	}
+
+	sum_groups(dst, part, N, G, st, skip);
+	return false;
+}
+
+bool Nufft::normal(float2* dst, const float2* src, const float2* sens, const int* skip, const CgApFuse* fuse)
 {
 	BAM_CHECK(has_psf, "Toeplitz normal operator without PSF");
 
@@ -1387,9 +1428,9 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 		launch_slice_inv(*this, out, sens, false, skip, "fft_toeplitz_slice_inv", pair_z);
 
 		if (out != dst)
-			sum_groups(dst, part.ptr, N, G, stream, skip);
+			return sum_groups_or_fuse(dst, part.ptr, N, G, stream, skip, fuse);
 
-		return;
+		return false;
 	}
 
 	const int cc = coil_group(S, C, n[act[0]], nullptr != sens);
@@ -1412,8 +1453,8 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 		launch_fft_axis(cfg, stream, ld, md, st);
 
 		if (out != dst)
-			sum_groups(dst, part.ptr, N, G, stream, skip);
-		return;
+			return sum_groups_or_fuse(dst, part.ptr, N, G, stream, skip, fuse);
+		return false;
 	}
 
 	{
@@ -1454,7 +1495,249 @@ void Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 	launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 
 	if (out != dst)
-		sum_groups(dst, part.ptr, N, G, stream, skip);
+		return sum_groups_or_fuse(dst, part.ptr, N, G, stream, skip, fuse);
+
+	return false;
+}
+
+// ---------------------------------------------------------------------------
+// Strang's circulant preconditioner (noncart/precond.c:46-71, 92-106)
+// ---------------------------------------------------------------------------
+
+__device__ inline float2 sep_fm(const SepTables& T, long i, const int n[3])
+{
+	const int x = (int)(i % n[0]);
+	const int y = (int)((i / n[0]) % n[1]);
+	const int z = (int)(i / ((long)n[0] * n[1]));
+	return cmul(cmul(T.fm[0][x], T.fm[1][y]), T.fm[2][z]);
+}
+
+__device__ inline float2 sep_lin(const SepTables& T, int s, long i, const int n[3])
+{
+	const int c[3] = { (int)(i % n[0]), (int)((i / n[0]) % n[1]), (int)(i / ((long)n[0] * n[1])) };
+	float2 v = make_float2(1.f, 0.f);
+	for (int d = 0; d < 3; d++)
+		if (T.bitpos[d] >= 0)
+			v = cmul(v, T.lin[d][(s >> T.bitpos[d]) & 1][c[d]]);
+	return v;
+}
+
+struct Int3 {
+	int v[3];
+};
+
+// tmp[s][i] = psf[s][i] conj(fm(i)) (ifftmod before the inverse FFT)
+__global__ void k_pre_load(SepTables T, Int3 n, long N, int S, const float2* __restrict__ psf, float2* __restrict__ tmp)
+{
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)S * N; e += (long)gridDim.x * blockDim.x)
+		tmp[e] = cmulc(psf[e], sep_fm(T, e % N, n.v));
+}
+
+// acc(i) = sum_s conj(fm(i)) tmp[s](i) conj(lin_s(i)) (ifftmod after the
+// inverse FFT, the cropped PSF of md_zfmacc2), then x fm(i) (fftmod before
+// the forward FFT)
+__global__ void k_pre_sum(SepTables T, Int3 n, long N, int S, const float2* __restrict__ tmp, float2* __restrict__ acc)
+{
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+		const float2 fm = sep_fm(T, i, n.v);
+		float2 a = make_float2(0.f, 0.f);
+		for (int s = 0; s < S; s++)
+			a = cadd(a, cmulc(cmulc(tmp[(long)s * N + i], fm), sep_lin(T, s, i, n.v)));
+		acc[i] = cmul(a, fm);
+	}
+}
+
+// pre(i) = |fm(i) acc(i)| + 1e-3 (fftmod after the forward FFT, md_zabs,
+// md_zsadd)
+__global__ void k_pre_abs(SepTables T, Int3 n, long N, float2* __restrict__ acc)
+{
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+		const float2 v = cmul(acc[i], sep_fm(T, i, n.v));
+		acc[i] = make_float2(sqrtf(v.x * v.x + v.y * v.y) + 1e-3f, 0.f);
+	}
+}
+
+// dst[c][i] = src[c][i] / pre[i]
+__global__ void k_pre_div(float2* __restrict__ dst, const float2* __restrict__ pre, long N, long total)
+{
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+		const float2 v = dst[e];
+		const float p = pre[e % N].x;
+		dst[e] = make_float2(v.x / p, v.y / p);
+	}
+}
+
+void Nufft::compute_precond(DevBuf<float2>& pre) const
+{
+	BAM_CHECK(toeplitz && has_psf, "nufft_precond_create: needs a Toeplitz operator with its PSF");
+	DevBuf<float2> tmp((size_t)S * N);
+	pre.alloc(N);
+	const Int3 nn{ { n[0], n[1], n[2] } };
+	const unsigned blk = (unsigned)std::min<long>(((long)S * N + 255) / 256, 8192);
+	const float sc = 1.f / sqrtf((float)N);
+
+	hipLaunchKernelGGL(k_pre_load, dim3(blk), dim3(256), 0, stream, tabs, nn, N, S, psf.ptr, tmp.ptr);
+	fft_nd(n, S, 7, +1.f, tmp.ptr, tmp.ptr, sc, stream);
+	hipLaunchKernelGGL(k_pre_sum, dim3(blk), dim3(256), 0, stream, tabs, nn, N, S, tmp.ptr, pre.ptr);
+	fft_nd(n, 1, 7, -1.f, pre.ptr, pre.ptr, sc, stream);
+	hipLaunchKernelGGL(k_pre_abs, dim3(blk), dim3(256), 0, stream, tabs, nn, N, pre.ptr);
+	BAM_HIP(hipGetLastError());
+	BAM_HIP(hipStreamSynchronize(stream));
+}
+
+// dst = IFFT(FFT(src) / pre) per coil, unnormalised FFTs over the image
+// dims (linop_fft_create, linops/someops.c:1732-1776): BART's
+// nufft_precond_apply (noncart/precond.c:92-106)
+void Nufft::apply_precond(const float2* pre, float2* dst, const float2* src, int ncoil) const
+{
+	fft_nd(n, ncoil, 7, -1.f, dst, src, 1.f, stream);
+	const long total = (long)ncoil * N;
+	const unsigned blk = (unsigned)std::min<long>((total + 255) / 256, 8192);
+	hipLaunchKernelGGL(k_pre_div, dim3(blk), dim3(256), 0, stream, dst, pre, N, total);
+	BAM_HIP(hipGetLastError());
+	fft_nd(n, ncoil, 7, +1.f, dst, dst, 1.f, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Normal operators of many 2-D plans in one launch per pass (batched CG)
+// ---------------------------------------------------------------------------
+
+// The three passes of Nufft::normal's 2-D path (first axis with maps and
+// linear phases; second axis, PSF, inverse; first axis back with the sums
+// over sub-grids and coil groups) for every plan of a size class, each pass
+// one launch of k_fft_axis_n_batch: the same work-groups, functors and
+// arithmetic as the plans' own launches, so the results are the same bits.
+struct NormalGroupSet {
+
+	struct Group {
+		int n0, n1;      // transform lengths of the first and second axis
+		int nb[3] = { 0, 0, 0 };
+		size_t lds[3] = { 0, 0, 0 };
+		DevBuf<FftItem<LdImgLin, NoMid, StGridS>> it1;
+		DevBuf<FftItem<LdGrid, MdPsf, StGrid>> it2;
+		DevBuf<FftItem<LdGridS, NoMid, StReduceS>> it3;
+		DevBuf<int2> bt[3];
+	};
+
+	std::vector<Group> groups;
+};
+
+bool normal_groupable(const Nufft& p, const float2* sens)
+{
+	if (2 != p.nact || p.slice || !p.toeplitz || !p.has_psf || nullptr == sens)
+		return false;
+
+	const int S = p.S, C = p.C;
+	const int cc = coil_group(S, C, p.n[p.act[0]], true);
+	GeomS gs = GeomS::make(p.n, p.act[0], S, C, cc);
+	VolAxis va = VolAxis::make(p.n, p.act[1], (long)S * C);
+	const FftLaunchCfg c1 = fft_config(p.axis[0], gs.va.transforms(), true, S * cc, -1.f, -1.f);
+	const FftLaunchCfg c2 = fft_config(p.axis[1], va.transforms(), va.contig(), 1, -1.f, +1.f);
+	return c1.fixed && c2.fixed && (C / cc <= 1 || (long)p.part.n >= (long)(C / cc) * p.N);
+}
+
+std::shared_ptr<NormalGroupSet> normal_groups_build(const std::vector<const Nufft*>& plans,
+						    const std::vector<const float2*>& sens,
+						    const std::vector<const float2*>& src, const std::vector<float2*>& dst,
+						    const std::vector<const int*>& skip, std::vector<int>& ngroups_out)
+{
+	auto set = std::make_shared<NormalGroupSet>();
+	std::vector<int> key_n0, key_n1;
+	std::vector<std::vector<int>> members;
+
+	for (size_t q = 0; q < plans.size(); q++) {
+		const Nufft& p = *plans[q];
+		const int n0 = p.n[p.act[0]], n1 = p.n[p.act[1]];
+		size_t g = 0;
+		while (g < members.size() && !(key_n0[g] == n0 && key_n1[g] == n1))
+			g++;
+		if (g == members.size()) {
+			key_n0.push_back(n0);
+			key_n1.push_back(n1);
+			members.emplace_back();
+		}
+		members[g].push_back((int)q);
+	}
+
+	set->groups.resize(members.size());
+	ngroups_out.assign(plans.size(), -1);
+
+	for (size_t g = 0; g < members.size(); g++) {
+
+		NormalGroupSet::Group& G = set->groups[g];
+		G.n0 = key_n0[g];
+		G.n1 = key_n1[g];
+		std::vector<FftItem<LdImgLin, NoMid, StGridS>> h1;
+		std::vector<FftItem<LdGrid, MdPsf, StGrid>> h2;
+		std::vector<FftItem<LdGridS, NoMid, StReduceS>> h3;
+		std::vector<int2> b1, b2, b3;
+
+		for (int j = 0; j < (int)members[g].size(); j++) {
+
+			const int q = members[g][j];
+			const Nufft& p = *plans[q];
+			const int S = p.S, C = p.C;
+			const int cc = coil_group(S, C, p.n[p.act[0]], true);
+			const int Gc = C / cc;
+			GeomS gs = GeomS::make(p.n, p.act[0], S, C, cc);
+			float2* out = (Gc > 1) ? p.part.ptr : dst[q];
+
+			FftLaunchCfg c1 = fft_config_skip(skip[q], p.axis[0], gs.va.transforms(), true, S * cc, -1.f, -1.f);
+			h1.push_back({ c1.L, LdImgLin{ gs, p.tabs, src[q], sens[q], false }, NoMid{},
+				       StGridS{ gs, p.tabs, p.grid.ptr, false } });
+
+			VolAxis va = VolAxis::make(p.n, p.act[1], (long)S * C);
+			FftLaunchCfg c2 = fft_config_skip(skip[q], p.axis[1], va.transforms(), va.contig(), 1, -1.f, +1.f);
+			h2.push_back({ c2.L, LdGrid{ va, p.grid.ptr }, MdPsf{ va, C, p.psf.ptr },
+				       StGrid{ va, p.tabs, p.grid.ptr, false } });
+
+			FftLaunchCfg c3 = fft_config_skip(skip[q], p.axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
+			h3.push_back({ c3.L, LdGridS{ gs, p.grid.ptr }, NoMid{}, StReduceS{ gs, p.tabs, out, sens[q], false } });
+
+			const FftLaunchCfg* cf[3] = { &c1, &c2, &c3 };
+			std::vector<int2>* bt[3] = { &b1, &b2, &b3 };
+			for (int ph = 0; ph < 3; ph++) {
+				BAM_CHECK(cf[ph]->fixed, "grouped normal: pass without a fixed-length kernel");
+				G.lds[ph] = std::max(G.lds[ph], cf[ph]->lds_bytes);
+				for (unsigned b = 0; b < cf[ph]->blocks; b++)
+					bt[ph]->push_back(make_int2(j, (int)b));
+			}
+
+			ngroups_out[q] = (int)g;
+		}
+
+		G.it1.alloc(h1.size());
+		G.it2.alloc(h2.size());
+		G.it3.alloc(h3.size());
+		BAM_HIP(hipMemcpy(G.it1.ptr, h1.data(), sizeof(h1[0]) * h1.size(), hipMemcpyHostToDevice));
+		BAM_HIP(hipMemcpy(G.it2.ptr, h2.data(), sizeof(h2[0]) * h2.size(), hipMemcpyHostToDevice));
+		BAM_HIP(hipMemcpy(G.it3.ptr, h3.data(), sizeof(h3[0]) * h3.size(), hipMemcpyHostToDevice));
+		const std::vector<int2>* bv[3] = { &b1, &b2, &b3 };
+		for (int ph = 0; ph < 3; ph++) {
+			G.nb[ph] = (int)bv[ph]->size();
+			G.bt[ph].alloc(bv[ph]->size());
+			BAM_HIP(hipMemcpy(G.bt[ph].ptr, bv[ph]->data(), sizeof(int2) * bv[ph]->size(), hipMemcpyHostToDevice));
+		}
+	}
+
+	return set;
+}
+
+void normal_groups_launch(const NormalGroupSet& set, hipStream_t st)
+{
+	for (const NormalGroupSet::Group& G : set.groups) {
+		launch_fft_axis_batch(G.n0, G.it1.ptr, G.bt[0].ptr, G.nb[0], G.lds[0], st, "fft_toeplitz_fwd");
+		launch_fft_axis_batch(G.n1, G.it2.ptr, G.bt[1].ptr, G.nb[1], G.lds[1], st, "fft_toeplitz");
+		launch_fft_axis_batch(G.n0, G.it3.ptr, G.bt[2].ptr, G.nb[2], G.lds[2], st, "fft_toeplitz_inv");
+	}
+}
+
+// the partial images of a plan's grouped normal operator: (pointer, groups)
+std::pair<const float2*, int> normal_group_partials(const Nufft& p, float2* dst)
+{
+	const int cc = coil_group(p.S, p.C, p.n[p.act[0]], true);
+	const int Gc = p.C / cc;
+	return (Gc > 1) ? std::make_pair((const float2*)p.part.ptr, Gc) : std::make_pair((const float2*)dst, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1555,6 +1838,7 @@ void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host,
 
 	Nufft p2;
 	p2.stream = stream;
+	p2.tune = false;  // one adjoint only
 	p2.create(n2, 1, M, traj2.data(), sqw.empty() ? nullptr : sqw.data(), false, periodic_psf, width, os, false);
 
 	long N2 = p2.N;
@@ -1581,8 +1865,26 @@ void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host,
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 	}
 
+	apply_real_psf();
 	BAM_HIP(hipStreamSynchronize(stream));
 	has_psf = true;
+}
+
+__global__ void k_real_part(float2* p, long n)
+{
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+		p[i].y = 0.f;
+}
+
+void Nufft::apply_real_psf()
+{
+	if (!real_psf)
+		return;
+
+	const long n = (long)S * N;
+	const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 4096);
+	hipLaunchKernelGGL(k_real_part, dim3(blocks), dim3(256), 0, stream, psf.ptr, n);
+	BAM_HIP(hipGetLastError());
 }
 
 }  // namespace bam

@@ -5,6 +5,11 @@
 #include "fft.hpp"
 #include "volume.hpp"
 
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
 namespace bam {
 
 constexpr int KB_SIZE = 1000;
@@ -100,6 +105,8 @@ struct GridPlan {
 	int nsplit = 0;
 	DevBuf<int> queue;       // adjoint work queue head
 	int ncu = 1;             // compute units of the device
+	int wpc = 2;             // persistent adjoint work-groups per compute unit (autotuned)
+	int cc_max = 4;          // largest coils-per-work-group the k-space buffer holds
 
 	FwdTiling ft;            // forward gather tiling (3-D)
 	DevBuf<int> fsm;         // forward order -> original sample index
@@ -132,6 +139,8 @@ struct SepTables {
 	const float2* fmxy;
 };
 
+struct CgApFuse;
+
 struct Nufft {
 
 	// geometry
@@ -157,6 +166,8 @@ struct Nufft {
 	GridPlan gp;
 	DevBuf<float2> psf;    // [S][N]
 	bool has_psf = false;
+	bool real_psf = false; // keep the real part of the PSF (conf.real, noncart/nufft.c:1128-1137)
+	bool tune = true;      // autotune the adjoint gridding launch when the plan is built
 	DevBuf<float2> grid;   // work [S][C][N]
 	DevBuf<float2> part;   // per-coil-group partial images [G][N]
 	AxisPlan axis[3];
@@ -180,15 +191,54 @@ struct Nufft {
 	// combined with coil maps
 	void forward(float2* ksp, const float2* cim, const float2* sens);
 	void adjoint(float2* cim, const float2* ksp, const float2* sens);
-	// Toeplitz; skip: optional device flag that turns the launches into no-ops
-	void normal(float2* dst, const float2* src, const float2* sens, const int* skip = nullptr);
+	// Toeplitz; skip: optional device flag that turns the launches into no-ops.
+	// fuse (CG): where the operator ends with a sum of coil-group partial
+	// images, that sum also takes the CG step dst += lambda src with its
+	// partials; returns whether it did.
+	bool normal(float2* dst, const float2* src, const float2* sens, const int* skip = nullptr,
+		    const CgApFuse* fuse = nullptr);
 	// PSF of the trajectory (periodic_psf: periodic gridding of the 2x
 	// grid, as nufft_set_traj always uses)
 	void compute_psf_from_traj(const float* traj, const float2* weights_host, bool periodic_psf);
+	// Strang's circulant preconditioner of the Toeplitz operator
+	// (noncart/precond.c:46-71): pre = |fftuc(sum_s ifftuc(psf_s)
+	// conj(linphase_s))| + 1e-3 over the image dims; apply: dst = IFFT(FFT(src)
+	// / pre) for ncoil coil images (unnormalised FFTs, precond.c:92-106)
+	void compute_precond(DevBuf<float2>& pre) const;
+	void apply_precond(const float2* pre, float2* dst, const float2* src, int ncoil) const;
+	// with real_psf: the imaginary part of the PSF set to zero (a real PSF
+	// multiplies both components of the grid alike, as BART's float PSF)
+	void apply_real_psf();
 };
+
+// Normal operators of several 2-D Toeplitz plans with coil maps, one launch
+// per pass and size class (batched CG).  normal_groupable: whether plan p
+// (with maps sens) can join; normal_groups_build: device tables for
+// plans[q] applied to src[q] -> dst[q] (or its partial images), skip flags
+// per plan, ngroups[q] = its size class; normal_groups_launch: the passes;
+// normal_group_partials: the partial images a plan's passes leave and their
+// count (dst itself and 1 without coil groups), to be summed by the caller.
+struct NormalGroupSet;
+bool normal_groupable(const Nufft& p, const float2* sens);
+std::shared_ptr<NormalGroupSet> normal_groups_build(const std::vector<const Nufft*>& plans,
+						    const std::vector<const float2*>& sens,
+						    const std::vector<const float2*>& src, const std::vector<float2*>& dst,
+						    const std::vector<const int*>& skip, std::vector<int>& ngroups);
+void normal_groups_launch(const NormalGroupSet& set, hipStream_t st);
+std::pair<const float2*, int> normal_group_partials(const Nufft& p, float2* dst);
 
 void grid_adjoint(const GridPlan& gp, const float* kb, float2* grid, const float2* ksp, const float2* weights,
 		  hipStream_t st);
+
+// Autotuner of the adjoint gridding launch: coils per work-group (4, 8, 16
+// up to the default) and persistent work-groups per compute unit (1, 2, 4)
+// are timed on the plan itself (the output does not depend on them) and
+// the fastest is kept, per shape class (dims, coils, samples) for the
+// process and in a cache file (BART_AMD_TUNE_FILE, default
+// $HOME/.cache/bart_amd_tune.txt; BART_AMD_AUTOTUNE=off: defaults).
+void tune_grid_adjoint(GridPlan& gp, const float* kb, float2* grid, hipStream_t st);
+// JSON list of the tuned shape classes of this process
+std::string tuning_report();
 void grid_forward(const GridPlan& gp, const float* kb, float2* ksp, const float2* grid, const float2* weights,
 		  hipStream_t st);
 
@@ -197,6 +247,10 @@ void fft_nd(const int dims[3], long batch, unsigned flags, float sign, float2* d
 
 void kb_table_host(double beta, float* table);
 void fill(float2* p, long n, float2 v, hipStream_t st, const int* skip = nullptr);
+// d += a x (vops->axpy)
+void axpy(float2* d, const float2* x, float a, long n, hipStream_t st, const int* skip = nullptr);
+// dst = sum_g part[g N ..], g = 0 .. G-1 in order (partial images)
+void sum_images(float2* dst, const float2* part, long N, int G, hipStream_t st, const int* skip);
 double calc_beta(float os, float width);
 
 }  // namespace bam

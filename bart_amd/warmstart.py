@@ -8,6 +8,7 @@ network runs on the device through the library (3x3 convolutions on the
 matrix cores); torch only holds device memory.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -84,16 +85,48 @@ class WarmNet:
         return float(loss)
 
 
-def to_channels(img, n):
-    """Complex [n*n] device image -> [1, 2, n, n] float32 (real, imaginary)."""
+def to_channels(img, n, scale=1.0):
+    """Complex [n*n] device image / scale -> [1, 2, n, n] float32 (real,
+    imaginary), packed by the library (bart_amd_warmnet_pack)."""
     import torch
-    return torch.view_as_real(img.reshape(n, n)).permute(2, 0, 1).contiguous().unsqueeze(0)
+    from . import _dims, _ptr
+    out = torch.empty((1, 2, n, n), dtype=torch.float32, device=img.device)
+    torch.cuda.synchronize()
+    _lib().bart_amd_warmnet_pack(DIMS, _dims([n, n]), _ptr(img.contiguous()), float(scale), _fptr(out))
+    _check()
+    return out
 
 
 def from_channels(t, n):
     """[1, 2, n, n] float32 -> complex [n*n] device image."""
     import torch
     return torch.view_as_complex(t[0].permute(1, 2, 0).contiguous()).reshape(n * n).contiguous()
+
+
+def network_input(op, b, n, lam):
+    """Network input channels [1, 2, n, n] of the first CG iterate from zero
+    x1 = alpha b, scaled by s = max |x1|, and s -- computed by the library
+    (bart_amd_warmnet_input)."""
+    import torch
+    from . import _dims, _ptr
+    inp = torch.empty((1, 2, n, n), dtype=torch.float32, device=b.device)
+    s = ctypes.c_float(0.0)
+    torch.cuda.synchronize()
+    _lib().bart_amd_warmnet_input(op.h, lam, DIMS, _dims(op.idims), _ptr(b), _fptr(inp), ctypes.byref(s))
+    _check()
+    return inp, float(s.value)
+
+
+def guess(net, op, b, lam):
+    """The learned initial guess x0 = s (x1 / s + net(x1 / s)), computed by
+    the library in one call (bart_amd_warmnet_guess)."""
+    import torch
+    from . import _dims, _ptr
+    x0 = torch.empty_like(b)
+    torch.cuda.synchronize()
+    _lib().bart_amd_warmnet_guess(net.h, op.h, lam, DIMS, _dims(op.idims), _ptr(b), _ptr(x0))
+    _check()
+    return x0
 
 
 # ---------------------------------------------------------------------------
@@ -129,21 +162,6 @@ def make_problem(seed, n=256, C=8, spokes=(32, 96), noise=1e-3, lam=0.01):
     return op, b, x
 
 
-def first_iterate(op, b, lam):
-    """x1 = alpha b, the first CG iterate from zero: the multiple of b that
-    the normal equations fit best; the network's input (scale-normalised)."""
-    from . import lsqr_conjgrad
-    x1, _, _ = lsqr_conjgrad(op, b, 1, lam, 0.0)
-    return x1
-
-
-def net_input(x1, n):
-    """Network input of the first iterate and its scale s = max |x1|."""
-    s = float(x1.abs().max())
-    s = s if s > 0 else 1.0
-    return to_channels(x1 / s, n), s
-
-
 def training_pairs(seeds, n=256, C=8, lam=0.01, maxiter=150, tol=1e-6, chunk=32):
     """(input, target) pairs: network input from x1, target the CG-SENSE
     solution to relative residual tol (at most maxiter iterations) scaled by
@@ -157,9 +175,9 @@ def training_pairs(seeds, n=256, C=8, lam=0.01, maxiter=150, tol=1e-6, chunk=32)
         bs = [p[1] for p in probs]
         sol, st, _ = lsqr_conjgrad_batch(ops, bs, maxiter, lam, tol)
         for (op, b, _), xsol, sti in zip(probs, sol, st):
-            inp, s = net_input(first_iterate(op, b, lam), n)
+            inp, s = network_input(op, b, n, lam)
             xs_in.append(inp)
-            xs_t.append(to_channels(xsol / s, n))
+            xs_t.append(to_channels(xsol, n, s))
             its.append(sti.iterations)
         for op in ops:
             op.free()
@@ -201,14 +219,11 @@ def solve_warm_and_cold(net, op, b, n, lam=0.01, tol=1e-4, maxiter=300):
     # it before either timed solve
     lsqr_conjgrad(op, b, 1, lam, 0.0)
     xc, stc, _ = lsqr_conjgrad(op, b, maxiter, lam, tol)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    x1 = first_iterate(op, b, lam)
-    inp, s = net_input(x1, n)
-    x0 = (from_channels(net.forward(inp), n) * s).contiguous()
-    e1.record()
+    guess(net, op, b, lam)  # warm-up (work buffers)
     torch.cuda.synchronize()
-    t_guess = e0.elapsed_time(e1)
+    t0 = time.perf_counter()
+    x0 = guess(net, op, b, lam)
+    t_guess = (time.perf_counter() - t0) * 1e3
     xw, stw, _ = lsqr_conjgrad(op, b, maxiter, lam, tol, x0=x0, fallback=True)
     diff = float((xw - xc).norm() / xc.norm())
     return dict(cold_it=stc.iterations, warm_it=stw.iterations, cold_ms=stc.time_ms,
@@ -222,7 +237,7 @@ def residual_problems(seeds, n=256, C=8, lam=0.01):
     out = []
     for sd in seeds:
         op, b, _ = make_problem(sd, n, C, lam=lam)
-        inp, s = net_input(first_iterate(op, b, lam), n)
+        inp, s = network_input(op, b, n, lam)
         out.append((op, b, inp, s, float(b.norm()) ** 2))
     return out
 

@@ -355,6 +355,8 @@ def run_batched(args, L, report):
         "batched_cg_ms": round(t_cg * 1e3, 3),
         "sequential_cg_ms": round(t_seq * 1e3, 3),
         "cg_iterations": rec[-1][2],
+        "grouped_launches": os.environ.get("BART_AMD_BATCH_GROUPS", "on") != "off",
+        "autotuned_adjoint_gridding": ba.tuning_report(),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -495,6 +497,9 @@ def main():
     ap.add_argument("--tol-solves", type=int, default=3, help="CG-SENSE solves to 1e-4 timed (median)")
     ap.add_argument("--tol-maxiter", type=int, default=200, help="iteration cap of the solves to 1e-4")
     ap.add_argument("--radial-solves", type=int, default=10, help="2-D radial solves to 1e-4 timed (median)")
+    ap.add_argument("--radial-warm-pairs", type=int, default=256,
+                    help="2-D radial warm start: training pairs of the same protocol (0: no warm start)")
+    ap.add_argument("--radial-warm-epochs", type=int, default=10, help="2-D radial warm start: training epochs")
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="hardware-counter target: run the adjoint, forward and SENSE normal operator twice "
@@ -743,6 +748,7 @@ def main():
         "fp64_nudft": accuracy,
         "normal_explicit_vs_toeplitz": normal_cmp,
         "kernels_ms_per_step": {k: round(v[0], 4) for k, v in sorted(per_step.items())},
+        "autotuned_adjoint_gridding": ba.tuning_report(),
     }
     if cpu_cg and radial and radial.get("cold"):
         out["cg_sense_gpu_over_cpu_radial2d"] = round(radial["cold"]["solves_per_s"] / cpu_cg["value"], 1)
@@ -754,8 +760,9 @@ def main():
 def radial_solves(args, dev):
     """CG-SENSE on the 2-D radial workload (256^2, 8 coils, 201 x 512,
     lambda 0.01) to relative residual 1e-4: A^H y then CG from zero, and
-    from the learned warm start when a trained network is present.  Also
-    the oracle's CPU solve of the same problem (about 10 s)."""
+    from the learned warm start (network trained in this run, unless
+    --radial-warm-pairs 0).  Also the oracle's CPU solve of the same problem
+    (about 10 s)."""
     import torch
     import bart_amd as ba
     w = workload("radial2d")
@@ -790,11 +797,71 @@ def radial_solves(args, dev):
                          solves_per_s=round(1.0 / t, 2)),
                warm=None)
     print(f"# radial2d cg-sense to 1e-4 (cold): {st.iterations} it, {t * 1e3:.3f} ms", file=sys.stderr, flush=True)
+    if args.radial_warm_pairs > 0:
+        out["warm"] = radial_warm(args, sense, w, y, b, dev)
+        print(f"# radial2d cg-sense to 1e-4 (warm): {out['warm']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu"] = cpu_cg_baseline(w, threads, y)
     sense.free()
     return out
+
+
+def radial_warm(args, sense, w, y, b, dev):
+    """The learned warm start for the 2-D radial solve: a network (6 layers,
+    48 channels) trained in this run on pairs of the same protocol --
+    the same trajectory, coils and lambda, other seeded ellipse phantoms with
+    noise -- (input: the first CG iterate scaled; target: the CG solution to
+    relative residual 1e-6), then the bench's own problem solved to 1e-4 from
+    the library's guess (bart_amd_warmnet_guess) with the zero-start fallback.
+    Times include A^H y and the guess."""
+    import torch
+    import bart_amd as ba
+    from bart_amd import inputs as I
+    from bart_amd import warmstart as WS
+
+    n = w["img"][0]
+    C, lam = w["C"], w["lam"]
+    M = w["R"] * w["S"]
+    t0 = time.perf_counter()
+    xin, xt = [], []
+    for k in range(args.radial_warm_pairs):
+        img = torch.from_numpy(I.ellipse_phantom(n, nell=10, seed=5000 + k).reshape(-1)).to(dev)
+        yk = sense.forward(img)
+        g = torch.Generator(device="cpu").manual_seed(5000 + k)
+        e = (torch.randn(M * C, generator=g) + 1j * torch.randn(M * C, generator=g)).to(torch.complex64)
+        yk = (yk + (1e-3 * float(yk.abs().max()) / np.sqrt(2.0)) * e.to(dev)).contiguous()
+        bk = sense.adjoint(yk)
+        inp, sc = WS.network_input(sense, bk, n, lam)
+        xs, _, _ = ba.lsqr_conjgrad(sense, bk, 150, lam, 1e-6)
+        xin.append(inp)
+        xt.append(WS.to_channels(xs, n, sc))
+    inputs, targets = torch.cat(xin), torch.cat(xt)
+    t_data = time.perf_counter() - t0
+    net = WS.WarmNet(48, seed=0)
+    t0 = time.perf_counter()
+    hist = WS.train(net, inputs, targets, epochs=args.radial_warm_epochs, batch=8, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+
+    def warm():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sense.adjoint(y, out=b)
+        x0 = WS.guess(net, sense, b, lam)
+        xs, st, _ = ba.lsqr_conjgrad(sense, b, 300, lam, 1e-4, x0=x0, fallback=True)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, st
+
+    warm()
+    rec = [warm() for _ in range(args.radial_solves)]
+    t = float(np.median([r[0] for r in rec]))
+    st = rec[-1][1]
+    net.free()
+    return dict(iterations=st.iterations, converged=st.status == 2, fell_back=bool(st.fell_back), ms=round(t * 1e3, 3),
+                solves_per_s=round(1.0 / t, 2), training_pairs=args.radial_warm_pairs,
+                training_epochs=args.radial_warm_epochs, train_loss_first_last=[hist[0], hist[-1]],
+                training_data_s=round(t_data, 2), training_s=round(t_train, 2))
 
 
 if __name__ == "__main__":

@@ -184,6 +184,24 @@ int lsqr_conjgrad(const struct linop_s* op, int maxiter, float l2lambda, float t
 		  BAM_COMPLEX* x, const BAM_COMPLEX* b, bool warmstart, bool warm_fallback, float* resid_trace,
 		  struct cg_stats_s* stats);
 
+/* The autotuner's choices (no counterpart in BART): the adjoint gridding
+ * launch of every NUFFT plan is tuned when the plan is built -- coils per
+ * work-group and persistent work-groups per compute unit, timed on the plan
+ * (results do not depend on them) -- and kept per shape class in a cache file
+ * (BART_AMD_TUNE_FILE; BART_AMD_AUTOTUNE=off keeps the defaults).  Writes a
+ * JSON list of this process's choices into buf (n bytes, truncated) and
+ * returns its full length, -1 on error. */
+int bart_amd_tuning_report(char* buf, int n);
+
+/* iter/monitor.c (iter_monitor_create) as pics -T uses it (src/pics.c:548-575):
+ * a truth image for the CG solves (lsqr_conjgrad) of op.  Each later solve
+ * writes err[k] = ||truth - x_k|| / ||truth|| for the iterate x_k that
+ * iteration k starts from, as conjgrad calls its monitor
+ * (iter/italgos.c:534); err is a host array of at least maxiter floats that
+ * stays valid while the monitor is set; truth NULL ends the monitoring.
+ * truth: host or device image of op's image dims.  Returns 0, -1 on error. */
+int bart_amd_cg_monitor(const struct linop_s* op, const BAM_COMPLEX* truth, float* err);
+
 /* lsqr_conjgrad without any device allocation: no counterpart in BART.  For
  * callers that cannot allocate (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  The same solve, with the same
@@ -199,6 +217,25 @@ int lsqr_conjgrad(const struct linop_s* op, int maxiter, float l2lambda, float t
 int lsqr_conjgrad_noalloc(const struct linop_s* op, int maxiter, float l2lambda, float tol, int N,
 			  const long img_dims[], BAM_COMPLEX* x, const BAM_COMPLEX* b, bool warmstart,
 			  bool warm_fallback, float* resid_trace, struct cg_stats_s* stats);
+
+/* noncart/precond.h -- replaces nufft_precond_create (noncart/precond.c:127-164):
+ * Strang's circulant preconditioner of a Toeplitz NUFFT (or SENSE) operator,
+ * pre = |fftuc(cropped PSF)| + 1e-3 (precond.c:46-71), applied as
+ * dst = IFFT(FFT(src) / pre) with unnormalised FFTs over the image dims
+ * (precond.c:92-106).  Released with operator_free. */
+struct operator_s;
+const struct operator_s* nufft_precond_create(const struct linop_s* nufft_op);
+void operator_free(const struct operator_s* op);
+/* the preconditioner on one vector of the operator's image dims (host or
+ * device); returns 0, -1 on error */
+int bart_amd_precond_apply(const struct operator_s* precond, BAM_COMPLEX* dst, const BAM_COMPLEX* src);
+/* lsqr2 with a preconditioner (iter/lsqr.c:126-142, as nufft -i -c uses it):
+ * CG on P (A^H A + l2lambda I) x = P b, P = precond, CG's own lambda 0; the
+ * stop rule of lsqr_conjgrad on the preconditioned residual.  x, b host or
+ * device vectors.  Returns the number of iterations, -1 on error. */
+int lsqr_conjgrad_precond(const struct linop_s* op, const struct operator_s* precond, int maxiter, float l2lambda,
+			  float tol, int N, const long img_dims[], BAM_COMPLEX* x, const BAM_COMPLEX* b, bool warmstart,
+			  float* resid_trace, struct cg_stats_s* stats);
 
 /* Batched CG: nprob independent problems (A_i^H A_i + l2lambda I) x_i = b_i,
  * each with its own operator ops[i] (image size, coils and trajectory may
@@ -261,6 +298,23 @@ double bart_amd_warmnet_train_step(struct bart_amd_warmnet* net, const float* x,
  * that forward's input */
 int bart_amd_warmnet_train_step_grad(struct bart_amd_warmnet* net, const float* x, const float* dout, int B, int H,
 				     int W, float lr);
+
+/* The network's input and the learned initial guess, computed on the
+ * device (no counterpart in BART; the x0 of pics -W, src/pics.c:580-594,
+ * iter/itop.c:54-66).  x1 = alpha b is the first CG iterate from zero on
+ * (A^H A + l2lambda I) x = b (alpha = ||b||^2 / b.(A^H A b + l2lambda b),
+ * as CG computes it; A^H A the operator's normal operator), s = max |x1|
+ * its scale, and the input channels [2][Y][X] (real, imaginary) of x1 / s.
+ * 2-D images of op's image dims; b, x0 host or device vectors; inp, out
+ * device arrays.  Each returns 0, -1 on error. */
+/* inp = channels of x1 / s, *scale = s */
+int bart_amd_warmnet_input(const struct linop_s* op, float l2lambda, int N, const long img_dims[],
+			   const BAM_COMPLEX* b, float* inp, float* scale);
+/* out = channels of img / scale (a training target; img and out device) */
+int bart_amd_warmnet_pack(int N, const long img_dims[], const BAM_COMPLEX* img, float scale, float* out);
+/* x0 = s (x1 / s + net(x1 / s)) */
+int bart_amd_warmnet_guess(struct bart_amd_warmnet* net, const struct linop_s* op, float l2lambda, int N,
+			   const long img_dims[], const BAM_COMPLEX* b, BAM_COMPLEX* x0);
 
 /* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
 enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };

@@ -111,6 +111,7 @@ def test_header_with_extra_sections(tmp_path):
     (["pics", "k", "s", "o"], "non-Cartesian"),
     (["pics", "--admm", "-t", "t", "k", "s", "o"], "admm"),
     (["nosuchtool"], "Unknown bart command"),
+    (["nufft", "--nufft-conf", "bogus", "t", "i", "o"], "Sub-option could not be parsed: bogus"),
 ])
 def test_cli_rejects_unsupported(tmp_path, args, msg):
     p = run_bart(args, tmp_path, check=False)

@@ -166,3 +166,23 @@ def test_batch_mixed_normal_operators(problems):
         x1, s1, t1 = ba.lsqr_conjgrad(o, bb, 15, 0.01, 1e-4)
         assert st[i].iterations == s1.iterations and same(xs[i], x1) and same(tr[i], t1), i
     op.free()
+
+
+def test_adjoint_gridding_autotuner_report():
+    # every plan's adjoint gridding launch is tuned when the plan is built
+    # (coils per work-group 4 .. default, work-groups per CU 1, 2, 4); the
+    # choice is reported per shape class and the adjoint keeps its accuracy
+    n, C, R, S = 96, 6, 192, 33
+    traj = I.radial_traj(R, S)
+    op = ba.nufft_create2(dims16(1, R, S, C), dims16(n, n, 1, C), dims16(3, R, S), ba.traj_array(traj),
+                          conf=ba.NufftConf.defaults(toeplitz=False))
+    rep = ba.tuning_report()
+    key = f"adj:{n}x{n}x1:S4:C{C}:M{R * S}"
+    mine = [e for e in rep if e["shape"].startswith(key)]
+    assert mine, rep
+    e = mine[-1]
+    assert e["coils_per_workgroup"] in (4, 8) and e["workgroups_per_cu"] in (1, 2, 4)
+    y = crandn((C, R * S), 71)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=False)
+    assert relerr(op.adjoint(y).reshape(C, 1, n, n), ref.adjoint(y)) < 1e-5
+    op.free()

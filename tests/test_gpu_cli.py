@@ -105,7 +105,26 @@ def test_cli_nufft_inverse(tmp_path):
         if it == 4:
             assert err < 1e-4
         else:
-            assert resid(x) < 2 * resid(xr)
+            # FP32 CG lags exact CG by a few steps in the energy norm it
+            # minimises: the tool's 20th iterate at least as good as the
+            # oracle's 17th (delayed convergence)
+            xr17, _, _, _ = ref.cgsense(ones, b, lam, 17, 0.0)
+
+            def phi(v):
+                v = v.reshape(1, n, n).astype(np.complex128)
+                return float(0.5 * np.vdot(v, ref.normal(v.astype(np.complex64)) + lam * v).real
+                             - np.vdot(b, v).real)
+
+            assert phi(x) <= phi(xr17)
+
+    # -c: Strang's circulant preconditioner (src/nufft.c:250-263): the tool's
+    # preconditioned CG equals the library call bit for bit
+    run_bart(["nufft", "-i", "-c", "-l", lam, "-m", 6, "t", "y", "xc"], tmp_path)
+    _, xc = read_raw(str(tmp_path / "xc"))
+    pre = ba.Precond(op)
+    xl, st, _ = ba.lsqr_conjgrad_precond(op, pre, bg, 6, lam, 0.0)
+    assert st.iterations == 6 and np.array_equal(xc, xl.reshape(-1))
+    pre.free()
     # -r: explicit A^H A instead of the Toeplitz operator (the same system up
     # to the accuracy of the Toeplitz embedding)
     run_bart(["nufft", "-i", "-r", "-l", lam, "-m", 4, "t", "y", "xr"], tmp_path)
@@ -145,6 +164,11 @@ def test_cli_pics_matches_oracle(tmp_path):
         r = b.astype(np.complex128) - ref.sense_normal(maps, x) - lam * x
         return np.linalg.norm(r) / np.linalg.norm(b)
 
+    def phi(x):
+        # CG minimises the energy norm of the error, 2 phi(x) + const
+        x = x.reshape(1, 64, 64).astype(np.complex128)
+        return float(0.5 * np.vdot(x, ref.sense_normal(maps, x) + lam * x).real - np.vdot(b, x).real)
+
     run_bart(["pics", "-l2", "-r", lam, "-i", 4, "-t", "t", "k", "s", "x4"], tmp_path)
     dims, x4 = read_raw(str(tmp_path / "x4"))
     assert dims == dims16(64, 64, 1)
@@ -156,8 +180,12 @@ def test_cli_pics_matches_oracle(tmp_path):
     run_bart(["pics", "-l2", "-r", lam, "-i", 30, "-t", "t", "k", "s", "x"], tmp_path)
     _, x = read_raw(str(tmp_path / "x"))
     xr, _, _, _ = ref.cgsense(maps, b, lam, 30, 0.0)
+    xr27, _, _, _ = ref.cgsense(maps, b, lam, 27, 0.0)
     print(f"pics -i 30: residual {resid(x):.2e} (oracle {resid(xr):.2e}), vs oracle {relerr(x, xr.reshape(-1)):.2e}")
-    assert resid(x) < 2 * resid(xr)
+    # FP32 CG lags exact CG by a few steps in the energy norm it minimises
+    # (delayed convergence): the tool's 30th iterate is at least as good as
+    # the oracle's 27th
+    assert phi(x) <= phi(xr27)
 
     # -S rescales the result by the data scale (the tool's scale comes from
     # its own A^H y, equal to this one to ~1e-6)
@@ -169,6 +197,27 @@ def test_cli_pics_matches_oracle(tmp_path):
     _, x1 = read_raw(str(tmp_path / "x1"))
     xr1, _, _, _ = ref.cgsense(maps, ref.sense_adjoint(maps, y), lam, 4, 0.0)
     assert relerr(x1, xr1.reshape(-1)) < 1e-4
+
+
+def test_cli_pics_truth_monitor(tmp_path):
+    # pics -T truth: one monitor line per iteration (iter/monitor.c:68, the
+    # error of the iterate the iteration starts from: 1 for the zero start),
+    # the truth compared as given (src/pics.c:548-575); the image is the same
+    # as without -T
+    ref, maps, y = make_pics_case(tmp_path, seed=43)
+    truth = I.ellipse_phantom(64, seed=43)
+    write_raw(str(tmp_path / "truth"), dims16(64, 64, 1), truth)
+    p = run_bart(["pics", "-S", "-w", 1, "-r", 0.01, "-i", 12, "-T", "truth", "-t", "t", "k", "s", "xm"], tmp_path)
+    lines = [ln for ln in p.stderr.splitlines() if ln.startswith("[Iter ")]
+    assert len(lines) == 12 and lines[0].startswith("[Iter 0001] Objective: -1.000000, Error: 1.000000")
+    errs = [float(ln.split("Error: ")[1]) for ln in lines]
+    assert errs[-1] < errs[1] < errs[0]
+    run_bart(["pics", "-S", "-w", 1, "-r", 0.01, "-i", 12, "-t", "t", "k", "s", "xp"], tmp_path)
+    assert np.array_equal(read_raw(str(tmp_path / "xm"))[1], read_raw(str(tmp_path / "xp"))[1])
+    # the error of the last monitored iterate (x_11) against the truth
+    run_bart(["pics", "-S", "-w", 1, "-r", 0.01, "-i", 11, "-t", "t", "k", "s", "x11"], tmp_path)
+    e11 = relerr(read_raw(str(tmp_path / "x11"))[1], truth.reshape(-1))
+    assert abs(errs[-1] / e11 - 1) < 1e-4
 
 
 def test_cli_pics_pattern_psf_and_warm_start(tmp_path):

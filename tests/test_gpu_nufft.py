@@ -457,6 +457,29 @@ def test_cgsense_tolerance_and_warm_start():
     assert np.array_equal(x4.reshape(-1), x.reshape(-1))
 
 
+def test_cg_monitor_errors_of_the_iterates():
+    # pics -T (iter/monitor.c): err[k] = ||truth - x_k|| / ||truth|| for the
+    # iterate x_k that iteration k starts from (x_0 = 0: error 1); x_k is
+    # the result of a solve stopped after k iterations; the monitor changes
+    # no bit of the solve
+    n, C, R, S = 64, 4, 128, 41
+    op, ref, maps, img, traj = make_sense(n, C, R, S, seed=34)
+    b = op.adjoint(ref.sense_forward(maps, img)).reshape(1, n, n)
+    x_plain, st_plain, tr_plain = ba.lsqr_conjgrad(op, b, 12, 0.01, 0.0)
+    mon = ba.CgMonitor(op, img, 12)
+    x, st, tr = ba.lsqr_conjgrad(op, b, 12, 0.01, 0.0)
+    err = mon.errors(st.iterations)
+    assert np.array_equal(x, x_plain) and np.array_equal(tr, tr_plain)
+    assert abs(err[0] - 1.0) < 1e-6
+    for k in (1, 2, 5, 11):
+        xk, _, _ = ba.lsqr_conjgrad(op, b, k, 0.01, 0.0)
+        assert abs(err[k] / relerr(xk.reshape(1, n, n), img) - 1) < 1e-5
+    assert err[11] < err[1]
+    mon.close()
+    x2, _, _ = ba.lsqr_conjgrad(op, b, 12, 0.01, 0.0)
+    assert np.array_equal(x2, x_plain)
+
+
 def test_device_pointers_with_torch():
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
@@ -481,3 +504,100 @@ def test_errors_are_reported():
     with pytest.raises(ba.BartError):
         ba.nufft_create2(dims16(1, 10, 1, 1), dims16(16, 16, 1, 1), dims16(3, 10, 1),
                          np.zeros(30, np.complex64), conf=ba.NufftConf.defaults(os=1.5))
+
+
+@pytest.mark.parametrize("img,C,mode", [((64, 64, 1), 8, "lowmem"), ((64, 64, 1), 6, "zero_overhead"),
+                                        ((32, 32, 32), 4, "lowmem"), ((64, 64, 64), 3, "zero_overhead")])
+def test_low_memory_modes(img, C, mode):
+    # --lowmem / --zero-mem (noncart/nufft.c:1629-2003): the plan holds a
+    # chunk of coils (stack S c N <= C N, or S N) and is applied chunk by
+    # chunk; NUFFT, SENSE and CG agree with the default mode to BART's
+    # tolerance for these modes (tests/nufft.mk:366-400, nrmse 1e-6), and
+    # the NUFFT of each coil is unchanged bit for bit
+    X, Y, Z = img
+    n = X
+    if Z == 1:
+        R, S = 2 * n, 33
+        traj = I.radial_traj(R, S)
+    else:
+        R, S = 2 * n, 60
+        traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
+    M = R * S
+    kw = {mode: True}
+    op0 = make_nufft(img, C, traj, R, S, toeplitz=True)
+    op1 = make_nufft(img, C, traj, R, S, toeplitz=True, **kw)
+    x = crandn((C, Z, Y, X), 61)
+    y = crandn((C, M), 62)
+    assert np.array_equal(op1.forward(x), op0.forward(x))
+    assert relerr(op1.adjoint(y), op0.adjoint(y)) < 1e-6
+    assert relerr(op1.normal(x), op0.normal(x)) < 1e-6
+    maps = I.gaussian_coils((Z, Y, X), C, seed=63)
+    conf0 = ba.NufftConf.defaults()
+    conf1 = ba.NufftConf.defaults(**kw)
+    s0 = ba.sense_nc_init(dims16(X, Y, Z, C), dims16(X, Y, Z, C), maps, dims16(1, R, S, C), dims16(3, R, S),
+                          ba.traj_array(traj), conf=conf0)
+    s1 = ba.sense_nc_init(dims16(X, Y, Z, C), dims16(X, Y, Z, C), maps, dims16(1, R, S, C), dims16(3, R, S),
+                          ba.traj_array(traj), conf=conf1)
+    im = crandn((Z, Y, X), 64)
+    assert np.array_equal(s1.forward(im), s0.forward(im))
+    b0, b1 = s0.adjoint(y), s1.adjoint(y)
+    assert relerr(b1, b0) < 1e-6
+    assert relerr(s1.normal(im), s0.normal(im)) < 1e-6
+    x0, st0, tr0 = ba.lsqr_conjgrad(s0, b0, 5, 0.01, 0.0)
+    x1, st1, tr1 = ba.lsqr_conjgrad(s1, b0, 5, 0.01, 0.0)
+    assert st1.iterations == 5 and relerr(x1, x0) < 1e-5
+    ref = O.Nufft(img, traj, C, toeplitz=True)
+    assert relerr(s1.normal(im).reshape(Z, Y, X), ref.sense_normal(maps, im)) < 1e-5
+    for o in (op0, op1, s0, s1):
+        o.free()
+
+
+@pytest.mark.parametrize("img,C", [((64, 64, 1), 2), ((32, 32, 32), 2)])
+def test_strang_preconditioner_and_preconditioned_cg(img, C):
+    # nufft_precond_create (noncart/precond.c:46-164) against the oracle's
+    # complex128 restatement; lsqr with the preconditioner (iter/lsqr.c:126-142):
+    # CG on P (A^H A + lambda I) x = P b, against complex128 CG on the
+    # oracle's operator and preconditioner
+    X, Y, Z = img
+    n = X
+    if Z == 1:
+        R, S = 2 * n, 41
+        traj = I.radial_traj(R, S)
+    else:
+        R, S = 2 * n, 80
+        traj = I.phyllotaxis_kooshball(spokes=S, readout=R)
+    op = make_nufft(img, C, traj, R, S, toeplitz=True)
+    ref = O.Nufft(img, traj, C, toeplitz=True)
+    pre = ba.Precond(op)
+    _, apply_o = O.precond_strang(ref)
+    x = crandn((C, Z, Y, X), 81)
+    assert relerr(pre.apply(x).reshape(C, Z, Y, X), apply_o(x)) < 1e-5
+    y = crandn((C, R * S), 82)
+    b = op.adjoint(y).reshape(C, Z, Y, X)
+    lam = 0.01
+    xg, st, trg = ba.lsqr_conjgrad_precond(op, pre, b, 5, lam, 0.0)
+    pn = lambda v: apply_o(ref.normal_fp64(v) + lam * np.asarray(v, np.complex128).reshape(C, Z, Y, X))
+    x64, tr64 = O.cg_fp64(pn, apply_o(b), 0.0, 5)
+    assert st.iterations == 5
+    assert float(np.max(np.abs(trg / tr64 - 1))) < 1e-4
+    assert relerr(xg.reshape(x64.shape), x64) < 1e-4
+    pre.free()
+    op.free()
+
+
+def test_real_psf():
+    # --nufft-conf real-psf (noncart/nufft.c:1128-1137): the PSF keeps its
+    # real part; the Toeplitz operator multiplies by it (against the oracle
+    # with the same real PSF)
+    n, C, R, S = 64, 2, 128, 41
+    traj = I.radial_traj(R, S)
+    op0 = make_nufft((n, n, 1), C, traj, R, S, toeplitz=True)
+    op1 = make_nufft((n, n, 1), C, traj, R, S, toeplitz=True, real=True)
+    p0, p1 = op0.get_psf(), op1.get_psf()
+    assert np.all(p1.imag == 0) and np.array_equal(p1.real, p0.real)
+    ref = O.Nufft((n, n, 1), traj, C, toeplitz=True)
+    ref.set_psf(ref.psf().real.astype(np.complex64))
+    x = crandn((C, 1, n, n), 91)
+    assert relerr(op1.normal(x).reshape(C, 1, n, n), ref.normal(x)) < 1e-5
+    op0.free()
+    op1.free()

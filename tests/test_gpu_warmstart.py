@@ -112,3 +112,36 @@ def test_train_step_with_given_output_gradient():
     b.train_step_grad(x, dout, 1e-3)
     assert np.allclose(a.get_params(), b.get_params(), rtol=0, atol=1e-7)
     assert np.allclose(a.get_grads(), b.get_grads(), rtol=1e-6, atol=1e-12)
+
+
+def test_network_input_and_guess_entries():
+    # bart_amd_warmnet_input / _guess: x1 = alpha b is CG's first iterate from
+    # zero (bitwise the 1-iteration solve), s = max |x1|, the channels of
+    # x1 / s, and x0 = s (x1 / s + net(x1 / s)); host pointers as well
+    import bart_amd as ba
+    from bart_amd import warmstart as WS
+    n = 64
+    op, b, _ = WS.make_problem(7, n=n, C=4)
+    lam = 0.01
+    x1, _, _ = ba.lsqr_conjgrad(op, b, 1, lam, 0.0)
+    inp, s = WS.network_input(op, b, n, lam)
+    s_ref = float(x1.abs().max())
+    assert abs(s / s_ref - 1) < 1e-6
+    ch = torch.stack([x1.real, x1.imag]).reshape(1, 2, n, n) / s
+    assert float((inp - ch).abs().max()) <= 1e-6 * float(ch.abs().max())
+    net = WS.WarmNet(8, seed=2)
+    x0 = WS.guess(net, op, b, lam)
+    y = net.forward(inp)
+    x0_ref = torch.complex(y[0, 0], y[0, 1]).reshape(-1) * s
+    assert float((x0 - x0_ref).abs().max()) <= 1e-6 * float(x0_ref.abs().max())
+    # host vectors through the same entry
+    bh = b.cpu().numpy()
+    xh = np.zeros_like(bh)
+    ba.lib().bart_amd_warmnet_guess(net.h, op.h, lam, ba.DIMS, ba._dims(op.idims), ba._ptr(bh), ba._ptr(xh))
+    ba._check()
+    assert np.array_equal(xh, x0.cpu().numpy())
+    # the training target packing
+    t = WS.to_channels(x1, n, s)
+    assert torch.equal(t, inp) or float((t - inp).abs().max()) <= 1e-6
+    net.free()
+    op.free()
