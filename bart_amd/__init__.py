@@ -382,16 +382,23 @@ def traj_array(traj_m3):
     return np.ascontiguousarray(t.reshape(-1, 3).astype(np.complex64))
 
 
-def nufft_create2(ksp_dims, cim_dims, traj_dims, traj, wgh_dims=None, weights=None, conf=None):
-    """noncart/nufft.h:48-57."""
+def nufft_create2(ksp_dims, cim_dims, traj_dims, traj, wgh_dims=None, weights=None, conf=None, bas_dims=None,
+                  basis=None):
+    """noncart/nufft.h:48-57.  With a subspace basis (bas_dims [1 1 1 1 1 T
+    K]): k-space [1 R S C 1 T], coil images [X Y Z C 1 1 K]."""
     conf = NufftConf.defaults() if conf is None else conf
     traj = _host_c64(traj) if not _is_torch(traj) else traj
     w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
+    bas = _host_c64(basis) if basis is not None and not _is_torch(basis) else basis
     if weights is not None and wgh_dims is None:
         wgh_dims = [1] + list(traj_dims[1:])
     h = lib().nufft_create2(DIMS, _dims(ksp_dims), _dims(cim_dims), _dims(traj_dims), _ptr(traj),
-                            _dims(wgh_dims) if wgh_dims is not None else None, _ptr(w), None, None, conf)
-    return Linop(h, ksp_dims, cim_dims)
+                            _dims(wgh_dims) if wgh_dims is not None else None, _ptr(w),
+                            _dims(bas_dims) if bas_dims is not None else None, _ptr(bas), conf)
+    odims = list(ksp_dims) + [1] * (DIMS - len(ksp_dims))
+    if basis is not None:
+        odims[6] = 1
+    return Linop(h, odims, cim_dims)
 
 
 def nufft_create(ksp_dims, cim_dims, traj_dims, traj, weights=None, conf=None):
@@ -441,19 +448,23 @@ def nudft_create(ksp_dims, cim_dims, traj_dims, traj):
 
 
 def sense_nc_init(max_dims, map_dims, maps, ksp_dims, traj_dims, traj, conf=None, wgs_dims=None, weights=None,
-                  nufft_op=False):
+                  nufft_op=False, basis_dims=None, basis=None):
     """sense/modelnc.c:28-67: A = NUFFT S, image -> k-space.  With nufft_op
-    the NUFFT operator sharing the plan (BART's fft_opp) is returned too."""
+    the NUFFT operator sharing the plan (BART's fft_opp) is returned too.
+    With a subspace basis (basis_dims [1 1 1 1 1 T K]): max_dims dim 6 = K,
+    k-space [1 R S C 1 T]."""
     conf = NufftConf.defaults() if conf is None else conf
     maps = _host_c64(maps) if not _is_torch(maps) else maps
     traj = _host_c64(traj) if not _is_torch(traj) else traj
     w = _host_c64(weights) if weights is not None and not _is_torch(weights) else weights
+    bas = _host_c64(basis) if basis is not None and not _is_torch(basis) else basis
     if weights is not None and wgs_dims is None:
         wgs_dims = [1] + list(traj_dims[1:3]) + [1] * 13
     fop = ctypes.c_void_p(0)
     h = lib().sense_nc_init(_dims(max_dims), _dims(map_dims), _ptr(maps), _dims(ksp_dims), _dims(traj_dims),
                             _ptr(traj), ctypes.byref(conf), _dims(wgs_dims) if wgs_dims is not None else None,
-                            _ptr(w), None, None, ctypes.byref(fop) if nufft_op else None, 0)
+                            _ptr(w), _dims(basis_dims) if basis_dims is not None else None, _ptr(bas),
+                            ctypes.byref(fop) if nufft_op else None, 0)
     img_dims = list(max_dims) + [1] * (DIMS - len(max_dims))
     img_dims[3] = 1
     img_dims[4] = 1

@@ -11,10 +11,11 @@
  * iter_conjgrad_defaults, iter/iter.c:51-60).
  * -s: the direct NUDFT instead of the NUFFT (noncart/nudft.c).
  * -c: Strang's circulant preconditioner for -i (noncart/precond.c).
+ * -B: a subspace basis [1 1 1 1 1 T K] (src/nufft.c:169-176, 208-212,
+ * 284-288): coil images [X Y Z C 1 1 K], k-space [1 R S C 1 T].
  * --nufft-conf: BART's sub-options.  -g is accepted (every path runs on
  * the GPU).  -1 (the image is the oversampled grid itself, gridded without
- * further oversampling), -B (basis) and -F/-T (B0) are rejected with an
- * error.
+ * further oversampling) and -F/-T (B0) are rejected with an error.
  */
 #include "tools.h"
 #include "cfl.h"
@@ -30,7 +31,8 @@ static void usage(void)
 {
 	fprintf(stderr,
 		"Usage: nufft [-a,-i] [-x x:y:z] [-t,-r] [-l lambda] [-m iter] [-P] [-s] [-g] [-p <pattern>]\n"
-		"             [-o os] [-w width] [--lowmem] [--zero-mem] [--no-precomp] <traj> <input> <output>\n"
+		"             [-o os] [-w width] [-c] [-B <basis>] [--lowmem] [--zero-mem] [--no-precomp]\n"
+		"             <traj> <input> <output>\n"
 		"\n"
 		"Perform non-uniform Fast Fourier Transform.\n");
 }
@@ -46,6 +48,7 @@ int main_nufft(int argc, char* argv[])
 	float lambda = 0.f;
 	int maxiter = 50;
 	const char* pattern_file = NULL;
+	const char* basis_file = NULL;
 	struct nufft_conf_s conf = nufft_conf_defaults;
 
 	enum { O_LOWMEM = 256, O_ZEROMEM, O_NOPRECOMP, O_NUFFTCONF };
@@ -86,7 +89,7 @@ int main_nufft(int argc, char* argv[])
 		case 'c': precond = true; break;
 		case 's': dft = true; break;
 		case '1': unsupported_option("nufft", "-1"); break;
-		case 'B': unsupported_option("nufft", "-B"); break;
+		case 'B': basis_file = optarg; break;
 		case 'F': unsupported_option("nufft", "-F"); break;
 		case 'T': unsupported_option("nufft", "-T"); break;
 		case O_NUFFTCONF: parse_nufft_conf(optarg, &conf); break;
@@ -145,6 +148,24 @@ int main_nufft(int argc, char* argv[])
 	if (dft && NULL != pattern)
 		cli_error("nufft: -s does not take a pattern (-p)");
 
+	long basis_dims[DIMS];
+	complex float* basis = NULL;
+
+	if (NULL != basis_file) {
+
+		basis = cfl_load(basis_file, DIMS, basis_dims);
+
+		for (int i = 0; i < DIMS; i++)
+			if (TE_DIM != i && COEFF_DIM != i && 1 != basis_dims[i])
+				cli_error("nufft: the basis may vary over dims 5 and 6 only");
+
+		if (dft)
+			cli_error("nufft: -s does not take a basis (-B)");
+
+		if (precond)
+			cli_error("nufft: -c does not take a basis (-B)");
+	}
+
 	if (inverse || adjoint) {
 
 		long ksp_dims[DIMS];
@@ -178,13 +199,20 @@ int main_nufft(int argc, char* argv[])
 		for (int i = 3; i < DIMS; i++)
 			coilim_dims[i] = ksp_dims[i];
 
+		if (NULL != basis) {
+
+			coilim_dims[COEFF_DIM] = basis_dims[COEFF_DIM];
+			coilim_dims[TE_DIM] = 1;
+		}
+
 		long n = cfl_size(DIMS, coilim_dims);
 		complex float* img = calloc((size_t)n, sizeof(complex float));
 
 		const struct linop_s* op = dft ? nudft_create(DIMS, 7, ksp_dims, coilim_dims, traj_dims, traj, NULL, NULL,
 							      NULL, NULL)
 					       : nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
-							       pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
+							       pattern ? pattern_dims : NULL, pattern,
+								       basis ? basis_dims : NULL, basis, conf);
 
 		if (inverse) {
 
@@ -231,13 +259,20 @@ int main_nufft(int argc, char* argv[])
 		for (int i = 0; i < DIMS; i++)
 			ksp_dims[i] = (i < 3) ? ((0 == i) ? 1 : traj_dims[i]) : coilim_dims[i];
 
+		if (NULL != basis) {
+
+			ksp_dims[TE_DIM] = basis_dims[TE_DIM];
+			ksp_dims[COEFF_DIM] = 1;
+		}
+
 		long m = cfl_size(DIMS, ksp_dims);
 		complex float* ksp = calloc((size_t)m, sizeof(complex float));
 
 		const struct linop_s* op = dft ? nudft_create(DIMS, 7, ksp_dims, coilim_dims, traj_dims, traj, NULL, NULL,
 							      NULL, NULL)
 					       : nufft_create2(DIMS, ksp_dims, coilim_dims, traj_dims, traj,
-							       pattern ? pattern_dims : NULL, pattern, NULL, NULL, conf);
+							       pattern ? pattern_dims : NULL, pattern,
+								       basis ? basis_dims : NULL, basis, conf);
 
 		linop_forward(op, DIMS, ksp_dims, ksp, DIMS, coilim_dims, img);
 		linop_free(op);
@@ -249,6 +284,7 @@ int main_nufft(int argc, char* argv[])
 
 	free(traj);
 	free(pattern);
+	free(basis);
 
 	return 0;
 }

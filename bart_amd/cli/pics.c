@@ -5,7 +5,7 @@
  *   pics -t traj [-l2] [-r lambda] [-i iter] [-p pattern] [-W img] [-S]
  *        [-w scaling] [-d level] [-g] [--gpu-gridding] [-U,--lowmem]
  *        [--no-toeplitz] [--psf_export file] [--psf_import file]
- *        [--tol t] kspace sensitivities output
+ *        [--tol t] [-B basis] [-T truth] kspace sensitivities output
  *
  * Steps as src/pics.c: the sampling pattern (from -p, else non-zero samples
  * over coils, misc/mri2.c:50) multiplies the k-space and is the NUFFT's
@@ -13,7 +13,9 @@
  * (sense/optcom.c:74-109) unless -w; then CG on (A^H A + lambda I) x = A^H y
  * (lsqr2 with an l2 penalty, grecon/italgo.c:84-101) from zero or the -W
  * image; -S multiplies the result by the scale.  --tol (not in BART, whose
- * CG runs a fixed iteration count) stops at |r| <= tol |A^H y|.
+ * CG runs a fixed iteration count) stops at |r| <= tol |A^H y|.  -B: a
+ * subspace basis [1 1 1 1 1 T K]; the image then has K coefficients in
+ * dim 6 (src/pics.c:292-333).
  * Other regularisers (-l1, -R, wavelets, TV, ...) and algorithms are rejected.
  */
 #include "tools.h"
@@ -31,7 +33,8 @@ static void usage(void)
 	fprintf(stderr,
 		"Usage: pics -t <traj> [-l2] [-r lambda] [-i iter] [-p <pattern>] [-W <img>] [-S] [-w scaling]\n"
 		"            [-d level] [-g] [--gpu-gridding] [-U,--lowmem] [--no-toeplitz]\n"
-		"            [--psf_export <file>] [--psf_import <file>] [--tol t] <kspace> <sensitivities> <output>\n"
+		"            [--psf_export <file>] [--psf_import <file>] [--tol t] [-B <basis>] [-T <truth>]\n"
+		"            <kspace> <sensitivities> <output>\n"
 		"\n"
 		"Parallel-imaging reconstruction (non-Cartesian CG-SENSE with l2 regularisation).\n");
 }
@@ -44,6 +47,7 @@ int main_pics(int argc, char* argv[])
 	const char* truth_file = NULL;
 	const char* psf_ofile = NULL;
 	const char* psf_ifile = NULL;
+	const char* basis_file = NULL;
 	float lambda = -1.f;
 	int maxiter = 30;
 	float scaling = 0.f;
@@ -111,6 +115,7 @@ int main_pics(int argc, char* argv[])
 		case 'D': case 'F': case 'J': case 'u': case 'C': case 'o':
 			break;
 		case 'T': truth_file = optarg; break;
+		case 'B': basis_file = optarg; break;
 
 		case O_UNSUP: unsupported_option("pics", lopts[li].name); break;
 		case 'h': usage(); return 0;
@@ -153,9 +158,35 @@ int main_pics(int argc, char* argv[])
 	long traj_dims[DIMS];
 	complex float* traj = cfl_load(traj_file, DIMS, traj_dims);
 
+	// subspace basis [1 1 1 1 1 T K] (src/pics.c:292-333): the image holds
+	// K coefficients in dim 6 instead of the T frames of dim 5
+	long basis_dims[DIMS];
+	complex float* basis = NULL;
+
+	if (NULL != basis_file) {
+
+		basis = cfl_load(basis_file, DIMS, basis_dims);
+
+		for (int i = 0; i < DIMS; i++)
+			if (TE_DIM != i && COEFF_DIM != i && 1 != basis_dims[i])
+				cli_error("pics: the basis may vary over dims 5 and 6 only");
+	}
+
 	long max_dims[DIMS];
 	memcpy(max_dims, ksp_dims, sizeof(max_dims));
 	memcpy(max_dims, map_dims, 5 * sizeof(long));
+
+	if (NULL != basis) {
+
+		if (1 != ksp_dims[COEFF_DIM])
+			cli_error("pics: k-space must have size 1 in dim 6 with a basis");
+
+		if (basis_dims[TE_DIM] != ksp_dims[TE_DIM])
+			cli_error("pics: basis and k-space do not match in dim 5");
+
+		max_dims[COEFF_DIM] = basis_dims[COEFF_DIM];
+		max_dims[TE_DIM] = 1;
+	}
 
 	long img_dims[DIMS];
 	memcpy(img_dims, max_dims, sizeof(img_dims));
@@ -225,7 +256,8 @@ int main_pics(int argc, char* argv[])
 
 	const struct linop_s* nufft_op = NULL;
 	const struct linop_s* forward_op = sense_nc_init(max_dims, map_dims, maps, ksp_dims, traj_dims, traj, &conf,
-							 pat_dims, pattern, NULL, NULL, &nufft_op, 0);
+							 pat_dims, pattern, basis ? basis_dims : NULL, basis,
+							 &nufft_op, 0);
 
 	if (NULL != psf_ofile) {
 
@@ -357,6 +389,7 @@ int main_pics(int argc, char* argv[])
 	free(b);
 	free(image);
 	free(pattern);
+	free(basis);
 	free(traj);
 	free(maps);
 	free(kspace);

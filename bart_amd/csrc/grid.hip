@@ -65,16 +65,6 @@ void kb_table_host(double beta, float* table)
 	}
 }
 
-// intlookup (noncart/grid.c:126-149): x in [0, 0.5]
-__device__ inline float kb_lookup(const float* table, float x)
-{
-	x *= 2.f;
-	float xs = x * (float)(KB_SIZE - 1);
-	int index = (int)xs;
-	index = min(max(index, 0), KB_SIZE - 1);
-	float fpart = xs - (float)index;
-	return (1.f - fpart) * table[index] + fpart * table[index + 1];
-}
 
 // conj(fftmod) of grid point (x, y, z) applied to an adjoint gridding result
 __device__ inline float2 apply_conj_fm(const GridGeom& g, float2 v, int x, int y, int z)

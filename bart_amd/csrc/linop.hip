@@ -122,6 +122,13 @@ struct Op {
 	// applied chunk by chunk; ctotal coils in all
 	int ctotal = 0, cchunk = 0;
 	DevBuf<float2> lm_part;       // per-chunk partial images of S^H (SENSE)
+	// subspace basis (K > 0): ncoil coils, K coefficients, T frames of Mf
+	// samples; the plan's channels are the coil-coefficient pairs.  bksp:
+	// coefficient k-space (ncoil K T Mf), bcim: coefficient coil images of
+	// a SENSE operator (ncoil K N)
+	int K = 0, T = 1, ncoil = 0;
+	long Mf = 0;
+	DevBuf<float2> bksp, bcim;
 	std::mutex& mtx() { return q->mtx; }
 	DevBuf<float2> stage_in, stage_out, kbuf;
 	std::unique_ptr<CgSolver> cg;
@@ -164,6 +171,19 @@ struct Op {
 			nudft->forward(d, s);
 			return;
 		}
+		if (K > 0) {
+			// maps, coefficient NUFFT, contraction with the basis
+			// (noncart/nufft.c:1450-1453)
+			const float2* m = maps();
+			const float2* cim = s;
+			if (nullptr != m) {
+				maps_expand(bcim.ptr, m, s, ncoil, K, nufft->N, nufft->stream);
+				cim = bcim.ptr;
+			}
+			nufft->forward(bksp.ptr, cim, nullptr);
+			basis_contract(d, bksp.ptr, nufft->bas.ptr, ncoil, K, T, Mf, nufft->stream);
+			return;
+		}
 		for (int k = 0; k < nchunks(); k++) {
 			const float2* m = maps();
 			nufft->forward(d + ksp_off(k), m ? s : s + cim_off(k), m ? m + cim_off(k) : nullptr);
@@ -175,6 +195,16 @@ struct Op {
 		BAM_CHECK(!normal_only, "NuFFT with normal operator only!");
 		if (OP_NUDFT == kind) {
 			nudft->adjoint(d, s);
+			return;
+		}
+		if (K > 0) {
+			// x conj(basis) (noncart/nufft.c:1483-1487), coefficient
+			// adjoint NUFFT, maps
+			const float2* m = maps();
+			basis_expand(bksp.ptr, s, nufft->bas.ptr, ncoil, K, T, Mf, nufft->stream);
+			nufft->adjoint((nullptr != m) ? bcim.ptr : d, bksp.ptr, nullptr);
+			if (nullptr != m)
+				maps_reduce(d, m, bcim.ptr, ncoil, K, nufft->N, nufft->stream);
 			return;
 		}
 		const float2* m = maps();
@@ -199,6 +229,18 @@ struct Op {
 		if (!toeplitz()) {
 			fwd(kbuf.ptr, s);
 			adj(d, kbuf.ptr);
+			return false;
+		}
+
+		if (K > 0) {
+			const float2* m = maps();
+			if (nullptr == m) {
+				nufft->normal_basis(d, s, skip);
+				return false;
+			}
+			maps_expand(bcim.ptr, m, s, ncoil, K, nufft->N, nufft->stream, skip);
+			nufft->normal_basis(bcim.ptr, bcim.ptr, skip);
+			maps_reduce(d, m, bcim.ptr, ncoil, K, nufft->N, nufft->stream, skip);
 			return false;
 		}
 
@@ -253,6 +295,7 @@ struct Op {
 	Nufft& plan(const char* what)
 	{
 		BAM_CHECK(OP_NUDFT != kind && nullptr != nufft, "%s: not a NUFFT operator", what);
+		BAM_CHECK(0 == K, "%s: not supported for an operator with a subspace basis", what);
 		return *nufft;
 	}
 };
@@ -365,20 +408,25 @@ static NufftDims parse_nufft_dims(int N, const long* ksp_dims, const long* cim_d
 	return r;
 }
 
+// upper_triag (noncart/nufft.c:1070-1074) stores the Hermitian K x K PSF
+// as its upper triangle; the full K x K storage kept here holds the same
+// values, so the option is accepted
 static void check_conf(const nufft_conf_s& conf, bool has_basis)
 {
-	BAM_CHECK(!has_basis, "nufft: subspace basis is not supported by this library");
+	(void)has_basis;
 	BAM_CHECK(2.f == conf.os, "nufft: only oversampling 2 is supported (chained NUFFT not implemented)");
 	BAM_CHECK(conf.decomp, "nufft: the non-decomposed oversampled grid (-1) is not supported");
 	BAM_CHECK(!conf.pcycle, "nufft: phase cycling is not supported");
-	BAM_CHECK(!conf.compress_psf && !conf.upper_triag,
-		  "nufft: compressed / upper-triangular PSF storage is not supported");
+	BAM_CHECK(!conf.compress_psf, "nufft: compressed PSF storage is not supported");
 	BAM_CHECK(0 == conf.cfft, "nufft: cfft flags are not supported");
 	BAM_CHECK(7 == (conf.flags & 7), "nufft: flags must include dims 0-2");
 }
 
-// weights of shape wgh_dims (broadcast over coils) -> per-sample vector
-static std::vector<float2> expand_weights(int N, const long* wgh_dims, const float2* w, const long* ksp_dims)
+// weights of shape wgh_dims (broadcast over coils) -> per-sample vector;
+// with T > 1 frames (subspace basis) the weights may vary over dim 5 and
+// the vector holds the frames' samples r + R (s + Sp t)
+static std::vector<float2> expand_weights(int N, const long* wgh_dims, const float2* w, const long* ksp_dims,
+					  long T = 1)
 {
 	std::vector<float2> out;
 
@@ -387,26 +435,118 @@ static std::vector<float2> expand_weights(int N, const long* wgh_dims, const flo
 
 	long R = ksp_dims[1], Sp = ksp_dims[2];
 	long wr = wgh_dims[1], ws = wgh_dims[2];
+	long wt = (N > 5) ? wgh_dims[5] : 1;
 
 	BAM_CHECK(1 == wgh_dims[0], "nufft: weights dim 0 must be 1");
 	BAM_CHECK((wr == R || wr == 1) && (ws == Sp || ws == 1), "Incompatible dimensions of k-space and weights!");
+	BAM_CHECK(wt == 1 || (wt == T && T > 1), "Incompatible dimensions of k-space and weights!");
 
 	for (int i = 3; i < N; i++)
-		BAM_CHECK(1 == wgh_dims[i], "nufft: weights must not vary over dims >= 3");
+		BAM_CHECK(1 == wgh_dims[i] || (5 == i && T > 1), "nufft: weights must not vary over dims >= 3");
 
-	std::vector<float2> h = to_host(w, wr * ws);
-	out.resize(R * Sp);
+	std::vector<float2> h = to_host(w, wr * ws * wt);
+	out.resize(R * Sp * T);
 
-	for (long s = 0; s < Sp; s++)
-		for (long r = 0; r < R; r++)
-			out[r + R * s] = h[(wr > 1 ? r : 0) + wr * (ws > 1 ? s : 0)];
+	for (long t = 0; t < T; t++)
+		for (long s = 0; s < Sp; s++)
+			for (long r = 0; r < R; r++)
+				out[r + R * (s + Sp * t)] = h[(wr > 1 ? r : 0) + wr * ((ws > 1 ? s : 0) + ws * (wt > 1 ? t : 0))];
 
 	return out;
 }
 
-static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims, const float2* traj,
-			 const long* wgh_dims, const float2* weights, const nufft_conf_s& conf)
+// NUFFT with a subspace basis (noncart/nufft.c:1018-1036, 1208-1258):
+// basis [1 1 1 1 1 T K], k-space [1 R S C 1 T (1|K)] (the operator's
+// output [1 R S C 1 T 1]), coil images [X Y Z C 1 1 K], trajectory
+// [3 R S 1 1 (1|T)], weights [1 (1|R) (1|S) 1 1 (1|T)].  The plan grids the
+// C K coefficient images at the T Mf samples of all frames.
+static Op* make_basis_op(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims, const float2* traj,
+			 const long* wgh_dims, const float2* weights, const nufft_conf_s& conf, const long* bas_dims,
+			 const float2* basis)
 {
+	BAM_CHECK(N >= 7 && N <= 32, "nufft: a subspace basis needs at least 7 dims (got %d)", N);
+
+	for (int i = 0; i < N; i++)
+		BAM_CHECK(1 == bas_dims[i] || 5 == i || 6 == i, "nufft: the basis may vary over dims 5 and 6 only");
+
+	const long T = bas_dims[5], K = bas_dims[6];
+	BAM_CHECK(K >= 1 && K <= BASIS_MAX, "nufft: %ld basis coefficients (at most %d)", K, BASIS_MAX);
+	BAM_CHECK(T >= 1, "nufft: bad basis dims");
+	BAM_CHECK(ksp_dims[5] == T, "nufft: k-space dim 5 (%ld) does not match the basis (%ld)", ksp_dims[5], T);
+	BAM_CHECK(1 == ksp_dims[6] || K == ksp_dims[6], "nufft: k-space dim 6 must be 1 or %ld", K);
+	BAM_CHECK(1 == cim_dims[5] && K == cim_dims[6],
+		  "nufft: coil images must have size 1 in dim 5 and %ld (coefficients) in dim 6", K);
+	BAM_CHECK(1 == traj_dims[6] && (1 == traj_dims[5] || T == traj_dims[5]),
+		  "nufft: trajectory dim 5 must be 1 or %ld, dim 6 must be 1", T);
+
+	long k1[32], c1[32], t1[32], out[32];
+
+	for (int i = 0; i < N; i++) {
+		k1[i] = (5 == i || 6 == i) ? 1 : ksp_dims[i];
+		c1[i] = (6 == i) ? 1 : cim_dims[i];
+		t1[i] = (5 == i) ? 1 : traj_dims[i];
+		out[i] = (6 == i) ? 1 : ksp_dims[i];
+	}
+
+	NufftDims nd = parse_nufft_dims(N, k1, c1, t1);
+	const long Mf = nd.M;
+	const long Mp = Mf * T;
+	const bool traj_t = (T > 1 && T == traj_dims[5]);
+
+	std::vector<float2> th = to_host(traj, 3 * Mf * (traj_t ? T : 1));
+	std::vector<float> tr(3 * Mp);
+
+	for (long t = 0; t < T; t++)
+		for (long m = 0; m < Mf; m++)
+			for (int d = 0; d < 3; d++)
+				tr[3 * (m + Mf * t) + d] = th[3 * (m + Mf * (traj_t ? t : 0)) + d].x;
+
+	std::vector<float2> w = expand_weights(N, wgh_dims, weights, ksp_dims, T);
+	std::vector<float2> B = to_host(basis, K * T);
+
+	auto op = new Op();
+	op->kind = OP_NUFFT;
+	op->N = N;
+	memcpy(op->odims, out, sizeof(long) * N);
+	memcpy(op->idims, cim_dims, sizeof(long) * N);
+	op->osize = md_size(N, out);
+	op->isize = md_size(N, cim_dims);
+	op->q = std::make_shared<Queue>();
+	op->stream = op->q->stream;
+	op->K = (int)K;
+	op->T = (int)T;
+	op->Mf = Mf;
+	op->ncoil = nd.C;
+	// the low-memory modes give the same results; with a basis the plan
+	// keeps all coils (no chunks)
+	op->ctotal = nd.C * (int)K;
+	op->cchunk = op->ctotal;
+
+	try {
+		op->nufft = std::make_shared<Nufft>();
+		op->nufft->stream = op->stream;
+		op->nufft->real_psf = conf.real;
+		op->nufft->set_basis((int)K, (int)T, Mf, B.data());
+		op->nufft->create(nd.img, nd.C * (int)K, Mp, tr.data(), w.empty() ? nullptr : w.data(), conf.toeplitz,
+				  conf.periodic, conf.width, conf.os, conf.toeplitz && !conf.nopsf);
+		op->bksp.alloc((size_t)nd.C * K * Mp);
+	} catch (...) {
+		delete op;
+		throw;
+	}
+
+	return op;
+}
+
+static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims, const float2* traj,
+			 const long* wgh_dims, const float2* weights, const nufft_conf_s& conf,
+			 const long* bas_dims = nullptr, const float2* basis = nullptr)
+{
+	if (nullptr != basis) {
+		BAM_CHECK(nullptr != bas_dims, "nufft: basis without dims");
+		return make_basis_op(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, conf, bas_dims, basis);
+	}
+
 	NufftDims nd = parse_nufft_dims(N, ksp_dims, cim_dims, traj_dims);
 
 
@@ -623,12 +763,11 @@ struct linop_s* nufft_create2(int N, const long ksp_dims[], const long cim_dims[
 			      const float _Complex* traj, const long wgh_dims[], const float _Complex* weights,
 			      const long bas_dims[], const float _Complex* basis, struct nufft_conf_s conf)
 {
-	(void)bas_dims;
 	return guarded([&]() -> struct linop_s* {
 		check_conf(conf, nullptr != basis);
 		BAM_CHECK(nullptr != traj, "nufft: trajectory is NULL");
 		Op* op = make_nufft_op(N, ksp_dims, cim_dims, traj_dims, (const float2*)traj, wgh_dims,
-				       (const float2*)weights, conf);
+				       (const float2*)weights, conf, bas_dims, (const float2*)basis);
 		return (struct linop_s*)op;
 	}, nullptr);
 }
@@ -693,7 +832,6 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 				    const float _Complex* basis, const struct linop_s** fft_opp,
 				    unsigned long shared_img_dims)
 {
-	(void)basis_dims;
 	return guarded([&]() -> const struct linop_s* {
 		const int N = 16;
 		nufft_conf_s conf = *_conf;
@@ -711,14 +849,26 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		for (int i = 0; i < 4; i++)
 			BAM_CHECK(map_dims[i] == coilim_dims[i], "sense_nc_init: map dims do not match (dim %d)", i);
 
+		// the coefficient dim of the k-space follows the image
+		// (sense/modelnc.c:41-43)
 		long ksp2[16];
 		memcpy(ksp2, ksp_dims, sizeof(ksp2));
+		ksp2[6] = max_dims[6];
 
 		Op* op = make_nufft_op(N, ksp2, coilim_dims, traj_dims, (const float2*)traj, wgs_dims,
-				       (const float2*)weights, conf);
+				       (const float2*)weights, conf, basis_dims, (const float2*)basis);
 		op->kind = OP_SENSE;
 		memcpy(op->idims, img_dims, sizeof(long) * N);
 		op->isize = md_size(N, img_dims);
+
+		if (op->K > 0) {
+			try {
+				op->bcim.alloc((size_t)op->ncoil * op->K * op->nufft->N);
+			} catch (...) {
+				delete op;
+				throw;
+			}
+		}
 
 		if (op->nchunks() > 1) {
 			try {
@@ -750,15 +900,28 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 			Op* nop = new Op();
 			nop->kind = OP_NUFFT;
 			nop->N = N;
-			memcpy(nop->odims, ksp2, sizeof(long) * N);
+			memcpy(nop->odims, op->odims, sizeof(long) * N);
 			memcpy(nop->idims, coilim_dims, sizeof(long) * N);
-			nop->osize = md_size(N, ksp2);
+			nop->osize = op->osize;
 			nop->isize = md_size(N, coilim_dims);
 			nop->q = op->q;
 			nop->stream = op->stream;
 			nop->nufft = op->nufft;
 			nop->ctotal = op->ctotal;
 			nop->cchunk = op->cchunk;
+			nop->K = op->K;
+			nop->T = op->T;
+			nop->Mf = op->Mf;
+			nop->ncoil = op->ncoil;
+			if (nop->K > 0) {
+				try {
+					nop->bksp.alloc((size_t)op->bksp.n);
+				} catch (...) {
+					delete nop;
+					delete op;
+					throw;
+				}
+			}
 			*fft_opp = (const struct linop_s*)nop;
 		}
 
@@ -1002,7 +1165,8 @@ float _Complex* compute_psf2(int N, const long psf_dims[], unsigned long flags, 
 	(void)bas_dims;
 	(void)lowmem;
 	return guarded([&]() -> float _Complex* {
-		BAM_CHECK(nullptr == basis && !upper_triag, "compute_psf2: subspace basis is not supported by this library");
+		(void)upper_triag;  // no basis: the PSF does not depend on it
+		BAM_CHECK(nullptr == basis, "compute_psf2: subspace basis is not supported by this library");
 		BAM_CHECK(7 == (flags & 7) && 0 == (flags & ~7ul), "compute_psf2: flags must be 7");
 		BAM_CHECK(N >= 4 && N <= 32, "compute_psf2: unsupported dimension count %d", N);
 		BAM_CHECK(nullptr != traj, "compute_psf2: trajectory is NULL");
@@ -1623,6 +1787,7 @@ static const float2* warm_input(Op* op, float l2lambda, int N, const long img_di
 	check_dims(what, N, img_dims, op->N, op->idims);
 	BAM_CHECK(OP_NUDFT != op->kind && 1 == op->nufft->n[2] && op->nufft->n[0] > 1 && op->nufft->n[1] > 1,
 		  "%s: 2-D images only", what);
+	BAM_CHECK(0 == op->K, "%s: not for an operator with a subspace basis", what);
 	BAM_CHECK(!op->toeplitz() || op->nufft->has_psf, "%s: Toeplitz operator without PSF", what);
 	const long npix = op->isize;
 
@@ -1747,6 +1912,7 @@ const struct operator_s* nufft_precond_create(const struct linop_s* _op)
 		BAM_CHECK(nullptr != op && OP_NUDFT != op->kind, "nufft_precond_create: not a NUFFT operator");
 		BAM_CHECK(op->toeplitz(), "nufft_precond_create: the operator must be in Toeplitz mode");
 		BAM_CHECK(1 == op->nchunks(), "nufft_precond_create: not with the low-memory modes");
+		BAM_CHECK(0 == op->K, "nufft_precond_create: not for an operator with a subspace basis");
 		std::lock_guard<std::mutex> lock(op->mtx());
 		auto p = new Precond{ op, {}, (int)(op->isize / op->nufft->N) };
 		try {

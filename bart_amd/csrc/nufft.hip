@@ -26,14 +26,14 @@ namespace bam {
 // ---------------------------------------------------------------------------
 
 // num/fft.c:64-90
-static double fftmod_phase(long length, long j)
+double fftmod_phase(long length, long j)
 {
 	long center1 = length / 2;
 	double shift = (double)center1 / (double)length;
 	return ((double)j - (double)center1 / 2.) * shift;
 }
 
-static std::complex<double> fftmod_value(double phase)
+std::complex<double> fftmod_value(double phase)
 {
 	double rem = phase - floor(phase);
 
@@ -64,7 +64,7 @@ static double bessel_i0_d(double x)
 }
 
 // noncart/grid.c:100-117, 522-531
-static double rolloff_1d(int d, int i, float os, double beta, double width)
+double rolloff_1d(int d, int i, float os, double beta, double width)
 {
 	if (1 == d)
 		return 1.;
@@ -1388,6 +1388,7 @@ static bool sum_groups_or_fuse(float2* dst, const float2* part, long N, int G, h
 bool Nufft::normal(float2* dst, const float2* src, const float2* sens, const int* skip, const CgApFuse* fuse)
 {
 	BAM_CHECK(has_psf, "Toeplitz normal operator without PSF");
+	BAM_CHECK(0 == nbasis, "normal: a plan with a subspace basis takes normal_basis");
 
 	if (slice) {
 
@@ -1498,6 +1499,89 @@ bool Nufft::normal(float2* dst, const float2* src, const float2* sens, const int
 		return sum_groups_or_fuse(dst, part.ptr, N, G, stream, skip, fuse);
 
 	return false;
+}
+
+void Nufft::set_basis(int K, int T, long Mf, const float2* B_host)
+{
+	BAM_CHECK(K >= 1 && K <= BASIS_MAX, "basis: %d coefficients (at most %d)", K, BASIS_MAX);
+	BAM_CHECK(T >= 1 && Mf >= 0, "basis: bad frame dims");
+	nbasis = K;
+	bT = T;
+	bMf = Mf;
+	bas.alloc((size_t)K * T);
+	BAM_HIP(hipMemcpy(bas.ptr, B_host, sizeof(float2) * K * T, hipMemcpyHostToDevice));
+}
+
+// Toeplitz normal operator with a subspace basis (noncart/nufft.c:1546-1625
+// with the K x K PSF of nufft_set_traj): per sub-grid s, FFT(linphase_s x_k)
+// for every coil and coefficient, the PSF product over the coefficients,
+// IFFT, conj(linphase_s), sum over s.  The last axis is transformed forward,
+// multiplied (basis_psf_apply) and transformed back in three launches.
+void Nufft::normal_basis(float2* dst, const float2* src, const int* skip)
+{
+	BAM_CHECK(has_psf && nbasis > 0, "normal_basis: no subspace PSF");
+	BAM_CHECK(0 == C % nbasis, "normal_basis: channels %d not a multiple of %d coefficients", C, nbasis);
+	const int Cc = C / nbasis;
+
+	if (slice) {
+
+		VolAxis va = VolAxis::make(n, 2, (long)S * C);
+		launch_slice_fwd(*this, src, nullptr, false, skip, "fft_toeplitz_slice_fwd", false);
+
+		FftLaunchCfg cf = fft_config_skip(skip, axis[2], va.transforms(), va.contig(), 1, -1.f, -1.f);
+		cf.name = "fft_toeplitz_basis_fwd";
+		LdGrid ld{ va, grid.ptr };
+		StGrid st{ va, tabs, grid.ptr, false };
+		launch_fft_axis(cf, stream, ld, NoMid{}, st);
+
+		basis_psf_apply(grid.ptr, psf.ptr, S, Cc, nbasis, N, stream, skip);
+
+		FftLaunchCfg ci = fft_config_skip(skip, axis[2], va.transforms(), va.contig(), 1, +1.f, +1.f);
+		ci.name = "fft_toeplitz_basis_inv";
+		launch_fft_axis(ci, stream, ld, NoMid{}, st);
+
+		launch_slice_inv(*this, dst, nullptr, false, skip, "fft_toeplitz_slice_inv", false);
+		return;
+	}
+
+	const int cc = coil_group(S, C, n[act[0]], false);
+	GeomS gs = GeomS::make(n, act[0], S, C, cc);
+
+	{
+		FftLaunchCfg cfg = fft_config_skip(skip, axis[0], gs.va.transforms(), true, S * cc, -1.f, -1.f);
+		cfg.name = "fft_toeplitz_basis_first";
+		LdImgLin ld{ gs, tabs, src, nullptr, false };
+		StGridS st{ gs, tabs, grid.ptr, false };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	for (int j = 1; j < nact; j++) {
+
+		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
+		FftLaunchCfg cfg = fft_config_skip(skip, axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
+		cfg.name = "fft_toeplitz_basis_fwd";
+		LdGrid ld{ va, grid.ptr };
+		StGrid st{ va, tabs, grid.ptr, false };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	basis_psf_apply(grid.ptr, psf.ptr, S, Cc, nbasis, N, stream, skip);
+
+	for (int j = nact - 1; j >= 1; j--) {
+
+		VolAxis va = VolAxis::make(n, act[j], (long)S * C);
+		FftLaunchCfg cfg = fft_config_skip(skip, axis[j], va.transforms(), va.contig(), 1, +1.f, +1.f);
+		cfg.name = "fft_toeplitz_basis_inv";
+		LdGrid ld{ va, grid.ptr };
+		StGrid st{ va, tabs, grid.ptr, false };
+		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
+	}
+
+	FftLaunchCfg cfg = fft_config_skip(skip, axis[0], gs.va.transforms(), true, S * cc, +1.f, +1.f);
+	cfg.name = "fft_toeplitz_basis_last";
+	LdGridS ld{ gs, grid.ptr };
+	StReduceS st{ gs, tabs, dst, nullptr, false };
+	launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1624,7 +1708,7 @@ struct NormalGroupSet {
 
 bool normal_groupable(const Nufft& p, const float2* sens)
 {
-	if (2 != p.nact || p.slice || !p.toeplitz || !p.has_psf || nullptr == sens)
+	if (2 != p.nact || p.slice || !p.toeplitz || !p.has_psf || nullptr == sens || p.nbasis > 0)
 		return false;
 
 	const int S = p.S, C = p.C;
@@ -1778,6 +1862,7 @@ struct StDecomp {
 	float scale;
 	int n1[3];        // sub-grid dims
 	long N1;
+	long vstride = 0; // output stride of the volumes (K x K basis PSFs)
 
 	__device__ void store(const float2* lds, const FftLaunch& L, long t0) const
 	{
@@ -1807,7 +1892,7 @@ struct StDecomp {
 				}
 			}
 
-			dst[(long)s * N1 + q[0] + (long)n1[0] * (q[1] + (long)n1[1] * q[2])] = x;
+			dst[(long)v * vstride + (long)s * N1 + q[0] + (long)n1[0] * (q[1] + (long)n1[1] * q[2])] = x;
 		});
 	}
 };
@@ -1836,19 +1921,29 @@ void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host,
 		}
 	}
 
+	// with a basis, J = K^2 channels whose k-space is conj(B_k) B_k' in
+	// place of the ones (compute_psf_int, noncart/nufft.c:535-586)
+	const int J = (nbasis > 0) ? nbasis * nbasis : 1;
+	BAM_CHECK(0 == nbasis || bMf * bT == M, "psf: basis frames (%d x %ld) do not match the samples (%ld)", bT, bMf,
+		  M);
+
 	Nufft p2;
 	p2.stream = stream;
 	p2.tune = false;  // one adjoint only
-	p2.create(n2, 1, M, traj2.data(), sqw.empty() ? nullptr : sqw.data(), false, periodic_psf, width, os, false);
+	p2.create(n2, J, M, traj2.data(), sqw.empty() ? nullptr : sqw.data(), false, periodic_psf, width, os, false);
 
 	long N2 = p2.N;
-	DevBuf<float2> ones(std::max<long>(M, 1));
-	DevBuf<float2> psft(N2);
+	DevBuf<float2> ones(std::max<long>((long)J * M, 1));
+	DevBuf<float2> psft((size_t)J * N2);
 
-	fill(ones.ptr, M, make_float2(1.f, 0.f), stream);
+	if (nbasis > 0)
+		basis_psf_vals(ones.ptr, bas.ptr, nbasis, bT, bMf, stream);
+	else
+		fill(ones.ptr, M, make_float2(1.f, 0.f), stream);
+
 	p2.adjoint(psft.ptr, ones.ptr, nullptr);
 
-	psf.alloc((size_t)S * N);
+	psf.alloc((size_t)J * S * N);
 
 	const float scale = 1.f / sqrtf((float)N2);
 	int last = p2.act[p2.nact - 1];
@@ -1856,12 +1951,12 @@ void Nufft::compute_psf_from_traj(const float* traj, const float2* weights_host,
 	for (int j = 0; j < p2.nact; j++) {
 
 		int a = p2.act[j];
-		VolAxis va = VolAxis::make(n2, a, 1);
+		VolAxis va = VolAxis::make(n2, a, J);
 		FftLaunchCfg cfg = fft_config(p2.axis[j], va.transforms(), va.contig(), 1, -1.f, -1.f);
 		cfg.name = "fft_psf";
 		LdFm2 ld{ va, p2.tabs, psft.ptr, 0 == j };
 		StDecomp st{ va, p2.tabs, (a == last) ? psf.ptr : psft.ptr, a == last, false, scale,
-			     { n[0], n[1], n[2] }, N };
+			     { n[0], n[1], n[2] }, N, (long)S * N };
 		launch_fft_axis(cfg, stream, ld, NoMid{}, st);
 	}
 
@@ -1881,7 +1976,7 @@ void Nufft::apply_real_psf()
 	if (!real_psf)
 		return;
 
-	const long n = (long)S * N;
+	const long n = (long)psf.n;
 	const unsigned blocks = (unsigned)std::min<long>((n + 255) / 256, 4096);
 	hipLaunchKernelGGL(k_real_part, dim3(blocks), dim3(256), 0, stream, psf.ptr, n);
 	BAM_HIP(hipGetLastError());

@@ -5,6 +5,7 @@
 #include "fft.hpp"
 #include "volume.hpp"
 
+#include <complex>
 #include <memory>
 #include <string>
 #include <utility>
@@ -14,6 +15,23 @@ namespace bam {
 
 constexpr int KB_SIZE = 1000;
 constexpr int MAX_TAPS = 8;  // taps per dimension (kernel width <= 7)
+
+// intlookup (noncart/grid.c:126-149): x in [0, 0.5]
+__device__ inline float kb_lookup(const float* table, float x)
+{
+	x *= 2.f;
+	float xs = x * (float)(KB_SIZE - 1);
+	int index = (int)xs;
+	index = min(max(index, 0), KB_SIZE - 1);
+	float fpart = xs - (float)index;
+	return (1.f - fpart) * table[index] + fpart * table[index + 1];
+}
+
+// host tables of nufft.hip: fftmod phase and value (num/fft.c:64-90) and the
+// rolloff of one dim (noncart/grid.c:100-117, 522-531)
+double fftmod_phase(long length, long j);
+std::complex<double> fftmod_value(double phase);
+double rolloff_1d(int d, int i, float os, double beta, double width);
 
 // Gridding configuration of the decomposition sub-grids
 // (noncart/grid.h:8-16 struct grid_conf_s, noncart/nufft.c:147-188).
@@ -209,6 +227,72 @@ struct Nufft {
 	// with real_psf: the imaginary part of the PSF set to zero (a real PSF
 	// multiplies both components of the grid alike, as BART's float PSF)
 	void apply_real_psf();
+
+	// Subspace basis (noncart/nufft.c:470-505, 1018-1036): nbasis = K
+	// coefficients (0: none).  The plan's C channels are the coil-coefficient
+	// pairs c + (C / K) k, its M samples the frames' samples m + Mf t (T
+	// frames of Mf samples); bas = B [K][T] on the device (set before
+	// create).  The PSF is then [K K][S][N], entry k K + k' the sub-grid PSFs
+	// of sum_m |w_m|^2 conj(B[t_m][k]) B[t_m][k'].
+	int nbasis = 0;
+	int bT = 1;
+	long bMf = 0;
+	DevBuf<float2> bas;
+	void set_basis(int K, int T, long Mf, const float2* B_host);
+	// Toeplitz normal operator of the coefficient images (C channels in and
+	// out, no maps): the K x K PSF product between the forward and inverse
+	// transforms of the last axis
+	void normal_basis(float2* dst, const float2* src, const int* skip = nullptr);
+};
+
+// Subspace basis kernels (basis.hip), layouts as described there; skip: an
+// optional device flag that turns the launch into a no-op.
+constexpr int BASIS_MAX = 16;
+void basis_contract(float2* dst, const float2* src, const float2* B, int C, int K, int T, long Mf, hipStream_t st,
+		    const int* skip = nullptr);
+void basis_expand(float2* dst, const float2* src, const float2* B, int C, int K, int T, long Mf, hipStream_t st,
+		  const int* skip = nullptr);
+void maps_expand(float2* dst, const float2* sens, const float2* x, int C, int K, long N, hipStream_t st,
+		 const int* skip = nullptr);
+void maps_reduce(float2* x, const float2* sens, const float2* src, int C, int K, long N, hipStream_t st,
+		 const int* skip = nullptr);
+void basis_psf_apply(float2* grid, const float2* psf, int S, int C, int K, long N, hipStream_t st,
+		     const int* skip = nullptr);
+void basis_psf_vals(float2* v, const float2* B, int K, int T, long Mf, hipStream_t st);
+
+// The chained NUFFT for os != 2 (noncart/nufft_chain.c:192-244): rolloff
+// into the centre of the round(os n) grid, centred unitary FFT, KB
+// interpolation on that grid with beta = calc_beta(os, width), weights times
+// os^(d/2).  The adjoint gridding gathers per grid point from samples sorted
+// by cells of CHAIN_CS points.  No Toeplitz operator (BART has none here).
+constexpr int CHAIN_CS = 4;
+constexpr int CHAIN_TAPS = 16;
+struct Chain {
+
+	int n[3], on[3], off[3], ncell[3];
+	long N = 0, N2 = 0, M = 0;
+	int C = 0;
+	int nd = 0;
+	float os = 2.f, width = 6.f;
+	double beta = 0.;
+	bool periodic = false;
+	float scale = 1.f;
+	bool has_weights = false;
+	DevBuf<float> kb;
+	DevBuf<float> roll;     // [N]
+	DevBuf<float2> fma;     // [N2] fftmod x N2^-1/4
+	DevBuf<float4> trj;     // [M] coordinates (original order)
+	DevBuf<float2> tmp;     // [M] weights x scale
+	DevBuf<float4> spos;    // sorted sample positions on the grid, w = original index
+	DevBuf<int> cstart;     // [cells + 1]
+	DevBuf<float2> grid;    // [C][N2]
+	DevBuf<float2> kwork;   // [C][M]
+	hipStream_t stream = 0;
+
+	void create(const int dims[3], int C, long M, const float* traj, const float2* weights_host, bool periodic,
+		    float width, float os);
+	void forward(float2* ksp, const float2* cim);
+	void adjoint(float2* cim, const float2* ksp);
 };
 
 // Normal operators of several 2-D Toeplitz plans with coil maps, one launch

@@ -60,10 +60,15 @@ struct nufft_conf_s {
 extern struct nufft_conf_s nufft_conf_defaults;
 
 /* noncart/nufft.h:48-57 -- replaces nufft_create2 (noncart/nufft.c:1174-1276).
- * Supported: os = 2, decomp = true, basis = NULL, kernel width <= 14,
- * 1 to 3 transformed dims (flags = 7), samples in dims 1-2 and coils in
- * dim 3.  Modes that only change memory use (lowmem, zero_overhead,
- * precomp*, pcycle = false) give the same results and are accepted. */
+ * Supported: os = 2, decomp = true, kernel width <= 14, 1 to 3
+ * transformed dims (flags = 7), samples in dims 1-2 and coils in dim 3.
+ * A subspace basis (bas_dims [1 1 1 1 1 T K], K <= 16,
+ * noncart/nufft.c:1018-1036) takes coil images [X Y Z C 1 1 K], k-space
+ * [1 R S C 1 T] (dim 6 1 or K; the operator's output has 1) and a
+ * trajectory and weights with dim 5 of 1 or T; its Toeplitz normal operator
+ * uses the K x K PSF.  Modes that only change memory use (lowmem,
+ * zero_overhead, precomp*, upper_triag, pcycle = false) give the same
+ * results and are accepted; compress_psf is rejected. */
 struct linop_s* nufft_create2(int N, const long ksp_dims[], const long cim_dims[], const long traj_dims[],
 			      const BAM_COMPLEX* traj, const long wgh_dims[], const BAM_COMPLEX* weights,
 			      const long bas_dims[], const BAM_COMPLEX* basis, struct nufft_conf_s conf);
@@ -75,6 +80,8 @@ struct linop_s* nufft_create(int N, const long ksp_dims[], const long cim_dims[]
 /* sense/modelnc.h -- replaces sense_nc_init (sense/modelnc.c:28-67): the
  * linear operator image -> k-space  A = NUFFT S  with normal operator
  * S^H T S (Toeplitz) when conf->toeplitz.  One set of maps (MAPS dim 1).
+ * With a basis, max_dims holds the K coefficients in dim 6 (and 1 in dim
+ * 5), as pics sets it (src/pics.c:326-333).
  * When fft_opp is not NULL it receives the NUFFT operator (coil images <->
  * k-space) sharing the same plan and PSF; free it with linop_free. */
 const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims[16], const BAM_COMPLEX* maps,

@@ -63,6 +63,14 @@ def lib():
         L.orc_nufft_subgrids.restype = ctypes.c_int
         L.orc_nufft_subgrids.argtypes = [vp]
         L.orc_nufft_compute_psf.argtypes = [c_long3, ctypes.c_long, vp, vp, vp]
+        L.orc_nufft_compute_psf_vals.argtypes = [c_long3, ctypes.c_long, vp, vp, vp, vp]
+        L.orc_nufft_normal_basis.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.orc_chain_create.restype = vp
+        L.orc_chain_create.argtypes = [c_long3, ctypes.c_long, vp, vp, ctypes.c_int, ctypes.c_bool, ctypes.c_float,
+                                       ctypes.c_float]
+        L.orc_chain_free.argtypes = [vp]
+        L.orc_chain_forward.argtypes = [vp, vp, vp]
+        L.orc_chain_adjoint.argtypes = [vp, vp, vp]
         L.orc_psf_grid_fp64.argtypes = [c_long3, ctypes.c_long, vp, vp, ctypes.c_int, vp]
         L.orc_nufft_grid_fp64.argtypes = [vp, ctypes.c_int, vp, vp]
         L.orc_sense_forward.argtypes = [vp, vp, vp, vp]
@@ -380,6 +388,139 @@ class Nufft:
         it = lib().orc_cgsense(self.h, _ptr(maps), lam, maxiter, tol, _ptr(x), _ptr(b),
                                _ptr(trace), _ptr(fin))
         return x, it, trace[:it], float(fin[0])
+
+
+class BasisNufft:
+    """BART's NUFFT with a subspace basis on the CPU
+    (noncart/nufft.c:470-505, 1018-1036, 1450-1453, 1483-1487, 1546-1625).
+
+    traj: complex64 [T or 1, Mf, 3] (one trajectory per frame, or one for
+    all); basis: [T, K] (B[t][k], BART dims [1 1 1 1 1 T K]); weights:
+    [T or 1, Mf] or None.  Coefficient coil images are [K, C, Z, Y, X];
+    frame k-space is [T, C, Mf].  The coefficient NUFFT is the plain one
+    with C K channels (c + C k) over the T Mf samples of all frames.
+    """
+
+    def __init__(self, dims, traj, basis, ncoils, weights=None, toeplitz=True, periodic=False):
+        self.B = _c64(basis)
+        self.T, self.K = self.B.shape
+        tr = _c64(traj)
+        tr = tr.reshape((1, -1, 3)) if tr.ndim == 2 else tr
+        self.Mf = tr.shape[1]
+        tr = np.broadcast_to(tr, (self.T, self.Mf, 3)).reshape(-1, 3)
+        self.traj = np.ascontiguousarray(tr)
+        w = None
+        if weights is not None:
+            w = _c64(weights).reshape(-1, self.Mf)
+            w = np.ascontiguousarray(np.broadcast_to(w, (self.T, self.Mf)).reshape(-1))
+        self.weights = w
+        self.C = int(ncoils)
+        self.dims = tuple(list(dims) + [1] * (3 - len(dims)))
+        # in Toeplitz mode the linear phases exclude the rolloff, which the
+        # forward and adjoint apply separately (as BART's plan does); the
+        # plain PSF the Toeplitz plan also computes is not used
+        self.nf = Nufft(self.dims, self.traj, self.C * self.K, weights=w, toeplitz=toeplitz, periodic=periodic)
+        self.S = self.nf.S
+        self.N = self.nf.N
+        self.psfB = self.compute_psf() if toeplitz else None
+
+    @property
+    def cim_shape(self):
+        return (self.K, self.C) + self.dims[::-1]
+
+    def forward(self, cim):
+        """[K, C, ...] -> [T, C, Mf]: dst[t][c] = sum_k B[t][k] NUFFT(x_kc)[t]
+        (md_ztenmul, nufft.c:1450-1453; weights inside the plain NUFFT)."""
+        tmp = self.nf.forward(_c64(cim).reshape(self.K * self.C, -1)).reshape(self.K, self.C, self.T, self.Mf)
+        out = np.zeros((self.T, self.C, self.Mf), np.complex64)
+        for k in range(self.K):
+            out += self.B[:, k][:, None, None] * tmp[k].transpose(1, 0, 2)
+        return out
+
+    def adjoint(self, ksp):
+        """[T, C, Mf] -> [K, C, ...]: x conj(B[t][k]) (md_ztenmulc,
+        nufft.c:1483-1487), then the plain adjoint NUFFT."""
+        ksp = _c64(ksp).reshape(self.T, self.C, self.Mf)
+        tmp = np.zeros((self.K, self.C, self.T, self.Mf), np.complex64)
+        for k in range(self.K):
+            tmp[k] = (ksp * np.conj(self.B[:, k])[:, None, None]).transpose(1, 0, 2)
+        return self.nf.adjoint(tmp.reshape(self.K * self.C, -1)).reshape(self.cim_shape)
+
+    def compute_psf(self):
+        """[K K, S, Z, Y, X]: entry k K + k' = the PSF of |w|^2 conj(B[t][k])
+        B[t][k'] (compute_psf_int with compute_square_basis,
+        nufft.c:470-505, 535-586)."""
+        d = list(self.dims)
+        out = np.zeros((self.K * self.K, self.S) + tuple(d[::-1]), np.complex64)
+        t_of = np.repeat(np.arange(self.T), self.Mf)
+        for k in range(self.K):
+            for q in range(self.K):
+                vals = np.ascontiguousarray((np.conj(self.B[t_of, k]) * self.B[t_of, q]).astype(np.complex64))
+                o = np.zeros((self.S,) + tuple(d[::-1]), np.complex64)
+                lib().orc_nufft_compute_psf_vals(_dims3(d), self.traj.shape[0], _ptr(self.traj),
+                                                 _ptr(self.weights), _ptr(vals), _ptr(o))
+                out[k * self.K + q] = o
+        return out
+
+    def normal(self, cim):
+        """Toeplitz normal operator with the K x K PSF (nufft.c:1546-1625)."""
+        cim = _c64(cim).reshape(self.cim_shape)
+        out = np.zeros(self.cim_shape, np.complex64)
+        psf = np.ascontiguousarray(self.psfB)
+        lib().orc_nufft_normal_basis(self.nf.h, self.K, _ptr(psf), _ptr(out), _ptr(cim))
+        return out
+
+    def sense_forward(self, maps, img):
+        """img [K, ...], maps [C, ...] -> [T, C, Mf] (maps2, sense/modelnc.c:57-59)."""
+        img = _c64(img).reshape((self.K, 1) + self.dims[::-1])
+        return self.forward(img * _c64(maps).reshape((1, self.C) + self.dims[::-1]))
+
+    def sense_adjoint(self, maps, ksp):
+        cim = self.adjoint(ksp)
+        return np.sum(cim * np.conj(_c64(maps).reshape((1, self.C) + self.dims[::-1])), axis=1)
+
+    def sense_normal(self, maps, img):
+        m = _c64(maps).reshape((1, self.C) + self.dims[::-1])
+        img = _c64(img).reshape((self.K, 1) + self.dims[::-1])
+        cim = self.normal(img * m)
+        return np.sum(cim * np.conj(m), axis=1)
+
+
+class ChainNufft:
+    """BART's chained NUFFT for oversampling factors other than 2
+    (noncart/nufft_chain.c:192-244): rolloff into the centre of the
+    round(os n) grid, centred unitary FFT, KB interpolation with beta =
+    calc_beta(os, width), weights times os^(d/2).  Arrays as :class:`Nufft`."""
+
+    def __init__(self, dims, traj, ncoils, weights=None, periodic=False, width=6.0, os_=1.5):
+        self.dims = tuple(list(dims) + [1] * (3 - len(dims)))
+        self.traj = _c64(traj).reshape(-1, 3)
+        self.M = self.traj.shape[0]
+        self.C = int(ncoils)
+        self.weights = _c64(weights).reshape(-1) if weights is not None else None
+        self.h = lib().orc_chain_create(_dims3(self.dims), self.M, _ptr(self.traj), _ptr(self.weights), self.C,
+                                        periodic, width, os_)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_chain_free(self.h)
+            self.h = None
+
+    @property
+    def cim_shape(self):
+        return (self.C,) + self.dims[::-1]
+
+    def forward(self, cim):
+        cim = _c64(cim).reshape(self.cim_shape)
+        out = np.zeros((self.C, self.M), np.complex64)
+        lib().orc_chain_forward(self.h, _ptr(out), _ptr(cim))
+        return out
+
+    def adjoint(self, ksp):
+        ksp = _c64(ksp).reshape(self.C, self.M)
+        out = np.zeros(self.cim_shape, np.complex64)
+        lib().orc_chain_adjoint(self.h, _ptr(out), _ptr(ksp))
+        return out
 
 
 def precond_strang(nf):

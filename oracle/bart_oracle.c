@@ -1008,12 +1008,85 @@ void orc_nufft_normal(const orc_nufft_t* nf, cf* dst, const cf* src)
 	}
 }
 
+/* Toeplitz normal operator with a subspace basis, noncart/nufft.c:1546-1625
+ * with the K x K PSF of nufft_set_traj (:1066-1079): nf has C K channels
+ * (coil c, coefficient k at c + C k); psfB is [K K][S][N], entry k K + k'
+ * multiplying coefficient k' into coefficient k (md_ztenmul over the
+ * coefficient dims, the sum over k' in order). */
+void orc_nufft_normal_basis(const orc_nufft_t* nf, int K, const cf* psfB, cf* dst, const cf* src)
+{
+	long N = nf->N;
+	int CK = nf->C;
+	int C = CK / K;
+	int S = nf->S;
+	cf* grid = nf->grid;
+
+	#pragma omp parallel for collapse(2) schedule(static)
+	for (int s = 0; s < S; s++)
+		for (int c = 0; c < CK; c++)
+			for (long i = 0; i < N; i++)
+				grid[i + N * (c + CK * (long)s)] = src[i + c * N] * nf->linphase[i + s * N];
+
+	orc_fft(nf->dims, (long)S * CK, -1, grid);
+
+	#pragma omp parallel for collapse(2) schedule(static)
+	for (int s = 0; s < S; s++)
+		for (int c = 0; c < C; c++) {
+
+			cf v[64];
+
+			for (long i = 0; i < N; i++) {
+
+				for (int k = 0; k < K; k++)
+					v[k] = grid[i + N * (c + C * k + CK * (long)s)];
+
+				for (int k = 0; k < K; k++) {
+
+					cf acc = 0.;
+
+					for (int q = 0; q < K; q++)
+						acc += psfB[i + N * (s + S * ((long)k * K + q))] * v[q];
+
+					grid[i + N * (c + C * k + CK * (long)s)] = acc;
+				}
+			}
+		}
+
+	orc_fft(nf->dims, (long)S * CK, +1, grid);
+
+	#pragma omp parallel for schedule(static)
+	for (long ci = 0; ci < (long)CK * N; ci++) {
+
+		long c = ci / N;
+		long i = ci % N;
+		cf acc = 0.;
+
+		for (int s = 0; s < S; s++)
+			acc += grid[i + N * (c + CK * (long)s)] * conjf(nf->linphase[i + s * N]);
+
+		dst[ci] = acc;
+	}
+}
+
 /* PSF, noncart/nufft.c:727-776 (compute_psf2) with :535-586
  * (compute_psf_int): adjoint NUFFT of |w|^2 (ones without weights) on the
  * 2x image with 2*traj and periodic gridding, centred unitary FFT over
  * the 2x grid (num/fft.c:471-475), then even/odd decomposition
  * (num/shuffle.c:84-130).  psf is [S][N]. */
+void orc_nufft_compute_psf_vals(const long dims[3], long M, const cf* traj, const cf* weights, const cf* vals,
+				cf* psf);
+
 void orc_nufft_compute_psf(const long dims[3], long M, const cf* traj, const cf* weights, cf* psf)
+{
+	orc_nufft_compute_psf_vals(dims, M, traj, weights, NULL, psf);
+}
+
+/* The same with the k-space vals[M] in place of the ones: one channel of the
+ * PSF of a subspace basis, whose adjoint NUFFT takes |w|^2 conj(sqr_basis)
+ * (compute_square_basis, noncart/nufft.c:470-505; nufft_apply_adjoint
+ * :1483-1487), vals = conj(B[t][k]) B[t][k'] for the PSF entry (k, k'). */
+void orc_nufft_compute_psf_vals(const long dims[3], long M, const cf* traj, const cf* weights, const cf* vals,
+				cf* psf)
 {
 	long dims2[3];
 
@@ -1050,7 +1123,7 @@ void orc_nufft_compute_psf(const long dims[3], long M, const cf* traj, const cf*
 	cf* ones = malloc(sizeof(cf) * (size_t)M);
 
 	for (long m = 0; m < M; m++)
-		ones[m] = 1.;
+		ones[m] = (NULL != vals) ? vals[m] : 1.;
 
 	cf* psft = malloc(sizeof(cf) * (size_t)N2);
 
@@ -1403,4 +1476,170 @@ void orc_set_num_threads(int n)
 #else
 	(void)n;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* The chained NUFFT for os != 2, noncart/nufft_chain.c:192-244 (selected   */
+/* by nufft_create2, noncart/nufft.c:1184-1199): rolloff into the centre of */
+/* the os-times grid (linop_kb_rolloff_create, :60-118, rolloff of          */
+/* grid.c:520-556), centred unitary FFT (linop_fftc_create,                 */
+/* linops/someops.c:1825-1879: fftmod(N^-1/4) x FFT x fftmod(N^-1/4)), KB   */
+/* interpolation on that grid (grid2H with os, width, beta =               */
+/* calc_beta(os, width)), and the weights times os^(d/2).  No Toeplitz.     */
+/* ------------------------------------------------------------------------ */
+
+struct orc_chain {
+
+	long dims[3], odims[3], off[3];
+	long N, N2, M;
+	int C;
+	float scale;		/* os^(d/2) */
+	struct orc_grid_conf conf;
+	cf* traj;
+	cf* tmp;		/* [M] weights x scale, or NULL */
+	cf* roll;		/* [N] */
+	cf* fma;		/* [N2] fftmod x N2^-1/4 */
+	cf* grid;		/* [C][N2] */
+	float table[KB_SIZE + 1];
+};
+
+typedef struct orc_chain orc_chain_t;
+
+orc_chain_t* orc_chain_create(const long dims[3], long M, const cf* traj, const cf* weights, int C, bool periodic,
+			      float width, float os)
+{
+	orc_chain_t* ch = calloc(1, sizeof(orc_chain_t));
+	int d = 0;
+
+	ch->N = ch->N2 = 1;
+
+	for (int i = 0; i < 3; i++) {
+
+		ch->dims[i] = dims[i];
+		ch->odims[i] = (dims[i] > 1) ? lround(os * dims[i]) : 1;
+		ch->off[i] = labs(ch->odims[i] / 2 - dims[i] / 2);
+		ch->N *= dims[i];
+		ch->N2 *= ch->odims[i];
+		d += (dims[i] > 1);
+	}
+
+	ch->M = M;
+	ch->C = C;
+	ch->conf = (struct orc_grid_conf){ .os = os, .width = width, .periodic = periodic };
+
+	/* the table and the rolloff use beta rounded to float, as the first
+	 * kb_init call of the forward (the rolloff's) makes it */
+	double beta = (double)(float)orc_calc_beta(os, width);
+	orc_kb_table(beta, ch->table);
+
+	ch->roll = malloc(sizeof(cf) * (size_t)ch->N);
+	orc_rolloff(os, width, beta, dims, ch->roll);
+
+	ch->fma = malloc(sizeof(cf) * (size_t)ch->N2);
+	orc_fftmod_table(ch->odims, false, ch->fma);
+	float s4 = 1.f / sqrtf(sqrtf((float)ch->N2));
+
+	for (long i = 0; i < ch->N2; i++)
+		ch->fma[i] *= s4;
+
+	ch->scale = powf(os, d / 2.f);
+
+	ch->traj = malloc(sizeof(cf) * 3 * (size_t)M);
+	memcpy(ch->traj, traj, sizeof(cf) * 3 * (size_t)M);
+
+	if (NULL != weights) {
+
+		ch->tmp = malloc(sizeof(cf) * (size_t)M);
+
+		for (long m = 0; m < M; m++)
+			ch->tmp[m] = weights[m] * ch->scale;
+	}
+
+	ch->grid = malloc(sizeof(cf) * (size_t)C * (size_t)ch->N2);
+	return ch;
+}
+
+void orc_chain_free(orc_chain_t* ch)
+{
+	if (NULL == ch)
+		return;
+
+	free(ch->traj);
+	free(ch->tmp);
+	free(ch->roll);
+	free(ch->fma);
+	free(ch->grid);
+	free(ch);
+}
+
+void orc_chain_forward(orc_chain_t* ch, cf* ksp, const cf* cim)
+{
+	long N = ch->N, N2 = ch->N2;
+	int C = ch->C;
+	cf* g = ch->grid;
+
+	memset(g, 0, sizeof(cf) * (size_t)C * (size_t)N2);
+
+	for (int c = 0; c < C; c++)
+		for (long z = 0; z < ch->dims[2]; z++)
+			for (long y = 0; y < ch->dims[1]; y++)
+				for (long x = 0; x < ch->dims[0]; x++) {
+
+					long i = x + ch->dims[0] * (y + ch->dims[1] * z);
+					long o = (x + ch->off[0]) + ch->odims[0] * ((y + ch->off[1]) + ch->odims[1] * (z + ch->off[2]));
+					g[c * N2 + o] = crealf(ch->roll[i]) * cim[c * N + i];
+				}
+
+	for (int c = 0; c < C; c++)
+		for (long o = 0; o < N2; o++)
+			g[c * N2 + o] *= ch->fma[o];
+
+	orc_fft(ch->odims, C, -1, g);
+
+	for (int c = 0; c < C; c++)
+		for (long o = 0; o < N2; o++)
+			g[c * N2 + o] *= ch->fma[o];
+
+	memset(ksp, 0, sizeof(cf) * (size_t)C * (size_t)ch->M);
+	orc_gridH(&ch->conf, ch->table, ch->M, ch->traj, C, ksp, ch->M, ch->odims, g, N2);
+
+	for (int c = 0; c < C; c++)
+		for (long m = 0; m < ch->M; m++)
+			ksp[c * ch->M + m] *= (NULL != ch->tmp) ? ch->tmp[m] : ch->scale;
+}
+
+void orc_chain_adjoint(orc_chain_t* ch, cf* cim, const cf* ksp)
+{
+	long N = ch->N, N2 = ch->N2, M = ch->M;
+	int C = ch->C;
+	cf* g = ch->grid;
+	cf* k2 = malloc(sizeof(cf) * (size_t)C * (size_t)M);
+
+	for (int c = 0; c < C; c++)
+		for (long m = 0; m < M; m++)
+			k2[c * M + m] = ksp[c * M + m] * ((NULL != ch->tmp) ? conjf(ch->tmp[m]) : ch->scale);
+
+	memset(g, 0, sizeof(cf) * (size_t)C * (size_t)N2);
+	orc_grid(&ch->conf, ch->table, M, ch->traj, ch->odims, C, g, N2, k2, M);
+	free(k2);
+
+	for (int c = 0; c < C; c++)
+		for (long o = 0; o < N2; o++)
+			g[c * N2 + o] *= conjf(ch->fma[o]);
+
+	orc_fft(ch->odims, C, +1, g);
+
+	for (int c = 0; c < C; c++)
+		for (long o = 0; o < N2; o++)
+			g[c * N2 + o] *= conjf(ch->fma[o]);
+
+	for (int c = 0; c < C; c++)
+		for (long z = 0; z < ch->dims[2]; z++)
+			for (long y = 0; y < ch->dims[1]; y++)
+				for (long x = 0; x < ch->dims[0]; x++) {
+
+					long i = x + ch->dims[0] * (y + ch->dims[1] * z);
+					long o = (x + ch->off[0]) + ch->odims[0] * ((y + ch->off[1]) + ch->odims[1] * (z + ch->off[2]));
+					cim[c * N + i] = crealf(ch->roll[i]) * g[c * N2 + o];
+				}
 }

@@ -52,3 +52,21 @@ def test_missing_library_raises(monkeypatch, tmp_path):
     monkeypatch.setattr(ba, "_LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ba.BartError):
         ba.lib()
+
+
+def test_basis_dims_are_checked_before_any_device_call():
+    # nufft_create2 with a subspace basis rejects dims that do not fit
+    # (image dim 6 = coefficients, k-space dim 5 = the basis frames)
+    import numpy as np
+    from bart_amd import inputs as I
+    d16 = lambda *d: list(d) + [1] * (16 - len(d))
+    n, Mf = 16, 128
+    traj = np.ascontiguousarray(I.radial_traj(32, 8).reshape(2, Mf, 3))
+    B = np.ones((2, 2), np.complex64)
+    conf = ba.NufftConf.defaults()
+    with pytest.raises(ba.BartError, match="coefficients"):
+        ba.nufft_create2(d16(1, Mf, 1, 1, 1, 2), d16(n, n, 1, 1), d16(3, Mf, 1, 1, 1, 2), traj, conf=conf,
+                         bas_dims=d16(1, 1, 1, 1, 1, 2, 2), basis=B)
+    with pytest.raises(ba.BartError, match="dim 5"):
+        ba.nufft_create2(d16(1, Mf, 1, 1, 1, 3), d16(n, n, 1, 1, 1, 1, 2), d16(3, Mf, 1, 1, 1, 2), traj,
+                         conf=conf, bas_dims=d16(1, 1, 1, 1, 1, 2, 2), basis=B)

@@ -244,3 +244,80 @@ def test_cli_pics_pattern_psf_and_warm_start(tmp_path):
     run_bart(["pics", "-r", lam, "-i", 200, "--tol", 1e-3, "-d", 3, "-t", "t", "k", "s", "x5"], tmp_path)
     _, x5 = read_raw(str(tmp_path / "x5"))
     assert np.all(np.isfinite(x5))
+
+
+def test_cli_pics_basis_equals_plain(tmp_path):
+    # tests/pics.mk:223-240 (test-pics-basis-noncart): each spoke is its own
+    # frame (delta pattern over spoke x frame), the basis is two equal
+    # coefficients of ones; the minimum-norm split gives each coefficient
+    # half of the plain reconstruction
+    run_bart(["traj", "-r", "-x256", "-D", "-y31", "t0"], tmp_path)
+    dims, t = read_raw(str(tmp_path / "t0"))
+    write_raw(str(tmp_path / "t"), dims, 0.5 * t)
+    n = 128
+    write_raw(str(tmp_path / "x"), dims16(n, n, 1, 1), I.ellipse_phantom(n, seed=21))
+    run_bart(["nufft", "t", "x", "k"], tmp_path)
+    kd, k = read_raw(str(tmp_path / "k"))
+    assert kd == dims16(1, 256, 31, 1)
+    p = np.eye(31, dtype=np.complex64)                       # [t][s]: dims 1 1 31 1 1 31
+    pk = (k.reshape(31, 256)[None, :, :] * p[:, :, None]).reshape(-1)
+    write_raw(str(tmp_path / "pk"), dims16(1, 256, 31, 1, 1, 31), pk)
+    p2 = np.broadcast_to(p[:, :, None], (31, 31, 256)).reshape(-1)
+    write_raw(str(tmp_path / "p2"), dims16(1, 256, 31, 1, 1, 31), p2)
+    write_raw(str(tmp_path / "o2"), dims16(1, 1, 1, 1, 1, 31, 2), np.ones(62, np.complex64))
+    write_raw(str(tmp_path / "c"), dims16(n, n, 1, 1), np.ones(n * n, np.complex64))
+    run_bart(["pics", "-r0.001", "-t", "t", "-p", "p2", "-B", "o2", "pk", "c", "reco1"], tmp_path)
+    run_bart(["pics", "-r0.001", "-t", "t", "k", "c", "reco"], tmp_path)
+    d1, reco1 = read_raw(str(tmp_path / "reco1"))
+    assert d1 == dims16(n, n, 1, 1, 1, 1, 2)
+    _, reco = read_raw(str(tmp_path / "reco"))
+    e = relerr(2.0 * reco1.reshape(2, -1)[0], reco)
+    print(f"pics -B vs plain pics: nrmse {e:.2e}")
+    assert e < 0.002
+
+
+def test_cli_pics_basis_frame_trajectories(tmp_path):
+    # tests/pics.mk:263-278 (test-pics-basis-noncart-memory): the spokes
+    # moved to the frame dim (a trajectory per frame)
+    run_bart(["traj", "-r", "-x128", "-o2.", "-D", "-y31", "t"], tmp_path)
+    dims, t = read_raw(str(tmp_path / "t"))
+    R = dims[1]
+    assert dims == dims16(3, R, 31)
+    n = 128
+    write_raw(str(tmp_path / "x"), dims16(n, n, 1, 1), I.ellipse_phantom(n, seed=22))
+    run_bart(["nufft", "-x", f"{n}:{n}:1", "t", "x", "k"], tmp_path)
+    _, k = read_raw(str(tmp_path / "k"))
+    write_raw(str(tmp_path / "t3"), dims16(3, R, 1, 1, 1, 31), t)
+    write_raw(str(tmp_path / "k1"), dims16(1, R, 1, 1, 1, 31), k)
+    write_raw(str(tmp_path / "o"), dims16(1, 1, 1, 1, 1, 31, 2), np.ones(62, np.complex64))
+    write_raw(str(tmp_path / "c"), dims16(n, n, 1, 1), np.ones(n * n, np.complex64))
+    run_bart(["pics", "-r0.001", "-t", "t3", "-B", "o", "k1", "c", "reco1"], tmp_path)
+    run_bart(["pics", "-r0.001", "-t", "t", "k", "c", "reco"], tmp_path)
+    _, reco1 = read_raw(str(tmp_path / "reco1"))
+    _, reco = read_raw(str(tmp_path / "reco"))
+    e = relerr(2.0 * reco1.reshape(2, -1)[0], reco)
+    print(f"pics -B (frame trajectories) vs plain pics: nrmse {e:.2e}")
+    assert e < 0.002
+
+
+def test_cli_nufft_basis_vs_oracle(tmp_path):
+    # nufft -B: forward of coefficient images and adjoint of frame k-space
+    n, C, R, spf, T, K = 32, 2, 64, 4, 3, 2
+    traj = I.radial_traj(R, T * spf).reshape(T, R * spf, 3)
+    rng = np.random.default_rng(5)
+    B = (rng.standard_normal((T, K)) + 1j * rng.standard_normal((T, K))).astype(np.complex64)
+    ref = O.BasisNufft((n, n, 1), traj, B, C, toeplitz=False)
+    write_raw(str(tmp_path / "t"), dims16(3, R, spf, 1, 1, T), traj)
+    write_raw(str(tmp_path / "b"), dims16(1, 1, 1, 1, 1, T, K), np.ascontiguousarray(B.T))
+    x = crandn((K, C, 1, n, n), 31)
+    write_raw(str(tmp_path / "x"), dims16(n, n, 1, C, 1, 1, K), x)
+    run_bart(["nufft", "-B", "b", "t", "x", "k"], tmp_path)
+    kd, k = read_raw(str(tmp_path / "k"))
+    assert kd == dims16(1, R, spf, C, 1, T)
+    assert relerr(k, ref.forward(x).reshape(-1)) < 1e-5
+    y = crandn((T, C, R * spf), 32)
+    write_raw(str(tmp_path / "y"), dims16(1, R, spf, C, 1, T), y)
+    run_bart(["nufft", "-a", "-B", "b", "-x", f"{n}:{n}:1", "t", "y", "a"], tmp_path)
+    ad, a = read_raw(str(tmp_path / "a"))
+    assert ad == dims16(n, n, 1, C, 1, 1, K)
+    assert relerr(a, ref.adjoint(y).reshape(-1)) < 1e-5

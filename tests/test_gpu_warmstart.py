@@ -145,3 +145,41 @@ def test_network_input_and_guess_entries():
     assert torch.equal(t, inp) or float((t - inp).abs().max()) <= 1e-6
     net.free()
     op.free()
+
+
+def test_warm_start_full_size():
+    # the learned warm start at its workload: 2-D radial 256^2, 8 coils,
+    # network of 48 channels.  Training pairs from the batched solver (CG to
+    # relative residual 1e-6), the network's forward output, loss and every
+    # parameter gradient against the oracle network on one full image, a
+    # few epochs of training, and CG to 1e-4 from the guess and from zero
+    from bart_amd import warmstart as WS
+    n, ch = 256, 48
+    inp, tgt, its = WS.training_pairs(list(range(100, 108)), n=n, C=8, maxiter=150, tol=1e-6, chunk=8)
+    assert inp.shape == (8, 2, n, n) and tgt.shape == (8, 2, n, n)
+    print("target CG iterations", its)
+    net = WS.WarmNet(ch, seed=0)
+    p = net.get_params().astype(np.float64)
+    x, t = inp[:1].contiguous(), tgt[:1].contiguous()
+    xh, th = x.cpu().numpy(), t.cpu().numpy()
+    y = net.forward(x).cpu().numpy()
+    ref, _ = R.net_forward(p, xh, ch)
+    e_fwd = relerr(y, ref)
+    g_ref, loss_ref = R.net_grads(p, xh, th, ch)
+    loss = net.train_step(x, t, 1e-3)
+    e_grad = relerr(net.get_grads(), g_ref)
+    print(f"256^2 x 48 channels: forward rel err {e_fwd:.1e}, loss {loss:.4e} (oracle {loss_ref:.4e}), "
+          f"gradient rel err {e_grad:.1e}")
+    assert e_fwd < 1e-4
+    assert abs(loss - loss_ref) < 1e-4 * loss_ref
+    assert e_grad < 1e-3
+    hist = WS.train(net, inp, tgt, epochs=6, batch=4, lr=1e-3)
+    print("epoch losses", " ".join(f"{h:.3e}" for h in hist))
+    assert np.isfinite(hist).all() and hist[-1] < hist[0]
+    op, b, _ = WS.make_problem(2000, n=n, C=8)
+    r = WS.solve_warm_and_cold(net, op, b, n)
+    print(r)
+    assert r["cold_status"] == 2 and r["warm_status"] == 2
+    assert r["warm_vs_cold"] < 0.1
+    op.free()
+    net.free()

@@ -369,3 +369,120 @@ def test_strang_preconditioner_restatement():
         col = O.nudft_adjoint((n, n, 1), traj, O.nudft_forward((n, n, 1), traj, e))[0, 0]
         exact = np.abs(np.fft.fft2(np.fft.ifftshift(col))) / n
         assert relerr(pre[0] - 1e-3, exact) < 2e-5
+
+
+# ---------------------------------------------------------------------------
+# Subspace basis (noncart/nufft.c:470-505, 1450-1453, 1483-1487, 1546-1625)
+# ---------------------------------------------------------------------------
+
+def _basis_case(T=3, K=2, n=32, R=64, spf=8, seed=0, weights=False):
+    rng = np.random.default_rng(seed)
+    traj = I.radial_traj(R, T * spf).reshape(T, R * spf, 3)
+    B = (rng.standard_normal((T, K)) + 1j * rng.standard_normal((T, K))).astype(np.complex64)
+    w = (0.5 + rng.random((T, R * spf))).astype(np.complex64) if weights else None
+    return traj, B, w
+
+
+def test_basis_with_one_coefficient_is_the_plain_nufft():
+    # K = 1, T = 1, B = 1: the contraction and the K x K PSF product are
+    # multiplications by one, so forward, adjoint and normal equal the plain
+    # operator's bit for bit
+    n = 16
+    traj = I.radial_traj(32, 7)
+    bn = O.BasisNufft((n, n, 1), traj, np.ones((1, 1), np.complex64), 2)
+    nf = O.Nufft((n, n, 1), traj, 2, toeplitz=True)
+    x = crandn((2, 1, n, n), 3)
+    assert np.array_equal(bn.forward(x[None])[0], nf.forward(x))
+    y = nf.forward(x)
+    assert np.array_equal(bn.adjoint(y[None])[0], nf.adjoint(y))
+    assert np.array_equal(bn.psfB[0], nf.psf())
+    assert np.array_equal(bn.normal(x[None])[0], nf.normal(x))
+
+
+@pytest.mark.parametrize("weights", [False, True])
+def test_basis_forward_is_the_frame_nudft_combination(weights):
+    # dst[t] = sum_k B[t][k] A_t x_k with A_t the NUDFT of frame t's
+    # trajectory (periodic gridding: the NUFFT's error is the KB one)
+    n = 32
+    traj, B, w = _basis_case(weights=weights)
+    T, K = B.shape
+    C = 2
+    bn = O.BasisNufft((n, n, 1), traj, B, C, weights=w, toeplitz=False, periodic=True)
+    x = crandn((K, C, 1, n, n), 5)
+    y = bn.forward(x)
+    for t in range(T):
+        ref = sum(B[t, k] * O.nudft_forward((n, n, 1), traj[t], x[k]) for k in range(K))
+        if w is not None:
+            ref = ref * w[t][None, :]
+        assert relerr(y[t], ref) < 2e-5
+
+
+def test_basis_adjoint_dot_test():
+    n = 32
+    traj, B, w = _basis_case(weights=True, seed=2)
+    bn = O.BasisNufft((n, n, 1), traj, B, 2, weights=w, toeplitz=False)
+    x = crandn((B.shape[1], 2, 1, n, n), 6)
+    y = crandn((B.shape[0], 2, traj.shape[1]), 7)
+    lhs = np.vdot(bn.forward(x).astype(np.complex128), y)
+    rhs = np.vdot(x.astype(np.complex128), bn.adjoint(y))
+    assert abs(lhs - rhs) < 1e-5 * abs(lhs)
+
+
+@pytest.mark.parametrize("weights", [False, True])
+def test_basis_toeplitz_equals_exact_normal(weights):
+    # (A^H A x)_k = sum_t sum_k' conj(B[t][k]) B[t][k'] A_t^H |w_t|^2 A_t x_k'
+    # with FP64 NUDFTs; complex B makes the K x K PSF Hermitian but not
+    # symmetric, so a transposed or conjugated PSF index fails here
+    n = 32
+    traj, B, w = _basis_case(weights=weights, seed=4)
+    T, K = B.shape
+    bn = O.BasisNufft((n, n, 1), traj, B, 1, weights=w, toeplitz=True)
+    x = crandn((K, 1, 1, n, n), 8)
+    ref = np.zeros((K, 1, 1, n, n), np.complex128)
+    for t in range(T):
+        ww = np.ones(traj.shape[1]) if w is None else np.abs(w[t]) ** 2
+        for k in range(K):
+            for q in range(K):
+                y = O.nudft_forward((n, n, 1), traj[t], x[q]) * ww[None, :]
+                ref[k] += np.conj(B[t, k]) * B[t, q] * O.nudft_adjoint((n, n, 1), traj[t], y)
+    assert relerr(bn.normal(x), ref) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Chained NUFFT, os != 2 (noncart/nufft_chain.c:192-244)
+# ---------------------------------------------------------------------------
+
+def test_chain_os15_w8_equals_os2_periodic():
+    # tests/nufft.mk:305-323 (test-nufft-forward-os / -adjoint-os): periodic
+    # NUFFTs with os 1.5, width 8 and with os 2, width 6 agree to 1e-5
+    n = 64
+    traj = I.radial_traj(64, 64)
+    x = I.ellipse_phantom(n, seed=3).reshape(1, 1, n, n)
+    a = O.ChainNufft((n, n, 1), traj, 1, periodic=True, width=8.0, os_=1.5)
+    b = O.Nufft((n, n, 1), traj, 1, toeplitz=False, periodic=True)
+    assert relerr(a.forward(x), b.forward(x)) < 1e-5
+    y = b.forward(x)
+    assert relerr(a.adjoint(y), b.adjoint(y)) < 1e-5
+
+
+@pytest.mark.parametrize("os_,width,dims", [(1.5, 8.0, (32, 32, 1)), (1.25, 6.0, (32, 32, 1)),
+                                            (1.5, 8.0, (16, 16, 16))])
+def test_chain_vs_nudft_and_dot(os_, width, dims):
+    # the chained NUFFT approximates the FP64 NUDFT (periodic: only the KB
+    # error); the adjoint is the adjoint (dot test, with weights)
+    if dims[2] > 1:
+        traj = I.phyllotaxis_kooshball(spokes=64, readout=32)
+    else:
+        traj = I.radial_traj(64, 21)
+    M = traj.shape[0]
+    w = (0.5 + np.random.default_rng(3).random(M)).astype(np.complex64)
+    a = O.ChainNufft(dims, traj, 2, periodic=True, width=width, os_=os_)
+    x = crandn((2,) + dims[::-1], 4)
+    ref = O.nudft_forward(dims, traj, x)
+    tol = 1e-5 if width >= 8 else 2e-3
+    assert relerr(a.forward(x), ref) < tol
+    aw = O.ChainNufft(dims, traj, 2, weights=w, periodic=False, width=width, os_=os_)
+    y = crandn((2, M), 5)
+    lhs = np.vdot(aw.forward(x).astype(np.complex128), y)
+    rhs = np.vdot(x.astype(np.complex128), aw.adjoint(y))
+    assert abs(lhs - rhs) < 1e-5 * abs(lhs)
