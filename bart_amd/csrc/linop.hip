@@ -113,6 +113,7 @@ struct Op {
 	std::shared_ptr<Queue> q;
 	std::shared_ptr<Nufft> nufft;
 	std::shared_ptr<Nudft> nudft;
+	std::shared_ptr<Chain> chain;  // os != 2 (nufft_create_chain); nufft is null then
 	std::shared_ptr<DevBuf<float2>> sens;
 	hipStream_t stream = 0;       // the handle's own stream
 	hipStream_t user_stream = 0;  // set by bart_amd_linop_set_stream (0: none)
@@ -171,6 +172,13 @@ struct Op {
 			nudft->forward(d, s);
 			return;
 		}
+		if (chain) {
+			const float2* m = maps();
+			if (nullptr != m)
+				maps_expand(bcim.ptr, m, s, chain->C, 1, chain->N, chain->stream);
+			chain->forward(d, (nullptr != m) ? bcim.ptr : s);
+			return;
+		}
 		if (K > 0) {
 			// maps, coefficient NUFFT, contraction with the basis
 			// (noncart/nufft.c:1450-1453)
@@ -197,6 +205,13 @@ struct Op {
 			nudft->adjoint(d, s);
 			return;
 		}
+		if (chain) {
+			const float2* m = maps();
+			chain->adjoint((nullptr != m) ? bcim.ptr : d, s);
+			if (nullptr != m)
+				maps_reduce(d, m, bcim.ptr, chain->C, 1, chain->N, chain->stream);
+			return;
+		}
 		if (K > 0) {
 			// x conj(basis) (noncart/nufft.c:1483-1487), coefficient
 			// adjoint NUFFT, maps
@@ -218,7 +233,7 @@ struct Op {
 		sum_images(d, lm_part.ptr, nufft->N, nchunks(), nufft->stream, nullptr);
 	}
 
-	bool toeplitz() const { return (OP_NUDFT != kind) && nufft->toeplitz; }
+	bool toeplitz() const { return (OP_NUDFT != kind) && nullptr != nufft && nufft->toeplitz; }
 
 	// A^H A: Toeplitz when available, else adjoint(forward(x)) through kbuf
 	// (linops/linop.c:489-525); kbuf must be allocated (prepare_normal)
@@ -278,23 +293,31 @@ struct Op {
 	{
 		if (nufft)
 			nufft->stream = s;
+		if (chain)
+			chain->stream = s;
 		if (nudft)
 			nudft->stream = s;
 	}
 
-	const void* plan_id() const { return nufft ? (const void*)nufft.get() : (const void*)nudft.get(); }
+	const void* plan_id() const
+	{
+		return nufft ? (const void*)nufft.get() : (chain ? (const void*)chain.get() : (const void*)nudft.get());
+	}
 
 	// relative cost of one normal operator: the sub-grid stack it transforms
 	double cost() const
 	{
 		if (OP_NUDFT == kind)
 			return (double)isize * (double)osize;
+		if (chain)
+			return (double)chain->C * (double)chain->N2;
 		return (double)nufft->S * nufft->C * nchunks() * (double)nufft->N;
 	}
 
 	Nufft& plan(const char* what)
 	{
-		BAM_CHECK(OP_NUDFT != kind && nullptr != nufft, "%s: not a NUFFT operator", what);
+		BAM_CHECK(OP_NUDFT != kind && (nullptr != nufft || nullptr != chain), "%s: not a NUFFT operator", what);
+		BAM_CHECK(nullptr == chain, "%s: not supported for the chained NUFFT (oversampling other than 2)", what);
 		BAM_CHECK(0 == K, "%s: not supported for an operator with a subspace basis", what);
 		return *nufft;
 	}
@@ -414,12 +437,23 @@ static NufftDims parse_nufft_dims(int N, const long* ksp_dims, const long* cim_d
 static void check_conf(const nufft_conf_s& conf, bool has_basis)
 {
 	(void)has_basis;
-	BAM_CHECK(2.f == conf.os, "nufft: only oversampling 2 is supported (chained NUFFT not implemented)");
 	BAM_CHECK(conf.decomp, "nufft: the non-decomposed oversampled grid (-1) is not supported");
 	BAM_CHECK(!conf.pcycle, "nufft: phase cycling is not supported");
 	BAM_CHECK(!conf.compress_psf, "nufft: compressed PSF storage is not supported");
 	BAM_CHECK(0 == conf.cfft, "nufft: cfft flags are not supported");
 	BAM_CHECK(7 == (conf.flags & 7), "nufft: flags must include dims 0-2");
+}
+
+// trajectory [3][M] (complex, host or device) -> host M x 3 reals
+static std::vector<float> traj_reals_of(const float2* traj, long M)
+{
+	std::vector<float2> th = to_host(traj, 3 * M);
+	std::vector<float> tr(3 * M);
+
+	for (long i = 0; i < 3 * M; i++)
+		tr[i] = th[i].x;
+
+	return tr;
 }
 
 // weights of shape wgh_dims (broadcast over coils) -> per-sample vector;
@@ -543,11 +577,42 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 			 const long* bas_dims = nullptr, const float2* basis = nullptr)
 {
 	if (nullptr != basis) {
+		BAM_CHECK(2.f == conf.os, "nufft: a subspace basis needs oversampling 2 (the chained NUFFT has none here)");
 		BAM_CHECK(nullptr != bas_dims, "nufft: basis without dims");
 		return make_basis_op(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, conf, bas_dims, basis);
 	}
 
 	NufftDims nd = parse_nufft_dims(N, ksp_dims, cim_dims, traj_dims);
+
+	if (2.f != conf.os) {
+
+		// the chained NUFFT (noncart/nufft.c:1184-1199): no Toeplitz mode,
+		// no memory modes
+		std::vector<float> tr = traj_reals_of(traj, nd.M);
+		std::vector<float2> w = expand_weights(N, wgh_dims, weights, ksp_dims);
+		auto op = new Op();
+		op->kind = OP_NUFFT;
+		op->N = N;
+		memcpy(op->odims, ksp_dims, sizeof(long) * N);
+		memcpy(op->idims, cim_dims, sizeof(long) * N);
+		op->osize = md_size(N, ksp_dims);
+		op->isize = md_size(N, cim_dims);
+		op->q = std::make_shared<Queue>();
+		op->stream = op->q->stream;
+		op->ctotal = op->cchunk = nd.C;
+
+		try {
+			op->chain = std::make_shared<Chain>();
+			op->chain->stream = op->stream;
+			op->chain->create(nd.img, nd.C, nd.M, tr.data(), w.empty() ? nullptr : w.data(), conf.periodic,
+					  conf.width, conf.os);
+		} catch (...) {
+			delete op;
+			throw;
+		}
+
+		return op;
+	}
 
 
 	std::vector<float2> th = to_host(traj, 3 * nd.M);
@@ -861,6 +926,15 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		memcpy(op->idims, img_dims, sizeof(long) * N);
 		op->isize = md_size(N, img_dims);
 
+		if (op->chain) {
+			try {
+				op->bcim.alloc((size_t)op->chain->C * op->chain->N);
+			} catch (...) {
+				delete op;
+				throw;
+			}
+		}
+
 		if (op->K > 0) {
 			try {
 				op->bcim.alloc((size_t)op->ncoil * op->K * op->nufft->N);
@@ -907,6 +981,7 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 			nop->q = op->q;
 			nop->stream = op->stream;
 			nop->nufft = op->nufft;
+			nop->chain = op->chain;
 			nop->ctotal = op->ctotal;
 			nop->cchunk = op->cchunk;
 			nop->K = op->K;
@@ -1502,7 +1577,7 @@ int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxite
 		bool grouped = groups_on && nprob > 1;
 
 		for (int i = 0; i < nprob && grouped; i++)
-			grouped = OP_SENSE == op[i]->kind && 1 == op[i]->nchunks() &&
+			grouped = OP_SENSE == op[i]->kind && 1 == op[i]->nchunks() && nullptr != op[i]->nufft &&
 				  normal_groupable(*op[i]->nufft, op[i]->maps());
 
 		// longest-processing-time assignment of the problems to the streams
@@ -1785,7 +1860,8 @@ static const float2* warm_input(Op* op, float l2lambda, int N, const long img_di
 				const char* what)
 {
 	check_dims(what, N, img_dims, op->N, op->idims);
-	BAM_CHECK(OP_NUDFT != op->kind && 1 == op->nufft->n[2] && op->nufft->n[0] > 1 && op->nufft->n[1] > 1,
+	BAM_CHECK(OP_NUDFT != op->kind && nullptr != op->nufft, "%s: not for the chained NUFFT", what);
+	BAM_CHECK(1 == op->nufft->n[2] && op->nufft->n[0] > 1 && op->nufft->n[1] > 1,
 		  "%s: 2-D images only", what);
 	BAM_CHECK(0 == op->K, "%s: not for an operator with a subspace basis", what);
 	BAM_CHECK(!op->toeplitz() || op->nufft->has_psf, "%s: Toeplitz operator without PSF", what);

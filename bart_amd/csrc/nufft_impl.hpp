@@ -264,12 +264,13 @@ void basis_psf_vals(float2* v, const float2* B, int K, int T, long Mf, hipStream
 // into the centre of the round(os n) grid, centred unitary FFT, KB
 // interpolation on that grid with beta = calc_beta(os, width), weights times
 // os^(d/2).  The adjoint gridding gathers per grid point from samples sorted
-// by cells of CHAIN_CS points.  No Toeplitz operator (BART has none here).
+// by cells of CHAIN_CS points (1 along a periodic dim that CHAIN_CS does not
+// divide).  No Toeplitz operator (BART has none here).
 constexpr int CHAIN_CS = 4;
 constexpr int CHAIN_TAPS = 16;
 struct Chain {
 
-	int n[3], on[3], off[3], ncell[3];
+	int n[3], on[3], off[3], ncell[3], cs[3];
 	long N = 0, N2 = 0, M = 0;
 	int C = 0;
 	int nd = 0;
@@ -283,7 +284,8 @@ struct Chain {
 	DevBuf<float2> fma;     // [N2] fftmod x N2^-1/4
 	DevBuf<float4> trj;     // [M] coordinates (original order)
 	DevBuf<float2> tmp;     // [M] weights x scale
-	DevBuf<float4> spos;    // sorted sample positions on the grid, w = original index
+	DevBuf<float4> spos;    // sorted sample positions on the grid
+	DevBuf<int> sidx;       // sorted sample -> original index
 	DevBuf<int> cstart;     // [cells + 1]
 	DevBuf<float2> grid;    // [C][N2]
 	DevBuf<float2> kwork;   // [C][M]

@@ -60,8 +60,10 @@ struct nufft_conf_s {
 extern struct nufft_conf_s nufft_conf_defaults;
 
 /* noncart/nufft.h:48-57 -- replaces nufft_create2 (noncart/nufft.c:1174-1276).
- * Supported: os = 2, decomp = true, kernel width <= 14, 1 to 3
- * transformed dims (flags = 7), samples in dims 1-2 and coils in dim 3.
+ * Supported: decomp = true, kernel width <= 14 (<= 15 with os != 2), 1
+ * to 3 transformed dims (flags = 7), samples in dims 1-2 and coils in dim
+ * 3.  os != 2 gives the chained NUFFT of noncart/nufft_chain.c:192-244
+ * (round(os n) grid, no Toeplitz operator, no basis).
  * A subspace basis (bas_dims [1 1 1 1 1 T K], K <= 16,
  * noncart/nufft.c:1018-1036) takes coil images [X Y Z C 1 1 K], k-space
  * [1 R S C 1 T] (dim 6 1 or K; the operator's output has 1) and a
