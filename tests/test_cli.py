@@ -105,7 +105,7 @@ def test_header_with_extra_sections(tmp_path):
 @pytest.mark.parametrize("args,msg", [
     (["nufft", "-s", "-p", "p", "t", "i", "o"], "ERROR"),
     (["nufft", "-a", "-i", "t", "i", "o"], "Adjoint and inverse"),
-    (["nufft", "-B", "b", "t", "i", "o"], "-B"),
+    (["nufft", "-F", "f", "t", "i", "o"], "-F"),
     (["pics", "-l1", "-t", "t", "k", "s", "o"], "-l1"),
     (["pics", "-R", "W:7:0:0.01", "-t", "t", "k", "s", "o"], "-R"),
     (["pics", "k", "s", "o"], "non-Cartesian"),
