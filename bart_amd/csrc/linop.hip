@@ -129,6 +129,10 @@ struct Op {
 	// a SENSE operator (ncoil K N)
 	int K = 0, T = 1, ncoil = 0;
 	long Mf = 0;
+	// batch dims >= 4 sharing the trajectory: nbatch images; a SENSE
+	// operator then applies its maps to each (bcim: the ncoil nbatch coil
+	// images), the plan has ncoil nbatch channels
+	int nbatch = 1;
 	DevBuf<float2> bksp, bcim;
 	std::mutex& mtx() { return q->mtx; }
 	DevBuf<float2> stage_in, stage_out, kbuf;
@@ -175,8 +179,13 @@ struct Op {
 		if (chain) {
 			const float2* m = maps();
 			if (nullptr != m)
-				maps_expand(bcim.ptr, m, s, chain->C, 1, chain->N, chain->stream);
+				maps_expand(bcim.ptr, m, s, ncoil, nbatch, chain->N, chain->stream);
 			chain->forward(d, (nullptr != m) ? bcim.ptr : s);
+			return;
+		}
+		if (nbatch > 1 && nullptr != maps()) {
+			maps_expand(bcim.ptr, maps(), s, ncoil, nbatch, nufft->N, nufft->stream);
+			nufft->forward(d, bcim.ptr, nullptr);
 			return;
 		}
 		if (K > 0) {
@@ -209,7 +218,12 @@ struct Op {
 			const float2* m = maps();
 			chain->adjoint((nullptr != m) ? bcim.ptr : d, s);
 			if (nullptr != m)
-				maps_reduce(d, m, bcim.ptr, chain->C, 1, chain->N, chain->stream);
+				maps_reduce(d, m, bcim.ptr, ncoil, nbatch, chain->N, chain->stream);
+			return;
+		}
+		if (nbatch > 1 && nullptr != maps()) {
+			nufft->adjoint(bcim.ptr, s, nullptr);
+			maps_reduce(d, maps(), bcim.ptr, ncoil, nbatch, nufft->N, nufft->stream);
 			return;
 		}
 		if (K > 0) {
@@ -244,6 +258,13 @@ struct Op {
 		if (!toeplitz()) {
 			fwd(kbuf.ptr, s);
 			adj(d, kbuf.ptr);
+			return false;
+		}
+
+		if (nbatch > 1 && nullptr != maps()) {
+			maps_expand(bcim.ptr, maps(), s, ncoil, nbatch, nufft->N, nufft->stream, skip);
+			nufft->normal(bcim.ptr, bcim.ptr, nullptr, skip);
+			maps_reduce(d, maps(), bcim.ptr, ncoil, nbatch, nufft->N, nufft->stream, skip);
 			return false;
 		}
 
@@ -403,6 +424,7 @@ struct NufftDims {
 	int img[3];
 	int C;
 	long M;
+	int B;  // batch: the product of dims >= 4 (trajectory and weights shared)
 };
 
 static NufftDims parse_nufft_dims(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims)
@@ -416,11 +438,22 @@ static NufftDims parse_nufft_dims(int N, const long* ksp_dims, const long* cim_d
 	BAM_CHECK(ksp_dims[3] == cim_dims[3], "nufft: coil dims of k-space (%ld) and image (%ld) differ",
 		  ksp_dims[3], cim_dims[3]);
 
-	for (int i = 4; i < N; i++)
-		BAM_CHECK(1 == ksp_dims[i] && 1 == cim_dims[i] && 1 == traj_dims[i],
-			  "nufft: dim %d must be 1 (batching over dims >= 4 is not supported)", i);
+	// dims >= 4 are batch dims with one trajectory for all (the NUFFT then
+	// treats them like coils: k-space and images match there,
+	// noncart/nufft.c:1229)
+	long B = 1;
+
+	for (int i = 4; i < N; i++) {
+		BAM_CHECK(ksp_dims[i] == cim_dims[i], "nufft: k-space and image differ in dim %d (%ld != %ld)", i,
+			  ksp_dims[i], cim_dims[i]);
+		BAM_CHECK(1 == traj_dims[i], "nufft: the trajectory may not vary over dim %d (one trajectory per batch)", i);
+		B *= ksp_dims[i];
+	}
+
+	BAM_CHECK(B >= 1 && B < (1l << 24), "nufft: bad batch size %ld", B);
 
 	NufftDims r;
+	r.B = (int)B;
 	for (int i = 0; i < 3; i++) {
 		BAM_CHECK(cim_dims[i] >= 1 && cim_dims[i] < (1l << 20), "nufft: bad image dim");
 		r.img[i] = (int)cim_dims[i];
@@ -458,7 +491,10 @@ static std::vector<float> traj_reals_of(const float2* traj, long M)
 
 // weights of shape wgh_dims (broadcast over coils) -> per-sample vector;
 // with T > 1 frames (subspace basis) the weights may vary over dim 5 and
-// the vector holds the frames' samples r + R (s + Sp t)
+// the vector holds the frames' samples r + R (s + Sp t).  Over batch dims
+// (>= 4, sharing the trajectory) the weights must be the same for every
+// batch entry (as the pattern pics estimates from repeated data is); they
+// may be given broadcast or in full.
 static std::vector<float2> expand_weights(int N, const long* wgh_dims, const float2* w, const long* ksp_dims,
 					  long T = 1)
 {
@@ -469,22 +505,45 @@ static std::vector<float2> expand_weights(int N, const long* wgh_dims, const flo
 
 	long R = ksp_dims[1], Sp = ksp_dims[2];
 	long wr = wgh_dims[1], ws = wgh_dims[2];
-	long wt = (N > 5) ? wgh_dims[5] : 1;
+	long wt = (N > 5 && T > 1) ? wgh_dims[5] : 1;
 
 	BAM_CHECK(1 == wgh_dims[0], "nufft: weights dim 0 must be 1");
 	BAM_CHECK((wr == R || wr == 1) && (ws == Sp || ws == 1), "Incompatible dimensions of k-space and weights!");
-	BAM_CHECK(wt == 1 || (wt == T && T > 1), "Incompatible dimensions of k-space and weights!");
+	BAM_CHECK(wt == 1 || wt == T, "Incompatible dimensions of k-space and weights!");
+	BAM_CHECK(1 == wgh_dims[3], "nufft: weights must not vary over the coil dim");
 
-	for (int i = 3; i < N; i++)
-		BAM_CHECK(1 == wgh_dims[i] || (5 == i && T > 1), "nufft: weights must not vary over dims >= 3");
+	long str[32];
+	long tot = 1;
 
-	std::vector<float2> h = to_host(w, wr * ws * wt);
+	for (int i = 0; i < N; i++) {
+		str[i] = tot;
+		tot *= wgh_dims[i];
+	}
+
+	for (int i = 4; i < N; i++)
+		BAM_CHECK(1 == wgh_dims[i] || wgh_dims[i] == ksp_dims[i], "Incompatible dimensions of k-space and weights!");
+
+	std::vector<float2> h = to_host(w, tot);
 	out.resize(R * Sp * T);
 
 	for (long t = 0; t < T; t++)
 		for (long s = 0; s < Sp; s++)
 			for (long r = 0; r < R; r++)
-				out[r + R * (s + Sp * t)] = h[(wr > 1 ? r : 0) + wr * ((ws > 1 ? s : 0) + ws * (wt > 1 ? t : 0))];
+				out[r + R * (s + Sp * t)] =
+					h[(wr > 1 ? r : 0) * str[1] + (ws > 1 ? s : 0) * str[2] + (wt > 1 ? t * str[5] : 0)];
+
+	// batch dims: every batch entry must hold the same weights
+	for (long e = 0; e < tot; e++) {
+		long rem = e, base = 0;
+		for (int i = 0; i < N; i++) {
+			const long c = rem % wgh_dims[i];
+			rem /= wgh_dims[i];
+			if (i < 4 || (5 == i && T > 1))
+				base += c * str[i];
+		}
+		BAM_CHECK(h[e].x == h[base].x && h[e].y == h[base].y,
+			  "nufft: weights that differ between batch entries (dims >= 4) are not supported");
+	}
 
 	return out;
 }
@@ -523,6 +582,7 @@ static Op* make_basis_op(int N, const long* ksp_dims, const long* cim_dims, cons
 	}
 
 	NufftDims nd = parse_nufft_dims(N, k1, c1, t1);
+	BAM_CHECK(1 == nd.B, "nufft: a subspace basis with further batch dims is not supported");
 	const long Mf = nd.M;
 	const long Mp = Mf * T;
 	const bool traj_t = (T > 1 && T == traj_dims[5]);
@@ -599,13 +659,15 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 		op->isize = md_size(N, cim_dims);
 		op->q = std::make_shared<Queue>();
 		op->stream = op->q->stream;
-		op->ctotal = op->cchunk = nd.C;
+		op->ctotal = op->cchunk = nd.C * nd.B;
+		op->ncoil = nd.C;
+		op->nbatch = nd.B;
 
 		try {
 			op->chain = std::make_shared<Chain>();
 			op->chain->stream = op->stream;
-			op->chain->create(nd.img, nd.C, nd.M, tr.data(), w.empty() ? nullptr : w.data(), conf.periodic,
-					  conf.width, conf.os);
+			op->chain->create(nd.img, nd.C * nd.B, nd.M, tr.data(), w.empty() ? nullptr : w.data(),
+					  conf.periodic, conf.width, conf.os);
 		} catch (...) {
 			delete op;
 			throw;
@@ -637,10 +699,13 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 	// time, C N grid points): here coil chunks whose stack S c N is at most
 	// C N for lowmem, and single coils (S N) for zero_overhead; the largest
 	// chunk size that divides C
-	op->ctotal = nd.C;
-	op->cchunk = nd.C;
+	op->ctotal = nd.C * nd.B;
+	op->cchunk = op->ctotal;
+	op->ncoil = nd.C;
+	op->nbatch = nd.B;
 
-	if (conf.lowmem || conf.zero_overhead) {
+	// (with batch dims the plan keeps all channels: the chunks are of coils)
+	if ((conf.lowmem || conf.zero_overhead) && 1 == nd.B) {
 		int S = 1;
 		for (int i = 0; i < 3; i++)
 			S *= (nd.img[i] > 1) ? 2 : 1;
@@ -880,7 +945,7 @@ const struct linop_s* nudft_create(int N, unsigned long flags, const long odims[
 		try {
 			op->nudft = std::make_shared<Nudft>();
 			op->nudft->stream = op->stream;
-			op->nudft->create(nd.img, nd.C, nd.M, tr.data());
+			op->nudft->create(nd.img, nd.C * nd.B, nd.M, tr.data());
 		} catch (...) {
 			delete op;
 			throw;
@@ -914,6 +979,9 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		for (int i = 0; i < 4; i++)
 			BAM_CHECK(map_dims[i] == coilim_dims[i], "sense_nc_init: map dims do not match (dim %d)", i);
 
+		for (int i = 4; i < N; i++)
+			BAM_CHECK(1 == map_dims[i], "sense_nc_init: maps varying over dim %d are not supported", i);
+
 		// the coefficient dim of the k-space follows the image
 		// (sense/modelnc.c:41-43)
 		long ksp2[16];
@@ -926,9 +994,9 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		memcpy(op->idims, img_dims, sizeof(long) * N);
 		op->isize = md_size(N, img_dims);
 
-		if (op->chain) {
+		if (op->chain || op->nbatch > 1) {
 			try {
-				op->bcim.alloc((size_t)op->chain->C * op->chain->N);
+				op->bcim.alloc((size_t)op->ncoil * op->nbatch * (op->chain ? op->chain->N : op->nufft->N));
 			} catch (...) {
 				delete op;
 				throw;
@@ -985,6 +1053,7 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 			nop->ctotal = op->ctotal;
 			nop->cchunk = op->cchunk;
 			nop->K = op->K;
+			nop->nbatch = op->nbatch;
 			nop->T = op->T;
 			nop->Mf = op->Mf;
 			nop->ncoil = op->ncoil;
@@ -1578,7 +1647,7 @@ int lsqr_conjgrad_batch(int nprob, const struct linop_s* const ops[], int maxite
 
 		for (int i = 0; i < nprob && grouped; i++)
 			grouped = OP_SENSE == op[i]->kind && 1 == op[i]->nchunks() && nullptr != op[i]->nufft &&
-				  normal_groupable(*op[i]->nufft, op[i]->maps());
+				  1 == op[i]->nbatch && normal_groupable(*op[i]->nufft, op[i]->maps());
 
 		// longest-processing-time assignment of the problems to the streams
 		const int K = grouped ? 1 : std::min(nprob, CG_BATCH_STREAMS);
@@ -1864,6 +1933,7 @@ static const float2* warm_input(Op* op, float l2lambda, int N, const long img_di
 	BAM_CHECK(1 == op->nufft->n[2] && op->nufft->n[0] > 1 && op->nufft->n[1] > 1,
 		  "%s: 2-D images only", what);
 	BAM_CHECK(0 == op->K, "%s: not for an operator with a subspace basis", what);
+	BAM_CHECK(1 == op->nbatch, "%s: not for an operator with batch dims", what);
 	BAM_CHECK(!op->toeplitz() || op->nufft->has_psf, "%s: Toeplitz operator without PSF", what);
 	const long npix = op->isize;
 

@@ -501,9 +501,17 @@ def test_errors_are_reported():
     with pytest.raises(ba.BartError):
         ba.nufft_create2(dims16(1, 10, 1, 2), dims16(16, 16, 1, 3), dims16(3, 10, 1),
                          np.zeros(30, np.complex64))
+    # the non-decomposed grid (nufft -1) and compressed PSF storage are not built
     with pytest.raises(ba.BartError):
         ba.nufft_create2(dims16(1, 10, 1, 1), dims16(16, 16, 1, 1), dims16(3, 10, 1),
-                         np.zeros(30, np.complex64), conf=ba.NufftConf.defaults(os=1.5))
+                         np.zeros(30, np.complex64), conf=ba.NufftConf.defaults(decomp=False))
+    with pytest.raises(ba.BartError):
+        ba.nufft_create2(dims16(1, 10, 1, 1), dims16(16, 16, 1, 1), dims16(3, 10, 1),
+                         np.zeros(30, np.complex64), conf=ba.NufftConf.defaults(compress_psf=True))
+    # the trajectory may not vary over batch dims
+    with pytest.raises(ba.BartError):
+        ba.nufft_create2(dims16(1, 10, 1, 1, 1, 2), dims16(16, 16, 1, 1, 1, 2), dims16(3, 10, 1, 1, 1, 2),
+                         np.zeros(60, np.complex64))
 
 
 @pytest.mark.parametrize("img,C,mode", [((64, 64, 1), 8, "lowmem"), ((64, 64, 1), 6, "zero_overhead"),
