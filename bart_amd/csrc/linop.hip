@@ -133,6 +133,9 @@ struct Op {
 	// operator then applies its maps to each (bcim: the ncoil nbatch coil
 	// images), the plan has ncoil nbatch channels
 	int nbatch = 1;
+	// frames with trajectories of their own: one plan each (nufft = the
+	// first), frame t's k-space and images at t times the frame slab
+	std::vector<std::shared_ptr<Nufft>> frames;
 	DevBuf<float2> bksp, bcim;
 	std::mutex& mtx() { return q->mtx; }
 	DevBuf<float2> stage_in, stage_out, kbuf;
@@ -176,6 +179,13 @@ struct Op {
 			nudft->forward(d, s);
 			return;
 		}
+		if (!frames.empty()) {
+			const float2* m = maps();
+			const long ci = (nullptr != m) ? nufft->N : (long)ncoil * nufft->N;
+			for (size_t t = 0; t < frames.size(); t++)
+				frames[t]->forward(d + (long)t * ncoil * nufft->M, s + (long)t * ci, m);
+			return;
+		}
 		if (chain) {
 			const float2* m = maps();
 			if (nullptr != m)
@@ -212,6 +222,13 @@ struct Op {
 		BAM_CHECK(!normal_only, "NuFFT with normal operator only!");
 		if (OP_NUDFT == kind) {
 			nudft->adjoint(d, s);
+			return;
+		}
+		if (!frames.empty()) {
+			const float2* m = maps();
+			const long ci = (nullptr != m) ? nufft->N : (long)ncoil * nufft->N;
+			for (size_t t = 0; t < frames.size(); t++)
+				frames[t]->adjoint(d + (long)t * ci, s + (long)t * ncoil * nufft->M, m);
 			return;
 		}
 		if (chain) {
@@ -258,6 +275,14 @@ struct Op {
 		if (!toeplitz()) {
 			fwd(kbuf.ptr, s);
 			adj(d, kbuf.ptr);
+			return false;
+		}
+
+		if (!frames.empty()) {
+			const float2* m = maps();
+			const long ci = (nullptr != m) ? nufft->N : (long)ncoil * nufft->N;
+			for (size_t t = 0; t < frames.size(); t++)
+				frames[t]->normal(d + (long)t * ci, s + (long)t * ci, m, skip);
 			return false;
 		}
 
@@ -316,6 +341,8 @@ struct Op {
 			nufft->stream = s;
 		if (chain)
 			chain->stream = s;
+		for (auto& f : frames)
+			f->stream = s;
 		if (nudft)
 			nudft->stream = s;
 	}
@@ -332,6 +359,8 @@ struct Op {
 			return (double)isize * (double)osize;
 		if (chain)
 			return (double)chain->C * (double)chain->N2;
+		if (!frames.empty())
+			return (double)frames.size() * nufft->S * nufft->C * (double)nufft->N;
 		return (double)nufft->S * nufft->C * nchunks() * (double)nufft->N;
 	}
 
@@ -339,6 +368,7 @@ struct Op {
 	{
 		BAM_CHECK(OP_NUDFT != kind && (nullptr != nufft || nullptr != chain), "%s: not a NUFFT operator", what);
 		BAM_CHECK(nullptr == chain, "%s: not supported for the chained NUFFT (oversampling other than 2)", what);
+		BAM_CHECK(frames.empty(), "%s: not supported for per-frame trajectories", what);
 		BAM_CHECK(0 == K, "%s: not supported for an operator with a subspace basis", what);
 		return *nufft;
 	}
@@ -632,6 +662,70 @@ static Op* make_basis_op(int N, const long* ksp_dims, const long* cim_dims, cons
 	return op;
 }
 
+// Batch dims with a trajectory per entry (frames, noncart/nufft.c:1229 with
+// trajectory dims > 1 beyond dim 3): one plan per frame, applied in frame
+// order to the frame's slab of k-space and images; the weights may vary
+// over the frames as well.
+static Op* make_frames_op(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims,
+			  const float2* traj, const long* wgh_dims, const float2* weights, const nufft_conf_s& conf,
+			  long F)
+{
+	long t1[32], w1[32];
+	long wstride = 0;
+	bool wframes = false;
+
+	for (int i = 0; i < N; i++) {
+		t1[i] = (i >= 4) ? 1 : traj_dims[i];
+		w1[i] = (nullptr != wgh_dims && i >= 4) ? 1 : (nullptr != wgh_dims ? wgh_dims[i] : 1);
+		if (nullptr != wgh_dims && i >= 4 && wgh_dims[i] > 1) {
+			BAM_CHECK(wgh_dims[i] == ksp_dims[i] && traj_dims[i] == ksp_dims[i],
+				  "Incompatible dimensions of k-space and weights!");
+			wframes = true;
+		}
+	}
+
+	NufftDims nd = parse_nufft_dims(N, ksp_dims, cim_dims, t1);
+	BAM_CHECK(nd.B == F, "nufft: frames and batch dims do not match");
+
+	if (wframes) {
+		wstride = 1;
+		for (int i = 0; i < 4; i++)
+			wstride *= wgh_dims[i];
+	}
+
+	auto op = new Op();
+	op->kind = OP_NUFFT;
+	op->N = N;
+	memcpy(op->odims, ksp_dims, sizeof(long) * N);
+	memcpy(op->idims, cim_dims, sizeof(long) * N);
+	op->osize = md_size(N, ksp_dims);
+	op->isize = md_size(N, cim_dims);
+	op->q = std::make_shared<Queue>();
+	op->stream = op->q->stream;
+	op->ctotal = op->cchunk = nd.C;
+	op->ncoil = nd.C;
+	op->nbatch = (int)F;
+
+	try {
+		for (long t = 0; t < F; t++) {
+			std::vector<float> tr = traj_reals_of(traj + 3 * nd.M * t, nd.M);
+			std::vector<float2> w = expand_weights(N, w1, weights ? weights + wstride * t : nullptr, ksp_dims);
+			auto p = std::make_shared<Nufft>();
+			p->stream = op->stream;
+			p->real_psf = conf.real;
+			p->create(nd.img, nd.C, nd.M, tr.data(), w.empty() ? nullptr : w.data(), conf.toeplitz,
+				  conf.periodic, conf.width, conf.os, conf.toeplitz && !conf.nopsf);
+			op->frames.push_back(p);
+		}
+	} catch (...) {
+		delete op;
+		throw;
+	}
+
+	op->nufft = op->frames[0];
+	return op;
+}
+
 static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, const long* traj_dims, const float2* traj,
 			 const long* wgh_dims, const float2* weights, const nufft_conf_s& conf,
 			 const long* bas_dims = nullptr, const float2* basis = nullptr)
@@ -640,6 +734,26 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 		BAM_CHECK(2.f == conf.os, "nufft: a subspace basis needs oversampling 2 (the chained NUFFT has none here)");
 		BAM_CHECK(nullptr != bas_dims, "nufft: basis without dims");
 		return make_basis_op(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, conf, bas_dims, basis);
+	}
+
+	// batch dims over which the trajectory varies (frames with trajectories
+	// of their own): one plan per frame
+	long F = 1;
+	bool shared = false;
+
+	for (int i = 4; i < N; i++) {
+		if (traj_dims[i] > 1) {
+			BAM_CHECK(traj_dims[i] == ksp_dims[i], "Incompatible dimensions of k-space and trajectory!");
+			F *= traj_dims[i];
+		} else if (ksp_dims[i] > 1) {
+			shared = true;
+		}
+	}
+
+	if (F > 1) {
+		BAM_CHECK(!shared, "nufft: batch dims with and without trajectories of their own cannot be mixed");
+		BAM_CHECK(2.f == conf.os, "nufft: per-frame trajectories need oversampling 2");
+		return make_frames_op(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, conf, F);
 	}
 
 	NufftDims nd = parse_nufft_dims(N, ksp_dims, cim_dims, traj_dims);
@@ -994,7 +1108,7 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 		memcpy(op->idims, img_dims, sizeof(long) * N);
 		op->isize = md_size(N, img_dims);
 
-		if (op->chain || op->nbatch > 1) {
+		if (op->chain || (op->nbatch > 1 && op->frames.empty())) {
 			try {
 				op->bcim.alloc((size_t)op->ncoil * op->nbatch * (op->chain ? op->chain->N : op->nufft->N));
 			} catch (...) {
@@ -1054,6 +1168,7 @@ const struct linop_s* sense_nc_init(const long max_dims[16], const long map_dims
 			nop->cchunk = op->cchunk;
 			nop->K = op->K;
 			nop->nbatch = op->nbatch;
+			nop->frames = op->frames;
 			nop->T = op->T;
 			nop->Mf = op->Mf;
 			nop->ncoil = op->ncoil;
@@ -2059,6 +2174,7 @@ const struct operator_s* nufft_precond_create(const struct linop_s* _op)
 		BAM_CHECK(op->toeplitz(), "nufft_precond_create: the operator must be in Toeplitz mode");
 		BAM_CHECK(1 == op->nchunks(), "nufft_precond_create: not with the low-memory modes");
 		BAM_CHECK(0 == op->K, "nufft_precond_create: not for an operator with a subspace basis");
+		BAM_CHECK(op->frames.empty(), "nufft_precond_create: not for per-frame trajectories");
 		std::lock_guard<std::mutex> lock(op->mtx());
 		auto p = new Precond{ op, {}, (int)(op->isize / op->nufft->N) };
 		try {

@@ -115,3 +115,65 @@ def test_cli_pics_batch_lowmem(tmp_path):
     spread = max(relerr(rr[i], rr[0]) for i in range(B5 * B6))
     print(f"pics -U vs pics over a {B5} x {B6} batch: {e:.1e}; spread over the batch {spread:.1e}")
     assert e < 5e-4 and spread < 1e-6
+
+
+def frame_trajs(F, R, S):
+    # golden-angle radial, a different set of spokes per frame
+    t = I.radial_traj(R, F * S)
+    return np.ascontiguousarray(t.reshape(F, R * S, 3))
+
+
+def test_nufft_frame_trajectories_vs_oracle():
+    # a trajectory per frame (dims >= 4 of the trajectory equal to the
+    # k-space's): frame t is gridded with its own trajectory and weights
+    import bart_amd as ba
+    n, C, F, R, S = 32, 2, 3, 64, 7
+    traj = frame_trajs(F, R, S)
+    M = R * S
+    w = (0.5 + np.random.default_rng(7).random((F, M))).astype(np.complex64)
+    op = ba.nufft_create2(d16(1, R, S, C, 1, F), d16(n, n, 1, C, 1, F), d16(3, R, S, 1, 1, F), traj,
+                          wgh_dims=d16(1, R, S, 1, 1, F), weights=w)
+    x = crandn((F, C, 1, n, n), 8)
+    y = crandn((F, C, M), 9)
+    f = op.forward(dev(x)).cpu().numpy().reshape(F, C, M)
+    a = op.adjoint(dev(y)).cpu().numpy().reshape(F, C, 1, n, n)
+    nx = op.normal(dev(x)).cpu().numpy().reshape(F, C, 1, n, n)
+    refs = [O.Nufft((n, n, 1), traj[t], C, weights=w[t], toeplitz=True) for t in range(F)]
+    e_f = relerr(f, np.stack([refs[t].forward(x[t]) for t in range(F)]))
+    e_a = relerr(a, np.stack([refs[t].adjoint(y[t]) for t in range(F)]))
+    e_n = relerr(nx, np.stack([refs[t].normal(x[t]) for t in range(F)]))
+    print(f"frames: forward {e_f:.1e} adjoint {e_a:.1e} normal {e_n:.1e}")
+    assert e_f < 1e-5 and e_a < 1e-5 and e_n < 1e-5
+    op.free()
+
+
+def test_sense_frame_trajectories_cgsense():
+    import bart_amd as ba
+    n, C, F, R, S = 16, 3, 3, 32, 9
+    traj = frame_trajs(F, R, S)
+    M = R * S
+    maps = I.gaussian_coils((1, n, n), C, seed=8)
+    op = ba.sense_nc_init(d16(n, n, 1, C, 1, F), d16(n, n, 1, C), maps, d16(1, R, S, C, 1, F),
+                          d16(3, R, S, 1, 1, F), traj)
+    refs = [O.Nufft((n, n, 1), traj[t], C, toeplitz=True) for t in range(F)]
+    x = crandn((F, 1, n, n), 10)
+    y = crandn((F, C, M), 11)
+    e_n = relerr(op.normal(dev(x)).cpu().numpy().reshape(F, n, n),
+                 np.stack([refs[t].sense_normal(maps, x[t])[0] for t in range(F)]))
+    assert e_n < 1e-5
+    lam = 0.05
+    b = op.adjoint(dev(y))
+    bh = b.cpu().numpy().reshape(F, n * n).astype(np.complex128)
+    xs = []
+    for t in range(F):
+        A = np.zeros((n * n, n * n), np.complex128)
+        for j in range(n * n):
+            e = np.zeros(n * n, np.complex64)
+            e[j] = 1
+            A[:, j] = refs[t].sense_normal(maps, e.reshape(1, n, n)).ravel()
+        xs.append(np.linalg.solve(A + lam * np.eye(n * n), bh[t]))
+    xg, st, _ = ba.lsqr_conjgrad(op, b, 300, lam, 1e-6)
+    e_cg = relerr(xg.cpu().numpy(), np.stack(xs))
+    print(f"frames SENSE normal {e_n:.1e}; CG {st.iterations} iterations, vs per-frame dense solves {e_cg:.1e}")
+    assert st.status == 2 and e_cg < 1e-4
+    op.free()

@@ -508,9 +508,9 @@ def test_errors_are_reported():
     with pytest.raises(ba.BartError):
         ba.nufft_create2(dims16(1, 10, 1, 1), dims16(16, 16, 1, 1), dims16(3, 10, 1),
                          np.zeros(30, np.complex64), conf=ba.NufftConf.defaults(compress_psf=True))
-    # the trajectory may not vary over batch dims
+    # batch dims with and without trajectories of their own are not mixed
     with pytest.raises(ba.BartError):
-        ba.nufft_create2(dims16(1, 10, 1, 1, 1, 2), dims16(16, 16, 1, 1, 1, 2), dims16(3, 10, 1, 1, 1, 2),
+        ba.nufft_create2(dims16(1, 10, 1, 1, 1, 2, 2), dims16(16, 16, 1, 1, 1, 2, 2), dims16(3, 10, 1, 1, 1, 2),
                          np.zeros(60, np.complex64))
 
 
