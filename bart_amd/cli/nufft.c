@@ -1,7 +1,7 @@
 /*
  * nufft.c -- "bart nufft" on the gfx950 library (src/nufft.c:44-330).
  *
- *   nufft [-a | -i] [-x x:y:z] [-t | -r] [-l lambda] [-m iter] [-P] [-g]
+ *   nufft [-a | -i] [-x x:y:z] [-t | -r] [-l lambda] [-m iter] [-P] [-g] [-1]
  *         [-p pattern] [-o 2] [-w width] [--lowmem] [--zero-mem]
  *         [--no-precomp] traj input output
  *
@@ -13,9 +13,11 @@
  * -c: Strang's circulant preconditioner for -i (noncart/precond.c).
  * -B: a subspace basis [1 1 1 1 1 T K] (src/nufft.c:169-176, 208-212,
  * 284-288): coil images [X Y Z C 1 1 K], k-space [1 R S C 1 T].
- * --nufft-conf: BART's sub-options.  -g is accepted (every path runs on
- * the GPU).  -1 (the image is the oversampled grid itself, gridded without
- * further oversampling) and -F/-T (B0) are rejected with an error.
+ * -1: the non-decomposed NUFFT whose image is the oversampled grid itself
+ * (an estimated image size is doubled, src/nufft.c:197-202); it has no
+ * Toeplitz operator (-i needs -r).  --nufft-conf: BART's sub-options.  -g
+ * is accepted (every path runs on the GPU).  -F/-T (B0) are rejected with
+ * an error.
  */
 #include "tools.h"
 #include "cfl.h"
@@ -30,7 +32,7 @@
 static void usage(void)
 {
 	fprintf(stderr,
-		"Usage: nufft [-a,-i] [-x x:y:z] [-t,-r] [-l lambda] [-m iter] [-P] [-s] [-g] [-p <pattern>]\n"
+		"Usage: nufft [-a,-i] [-x x:y:z] [-t,-r] [-l lambda] [-m iter] [-P] [-s] [-g] [-1] [-p <pattern>]\n"
 		"             [-o os] [-w width] [-c] [-B <basis>] [--lowmem] [--zero-mem] [--no-precomp]\n"
 		"             <traj> <input> <output>\n"
 		"\n"
@@ -88,7 +90,7 @@ int main_nufft(int argc, char* argv[])
 		case O_NOPRECOMP: precomp = false; break;
 		case 'c': precond = true; break;
 		case 's': dft = true; break;
-		case '1': unsupported_option("nufft", "-1"); break;
+		case '1': conf.decomp = false; break;
 		case 'B': basis_file = optarg; break;
 		case 'F': unsupported_option("nufft", "-F"); break;
 		case 'T': unsupported_option("nufft", "-T"); break;
@@ -186,7 +188,7 @@ int main_nufft(int argc, char* argv[])
 		if (0 == coilim_vec[0] * coilim_vec[1] * coilim_vec[2]) {
 
 			for (int i = 0; i < 3; i++)
-				coilim_dims[i] = coilest[i];
+				coilim_dims[i] = (!conf.decomp && coilest[i] > 1) ? 2 * coilest[i] : coilest[i];
 
 			debug_printf(2, "Est. image size: %ldx%ldx%ld\n", coilim_dims[0], coilim_dims[1], coilim_dims[2]);
 

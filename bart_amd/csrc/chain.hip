@@ -1,6 +1,8 @@
 // chain.hip -- the chained NUFFT for oversampling factors other than 2
 // (noncart/nufft_chain.c:192-244, selected by nufft_create2 for os != 2,
-// noncart/nufft.c:1184-1199):
+// noncart/nufft.c:1184-1199), and in the same form the non-decomposed
+// NUFFT of decomp = false (nufft -1, :898-935: the image is the oversampled
+// grid, no padding, rolloff with os = 1):
 //   forward: x -> rolloff into the centre of the round(os n) grid
 //            (linop_kb_rolloff_create, nufft_chain.c:60-118) -> centred
 //            unitary FFT (fftmod(N2^-1/4) FFT fftmod(N2^-1/4),
@@ -368,7 +370,7 @@ __global__ void k_chain_grid(ChainGeom g, const float* __restrict__ kbt, const f
 }
 
 void Chain::create(const int dims[3], int C_, long M_, const float* traj, const float2* weights_host, bool periodic_,
-		   float width_, float os_)
+		   float width_, float os_, bool pad)
 {
 	BAM_CHECK(os_ >= 1.f, "nufft: oversampling %f below 1", os_);
 	BAM_CHECK(width_ > 0.f && (int)floorf(width_) + 1 <= CHAIN_TAPS, "nufft: kernel width %f not supported",
@@ -385,7 +387,7 @@ void Chain::create(const int dims[3], int C_, long M_, const float* traj, const 
 	for (int d = 0; d < 3; d++) {
 
 		n[d] = dims[d];
-		on[d] = (n[d] > 1) ? (int)lround(os * n[d]) : 1;
+		on[d] = (n[d] > 1 && pad) ? (int)lround(os * n[d]) : n[d];
 		off[d] = std::abs(on[d] / 2 - n[d] / 2);
 		BAM_CHECK(1 == on[d] || (on[d] >= n[d] && on[d] > (int)floorf(width) + 1 && fft_size_supported(on[d])),
 			  "nufft: oversampled size %d of dim %d not supported", on[d], d);
@@ -411,9 +413,10 @@ void Chain::create(const int dims[3], int C_, long M_, const float* traj, const 
 	for (int z = 0; z < n[2]; z++)
 		for (int y = 0; y < n[1]; y++)
 			for (int x = 0; x < n[0]; x++) {
-				const double rx = (n[0] > 1) ? rolloff_1d(n[0], x, os, beta, width) : 1.;
-				const double ry = (n[1] > 1) ? rolloff_1d(n[1], y, os, beta, width) : 1.;
-				const double rz = (n[2] > 1) ? rolloff_1d(n[2], z, os, beta, width) : 1.;
+				const float ros = pad ? os : 1.f;
+				const double rx = (n[0] > 1) ? rolloff_1d(n[0], x, ros, beta, width) : 1.;
+				const double ry = (n[1] > 1) ? rolloff_1d(n[1], y, ros, beta, width) : 1.;
+				const double rz = (n[2] > 1) ? rolloff_1d(n[2], z, ros, beta, width) : 1.;
 				rh[x + (long)n[0] * (y + (long)n[1] * z)] = (float)(rx * ry * rz);
 			}
 

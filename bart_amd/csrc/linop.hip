@@ -500,7 +500,6 @@ static NufftDims parse_nufft_dims(int N, const long* ksp_dims, const long* cim_d
 static void check_conf(const nufft_conf_s& conf, bool has_basis)
 {
 	(void)has_basis;
-	BAM_CHECK(conf.decomp, "nufft: the non-decomposed oversampled grid (-1) is not supported");
 	BAM_CHECK(!conf.pcycle, "nufft: phase cycling is not supported");
 	BAM_CHECK(!conf.compress_psf, "nufft: compressed PSF storage is not supported");
 	BAM_CHECK(0 == conf.cfft, "nufft: cfft flags are not supported");
@@ -731,7 +730,8 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 			 const long* bas_dims = nullptr, const float2* basis = nullptr)
 {
 	if (nullptr != basis) {
-		BAM_CHECK(2.f == conf.os, "nufft: a subspace basis needs oversampling 2 (the chained NUFFT has none here)");
+		BAM_CHECK(2.f == conf.os && conf.decomp,
+			  "nufft: a subspace basis needs the decomposed NUFFT with oversampling 2");
 		BAM_CHECK(nullptr != bas_dims, "nufft: basis without dims");
 		return make_basis_op(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, conf, bas_dims, basis);
 	}
@@ -752,16 +752,19 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 
 	if (F > 1) {
 		BAM_CHECK(!shared, "nufft: batch dims with and without trajectories of their own cannot be mixed");
-		BAM_CHECK(2.f == conf.os, "nufft: per-frame trajectories need oversampling 2");
+		BAM_CHECK(2.f == conf.os && conf.decomp, "nufft: per-frame trajectories need the decomposed NUFFT with oversampling 2");
 		return make_frames_op(N, ksp_dims, cim_dims, traj_dims, traj, wgh_dims, weights, conf, F);
 	}
 
 	NufftDims nd = parse_nufft_dims(N, ksp_dims, cim_dims, traj_dims);
 
-	if (2.f != conf.os) {
+	if (2.f != conf.os || !conf.decomp) {
 
-		// the chained NUFFT (noncart/nufft.c:1184-1199): no Toeplitz mode,
-		// no memory modes
+		// the chained NUFFT (noncart/nufft.c:1184-1199), or the
+		// non-decomposed one (decomp = false, :851, 898-935: no Toeplitz
+		// mode); no memory modes
+		BAM_CHECK(2.f != conf.os || !conf.toeplitz,
+			  "nufft: the non-decomposed grid (-1) has no Toeplitz operator (nufft.c:851)");
 		std::vector<float> tr = traj_reals_of(traj, nd.M);
 		std::vector<float2> w = expand_weights(N, wgh_dims, weights, ksp_dims);
 		auto op = new Op();
@@ -781,7 +784,7 @@ static Op* make_nufft_op(int N, const long* ksp_dims, const long* cim_dims, cons
 			op->chain = std::make_shared<Chain>();
 			op->chain->stream = op->stream;
 			op->chain->create(nd.img, nd.C * nd.B, nd.M, tr.data(), w.empty() ? nullptr : w.data(),
-					  conf.periodic, conf.width, conf.os);
+					  conf.periodic, conf.width, conf.os, 2.f != conf.os);
 		} catch (...) {
 			delete op;
 			throw;

@@ -291,8 +291,11 @@ struct Chain {
 	DevBuf<float2> kwork;   // [C][M]
 	hipStream_t stream = 0;
 
+	// pad = false: the non-decomposed NUFFT of decomp = false
+	// (noncart/nufft.c:898-935): the image is the oversampled grid itself,
+	// rolloff with os = 1 over it, positions with os
 	void create(const int dims[3], int C, long M, const float* traj, const float2* weights_host, bool periodic,
-		    float width, float os);
+		    float width, float os, bool pad = true);
 	void forward(float2* ksp, const float2* cim);
 	void adjoint(float2* cim, const float2* ksp);
 };

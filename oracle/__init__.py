@@ -67,7 +67,7 @@ def lib():
         L.orc_nufft_normal_basis.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.orc_chain_create.restype = vp
         L.orc_chain_create.argtypes = [c_long3, ctypes.c_long, vp, vp, ctypes.c_int, ctypes.c_bool, ctypes.c_float,
-                                       ctypes.c_float]
+                                       ctypes.c_float, ctypes.c_bool]
         L.orc_chain_free.argtypes = [vp]
         L.orc_chain_forward.argtypes = [vp, vp, vp]
         L.orc_chain_adjoint.argtypes = [vp, vp, vp]
@@ -490,16 +490,19 @@ class ChainNufft:
     """BART's chained NUFFT for oversampling factors other than 2
     (noncart/nufft_chain.c:192-244): rolloff into the centre of the
     round(os n) grid, centred unitary FFT, KB interpolation with beta =
-    calc_beta(os, width), weights times os^(d/2).  Arrays as :class:`Nufft`."""
+    calc_beta(os, width), weights times os^(d/2).  Arrays as :class:`Nufft`.
+    pad=False: the non-decomposed NUFFT (decomp = false, nufft -1,
+    noncart/nufft.c:898-935): the image is the oversampled grid itself, the
+    rolloff is taken with os = 1 over it."""
 
-    def __init__(self, dims, traj, ncoils, weights=None, periodic=False, width=6.0, os_=1.5):
+    def __init__(self, dims, traj, ncoils, weights=None, periodic=False, width=6.0, os_=1.5, pad=True):
         self.dims = tuple(list(dims) + [1] * (3 - len(dims)))
         self.traj = _c64(traj).reshape(-1, 3)
         self.M = self.traj.shape[0]
         self.C = int(ncoils)
         self.weights = _c64(weights).reshape(-1) if weights is not None else None
         self.h = lib().orc_chain_create(_dims3(self.dims), self.M, _ptr(self.traj), _ptr(self.weights), self.C,
-                                        periodic, width, os_)
+                                        periodic, width, os_, pad)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -1505,8 +1505,13 @@ struct orc_chain {
 
 typedef struct orc_chain orc_chain_t;
 
+/* pad = false: the non-decomposed NUFFT of nufft_create2 with decomp =
+ * false (noncart/nufft.c:898-935, 1432-1441): the image is the
+ * oversampled grid itself (no padding), the rolloff is taken with os = 1
+ * over it, the positions with os, and the scale 2^(d/2) folded into the
+ * linear phase is the os^(d/2) of the chain. */
 orc_chain_t* orc_chain_create(const long dims[3], long M, const cf* traj, const cf* weights, int C, bool periodic,
-			      float width, float os)
+			      float width, float os, bool pad)
 {
 	orc_chain_t* ch = calloc(1, sizeof(orc_chain_t));
 	int d = 0;
@@ -1516,7 +1521,7 @@ orc_chain_t* orc_chain_create(const long dims[3], long M, const cf* traj, const 
 	for (int i = 0; i < 3; i++) {
 
 		ch->dims[i] = dims[i];
-		ch->odims[i] = (dims[i] > 1) ? lround(os * dims[i]) : 1;
+		ch->odims[i] = (dims[i] > 1 && pad) ? lround(os * dims[i]) : dims[i];
 		ch->off[i] = labs(ch->odims[i] / 2 - dims[i] / 2);
 		ch->N *= dims[i];
 		ch->N2 *= ch->odims[i];
@@ -1533,7 +1538,7 @@ orc_chain_t* orc_chain_create(const long dims[3], long M, const cf* traj, const 
 	orc_kb_table(beta, ch->table);
 
 	ch->roll = malloc(sizeof(cf) * (size_t)ch->N);
-	orc_rolloff(os, width, beta, dims, ch->roll);
+	orc_rolloff(pad ? os : 1.f, width, beta, dims, ch->roll);
 
 	ch->fma = malloc(sizeof(cf) * (size_t)ch->N2);
 	orc_fftmod_table(ch->odims, false, ch->fma);

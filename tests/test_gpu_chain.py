@@ -132,3 +132,52 @@ def test_chain_sense_cgsense():
     print(f"chained SENSE normal {e_n:.1e}; CG-SENSE {st.iterations} iterations, vs dense solve {e_cg:.1e}")
     assert st.status == 2 and e_cg < 1e-4
     op.free()
+
+
+def test_non_decomposed_grid_vs_oracle():
+    # decomp = false (nufft -1): the image is the oversampled grid itself
+    import bart_amd as ba
+    n, C = 64, 2
+    traj = I.radial_traj(64, 31)
+    M = traj.shape[0]
+    w = (0.5 + np.random.default_rng(3).random(M)).astype(np.complex64)
+    conf = ba.NufftConf.defaults(decomp=False, toeplitz=False)
+    op = ba.nufft_create2(d16(1, M, 1, C), d16(n, n, 1, C), d16(3, M, 1), np.ascontiguousarray(traj),
+                          wgh_dims=d16(1, M, 1), weights=w, conf=conf)
+    ref = O.ChainNufft((n, n, 1), traj, C, weights=w, width=6.0, os_=2.0, pad=False)
+    x = crandn((C, 1, n, n), 12)
+    y = crandn((C, M), 13)
+    e_f = relerr(op.forward(dev(x)).cpu().numpy(), ref.forward(x))
+    e_a = relerr(op.adjoint(dev(y)).cpu().numpy(), ref.adjoint(y))
+    print(f"non-decomposed: forward {e_f:.1e} adjoint {e_a:.1e}")
+    assert e_f < 1e-5 and e_a < 1e-5
+    op.free()
+    with pytest.raises(ba.BartError, match="Toeplitz"):
+        ba.nufft_create2(d16(1, M, 1, C), d16(n, n, 1, C), d16(3, M, 1), np.ascontiguousarray(traj),
+                         conf=ba.NufftConf.defaults(decomp=False))
+
+
+def test_cli_nufft_over(tmp_path):
+    # tests/nufft.mk:336-347 (test-nufft-over): nufft -1 of the zero-padded
+    # image equals nufft of the image; nufft -a -1 cropped equals nufft -a
+    run_bart(["traj", "-r", "-x128", "-y128", "t"], tmp_path)
+    n = 128
+    x = I.ellipse_phantom(n, seed=14).reshape(n, n)
+    x2 = np.zeros((2 * n, 2 * n), np.complex64)
+    x2[n // 2:n // 2 + n, n // 2:n // 2 + n] = x
+    write_raw(str(tmp_path / "x"), dims16(n, n, 1), x)
+    write_raw(str(tmp_path / "x2"), dims16(2 * n, 2 * n, 1), x2)
+    run_bart(["nufft", "-r", "t", "x", "k1"], tmp_path)
+    run_bart(["nufft", "-1", "-r", "t", "x2", "k2"], tmp_path)
+    _, k1 = read_raw(str(tmp_path / "k1"))
+    _, k2 = read_raw(str(tmp_path / "k2"))
+    e_f = relerr(k2, k1)
+    run_bart(["nufft", "-a", "t", "k1", "r1"], tmp_path)
+    run_bart(["nufft", "-a", "-1", "-r", "t", "k1", "r2a"], tmp_path)
+    d1, r1 = read_raw(str(tmp_path / "r1"))
+    d2, r2a = read_raw(str(tmp_path / "r2a"))
+    assert d1 == dims16(n, n, 1) and d2 == dims16(2 * n, 2 * n, 1)
+    r2 = r2a.reshape(2 * n, 2 * n)[n // 2:n // 2 + n, n // 2:n // 2 + n]
+    e_a = relerr(r2, r1)
+    print(f"nufft -1 of the padded image vs nufft: forward {e_f:.1e}, adjoint cropped {e_a:.1e}")
+    assert e_f < 1e-6 and e_a < 1e-6

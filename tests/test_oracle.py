@@ -486,3 +486,19 @@ def test_chain_vs_nudft_and_dot(os_, width, dims):
     lhs = np.vdot(aw.forward(x).astype(np.complex128), y)
     rhs = np.vdot(x.astype(np.complex128), aw.adjoint(y))
     assert abs(lhs - rhs) < 1e-5 * abs(lhs)
+
+
+def test_non_decomposed_grid_of_padded_image_is_the_nufft():
+    # tests/nufft.mk:336-347 (test-nufft-over): nufft -1 (decomp = false,
+    # the image is the oversampled grid) of the zero-padded image equals the
+    # NUFFT of the image, and its adjoint cropped equals the adjoint (1e-6)
+    n = 64
+    traj = I.radial_traj(2 * n, 2 * n)
+    x = I.ellipse_phantom(n, seed=1).reshape(1, 1, n, n)
+    x2 = np.zeros((1, 1, 2 * n, 2 * n), np.complex64)
+    x2[:, :, n // 2:n // 2 + n, n // 2:n // 2 + n] = x
+    a = O.Nufft((n, n, 1), traj, 1, toeplitz=False)
+    b = O.ChainNufft((2 * n, 2 * n, 1), traj, 1, width=6.0, os_=2.0, pad=False)
+    ka = a.forward(x)
+    assert relerr(b.forward(x2), ka) < 1e-6
+    assert relerr(b.adjoint(ka)[:, :, n // 2:n // 2 + n, n // 2:n // 2 + n], a.adjoint(ka)) < 1e-6
