@@ -202,6 +202,10 @@ def lib():
     L.bart_amd_warmnet_train_step.argtypes = [vp, vp, vp, ci, ci, ci, ctypes.c_float]
     L.bart_amd_warmnet_train_step_grad.restype = ci
     L.bart_amd_warmnet_train_step_grad.argtypes = [vp, vp, vp, ci, ci, ci, ctypes.c_float]
+    L.extract_calib.restype = vp
+    L.extract_calib.argtypes = [vp, vp, vp, vp, ctypes.c_bool]
+    L.calib.restype = None
+    L.calib.argtypes = [vp, vp, vp, vp, ci, vp, vp, vp]
     L.bart_amd_set_error_mode.argtypes = [ctypes.c_int]
     L.bart_amd_last_error.restype = ctypes.c_char_p
     L.bart_amd_clear_error.restype = None
@@ -512,6 +516,77 @@ class Precond:
         if getattr(self, "h", None):
             lib().operator_free(self.h)
             self.h = None
+
+
+class EcalibConf(ctypes.Structure):
+    """struct ecalib_conf (calib/calib.h:12-34)."""
+    _fields_ = [
+        ("kdims", ctypes.c_long * 3),
+        ("threshold", ctypes.c_float),
+        ("numsv", ctypes.c_int),
+        ("percentsv", ctypes.c_float),
+        ("weighting", ctypes.c_bool),
+        ("softcrop", ctypes.c_bool),
+        ("crop", ctypes.c_float),
+        ("orthiter", ctypes.c_bool),
+        ("num_orthiter", ctypes.c_int),
+        ("usegpu", ctypes.c_bool),
+        ("perturb", ctypes.c_float),
+        ("intensity", ctypes.c_bool),
+        ("rotphase", ctypes.c_bool),
+        ("var", ctypes.c_float),
+        ("automate", ctypes.c_bool),
+        ("phase_normalize", ctypes.c_bool),
+        ("econdim", ctypes.c_int),
+        ("nystroem", ctypes.c_bool),
+        ("nystroem_os", ctypes.c_long),
+        ("nystroem_K", ctypes.c_long),
+    ]
+
+    @classmethod
+    def defaults(cls, **kw):
+        """ecalib_defaults (calib/calib.c:686-708) with overrides."""
+        c = cls()
+        ctypes.memmove(ctypes.byref(c), ctypes.byref(cls.in_dll(lib(), "ecalib_defaults")), ctypes.sizeof(cls))
+        for k, v in kw.items():
+            if "kdims" == k:
+                c.kdims = (ctypes.c_long * 3)(*v)
+            else:
+                setattr(c, k, v)
+        return c
+
+
+def ecalib(ksp, maps=2, calsize=(24, 24, 24), conf=None):
+    """ESPIRiT (src/ecalib.c:47-263): Cartesian k-space [C, Z, Y, X] ->
+    (maps [M, C, Z, Y, X], eigenvalue maps [M, Z, Y, X], singular values),
+    via extract_calib and calib of the C ABI."""
+    conf = EcalibConf.defaults() if conf is None else conf
+    ksp = np.ascontiguousarray(ksp, np.complex64)
+    C, Z, Y, X = ksp.shape
+    kd = _dims([X, Y, Z, C])
+    cd = (ctypes.c_long * DIMS)()
+    cs = (ctypes.c_long * 3)(*calsize)
+    p = lib().extract_calib(cd, cs, kd, _ptr(ksp), False)
+    _check()
+    if not p:
+        raise BartError("extract_calib failed")
+    try:
+        for i in range(3):
+            if cd[i] == 1 and [X, Y, Z][i] != 1:
+                raise BartError("Calibration region not found!")
+        kdims = [conf.kdims[i] if [X, Y, Z][i] > 1 else 1 for i in range(3)]
+        conf.kdims = (ctypes.c_long * 3)(*kdims)
+        od = [X if kdims[0] > 1 else 1, Y if kdims[1] > 1 else 1, Z if kdims[2] > 1 else 1, C, maps]
+        out = np.zeros([maps, C] + od[2::-1], np.complex64)
+        ev = np.zeros([maps] + od[2::-1], np.complex64)
+        N = int(np.prod(kdims)) * C
+        sv = np.zeros(N, np.float32)
+        lib().calib(ctypes.byref(conf), _dims(od), _ptr(out), _ptr(ev), N,
+                    sv.ctypes.data_as(ctypes.c_void_p), cd, ctypes.c_void_p(p))
+        _check()
+    finally:
+        ctypes.CDLL(None).free(ctypes.c_void_p(p))
+    return out, ev.real.copy(), sv
 
 
 def lsqr_conjgrad_precond(op, pre, b, maxiter, lam=0.0, tol=0.0):

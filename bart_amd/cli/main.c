@@ -17,6 +17,7 @@ struct tool {
 };
 
 static const struct tool tools[] = {
+	{ "ecalib", main_ecalib },
 	{ "nufft", main_nufft },
 	{ "pics", main_pics },
 	{ "traj", main_traj },

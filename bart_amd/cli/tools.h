@@ -12,6 +12,7 @@
 /* dims of BART's 16-dim layout (misc/mri.h:9-27) */
 enum { READ_DIM, PHS1_DIM, PHS2_DIM, COIL_DIM, MAPS_DIM, TE_DIM, COEFF_DIM };
 
+int main_ecalib(int argc, char* argv[]);
 int main_nufft(int argc, char* argv[]);
 int main_pics(int argc, char* argv[]);
 int main_traj(int argc, char* argv[]);

@@ -5,6 +5,7 @@
 
 #include "cg.hpp"
 #include "cnn.hpp"
+#include "espirit.hpp"
 #include "nudft.hpp"
 #include "nufft_impl.hpp"
 
@@ -2271,6 +2272,103 @@ int bart_amd_precond_apply(const struct operator_s* _pre, float _Complex* dst, c
 		BAM_HIP(hipStreamSynchronize(op->stream));
 		return 0;
 	}, -1);
+}
+
+// ---------------------------------------------------------------------------
+// ESPIRiT (calib/calib.c:686-807, misc/mri2.c:171-224)
+// ---------------------------------------------------------------------------
+
+const struct ecalib_conf ecalib_defaults = {
+	.kdims = { 6, 6, 6 },
+	.threshold = 0.001f,
+	.numsv = -1,
+	.percentsv = -1.f,
+	.weighting = false,
+	.softcrop = false,
+	.crop = 0.8f,
+	.orthiter = true,
+	.num_orthiter = 30,
+	.usegpu = false,
+	.perturb = -1.f,
+	.intensity = false,
+	.rotphase = true,
+	.var = -1.f,
+	.automate = false,
+	.phase_normalize = false,
+	.econdim = -2,
+	.nystroem = false,
+	.nystroem_os = 20,
+	.nystroem_K = 40,
+};
+
+float _Complex* extract_calib(long caldims[16], const long calsize[3], const long in_dims[16],
+			      const float _Complex* in_data, bool fixed)
+{
+	return guarded([&]() -> float _Complex* {
+		BAM_CHECK(nullptr != in_data, "extract_calib: NULL data");
+		BAM_CHECK(1 == in_dims[4], "MAPS dimension is not of size one.");
+		for (int i = 5; i < 16; i++)
+			BAM_CHECK(1 == in_dims[i], "extract_calib: dims >= 5 must be 1");
+		BAM_CHECK(!fixed, "extract_calib: the fixed-size region is not supported");
+		const int C = (int)in_dims[3];
+		const long n = md_size(4, in_dims);
+		std::vector<float2> h = to_host((const float2*)in_data, n);
+		long cd[3];
+		std::vector<float2> cal = extract_calib_host(calsize, in_dims, C, h.data(), cd);
+		for (int i = 0; i < 16; i++)
+			caldims[i] = (i < 3) ? cd[i] : ((3 == i) ? C : 1);
+		float2* out = (float2*)malloc(sizeof(float2) * cal.size());
+		BAM_CHECK(nullptr != out, "extract_calib: out of host memory");
+		memcpy(out, cal.data(), sizeof(float2) * cal.size());
+		return (float _Complex*)out;
+	}, nullptr);
+}
+
+void calib(const struct ecalib_conf* conf, const long out_dims[16], float _Complex* out_data, float _Complex* eptr,
+	   int SN, float svals[], const long calreg_dims[16], const float _Complex* data)
+{
+	guarded_void([&]() {
+		BAM_CHECK(nullptr != conf && nullptr != out_data && nullptr != data, "calib: NULL argument");
+		BAM_CHECK(!conf->weighting && !conf->automate && -1 == conf->numsv && -1.f == conf->percentsv,
+			  "calib: soft weighting and automatic or count thresholds are not supported");
+		BAM_CHECK(!conf->intensity && !conf->phase_normalize && !conf->nystroem && !conf->softcrop,
+			  "calib: intensity and phase normalisation, Nystroem and soft crop are not supported");
+		BAM_CHECK(conf->orthiter && conf->perturb <= 0.f, "calib: only the orthogonal iteration is supported");
+		BAM_CHECK(conf->crop >= 0.f, "calib: the automatic crop threshold is not supported");
+		for (int i = 5; i < 16; i++)
+			BAM_CHECK(1 == out_dims[i] && 1 == calreg_dims[i], "calib: dims >= 5 must be 1");
+		BAM_CHECK(out_dims[3] == calreg_dims[3], "calib: coil dims differ");
+		const int C = (int)out_dims[3];
+		const int M = (int)out_dims[4];
+		const long N = conf->kdims[0] * conf->kdims[1] * conf->kdims[2] * C;
+		BAM_CHECK(nullptr == svals || SN >= (int)std::min<long>(N, 1 << 30), "calib: svals too short");
+
+		EspiritConf ec;
+		for (int d = 0; d < 3; d++)
+			ec.kdims[d] = (1 == out_dims[d]) ? 1 : conf->kdims[d];
+		ec.threshold = conf->threshold;
+		ec.crop = conf->crop;
+		ec.softcrop = conf->softcrop;
+		ec.num_orthiter = conf->num_orthiter;
+		ec.rotphase = conf->rotphase;
+
+		const long img[3] = { out_dims[0], out_dims[1], out_dims[2] };
+		const long cd[3] = { calreg_dims[0], calreg_dims[1], calreg_dims[2] };
+		std::vector<float2> cal = to_host((const float2*)data, md_size(4, calreg_dims));
+
+		const long np = img[0] * img[1] * img[2];
+		hipStream_t st = default_stream();
+		DevBuf<float2> dm((size_t)M * C * np), de((size_t)M * np);
+		std::vector<float> sv(N);
+		espirit_calib(ec, img, C, M, cd, cal.data(), dm.ptr, de.ptr, sv.data(), st);
+
+		BAM_HIP(hipMemcpy(out_data, dm.ptr, sizeof(float2) * M * C * np, hipMemcpyDefault));
+		if (nullptr != eptr)
+			BAM_HIP(hipMemcpy(eptr, de.ptr, sizeof(float2) * M * np, hipMemcpyDefault));
+		if (nullptr != svals)
+			for (long i = 0; i < N && i < SN; i++)
+				svals[i] = sv[i];
+	});
 }
 
 }  // extern "C"

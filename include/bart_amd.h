@@ -342,6 +342,55 @@ int bart_amd_profile_report(char* buf, int len);
 /* Library version string. */
 const char* bart_amd_version(void);
 
+/* calib/calib.h:12-34 -- struct ecalib_conf, field for field */
+struct ecalib_conf {
+	long kdims[3];
+	float threshold;
+	int numsv;
+	float percentsv;
+	bool weighting;
+	bool softcrop;
+	float crop;
+	bool orthiter;
+	int num_orthiter;
+	bool usegpu;
+	float perturb;
+	bool intensity;
+	bool rotphase;
+	float var;
+	bool automate;
+	bool phase_normalize;
+	int econdim;
+	bool nystroem;
+	long nystroem_os;
+	long nystroem_K;
+};
+
+/* calib/calib.h:36 (values of calib/calib.c:686-708) */
+extern const struct ecalib_conf ecalib_defaults;
+
+/* misc/mri2.h -- replaces extract_calib (misc/mri2.c:171-224): the
+ * calibration region of Cartesian k-space [X Y Z C] (host or device): the
+ * centred box that grows while fully sampled up to calsize, then moved along
+ * the readout to maximal energy.  Returns a malloc'ed host array of
+ * caldims; fixed = true is not supported. */
+BAM_COMPLEX* extract_calib(long caldims[16], const long calsize[3], const long in_dims[16],
+			   const BAM_COMPLEX* in_data, bool fixed);
+
+/* calib/calib.h:38-39 -- replaces calib (calib/calib.c:801-807, calib2
+ * :715-798): ESPIRiT maps out [X Y Z C M] and eigenvalue maps eptr
+ * [X Y Z 1 M] (may be NULL) from the calibration region [cx cy cz C]; SN
+ * singular values into svals (may be NULL).  Pointers host or device.  The
+ * Gram matrix, its eigenvectors (rocSOLVER), the image-space covariance, its
+ * sinc interpolation and the point-wise orthogonal iteration run on the GPU.
+ * Supported: threshold and crop (hard), orthiter, rotphase; weighting,
+ * automatic thresholds, intensity / phase normalisation, Nystroem and soft
+ * crop are rejected.  The phase reference (the channel principal component)
+ * has its first entry made real, so maps agree with BART's up to one global
+ * phase per map. */
+void calib(const struct ecalib_conf* conf, const long out_dims[16], BAM_COMPLEX* out_data, BAM_COMPLEX* eptr,
+	   int SN, float svals[], const long calreg_dims[16], const BAM_COMPLEX* data);
+
 #ifdef __cplusplus
 }
 #endif
