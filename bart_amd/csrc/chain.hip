@@ -12,7 +12,8 @@
 //   adjoint: the adjoint of every step, in reverse.
 // The interpolation is a gather per sample; its adjoint is a gather per
 // grid point over the samples of the nearby cells (samples sorted by cells
-// of CHAIN_CS points), so no atomics are used and the result is
+// of CHAIN_CS points), a tile of points per work-group walking the tile's
+// samples in LDS batches, so no atomics are used and the result is
 // deterministic.
 #include "nufft_impl.hpp"
 #include "prof.hpp"
@@ -30,6 +31,64 @@ struct ChainGeom {
 	float os, width;
 	bool periodic;
 };
+
+// tiles of the adjoint gather: extent t per dim (product <= 256) and
+// number of tiles nt per dim
+struct ChainTiles {
+	int t[3], nt[3];
+};
+
+static ChainTiles tiles(const Chain& c)
+{
+	int act = 0;
+	for (int d = 0; d < 3; d++)
+		act += (c.on[d] > 1) ? 1 : 0;
+
+	ChainTiles tl;
+	int k = 0;
+
+	for (int d = 0; d < 3; d++) {
+		tl.t[d] = 1;
+		if (c.on[d] > 1) {
+			tl.t[d] = (3 == act) ? ((2 == k) ? 4 : 8) : ((2 == act) ? 16 : 256);
+			k++;
+		}
+		tl.nt[d] = (c.on[d] + tl.t[d] - 1) / tl.t[d];
+	}
+
+	return tl;
+}
+
+// cells whose samples can reach some grid index of [a, b] along dim d: the
+// cells within width / 2 of the range plus one on each side (rounding of
+// the positions), wrapped (periodic) or clamped, every cell at most once;
+// returns the count, fills the first cell
+static int tile_span(const Chain& c, int d, int a, int b, int& first)
+{
+	if (1 == c.on[d]) {
+		first = 0;
+		return 1;
+	}
+
+	const int nc = c.ncell[d];
+	int lo = (int)floorf(((float)a - 0.5f * c.width) / (float)c.cs[d]) - 1;
+	int hi = (int)floorf(((float)b + 0.5f * c.width) / (float)c.cs[d]) + 1;
+
+	if (hi - lo + 1 >= nc) {
+		first = 0;
+		return nc;
+	}
+
+	if (!c.periodic) {
+		lo = std::max(lo, 0);
+		hi = std::min(hi, nc - 1);
+		first = lo;
+		return std::max(hi - lo + 1, 0);
+	}
+
+	first = ((lo % nc) + nc) % nc;
+	return hi - lo + 1;
+}
 
 static ChainGeom geom(const Chain& c)
 {
@@ -58,7 +117,7 @@ static unsigned nblocks(long n) { return (unsigned)std::min<long>(std::max<long>
 // sample position on the grid of dim d: os k + on / 2 (noncart/grid.c:181-187)
 __device__ inline float chain_pos(const ChainGeom& g, int d, float k)
 {
-	return (g.on[d] > 1) ? __fadd_rn(__fmul_rn(g.os, k), (float)(g.on[d] / 2)) : __fmul_rn(g.os, k);
+	return grid_pos(g.os, k, 0.f, (g.on[d] > 1) ? (float)(g.on[d] / 2) : 0.f);
 }
 
 // first tap, tap count and the index offset of one dim (noncart/grid.c:415-441):
@@ -222,150 +281,221 @@ __global__ void k_chain_gridH(ChainGeom g, const float* __restrict__ kbt, const 
 	}
 }
 
-// kw[c][m] = ksp[c][m] conj(weights x scale) (or x scale): the adjoint of
-// the last diagonal
-__global__ void k_chain_kscale(ChainGeom g, const float2* __restrict__ tmp, float scale,
-			       const float2* __restrict__ ksp, float2* __restrict__ kw)
+constexpr int CHAIN_CC = 8;   // coils per work-group of the adjoint gather
+constexpr int CHAIN_TB = 128; // samples per batch of the adjoint gather
+constexpr int CHAIN_SMAX = 1024;  // samples per work-group of the adjoint gather
+
+// kws[(ch M + s) CHAIN_CC + q] = ksp[c][m(s)] conj(weights x scale) (or
+// x scale), c = ch CHAIN_CC + q, m(s) the original index of sorted sample s
+// (0 past the last coil): the adjoint of the last diagonal, written in cell
+// order with the coils of a chunk side by side
+__global__ void k_chain_kscale(ChainGeom g, const int* __restrict__ sidx, const float2* __restrict__ tmp, float scale,
+			       const float2* __restrict__ ksp, float2* __restrict__ kws)
 {
-	const long total = g.M * g.C;
+	const int nch = (g.C + CHAIN_CC - 1) / CHAIN_CC;
+	const long total = g.M * nch;
 
-	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
-		kw[e] = (nullptr != tmp) ? cmulc(ksp[e], tmp[e % g.M]) : cscale(ksp[e], scale);
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+		const int c0 = (int)(e / g.M) * CHAIN_CC;
+		const long m = sidx[e % g.M];
+
+		for (int q = 0; q < CHAIN_CC; q++) {
+			float2 v = make_float2(0.f, 0.f);
+			if (c0 + q < g.C) {
+				const float2 x = ksp[(long)(c0 + q) * g.M + m];
+				v = (nullptr != tmp) ? cmulc(x, tmp[m]) : cscale(x, scale);
+			}
+			kws[e * CHAIN_CC + q] = v;
+		}
+	}
 }
-
-// weight of grid index o along dim d for a sample at p (0 if o is not
-// one of its taps)
-__device__ inline float tap_at(const ChainGeom& g, const float* kb, int d, float p, int o, bool& hit)
-{
-	if (1 == g.on[d]) {
-		hit = true;
-		return tap_weight(g, kb, d, 0, p);
-	}
-
-	const TapRange r = tap_range(g, d, p);
-	hit = false;
-
-	if (r.cnt <= 0)
-		return 0.f;
-
-	int j;
-
-	if (g.periodic) {
-		const int base = (r.sti + r.off) % g.on[d];
-		const int delta = ((o - base) % g.on[d] + g.on[d]) % g.on[d];
-		if (delta >= r.cnt)
-			return 0.f;
-		j = r.sti + delta;
-	} else {
-		if (o < r.sti || o >= r.sti + r.cnt)
-			return 0.f;
-		j = o;
-	}
-
-	hit = true;
-	return tap_weight(g, kb, d, j, p);
-}
-
-// cell span of grid index o along dim d: the cells whose samples can reach
-// o, wrapped (periodic) or clamped; returns the count, fills the first cell
-__device__ inline int cell_span(const ChainGeom& g, int d, int o, int& first)
-{
-	if (1 == g.on[d]) {
-		first = 0;
-		return 1;
-	}
-
-	const int nc = g.ncell[d];
-	int lo = (int)floorf(((float)o - 0.5f * g.width) / (float)g.cs[d]) - 1;
-	int hi = (int)floorf(((float)o + 0.5f * g.width) / (float)g.cs[d]) + 1;
-
-	if (hi - lo + 1 >= nc) {
-		first = 0;
-		return nc;
-	}
-
-	if (!g.periodic) {
-		lo = max(lo, 0);
-		hi = min(hi, nc - 1);
-		first = lo;
-		return max(hi - lo + 1, 0);
-	}
-
-	first = ((lo % nc) + nc) % nc;
-	return hi - lo + 1;
-}
-
-constexpr int CHAIN_CC = 8;  // coils per thread of the adjoint gather
 
 // G[c][o] = sum over the samples of the nearby cells of w(o) kw[c][m]
-// (the adjoint of k_chain_gridH: grid2, noncart/grid.c:202-253)
-__global__ void k_chain_grid(ChainGeom g, const float* __restrict__ kbt, const float4* __restrict__ spos,
-			     const int* __restrict__ sidx, const int* __restrict__ cstart,
-			     const float2* __restrict__ kw, float2* __restrict__ G)
+// (the adjoint of k_chain_gridH: grid2, noncart/grid.c:202-253).
+// A work-group owns a tile of grid points (one per thread), a chunk of
+// CHAIN_CC coils and a slice [k CHAIN_SMAX, (k + 1) CHAIN_SMAX) of the
+// samples of the cells that can reach the tile (taken in cell order: z, y,
+// x, then sample order).  It walks its samples in batches: one thread per
+// sample evaluates its taps (first grid index, count and KB weights per dim)
+// into LDS and the coil values come in as one contiguous run; then every
+// point adds the samples whose taps cover it, in order.  A tile with one
+// slice writes the grid; the slices of a denser tile write partial tiles
+// that k_chain_sum adds in slice order, so the result is deterministic and
+// needs no atomics.
+__global__ __launch_bounds__(256) void k_chain_grid(ChainGeom g, ChainTiles tl, const float* __restrict__ kbt,
+						    const int4* __restrict__ work, const int* __restrict__ span,
+						    const float4* __restrict__ spos, const int* __restrict__ cstart,
+						    const float2* __restrict__ kws, float2* __restrict__ G,
+						    float2* __restrict__ part, long nslots)
 {
 	__shared__ float kb[KB_SIZE + 1];
+	__shared__ int sbase[CHAIN_TB][3];
+	__shared__ int scnt[CHAIN_TB][3];
+	__shared__ float sw[CHAIN_TB][3][CHAIN_TAPS];
+	__shared__ float2 sv[CHAIN_TB * CHAIN_CC];
 
 	for (int i = threadIdx.x; i < KB_SIZE + 1; i += blockDim.x)
 		kb[i] = kbt[i];
 
-	__syncthreads();
+	const int4 wi = work[blockIdx.x];  // tile, slice, slices of the tile, first partial slot
+	const int ch = blockIdx.y;
+	const int c0 = ch * CHAIN_CC;
+	const long b = wi.x;
+	const int tc[3] = { (int)(b % tl.nt[0]), (int)((b / tl.nt[0]) % tl.nt[1]), (int)(b / ((long)tl.nt[0] * tl.nt[1])) };
+	const int l = threadIdx.x;
+	const int lc[3] = { l % tl.t[0], (l / tl.t[0]) % tl.t[1], l / (tl.t[0] * tl.t[1]) };
+	int o[3], first[3], count[3];
+	bool inside = l < tl.t[0] * tl.t[1] * tl.t[2];
 
-	const int nch = (g.C + CHAIN_CC - 1) / CHAIN_CC;
-	const long total = g.N2 * nch;
+	for (int d = 0; d < 3; d++) {
+		o[d] = tc[d] * tl.t[d] + lc[d];
+		inside = inside && o[d] < g.on[d];
+		first[d] = span[6 * b + d];
+		count[d] = span[6 * b + 3 + d];
+	}
 
-	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+	// this work-group's slice of the tile's samples
+	const long v0 = (long)wi.y * CHAIN_SMAX;
+	const long v1 = v0 + CHAIN_SMAX;
+	long vo = 0;  // samples of the tile before the current run
 
-		const long o = e % g.N2;
-		const int c0 = (int)(e / g.N2) * CHAIN_CC;
-		const int oc[3] = { (int)(o % g.on[0]), (int)((o / g.on[0]) % g.on[1]), (int)(o / ((long)g.on[0] * g.on[1])) };
-		int f[3], nspan[3];
+	float2 acc[CHAIN_CC];
 
-		for (int d = 0; d < 3; d++)
-			nspan[d] = cell_span(g, d, oc[d], f[d]);
+	for (int q = 0; q < CHAIN_CC; q++)
+		acc[q] = make_float2(0.f, 0.f);
 
-		float2 acc[CHAIN_CC];
+	for (int cz = 0; cz < count[2]; cz++)
+		for (int cy = 0; cy < count[1]; cy++) {
 
-		for (int q = 0; q < CHAIN_CC; q++)
-			acc[q] = make_float2(0.f, 0.f);
+			const int iz = (first[2] + cz) % g.ncell[2];
+			const int iy = (first[1] + cy) % g.ncell[1];
+			const long row = (long)g.ncell[0] * (iy + (long)g.ncell[1] * iz);
+			// the x cells as up to two contiguous runs (wrapped)
+			const int n0 = min(count[0], g.ncell[0] - first[0]);
+			const int runs[2][2] = { { first[0], n0 }, { 0, count[0] - n0 } };
 
-		for (int cz = 0; cz < nspan[2]; cz++)
-			for (int cy = 0; cy < nspan[1]; cy++)
-				for (int cx = 0; cx < nspan[0]; cx++) {
+			for (int r = 0; r < 2; r++) {
 
-					const int ix = (f[0] + cx) % g.ncell[0];
-					const int iy = (f[1] + cy) % g.ncell[1];
-					const int iz = (f[2] + cz) % g.ncell[2];
-					const long cell = ix + (long)g.ncell[0] * (iy + (long)g.ncell[1] * iz);
+				if (runs[r][1] <= 0)
+					continue;
 
-					for (int s = cstart[cell]; s < cstart[cell + 1]; s++) {
+				const int r0 = cstart[row + runs[r][0]];
+				const int r1 = cstart[row + runs[r][0] + runs[r][1]];
+				// the part of [r0, r1) inside the slice
+				const int s0 = (int)(r0 + std::min<long>(std::max<long>(v0 - vo, 0), r1 - r0));
+				const int s1 = (int)(r0 + std::min<long>(std::max<long>(v1 - vo, 0), r1 - r0));
+				vo += r1 - r0;
 
-						const float4 p = spos[s];
-						bool hx, hy, hz;
-						const float wz = tap_at(g, kb, 2, p.z, oc[2], hz);
-						if (!hz)
+				for (int sb = s0; sb < s1; sb += CHAIN_TB) {
+
+					const int nb = min(CHAIN_TB, s1 - sb);
+
+					__syncthreads();  // the previous batch has been read
+
+					if (threadIdx.x < nb) {
+						const int j = threadIdx.x;
+						const float4 pp = spos[sb + j];
+						const float p[3] = { pp.x, pp.y, pp.z };
+
+						for (int d = 0; d < 3; d++) {
+							const TapRange tr = tap_range(g, d, p[d]);
+							const int cnt = max(tr.cnt, 0);
+							sbase[j][d] = (1 == g.on[d]) ? 0
+								     : (g.periodic ? (tr.sti + tr.off) % g.on[d] : tr.sti);
+							scnt[j][d] = cnt;
+							for (int t = 0; t < cnt; t++)
+								sw[j][d][t] = tap_weight(g, kb, d, tr.sti + t, p[d]);
+						}
+					}
+
+					const float2* src = kws + ((long)ch * g.M + sb) * CHAIN_CC;
+
+					for (int i = threadIdx.x; i < nb * CHAIN_CC; i += blockDim.x)
+						sv[i] = src[i];
+
+					__syncthreads();
+
+					if (!inside)
+						continue;
+
+					for (int j = 0; j < nb; j++) {
+
+						int dl[3];
+						bool hit = true;
+
+						for (int d = 2; d >= 0; d--) {
+							int e = o[d] - sbase[j][d];
+							if (g.periodic && e < 0)
+								e += g.on[d];
+							dl[d] = e;
+							hit = hit && (unsigned)e < (unsigned)scnt[j][d];
+						}
+
+						if (!hit)
 							continue;
-						const float wy = tap_at(g, kb, 1, p.y, oc[1], hy);
-						if (!hy)
-							continue;
-						const float wx = tap_at(g, kb, 0, p.x, oc[0], hx);
-						if (!hx)
-							continue;
 
-						const float w = ((1.f * wz) * wy) * wx;
-						const long m = sidx[s];
+						const float w = ((1.f * sw[j][2][dl[2]]) * sw[j][1][dl[1]]) * sw[j][0][dl[0]];
 
-						for (int q = 0; q < CHAIN_CC; q++)
-							if (c0 + q < g.C) {
-								const float2 v = kw[(long)(c0 + q) * g.M + m];
-								acc[q].x += v.x * w;
-								acc[q].y += v.y * w;
-							}
+						for (int q = 0; q < CHAIN_CC; q++) {
+							const float2 v = sv[j * CHAIN_CC + q];
+							acc[q].x += v.x * w;
+							acc[q].y += v.y * w;
+						}
 					}
 				}
+			}
+		}
 
+	if (1 == wi.z) {
+		if (inside) {
+			const long oi = o[0] + (long)g.on[0] * (o[1] + (long)g.on[1] * o[2]);
+			for (int q = 0; q < CHAIN_CC; q++)
+				if (c0 + q < g.C)
+					G[(long)(c0 + q) * g.N2 + oi] = acc[q];
+		}
+	} else {
+		float2* pp = part + ((long)ch * nslots + wi.w + wi.y) * (CHAIN_CC * 256);
 		for (int q = 0; q < CHAIN_CC; q++)
-			if (c0 + q < g.C)
-				G[(long)(c0 + q) * g.N2 + o] = acc[q];
+			pp[q * 256 + l] = acc[q];
+	}
+}
+
+// G[c][o] = sum over the slices k of the partial tiles, in order (tiles
+// whose samples were split over several work-groups)
+__global__ __launch_bounds__(256) void k_chain_sum(ChainGeom g, ChainTiles tl, const int4* __restrict__ split,
+						   const float2* __restrict__ part, long nslots, float2* __restrict__ G)
+{
+	const int4 si = split[blockIdx.x];  // tile, slices, first partial slot
+	const int ch = blockIdx.y;
+	const int c0 = ch * CHAIN_CC;
+	const long b = si.x;
+	const int tc[3] = { (int)(b % tl.nt[0]), (int)((b / tl.nt[0]) % tl.nt[1]), (int)(b / ((long)tl.nt[0] * tl.nt[1])) };
+	const int l = threadIdx.x;
+	const int lc[3] = { l % tl.t[0], (l / tl.t[0]) % tl.t[1], l / (tl.t[0] * tl.t[1]) };
+	int o[3];
+	bool inside = l < tl.t[0] * tl.t[1] * tl.t[2];
+
+	for (int d = 0; d < 3; d++) {
+		o[d] = tc[d] * tl.t[d] + lc[d];
+		inside = inside && o[d] < g.on[d];
+	}
+
+	if (!inside)
+		return;
+
+	const long oi = o[0] + (long)g.on[0] * (o[1] + (long)g.on[1] * o[2]);
+	const float2* pp = part + ((long)ch * nslots + si.z) * (CHAIN_CC * 256) + l;
+
+	for (int q = 0; q < CHAIN_CC; q++) {
+		if (c0 + q >= g.C)
+			break;
+		float2 acc = pp[q * 256];
+		for (int k = 1; k < si.y; k++) {
+			const float2 v = pp[(long)k * (CHAIN_CC * 256) + q * 256];
+			acc.x += v.x;
+			acc.y += v.y;
+		}
+		G[(long)(c0 + q) * g.N2 + oi] = acc;
 	}
 }
 
@@ -465,7 +595,7 @@ void Chain::create(const int dims[3], int C_, long M_, const float* traj, const 
 
 		for (int d = 0; d < 3; d++) {
 			const float k = traj[3 * m + d];
-			float v = os * k;
+			float v = (float)((double)os * (double)k);
 			if (on[d] > 1)
 				v = v + (float)(on[d] / 2);
 			p[d] = v;
@@ -512,7 +642,61 @@ void Chain::create(const int dims[3], int C_, long M_, const float* traj, const 
 	BAM_HIP(hipMemcpy(cstart.ptr, cnt.data(), sizeof(int) * (ncells + 1), hipMemcpyHostToDevice));
 
 	grid.alloc((size_t)C * N2);
-	kwork.alloc(std::max<long>((long)C * M, 1));
+	kwork.alloc(std::max<long>((long)((C + CHAIN_CC - 1) / CHAIN_CC) * CHAIN_CC * M, 1));
+
+	// adjoint work: per tile its cell span and sample count, split into
+	// slices of at most CHAIN_SMAX samples
+	const ChainTiles tl = tiles(*this);
+	const long ntiles = (long)tl.nt[0] * tl.nt[1] * tl.nt[2];
+	BAM_CHECK(ntiles < (1L << 30), "nufft: grid too large for the adjoint gather");
+	std::vector<int> sp6((size_t)ntiles * 6);
+	std::vector<int4> wl, sl;
+	long slots = 0;
+
+	for (long t = 0; t < ntiles; t++) {
+
+		const long tc[3] = { t % tl.nt[0], (t / tl.nt[0]) % tl.nt[1], t / ((long)tl.nt[0] * tl.nt[1]) };
+		int f[3], n[3];
+
+		for (int d = 0; d < 3; d++) {
+			const int a = (int)tc[d] * tl.t[d];
+			n[d] = tile_span(*this, d, a, std::min(a + tl.t[d], on[d]) - 1, f[d]);
+			sp6[6 * t + d] = f[d];
+			sp6[6 * t + 3 + d] = n[d];
+		}
+
+		long ns = 0;
+		for (int cz = 0; cz < n[2]; cz++)
+			for (int cy = 0; cy < n[1]; cy++) {
+				const long row = (long)ncell[0] * ((f[1] + cy) % ncell[1] + (long)ncell[1] * ((f[2] + cz) % ncell[2]));
+				const int n0 = std::min(n[0], ncell[0] - f[0]);
+				ns += cnt[row + f[0] + n0] - cnt[row + f[0]];
+				ns += cnt[row + (n[0] - n0)] - cnt[row];
+			}
+
+		const int K = (int)std::max<long>((ns + CHAIN_SMAX - 1) / CHAIN_SMAX, 1);
+
+		for (int k = 0; k < K; k++)
+			wl.push_back(make_int4((int)t, k, K, (K > 1) ? (int)slots : -1));
+
+		if (K > 1) {
+			sl.push_back(make_int4((int)t, K, (int)slots, 0));
+			slots += K;
+		}
+	}
+
+	BAM_CHECK(wl.size() < (1UL << 31) && slots < (1L << 30), "nufft: too many adjoint work items");
+	nwork = (long)wl.size();
+	nsplit = (long)sl.size();
+	nslots = slots;
+	cspan.alloc(sp6.size());
+	BAM_HIP(hipMemcpy(cspan.ptr, sp6.data(), sizeof(int) * sp6.size(), hipMemcpyHostToDevice));
+	cwork.alloc(wl.size());
+	BAM_HIP(hipMemcpy(cwork.ptr, wl.data(), sizeof(int4) * wl.size(), hipMemcpyHostToDevice));
+	csplit.alloc(std::max<size_t>(sl.size(), 1));
+	if (!sl.empty())
+		BAM_HIP(hipMemcpy(csplit.ptr, sl.data(), sizeof(int4) * sl.size(), hipMemcpyHostToDevice));
+	cpart.alloc((size_t)std::max<long>(slots, 1) * ((C + CHAIN_CC - 1) / CHAIN_CC) * CHAIN_CC * 256);
 }
 
 void Chain::forward(float2* ksp, const float2* cim)
@@ -538,16 +722,25 @@ void Chain::adjoint(float2* cim, const float2* ksp)
 	const ChainGeom g = geom(*this);
 	ProfScope prof("chain_adjoint", stream);
 
+	const long nch = (C + CHAIN_CC - 1) / CHAIN_CC;
+
 	if (M > 0) {
-		hipLaunchKernelGGL(k_chain_kscale, dim3(nblocks(M * C)), dim3(256), 0, stream, g,
+		hipLaunchKernelGGL(k_chain_kscale, dim3(nblocks(M * nch)), dim3(256), 0, stream, g, sidx.ptr,
 				   has_weights ? tmp.ptr : nullptr, scale, ksp, kwork.ptr);
 		BAM_HIP(hipGetLastError());
 	}
 
-	const long nch = (C + CHAIN_CC - 1) / CHAIN_CC;
-	hipLaunchKernelGGL(k_chain_grid, dim3(nblocks(N2 * nch)), dim3(256), 0, stream, g, kb.ptr, spos.ptr,
-			   sidx.ptr, cstart.ptr, kwork.ptr, grid.ptr);
+	const ChainTiles tl = tiles(*this);
+	BAM_CHECK(nch < 65536, "nufft: too many coils for the adjoint gather");
+	hipLaunchKernelGGL(k_chain_grid, dim3((unsigned)nwork, (unsigned)nch), dim3(256), 0, stream, g, tl, kb.ptr,
+			   cwork.ptr, cspan.ptr, spos.ptr, cstart.ptr, kwork.ptr, grid.ptr, cpart.ptr, nslots);
 	BAM_HIP(hipGetLastError());
+
+	if (nsplit > 0) {
+		hipLaunchKernelGGL(k_chain_sum, dim3((unsigned)nsplit, (unsigned)nch), dim3(256), 0, stream, g, tl,
+				   csplit.ptr, cpart.ptr, nslots, grid.ptr);
+		BAM_HIP(hipGetLastError());
+	}
 	hipLaunchKernelGGL(k_chain_mul, dim3(nblocks(N2 * C)), dim3(256), 0, stream, g, fma.ptr, grid.ptr, true);
 	BAM_HIP(hipGetLastError());
 	fft_nd(on, C, 7, +1.f, grid.ptr, grid.ptr, 1.f, stream);

@@ -87,16 +87,13 @@ constexpr int MAX_RANGES = 256;
 // visits (samples of a tile's cells and their ring) per adjoint work item
 constexpr int ADJ_MAXV = 2048;
 
-// sample position on the grid, pos = os (k + shift) + n / 2, rounded after
-// each operation as in noncart/grid.c:181-187
-#pragma clang fp contract(off)
+// sample position on the grid, pos = os (k + shift) + n / 2, rounded as
+// noncart/grid.c:181-187 rounds it (the device twin is grid_pos)
 static float host_pos(const GridGeom& g, int s, int d, float k)
 {
-	float a = k + g.shift[s][d];
-	float b = g.os * a;
+	const float b = (float)((double)g.os * ((double)k + (double)g.shift[s][d]));
 	return b + (float)(g.n[d] / 2);
 }
-#pragma clang fp contract(on)
 
 // Adjoint tiling.  A wave owns a block of every sub-grid: 2 x 2 x 2 points
 // in 3-d (a 4 x 4 x 4 block of the 2x oversampled grid), 4 x 4 in 2-d, 32 in
@@ -647,7 +644,7 @@ __device__ inline int build_ranges(const GridGeom& g, const int* __restrict__ cs
 
 __device__ inline float sample_pos_v(const GridGeom& g, int d, int v, float k)
 {
-	return __fadd_rn(__fmul_rn(g.os, __fadd_rn(k, g.shv[d][v])), (float)(g.n[d] / 2));
+	return grid_pos(g.os, k, g.shv[d][v], (float)(g.n[d] / 2));
 }
 
 template <int TP> struct Taps1 {
@@ -956,7 +953,7 @@ __global__ __launch_bounds__(1024) void k_grid_adj(GridGeom g, GridTiling t, con
 				const bool v1 = (1 == var) && (sel3(d, g.nshv[0], g.nshv[1], g.nshv[2]) > 1);
 				const float sh = v1 ? sel3(d, g.shv[0][1], g.shv[1][1], g.shv[2][1])
 						    : sel3(d, g.shv[0][0], g.shv[1][0], g.shv[2][0]);
-				const float pos = __fadd_rn(__fmul_rn(g.os, __fadd_rn(kd, sh)), (float)(n / 2));
+				const float pos = grid_pos(g.os, kd, sh, (float)(n / 2));
 				int sti = (int)ceil((double)pos - 0.5 * (double)g.width);
 				int eni = (int)floor((double)pos + 0.5 * (double)g.width);
 

@@ -16,15 +16,27 @@ namespace bam {
 constexpr int KB_SIZE = 1000;
 constexpr int MAX_TAPS = 8;  // taps per dimension (kernel width <= 7)
 
-// intlookup (noncart/grid.c:126-149): x in [0, 0.5]
+// sample position on a grid of n points, os (k + shift) + n / 2, rounded
+// as noncart/grid.c:181-187 rounds it: os (k + shift) in double, rounded to
+// float, then the float sum with n / 2
+__device__ inline float grid_pos(float os, float k, float shift, float half)
+{
+	const float u = (float)((double)os * ((double)k + (double)shift));
+	return u + half;
+}
+
+// intlookup (noncart/grid.c:126-149): x in [0, 0.5]; the interpolation
+// (lerp, :120-123) in double.  No fused multiply-add: the products and
+// sums are rounded one by one, as in the reference's build.
 __device__ inline float kb_lookup(const float* table, float x)
 {
+#pragma clang fp contract(off)
 	x *= 2.f;
-	float xs = x * (float)(KB_SIZE - 1);
+	const float xs = x * (float)(KB_SIZE - 1);
 	int index = (int)xs;
 	index = min(max(index, 0), KB_SIZE - 1);
-	float fpart = xs - (float)index;
-	return (1.f - fpart) * table[index] + fpart * table[index + 1];
+	const double f = (double)(xs - (float)index);
+	return (float)((1. - f) * (double)table[index] + f * (double)table[index + 1]);
 }
 
 // host tables of nufft.hip: fftmod phase and value (num/fft.c:64-90) and the
@@ -288,7 +300,12 @@ struct Chain {
 	DevBuf<int> sidx;       // sorted sample -> original index
 	DevBuf<int> cstart;     // [cells + 1]
 	DevBuf<float2> grid;    // [C][N2]
-	DevBuf<float2> kwork;   // [C][M]
+	DevBuf<float2> kwork;   // [C / 8][M][8] scaled k-space in cell order
+	DevBuf<int> cspan;      // adjoint: per tile its first cell and cell count per dim
+	DevBuf<int4> cwork;     // adjoint work items (tile, slice, slices, first partial slot)
+	DevBuf<int4> csplit;    // tiles split into slices (tile, slices, first partial slot)
+	DevBuf<float2> cpart;   // partial tiles of the split tiles
+	long nwork = 0, nsplit = 0, nslots = 0;
 	hipStream_t stream = 0;
 
 	// pad = false: the non-decomposed NUFFT of decomp = false

@@ -484,7 +484,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="kooshball", choices=["kooshball", "radial2d", "spiral", "batched", "warmstart"])
+    ap.add_argument("--config", default="kooshball", choices=["kooshball", "radial2d", "spiral", "batched", "warmstart", "widened"])
     ap.add_argument("--ws-train", type=int, default=1024, help="warmstart: training pairs")
     ap.add_argument("--ws-eval", type=int, default=32, help="warmstart: held-out problems")
     ap.add_argument("--ws-epochs", type=int, default=40, help="warmstart: training epochs")
@@ -528,6 +528,10 @@ def main():
 
     if args.config == "warmstart":
         run_warmstart(args)
+        return
+
+    if args.config == "widened":
+        run_widened(args, L, torch.device("cuda:0"))
         return
 
     t_gen = time.perf_counter()
@@ -755,6 +759,210 @@ def main():
     elif cpu_cg and Z == 1:
         out["cg_sense_gpu_over_cpu"] = round(1.0 / t_tol / cpu_cg["value"], 1)
     print(json.dumps(out), flush=True)
+
+
+def _median_ms(fn, reps):
+    """Median wall time of fn() in ms over reps calls after one untimed
+    call, each bracketed by device synchronisations."""
+    import torch
+    fn()
+    ts = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)) * 1e3
+
+
+def _cpu_ms(fn, budget_s=4.0, max_reps=50):
+    """Mean wall time of fn() in ms, repeated for about budget_s."""
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= budget_s or reps >= max_reps:
+            return dt / reps * 1e3, reps
+
+
+def _frac(nbytes, ms):
+    return round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+
+def run_widened(args, L, dev):
+    """The rows beyond the plain NUFFT / CG-SENSE path, each on a named
+    workload, timed on the GPU (median wall time with the device
+    synchronised, inputs resident) beside the oracle on the host cores:
+      * subspace basis: 2-D radial 256^2, 8 coils, 64 frames of 4 spokes x 512
+        samples, K = 4 coefficients: NUFFT adjoint + forward, the Toeplitz
+        normal operator with the K x K PSF, 30 CG-SENSE iterations;
+      * chained NUFFT (os 1.5, width 6) on the 2-D radial 256^2 / 8 coils /
+        201 x 512 workload: adjoint + forward;
+      * Strang preconditioner: the same workload's inverse NUFFT (nufft -i) to
+        relative residual 1e-4 with and without it;
+      * ESPIRiT: 2-D 256^2 / 8 coils and 3-D 128^3 / 16 coils, 2 maps, 24^3
+        calibration region, 6^3 kernel (BART's defaults), maps and
+        eigenvalue maps into device memory.
+    Each row carries its compulsory bytes (inputs read once, outputs written
+    once) and the fraction of the HBM peak that its time corresponds to.
+    Prints one JSON line."""
+    import ctypes
+    import torch
+    import bart_amd as ba
+    from bart_amd import inputs as I
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu = not args.no_cpu
+    if cpu:
+        import oracle as O
+        O.lib().orc_set_num_threads(threads)
+    rows = {}
+
+    def crandn(shape, seed):
+        r = np.random.default_rng(seed)
+        return (r.standard_normal(shape) + 1j * r.standard_normal(shape)).astype(np.complex64)
+
+    def on_dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.complex64).ravel()).to(dev)
+
+    # -- subspace basis --------------------------------------------------
+    n, C, T, spf, R, K = 256, 8, 64, 4, 512, 4
+    N = n * n
+    Mf = spf * R
+    traj = I.radial_traj(R, T * spf).reshape(T, Mf, 3)
+    rng = np.random.default_rng(11)
+    B = np.linalg.qr(rng.standard_normal((T, K)) + 1j * rng.standard_normal((T, K)))[0].astype(np.complex64)
+    conf = ba.NufftConf.defaults()
+    bas = np.ascontiguousarray(B.T)
+    t0 = time.perf_counter()
+    op = ba.nufft_create2(dims16(1, Mf, 1, C, 1, T), dims16(n, n, 1, C, 1, 1, K), dims16(3, Mf, 1, 1, 1, T),
+                          np.ascontiguousarray(traj), conf=conf, bas_dims=dims16(1, 1, 1, 1, 1, T, K), basis=bas)
+    t_plan = time.perf_counter() - t0
+    x = on_dev(crandn((K, C, 1, n, n), 12))
+    y = on_dev(crandn((T, C, Mf), 13))
+    xo = torch.empty_like(x)
+    yo = torch.empty_like(y)
+    t_fa = _median_ms(lambda: (op.adjoint(y, out=xo), op.forward(xo, out=yo)), 10)
+    t_n = _median_ms(lambda: op.normal(x, out=xo), 20)
+    maps = I.gaussian_coils((1, n, n), C, seed=14)
+    sense = ba.sense_nc_init(dims16(n, n, 1, C, 1, 1, K), dims16(n, n, 1, C), maps, dims16(1, Mf, 1, C, 1, T),
+                             dims16(3, Mf, 1, 1, 1, T), np.ascontiguousarray(traj), basis_dims=dims16(1, 1, 1, 1, 1, T, K),
+                             basis=bas)
+    b = sense.adjoint(y)
+    t_cg = _median_ms(lambda: ba.lsqr_conjgrad(sense, b, 30, 0.01, 0.0), 5)
+    nb_fa = 2 * (8 * C * T * Mf + 4 * 2 * T * Mf + 8 * C * K * N)
+    nb_n = 2 * 8 * C * K * N + 8 * K * K * 4 * N
+    row = dict(workload=f"2d_radial_subspace_{n}^2_{C}coils_{T}frames_{spf}x{R}_K{K}", plan_s=round(t_plan, 2),
+               adj_fwd_ms=round(t_fa, 3), adj_fwd_msamples_s=round(2 * C * T * Mf / t_fa / 1e3, 1),
+               adj_fwd_bytes=nb_fa, adj_fwd_frac=_frac(nb_fa, t_fa),
+               normal_ms=round(t_n, 3), normal_bytes=nb_n, normal_frac=_frac(nb_n, t_n),
+               cg_sense_30_ms=round(t_cg, 3))
+    if cpu:
+        ref = O.BasisNufft((n, n, 1), traj, B, C, toeplitz=True)
+        xh = crandn((K, C, 1, n, n), 12)
+        yh = crandn((T, C, Mf), 13)
+        c_fa, r1 = _cpu_ms(lambda: ref.forward(ref.adjoint(yh)))
+        c_n, r2 = _cpu_ms(lambda: ref.normal(xh))
+        row["cpu"] = dict(adj_fwd_ms=round(c_fa, 1), normal_ms=round(c_n, 1), cores=threads, kind="port",
+                          sample=f"oracle BasisNufft, same workload ({r1} / {r2} repetitions)")
+    rows["basis"] = row
+    print(f"# basis: {row}", file=sys.stderr, flush=True)
+    sense.free()
+    op.free()
+
+    # -- chained NUFFT and the preconditioner (2-D radial) -----------------
+    w = workload("radial2d")
+    X, Y, Z = w["img"]
+    C, R, S = w["C"], w["R"], w["S"]
+    M, N = R * S, X * Y * Z
+    tr = ba.traj_array(w["traj"])
+    conf = ba.NufftConf.defaults(toeplitz=False)
+    conf.os = 1.5
+    op = ba.nufft_create2(dims16(1, R, S, C), dims16(X, Y, Z, C), dims16(3, R, S), tr, conf=conf)
+    y = on_dev(crandn((C, M), 21))
+    ci = torch.empty(C * N, dtype=torch.complex64, device=dev)
+    ko = torch.empty_like(y)
+    t_fa = _median_ms(lambda: (op.adjoint(y, out=ci), op.forward(ci, out=ko)), 20)
+    nb = 2 * (8 * C * M + 4 * 2 * M + 8 * C * N)
+    row = dict(workload=w["name"] + "_os1.5_width6", adj_fwd_ms=round(t_fa, 3),
+               adj_fwd_msamples_s=round(2 * C * M / t_fa / 1e3, 1), adj_fwd_bytes=nb, adj_fwd_frac=_frac(nb, t_fa))
+    if cpu:
+        ref = O.ChainNufft(w["img"], w["traj"], C, os_=1.5, width=6.0)
+        yh = crandn((C, M), 21)
+        c_fa, r1 = _cpu_ms(lambda: ref.forward(ref.adjoint(yh)))
+        row["cpu"] = dict(adj_fwd_ms=round(c_fa, 1), cores=threads, kind="port",
+                          sample=f"oracle ChainNufft, same workload ({r1} repetitions)")
+    rows["chain"] = row
+    print(f"# chain: {row}", file=sys.stderr, flush=True)
+    op.free()
+
+    conf = ba.NufftConf.defaults()
+    op = ba.nufft_create2(dims16(1, R, S, C), dims16(X, Y, Z, C), dims16(3, R, S), tr, conf=conf)
+    maps = np.asarray(w["maps"]).reshape(C, N)
+    yk = op.forward(on_dev(maps * w["x"].reshape(1, N)))
+    bh = op.adjoint(yk).cpu().numpy()
+    pre = ba.Precond(op)
+    res = {}
+    for name, fn in (("plain", lambda: ba.lsqr_conjgrad(op, bh, 300, 0.01, 1e-4)),
+                     ("strang", lambda: ba.lsqr_conjgrad_precond(op, pre, bh, 300, 0.01, 1e-4))):
+        t = _median_ms(fn, 3)
+        st = fn()[1]
+        res[name] = dict(iterations=st.iterations, ms=round(t, 3))
+    rows["precond"] = dict(workload=w["name"] + "_inverse_nufft_lambda0.01_tol1e-4", **res)
+    print(f"# precond: {rows['precond']}", file=sys.stderr, flush=True)
+    pre.free()
+    op.free()
+
+    # -- ESPIRiT -----------------------------------------------------------
+    L.extract_calib.restype = ctypes.c_void_p
+    L.extract_calib.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_bool]
+    L.calib.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+
+    def cart_ksp(dims, C, seed):
+        X, Y, Z = dims
+        img = (I.ellipse_phantom(X, seed=seed) if Z == 1 else I.ellipsoid_phantom(X, nell=8, seed=seed))
+        Sm = I.gaussian_coils((Z, Y, X), C, seed=seed + 1)
+        ax = (1, 2, 3) if Z > 1 else (2, 3)
+        k = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(Sm * img[None], axes=ax), axes=ax, norm="ortho"), axes=ax)
+        return k.astype(np.complex64)
+
+    for dims, C in (((256, 256, 1), 8), ((128, 128, 128), 16)):
+        X, Y, Z = dims
+        Mp = 2
+        ksp = cart_ksp(dims, C, 31)
+        kd = (ctypes.c_long * 16)(*dims16(X, Y, Z, C))
+        cd = (ctypes.c_long * 16)()
+        cs = (ctypes.c_long * 3)(24, 24, 24)
+        ec = ba.EcalibConf.defaults(kdims=[6 if v > 1 else 1 for v in dims])
+        od = (ctypes.c_long * 16)(*dims16(X, Y, Z, C, Mp))
+        Np = X * Y * Z
+        mo = torch.empty(Mp * C * Np, dtype=torch.complex64, device=dev)
+        eo = torch.empty(Mp * Np, dtype=torch.complex64, device=dev)
+        nsv = 216 * C
+
+        def espirit():
+            p = L.extract_calib(cd, cs, kd, ksp.ctypes.data, False)
+            L.calib(ctypes.byref(ec), od, mo.data_ptr(), eo.data_ptr(), 0, None, cd, p)
+            ctypes.CDLL(None).free(ctypes.c_void_p(p))
+
+        t_e = _median_ms(espirit, 3)
+        ba._check()
+        nb = 8 * C * Np + 8 * Mp * C * Np + 8 * Mp * Np
+        row = dict(workload=f"espirit_{X}x{Y}x{Z}_{C}coils_{Mp}maps_cal24_kernel6", ms=round(t_e, 2),
+                   bytes=nb, frac=_frac(nb, t_e))
+        if cpu and Z == 1:
+            from oracle import espirit as E
+            c_e, r1 = _cpu_ms(lambda: E.ecalib(ksp, maps=Mp), budget_s=8.0, max_reps=5)
+            row["cpu"] = dict(ms=round(c_e, 1), cores=threads, kind="port",
+                              sample=f"oracle ecalib (float64 numpy), same workload ({r1} repetitions)")
+        rows["espirit_" + ("2d" if Z == 1 else "3d")] = row
+        print(f"# espirit: {row}", file=sys.stderr, flush=True)
+
+    print(json.dumps(dict(metric="rows beyond the NUFFT / CG-SENSE path (basis, chain, preconditioner, ESPIRiT)",
+                          n_gpus=1, data="synthetic", rows=rows)))
 
 
 def radial_solves(args, dev):

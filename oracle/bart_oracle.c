@@ -211,7 +211,9 @@ static void sample_pos(const struct orc_grid_conf* conf, const long gdims[3], co
 {
 	for (int j = 0; j < 3; j++) {
 
-		pos[j] = conf->os * (crealf(tr[j]) + conf->shift[j]);
+		/* creal() of grid.c:181-183 widens to double: the sum and the
+		 * product are double, rounded to float once */
+		pos[j] = (float)(conf->os * ((double)crealf(tr[j]) + conf->shift[j]));
 		pos[j] += (gdims[j] > 1) ? ((float)(gdims[j] / 2)) : 0.f;
 	}
 }

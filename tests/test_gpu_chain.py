@@ -78,6 +78,24 @@ def test_chain_vs_oracle(dims, kind, os_, width, periodic, weights):
     op.free()
 
 
+def test_chain_full_size_radial_vs_oracle():
+    # 2-D golden-angle radial 256^2, 201 x 512, os 1.5, 12 coils: the dense
+    # centre tiles of the adjoint are split over several work-groups (slices
+    # of 1024 samples summed in order) and the coils over two chunks of 8
+    n, C, R, S = 256, 12, 512, 201
+    traj = I.radial_traj(R, S)
+    ref = O.ChainNufft((n, n, 1), traj, C, os_=1.5, width=6.0)
+    op = make((n, n, 1), traj, C, os_=1.5, width=6.0)
+    x = crandn((C, 1, n, n), 5)
+    y = crandn((C, R * S), 6)
+    a = op.adjoint(dev(y)).cpu().numpy()
+    e_f, e_a = relerr(op.forward(dev(x)).cpu().numpy(), ref.forward(x)), relerr(a, ref.adjoint(y))
+    print(f"256^2 radial os 1.5, 12 coils: forward {e_f:.1e} adjoint {e_a:.1e}")
+    assert e_f < 1e-5 and e_a < 1e-5
+    assert np.array_equal(op.adjoint(dev(y)).cpu().numpy(), a)
+    op.free()
+
+
 def test_cli_nufft_os_equals_os2(tmp_path):
     # tests/nufft.mk:305-323: periodic forward and adjoint with os 1.5 and
     # width 8 agree with os 2 and width 6 to 1e-5
