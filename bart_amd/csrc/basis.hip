@@ -108,6 +108,7 @@ __global__ void k_maps_reduce(float2* __restrict__ x, const float2* __restrict__
 // grid[s][c + C k][i] = sum_k' psf[k K + k'][s][i] grid[s][c + C k'][i]
 // (md_ztenmul over the coefficient dims, noncart/nufft.c:1594-1602); the
 // stack holds S x (C K) volumes of N points
+template <int KK>  // KK >= K: register arrays of fixed size
 __global__ void k_basis_psf(float2* __restrict__ grid, const float2* __restrict__ psf, int S, int C, int K, long N,
 			    const int* skip)
 {
@@ -122,16 +123,24 @@ __global__ void k_basis_psf(float2* __restrict__ grid, const float2* __restrict_
 		const long sc = e / N;
 		const int c = (int)(sc % C);
 		const int s = (int)(sc / C);
-		float2 v[BASIS_MAX];
+		float2 v[KK];
 
-		for (int k = 0; k < K; k++)
-			v[k] = grid[((long)s * CK + c + (long)C * k) * N + i];
+#pragma unroll
+		for (int k = 0; k < KK; k++)
+			v[k] = (k < K) ? grid[((long)s * CK + c + (long)C * k) * N + i] : make_float2(0.f, 0.f);
 
-		for (int k = 0; k < K; k++) {
+#pragma unroll
+		for (int k = 0; k < KK; k++) {
+
+			if (k >= K)
+				break;
+
 			float2 acc = make_float2(0.f, 0.f);
 
-			for (int q = 0; q < K; q++)
-				acc = cadd(acc, cmul(psf[(((long)k * K + q) * S + s) * N + i], v[q]));
+#pragma unroll
+			for (int q = 0; q < KK; q++)
+				if (q < K)
+					acc = cadd(acc, cmul(psf[(((long)k * K + q) * S + s) * N + i], v[q]));
 
 			grid[((long)s * CK + c + (long)C * k) * N + i] = acc;
 		}
@@ -189,7 +198,8 @@ void maps_reduce(float2* x, const float2* sens, const float2* src, int C, int K,
 void basis_psf_apply(float2* grid, const float2* psf, int S, int C, int K, long N, hipStream_t st, const int* skip)
 {
 	BAM_CHECK(K >= 1 && K <= BASIS_MAX, "basis: %d coefficients (at most %d)", K, BASIS_MAX);
-	hipLaunchKernelGGL(k_basis_psf, dim3(blocks_for((long)S * C * N)), dim3(256), 0, st, grid, psf, S, C, K, N,
+	auto kern = (K <= 2) ? k_basis_psf<2> : (K <= 4) ? k_basis_psf<4> : (K <= 8) ? k_basis_psf<8> : k_basis_psf<BASIS_MAX>;
+	hipLaunchKernelGGL(kern, dim3(blocks_for((long)S * C * N)), dim3(256), 0, st, grid, psf, S, C, K, N,
 			   skip);
 	BAM_HIP(hipGetLastError());
 }

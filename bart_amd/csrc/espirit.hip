@@ -158,38 +158,84 @@ __device__ inline float2 cas_entry(const CasGeom& g, const float2* cal, long p, 
 	return cal[(((long)c * g.cz + qz + oz) * g.cy + qy + oy) * g.cx + qx + ox];
 }
 
-// G[i][j] = sum_p conj(A[p][i]) A[p][j], one 16 x 16 tile per work-group
+// G[i][j] = sum_p conj(A[p][i]) A[p][j] over the rows p of the calibration
+// matrix, in double.  A work-group computes a 64 x 64 tile of the upper
+// block triangle (each thread 4 x 4 entries, rows ti + 16 a, columns
+// tj + 16 b) from 16 rows of A at a time staged in LDS, and writes the
+// mirrored tile as its conjugate, so G is Hermitian bit for bit.
+constexpr int CG_T = 64;   // tile edge
+constexpr int CG_KP = 16;  // rows of A per LDS stage
+
 __global__ __launch_bounds__(256) void k_cas_gram(CasGeom g, const float2* __restrict__ cal, float2* __restrict__ G)
 {
-	__shared__ float2 Ai[32][17], Aj[32][17];
-	const int ti = threadIdx.y, tj = threadIdx.x;
-	const int i = blockIdx.y * 16 + ti, j = blockIdx.x * 16 + tj;
-	const int t = ti * 16 + tj;
-	double re = 0., im = 0.;
+	__shared__ float2 As[CG_KP][CG_T], Bs[CG_KP][CG_T];
 
-	for (long p0 = 0; p0 < g.P; p0 += 32) {
+	// block -> (bi, bj), bj >= bi, row by row of the upper block triangle
+	const int nb = (g.N + CG_T - 1) / CG_T;
+	int bi = 0;
+	long rem = blockIdx.x;
+	while (rem >= nb - bi) {
+		rem -= nb - bi;
+		bi++;
+	}
+	const int bj = bi + (int)rem;
 
-		for (int q = t; q < 32 * 16; q += 256) {
-			const int pp = q / 16, kk = q % 16;
+	const int ti = threadIdx.x / 16, tj = threadIdx.x % 16;
+	double re[4][4], im[4][4];
+
+#pragma unroll
+	for (int a = 0; a < 4; a++)
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			re[a][c] = 0.;
+			im[a][c] = 0.;
+		}
+
+	for (long p0 = 0; p0 < g.P; p0 += CG_KP) {
+
+		for (int q = threadIdx.x; q < CG_KP * CG_T; q += 256) {
+			const int pp = q / CG_T, kk = q % CG_T;
 			const long p = p0 + pp;
-			const int ki = blockIdx.y * 16 + kk, kj = blockIdx.x * 16 + kk;
-			Ai[pp][kk] = (p < g.P && ki < g.N) ? cas_entry(g, cal, p, ki) : make_float2(0.f, 0.f);
-			Aj[pp][kk] = (p < g.P && kj < g.N) ? cas_entry(g, cal, p, kj) : make_float2(0.f, 0.f);
+			const int ki = bi * CG_T + kk, kj = bj * CG_T + kk;
+			As[pp][kk] = (p < g.P && ki < g.N) ? cas_entry(g, cal, p, ki) : make_float2(0.f, 0.f);
+			Bs[pp][kk] = (p < g.P && kj < g.N) ? cas_entry(g, cal, p, kj) : make_float2(0.f, 0.f);
 		}
 
 		__syncthreads();
 
-		for (int pp = 0; pp < 32; pp++) {
-			const float2 a = Ai[pp][ti], b = Aj[pp][tj];
-			re += (double)a.x * b.x + (double)a.y * b.y;
-			im += (double)a.x * b.y - (double)a.y * b.x;
+		for (int pp = 0; pp < CG_KP; pp++) {
+
+			float2 x[4], y[4];
+
+#pragma unroll
+			for (int a = 0; a < 4; a++) {
+				x[a] = As[pp][ti + 16 * a];
+				y[a] = Bs[pp][tj + 16 * a];
+			}
+
+#pragma unroll
+			for (int a = 0; a < 4; a++)
+#pragma unroll
+				for (int c = 0; c < 4; c++) {
+					re[a][c] += (double)x[a].x * y[c].x + (double)x[a].y * y[c].y;
+					im[a][c] += (double)x[a].x * y[c].y - (double)x[a].y * y[c].x;
+				}
 		}
 
 		__syncthreads();
 	}
 
-	if (i < g.N && j < g.N)
-		G[(long)i * g.N + j] = make_float2((float)re, (float)im);
+#pragma unroll
+	for (int a = 0; a < 4; a++)
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			const int i = bi * CG_T + ti + 16 * a, j = bj * CG_T + tj + 16 * c;
+			if (i < g.N && j < g.N) {
+				G[(long)i * g.N + j] = make_float2((float)re[a][c], (float)im[a][c]);
+				if (bi != bj)
+					G[(long)j * g.N + i] = make_float2((float)re[a][c], -(float)im[a][c]);
+			}
+		}
 }
 
 struct PadGeom {
@@ -316,90 +362,130 @@ __device__ inline float2 hval(const MapsGeom& g, const float2* cov, long p, int 
 	return cconj(cov[((long)k * (k + 1) / 2 + j) * g.np + p]);
 }
 
-// per point: orthogonal iteration (rows times H, gram_schmidt from the last
-// row), maps u = row M - 1 - u with eigenvalue val, crop, phase of the
-// principal component
-__global__ void k_eigenmaps(MapsGeom g, const float2* __restrict__ cov, const float2* __restrict__ rot,
-			    float2* __restrict__ maps, float2* __restrict__ emaps)
+// sum over the G lanes of a group (butterfly, fixed order)
+template <int G>
+__device__ inline double group_sum(double v)
 {
-	for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < g.np; p += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+	for (int m = G / 2; m >= 1; m /= 2)
+		v += __shfl_xor(v, m, G);
+	return v;
+}
 
-		float2 out[ESP_MAXM][ESP_MAXC];
-		float2 tmp[ESP_MAXC];
-		float val[ESP_MAXM];
+// per point: orthogonal iteration (rows times H, gram_schmidt from the last
+// row, num/linalg.c:1296-1314), maps u = row M - 1 - u with eigenvalue val,
+// crop, phase of the principal component.  A group of G >= C lanes serves
+// one point: lane k holds column k of H in registers and component k of
+// each row; a row times H takes the row's components from the other lanes
+// by shuffles, the dot products of the Gram-Schmidt step are group sums.
+// Products and sums in double, as the single-lane form.
+template <int G>
+__global__ __launch_bounds__(256) void k_eigenmaps(MapsGeom g, const float2* __restrict__ cov,
+						   const float2* __restrict__ rot, float2* __restrict__ maps,
+						   float2* __restrict__ emaps)
+{
+	const int k = threadIdx.x % G;
+	const long p0 = ((long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+	const bool live = p0 < g.np;
+	const long p = live ? p0 : g.np - 1;  // idle groups run on the last point and store nothing
+	const bool col = k < g.C;
 
-		for (int u = 0; u < g.M; u++) {
-			val[u] = 0.f;
-			for (int c = 0; c < g.C; c++)
-				out[u][c] = make_float2((u == c) ? 1.f : 0.f, 0.f);
-		}
+	float2 h[G];
 
-		for (int it = 0; it < g.iters; it++) {
+#pragma unroll
+	for (int j = 0; j < G; j++)
+		h[j] = (col && j < g.C) ? hval(g, cov, p, j, k) : make_float2(0.f, 0.f);
 
-			for (int u = 0; u < g.M; u++) {
+	float2 row[ESP_MAXM];
+	float val[ESP_MAXM];
 
-				for (int j = 0; j < g.C; j++)
-					tmp[j] = out[u][j];
+#pragma unroll
+	for (int u = 0; u < ESP_MAXM; u++) {
+		row[u] = make_float2((u == k) ? 1.f : 0.f, 0.f);
+		val[u] = 0.f;
+	}
 
-				for (int k = 0; k < g.C; k++) {
-					double re = 0., im = 0.;
-					for (int j = 0; j < g.C; j++) {
-						const float2 a = tmp[j], h = hval(g, cov, p, j, k);
-						re += (double)a.x * h.x - (double)a.y * h.y;
-						im += (double)a.x * h.y + (double)a.y * h.x;
-					}
-					out[u][k] = make_float2((float)re, (float)im);
-				}
+	for (int it = 0; it < g.iters; it++) {
+
+#pragma unroll
+		for (int u = 0; u < ESP_MAXM; u++) {
+
+			if (u >= g.M)
+				break;
+
+			const float2 t = row[u];
+			double re = 0., im = 0.;
+
+#pragma unroll
+			for (int j = 0; j < G; j++) {
+				const float2 a = make_float2(__shfl(t.x, j, G), __shfl(t.y, j, G));
+				re += (double)a.x * h[j].x - (double)a.y * h[j].y;
+				im += (double)a.x * h[j].y + (double)a.y * h[j].x;
 			}
 
-			for (int r = g.M - 1; r >= 0; r--) {
-
-				for (int j = r + 1; j < g.M; j++) {
-					double sr = 0., si = 0.;
-					for (int k = 0; k < g.C; k++) {
-						const float2 a = out[r][k], b = out[j][k];
-						sr += (double)a.x * b.x + (double)a.y * b.y;
-						si += (double)a.y * b.x - (double)a.x * b.y;
-					}
-					const float2 s = make_float2((float)sr, (float)si);
-					for (int k = 0; k < g.C; k++)
-						out[r][k] = csub(out[r][k], cmul(s, out[j][k]));
-				}
-
-				double nn = 0.;
-				for (int k = 0; k < g.C; k++)
-					nn += (double)out[r][k].x * out[r][k].x + (double)out[r][k].y * out[r][k].y;
-				val[r] = sqrtf((float)nn);
-				for (int k = 0; k < g.C; k++)
-					out[r][k] = make_float2(out[r][k].x / val[r], out[r][k].y / val[r]);
-			}
+			row[u] = make_float2((float)re, (float)im);
 		}
 
-		for (int u = 0; u < g.M; u++) {
+#pragma unroll
+		for (int r = ESP_MAXM - 1; r >= 0; r--) {
 
-			const int row = g.M - 1 - u;
-			const float e = val[row];
-			const float w = (fabsf(e) >= g.crop) ? 1.f : 0.f;
-			float2 ph = make_float2(1.f, 0.f);
+			if (r >= g.M)
+				continue;
 
-			if (g.rotphase) {
-				double tr = 0., ti = 0.;
-				for (int c = 0; c < g.C; c++) {
-					const float2 a = cscale(out[row][c], w), b = rot[c];
-					tr += (double)a.x * b.x + (double)a.y * b.y;
-					ti += (double)a.y * b.x - (double)a.x * b.y;
-				}
-				const double m = sqrt(tr * tr + ti * ti);
-				if (m > 0.)
-					ph = make_float2((float)(tr / m), (float)(ti / m));
+#pragma unroll
+			for (int j = r + 1; j < ESP_MAXM; j++) {
+
+				if (j >= g.M)
+					break;
+
+				const float2 a = row[r], b = row[j];
+				const float sr = (float)group_sum<G>((double)a.x * b.x + (double)a.y * b.y);
+				const float si = (float)group_sum<G>((double)a.y * b.x - (double)a.x * b.y);
+				row[r] = csub(row[r], cmul(make_float2(sr, si), row[j]));
 			}
 
-			for (int c = 0; c < g.C; c++)
-				maps[((long)u * g.C + c) * g.np + p] = cmulc(cscale(out[row][c], w), ph);
-
-			if (nullptr != emaps)
-				emaps[(long)u * g.np + p] = make_float2(e, 0.f);
+			const double nn = group_sum<G>((double)row[r].x * row[r].x + (double)row[r].y * row[r].y);
+			val[r] = sqrtf((float)nn);
+			row[r] = make_float2(row[r].x / val[r], row[r].y / val[r]);
 		}
+	}
+
+	const float2 rc = col ? rot[k] : make_float2(0.f, 0.f);
+
+#pragma unroll
+	for (int u = 0; u < ESP_MAXM; u++) {
+
+		if (u >= g.M)
+			break;
+
+		// map u is row M - 1 - u
+		float2 a = make_float2(0.f, 0.f);
+		float e = 0.f;
+
+#pragma unroll
+		for (int r = 0; r < ESP_MAXM; r++)
+			if (r == g.M - 1 - u) {
+				a = row[r];
+				e = val[r];
+			}
+
+		const float w = (fabsf(e) >= g.crop) ? 1.f : 0.f;
+		a = cscale(a, w);
+		float2 ph = make_float2(1.f, 0.f);
+
+		if (g.rotphase) {
+			const double tr = group_sum<G>((double)a.x * rc.x + (double)a.y * rc.y);
+			const double ti = group_sum<G>((double)a.y * rc.x - (double)a.x * rc.y);
+			const double m = sqrt(tr * tr + ti * ti);
+			if (m > 0.)
+				ph = make_float2((float)(tr / m), (float)(ti / m));
+		}
+
+		if (live && col)
+			maps[((long)u * g.C + k) * g.np + p] = cmulc(a, ph);
+
+		if (live && 0 == k && nullptr != emaps)
+			emaps[(long)u * g.np + p] = make_float2(e, 0.f);
 	}
 }
 
@@ -478,7 +564,8 @@ int espirit_calib(const EspiritConf& conf, const long img[3], int C, int maps, c
 	DevBuf<float2> dcal(ncal);
 	BAM_HIP(hipMemcpyAsync(dcal.ptr, cal, sizeof(float2) * ncal, hipMemcpyHostToDevice, st));
 	DevBuf<float2> G((size_t)N * N);
-	hipLaunchKernelGGL(k_cas_gram, dim3((N + 15) / 16, (N + 15) / 16), dim3(16, 16), 0, st, cg, dcal.ptr, G.ptr);
+	const long nbt = (N + CG_T - 1) / CG_T;
+	hipLaunchKernelGGL(k_cas_gram, dim3((unsigned)(nbt * (nbt + 1) / 2)), dim3(256), 0, st, cg, dcal.ptr, G.ptr);
 	BAM_HIP(hipGetLastError());
 
 	// 2. eigenvectors: the column-major routine reads the row-major Gram
@@ -632,7 +719,13 @@ int espirit_calib(const EspiritConf& conf, const long img[3], int C, int maps, c
 	mg.iters = conf.num_orthiter;
 	mg.crop = conf.crop;
 	mg.rotphase = conf.rotphase;
-	hipLaunchKernelGGL(k_eigenmaps, dim3(nblk(mg.np)), dim3(256), 0, st, mg, a.ptr, drot.ptr, maps_out, emaps);
+	// lanes per point: the smallest group of 4, 8, 16 or 32 that holds C
+	const int lanes = (C <= 4) ? 4 : (C <= 8) ? 8 : (C <= 16) ? 16 : 32;
+	const long nb = (mg.np * lanes + 255) / 256;
+	BAM_CHECK(nb < (1L << 31), "ecalib: image too large");
+	auto kern = (4 == lanes) ? k_eigenmaps<4> : (8 == lanes) ? k_eigenmaps<8> : (16 == lanes) ? k_eigenmaps<16>
+											 : k_eigenmaps<32>;
+	hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, st, mg, a.ptr, drot.ptr, maps_out, emaps);
 	BAM_HIP(hipGetLastError());
 	BAM_HIP(hipStreamSynchronize(st));
 

@@ -1,7 +1,8 @@
 """GPU: ESPIRiT (extract_calib + calib of the C ABI, bart ecalib) against the
 oracle's float64 restatement (oracle/espirit.py): the number of kernels and
 the singular values, the maps and eigenvalue maps away from the crop
-boundary (the crop compares an eigenvalue with 0.8 and flips where FP32
+boundary (coil counts 4 to 24: every lane-group width of the point-wise
+iteration) (the crop compares an eigenvalue with 0.8 and flips where FP32
 and FP64 land on either side), 2-D and 3-D, one and two maps, with and
 without the phase rotation; the maps explain the coil images as the
 reference's test requires (tests/ecalib.mk:3-9); the command-line tool
@@ -45,7 +46,8 @@ def compare(sens, ev, rs, re, margin=0.05, crop=0.8):
 
 
 @pytest.mark.parametrize("dims,C,maps,rot", [((64, 64, 1), 8, 1, True), ((64, 64, 1), 8, 2, True),
-                                              ((96, 80, 1), 12, 2, False), ((32, 32, 32), 6, 1, True)])
+                                              ((96, 80, 1), 12, 2, False), ((32, 32, 32), 6, 1, True),
+                                              ((64, 64, 1), 4, 2, True), ((64, 64, 1), 24, 3, True)])
 def test_ecalib_vs_oracle(dims, C, maps, rot):
     import bart_amd as ba
     img, S, ksp = case(dims, C, 7)
