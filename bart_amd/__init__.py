@@ -161,6 +161,11 @@ def lib():
     L.bart_amd_warmnet_pack.argtypes = [ctypes.c_int, lp, vp, ctypes.c_float, vp]
     L.bart_amd_warmnet_guess.restype = ctypes.c_int
     L.bart_amd_warmnet_guess.argtypes = [vp, vp, ctypes.c_float, ctypes.c_int, lp, vp, vp]
+    L.bart_amd_warmnet_input_dir.restype = ctypes.c_int
+    L.bart_amd_warmnet_input_dir.argtypes = [vp, ctypes.c_float, ctypes.c_int, lp, vp, vp, vp,
+                                             ctypes.POINTER(ctypes.c_float)]
+    L.bart_amd_warmnet_guess_dir.restype = ctypes.c_int
+    L.bart_amd_warmnet_guess_dir.argtypes = [vp, vp, ctypes.c_float, ctypes.c_int, lp, vp, vp, vp]
     L.nufft_precond_create.restype = vp
     L.nufft_precond_create.argtypes = [vp]
     L.operator_free.restype = None

@@ -619,79 +619,91 @@ void WarmNet::train_step_grad(const float* x, const float* dout, int B, int H, i
 constexpr int GS_BLOCKS = 256;
 constexpr int GS_THREADS = 256;
 
-// per work-group: sum b.b, sum b.(A^H A b + lambda b) (float products, FP64
-// sums, as the CG's start and first p.Ap) and max |b|^2
-__global__ __launch_bounds__(GS_THREADS) void k_guess_partials(const float* __restrict__ b,
-							       const float* __restrict__ Ab, float lambda, long n,
+// partial sums of the first step along a direction d for the right-hand
+// side b: d^H b (re, im), d^H (A^H A d + lambda d) and max |d|^2, per block
+// (FP64; d = b gives CG's first step from zero)
+__global__ __launch_bounds__(GS_THREADS) void k_guess_partials(const float* __restrict__ d, const float* __restrict__ b,
+							       const float* __restrict__ Ad, float lambda, long n,
 							       double* __restrict__ part)
 {
-	__shared__ double sh[3][GS_THREADS / 64];
-	double bb = 0., bab = 0., mx = 0.;
+	__shared__ double sh[4][GS_THREADS / 64];
+	double db = 0., dbi = 0., dad = 0., mx = 0.;
 
 	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+		const float2 dv = ((const float2*)d)[i];
 		const float2 bv = ((const float2*)b)[i];
-		const float2 av = ((const float2*)Ab)[i];
-		const float a0 = av.x + lambda * bv.x;
-		const float a1 = av.y + lambda * bv.y;
-		bb += (double)(bv.x * bv.x) + (double)(bv.y * bv.y);
-		bab += (double)(bv.x * a0) + (double)(bv.y * a1);
-		mx = fmax(mx, (double)bv.x * bv.x + (double)bv.y * bv.y);
+		const float2 av = ((const float2*)Ad)[i];
+		const float a0 = av.x + lambda * dv.x;
+		const float a1 = av.y + lambda * dv.y;
+		db += (double)(dv.x * bv.x) + (double)(dv.y * bv.y);
+		dbi += (double)(dv.x * bv.y) - (double)(dv.y * bv.x);
+		dad += (double)(dv.x * a0) + (double)(dv.y * a1);
+		mx = fmax(mx, (double)dv.x * dv.x + (double)dv.y * dv.y);
 	}
 
 	for (int off = 32; off > 0; off >>= 1) {
-		bb += __shfl_down(bb, off, 64);
-		bab += __shfl_down(bab, off, 64);
+		db += __shfl_down(db, off, 64);
+		dbi += __shfl_down(dbi, off, 64);
+		dad += __shfl_down(dad, off, 64);
 		mx = fmax(mx, __shfl_down(mx, off, 64));
 	}
 
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 
 	if (0 == lane) {
-		sh[0][wid] = bb;
-		sh[1][wid] = bab;
+		sh[0][wid] = db;
+		sh[1][wid] = dad;
 		sh[2][wid] = mx;
+		sh[3][wid] = dbi;
 	}
 
 	__syncthreads();
 
 	if (0 == threadIdx.x) {
-		double a = 0., c = 0., m = 0.;
+		double a = 0., c = 0., m = 0., ai = 0.;
 		for (int w = 0; w < GS_THREADS / 64; w++) {
 			a += sh[0][w];
 			c += sh[1][w];
 			m = fmax(m, sh[2][w]);
+			ai += sh[3][w];
 		}
 		part[blockIdx.x] = a;
 		part[GS_BLOCKS + blockIdx.x] = c;
 		part[2 * GS_BLOCKS + blockIdx.x] = m;
+		part[3 * GS_BLOCKS + blockIdx.x] = ai;
 	}
 }
 
-// sc = { alpha, s, 1 / s }: x1 = alpha b, s = max |x1| (1 when x1 = 0)
+// sc = { re alpha, s, 1 / s, im alpha }: x1 = alpha d with alpha =
+// d^H b / d^H (A^H A + lambda I) d, s = max |x1| (1 when x1 = 0)
 __global__ void k_guess_scalars(const double* __restrict__ part, float* __restrict__ sc)
 {
 	if (0 != threadIdx.x)
 		return;
 
-	double a = 0., c = 0., m = 0.;
+	double a = 0., c = 0., m = 0., ai = 0.;
 	for (int i = 0; i < GS_BLOCKS; i++) {
 		a += part[i];
 		c += part[GS_BLOCKS + i];
 		m = fmax(m, part[2 * GS_BLOCKS + i]);
+		ai += part[3 * GS_BLOCKS + i];
 	}
 
 	const float rs = (float)a;
+	const float rsi = (float)ai;
 	const float pAp = (float)c;
 	const float alpha = (0.f != pAp) ? rs / pAp : 0.f;
-	float s = fabsf(alpha) * (float)sqrt(m);
+	const float alpha_i = (0.f != pAp) ? rsi / pAp : 0.f;
+	float s = sqrtf(alpha * alpha + alpha_i * alpha_i) * (float)sqrt(m);
 	if (!(s > 0.f))
 		s = 1.f;
 	sc[0] = alpha;
 	sc[1] = s;
 	sc[2] = 1.f / s;
+	sc[3] = alpha_i;
 }
 
-// ch[0][i] = re(v_i) f, ch[1][i] = im(v_i) f with v = alpha b (mode 0) or
+// ch[0][i] = re(v_i) f, ch[1][i] = im(v_i) f with v = alpha d (mode 0) or
 // v = img (mode 1; f = 1 / scale given)
 __global__ void k_guess_pack(const float2* __restrict__ v, const float* __restrict__ sc, float f, long npix,
 			     int mode, float* __restrict__ ch)
@@ -700,7 +712,7 @@ __global__ void k_guess_pack(const float2* __restrict__ v, const float* __restri
 		float2 x = v[i];
 		float g = f;
 		if (0 == mode) {
-			x = make_float2(sc[0] * x.x, sc[0] * x.y);
+			x = make_float2(sc[0] * x.x - sc[3] * x.y, sc[0] * x.y + sc[3] * x.x);
 			g = sc[2];
 		}
 		ch[i] = x.x * g;
@@ -716,11 +728,11 @@ __global__ void k_guess_unpack(const float* __restrict__ ch, const float* __rest
 		x0[i] = make_float2(sc[1] * ch[i], sc[1] * ch[npix + i]);
 }
 
-void guess_scalars(const float2* b, const float2* Ab, float lambda, long npix, double* part, float* sc,
-		   hipStream_t st)
+void guess_scalars(const float2* d, const float2* b, const float2* Ad, float lambda, long npix, double* part,
+		   float* sc, hipStream_t st)
 {
-	hipLaunchKernelGGL(k_guess_partials, dim3(GS_BLOCKS), dim3(GS_THREADS), 0, st, (const float*)b,
-			   (const float*)Ab, lambda, npix, part);
+	hipLaunchKernelGGL(k_guess_partials, dim3(GS_BLOCKS), dim3(GS_THREADS), 0, st, (const float*)d,
+			   (const float*)b, (const float*)Ad, lambda, npix, part);
 	hipLaunchKernelGGL(k_guess_scalars, dim3(1), dim3(64), 0, st, part, sc);
 	BAM_HIP(hipGetLastError());
 }

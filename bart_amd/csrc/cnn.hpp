@@ -15,15 +15,17 @@ void conv3x3_bwd_data(const float* dy, int B, int Cout, int H, int W, const floa
 void conv3x3_bwd_weights(const float* x, const float* dy, int B, int Cin, int Cout, int H, int W, float* dw,
 			 float* db, hipStream_t st);
 
-// Input and guess of the warm-start network.  guess_scalars: sc = { alpha,
-// s, 1 / s } of the first CG iterate from zero x1 = alpha b (alpha =
-// ||b||^2 / b.(A^H A b + lambda b), as CG computes it) and its scale s =
-// max |x1|; part: 3 x 256 doubles.  guess_pack: channels [2][npix] (real,
-// imaginary) of x1 / s (scaled_first_iterate, v = b) or of v / scale
-// (f = 1 / scale).  guess_unpack: x0 = s (ch[0] + i ch[1]).
-constexpr int GUESS_PARTS = 3 * 256;
-void guess_scalars(const float2* b, const float2* Ab, float lambda, long npix, double* part, float* sc,
-		   hipStream_t st);
+// Input and guess of the warm-start network.  guess_scalars: sc = { re
+// alpha, s, 1 / s, im alpha } of the step along a direction d for the
+// right-hand side b, x1 = alpha d with alpha = d^H b / d^H (A^H A d +
+// lambda d) (d = b: CG's first iterate from zero, alpha as CG computes it),
+// given Ad = A^H A d, and its scale s = max |x1|; part: 4 x 256 doubles.
+// guess_pack: channels [2][npix] (real, imaginary) of x1 / s
+// (scaled_first_iterate, v = d) or of v / scale (f = 1 / scale).
+// guess_unpack: x0 = s (ch[0] + i ch[1]).
+constexpr int GUESS_PARTS = 4 * 256;
+void guess_scalars(const float2* d, const float2* b, const float2* Ad, float lambda, long npix, double* part,
+		   float* sc, hipStream_t st);
 void guess_pack(const float2* v, const float* sc, float f, long npix, bool scaled_first_iterate, float* ch,
 		hipStream_t st);
 void guess_unpack(const float* ch, const float* sc, long npix, float2* x0, hipStream_t st);

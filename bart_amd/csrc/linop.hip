@@ -143,7 +143,7 @@ struct Op {
 	std::unique_ptr<CgSolver> cg;
 	float* mon_err = nullptr;     // host array of bart_amd_cg_monitor (null: off)
 	// warm-start guess: A^H A b, partial sums, scalars, network channels in / out
-	DevBuf<float2> g_ab, g_b, g_x;
+	DevBuf<float2> g_ab, g_b, g_d, g_x;
 	DevBuf<double> g_part;
 	DevBuf<float> g_sc, g_in, g_out;
 
@@ -2042,10 +2042,11 @@ int bart_amd_warmnet_train_step_grad(struct bart_amd_warmnet* net, const float* 
 	}, -1);
 }
 
-// first CG iterate from zero, its scale and the network channels of x1 / s
-// in op->g_in (b: host or device); returns the device pointer of b used
-static const float2* warm_input(Op* op, float l2lambda, int N, const long img_dims[], const float _Complex* b,
-				const char* what)
+// the step along d (d = b when NULL: CG's first iterate from zero), its
+// scale and the network channels of x1 / s in op->g_in (b, d: host or
+// device)
+static void warm_input(Op* op, float l2lambda, int N, const long img_dims[], const float _Complex* b,
+		       const float _Complex* d, const char* what)
 {
 	check_dims(what, N, img_dims, op->N, op->idims);
 	BAM_CHECK(OP_NUDFT != op->kind && nullptr != op->nufft, "%s: not for the chained NUFFT", what);
@@ -2075,15 +2076,26 @@ static const float2* warm_input(Op* op, float l2lambda, int N, const long img_di
 		bd = op->g_b.ptr;
 	}
 
+	const float2* dd = bd;
+
+	if (nullptr != d) {
+		dd = (const float2*)d;
+		if (!on_device(d)) {
+			if ((long)op->g_d.n < npix)
+				op->g_d.alloc(npix);
+			BAM_HIP(hipMemcpyAsync(op->g_d.ptr, d, sizeof(float2) * npix, hipMemcpyHostToDevice, op->stream));
+			dd = op->g_d.ptr;
+		}
+	}
+
 	op->prepare_normal();
-	op->nrm(op->g_ab.ptr, bd, nullptr);
-	guess_scalars(bd, op->g_ab.ptr, l2lambda, npix, op->g_part.ptr, op->g_sc.ptr, op->stream);
-	guess_pack(bd, op->g_sc.ptr, 1.f, npix, true, op->g_in.ptr, op->stream);
-	return bd;
+	op->nrm(op->g_ab.ptr, dd, nullptr);
+	guess_scalars(dd, bd, op->g_ab.ptr, l2lambda, npix, op->g_part.ptr, op->g_sc.ptr, op->stream);
+	guess_pack(dd, op->g_sc.ptr, 1.f, npix, true, op->g_in.ptr, op->stream);
 }
 
-int bart_amd_warmnet_input(const struct linop_s* _op, float l2lambda, int N, const long img_dims[],
-			   const float _Complex* b, float* inp, float* scale)
+int bart_amd_warmnet_input_dir(const struct linop_s* _op, float l2lambda, int N, const long img_dims[],
+			       const float _Complex* b, const float _Complex* d, float* inp, float* scale)
 {
 	return guarded([&]() -> int {
 		Op* op = (Op*)_op;
@@ -2092,7 +2104,7 @@ int bart_amd_warmnet_input(const struct linop_s* _op, float l2lambda, int N, con
 		std::lock_guard<std::mutex> lock(op->mtx());
 		if (0 != op->user_stream)
 			BAM_HIP(hipStreamSynchronize(op->user_stream));
-		warm_input(op, l2lambda, N, img_dims, b, "bart_amd_warmnet_input");
+		warm_input(op, l2lambda, N, img_dims, b, d, "bart_amd_warmnet_input");
 		BAM_HIP(hipMemcpyAsync(inp, op->g_in.ptr, sizeof(float) * 2 * op->isize, hipMemcpyDeviceToDevice,
 				       op->stream));
 		float sc[3];
@@ -2102,6 +2114,12 @@ int bart_amd_warmnet_input(const struct linop_s* _op, float l2lambda, int N, con
 			*scale = sc[1];
 		return 0;
 	}, -1);
+}
+
+int bart_amd_warmnet_input(const struct linop_s* op, float l2lambda, int N, const long img_dims[],
+			   const float _Complex* b, float* inp, float* scale)
+{
+	return bart_amd_warmnet_input_dir(op, l2lambda, N, img_dims, b, nullptr, inp, scale);
 }
 
 int bart_amd_warmnet_pack(int N, const long img_dims[], const float _Complex* img, float scale, float* out)
@@ -2117,8 +2135,9 @@ int bart_amd_warmnet_pack(int N, const long img_dims[], const float _Complex* im
 	}, -1);
 }
 
-int bart_amd_warmnet_guess(struct bart_amd_warmnet* _net, const struct linop_s* _op, float l2lambda, int N,
-			   const long img_dims[], const float _Complex* b, float _Complex* x0)
+int bart_amd_warmnet_guess_dir(struct bart_amd_warmnet* _net, const struct linop_s* _op, float l2lambda, int N,
+			       const long img_dims[], const float _Complex* b, const float _Complex* d,
+			       float _Complex* x0)
 {
 	return guarded([&]() -> int {
 		Op* op = (Op*)_op;
@@ -2128,7 +2147,7 @@ int bart_amd_warmnet_guess(struct bart_amd_warmnet* _net, const struct linop_s* 
 		std::lock_guard<std::mutex> lock(op->mtx());
 		if (0 != op->user_stream)
 			BAM_HIP(hipStreamSynchronize(op->user_stream));
-		warm_input(op, l2lambda, N, img_dims, b, "bart_amd_warmnet_guess");
+		warm_input(op, l2lambda, N, img_dims, b, d, "bart_amd_warmnet_guess");
 		net->forward(op->g_in.ptr, 1, op->nufft->n[1], op->nufft->n[0], op->g_out.ptr, op->stream);
 
 		const long npix = op->isize;
@@ -2149,6 +2168,12 @@ int bart_amd_warmnet_guess(struct bart_amd_warmnet* _net, const struct linop_s* 
 		BAM_HIP(hipStreamSynchronize(op->stream));
 		return 0;
 	}, -1);
+}
+
+int bart_amd_warmnet_guess(struct bart_amd_warmnet* net, const struct linop_s* op, float l2lambda, int N,
+			   const long img_dims[], const float _Complex* b, float _Complex* x0)
+{
+	return bart_amd_warmnet_guess_dir(net, op, l2lambda, N, img_dims, b, nullptr, x0);
 }
 
 int bart_amd_tuning_report(char* buf, int n)

@@ -4,8 +4,11 @@ and CG from the network's guess with the zero start as fallback.
 
 No counterpart in BART: pics takes an initial guess with -W
 (src/pics.c:580-594, iter/itop.c:54-66); this module proposes it.  The
-network runs on the device through the library (3x3 convolutions on the
-matrix cores); torch only holds device memory.
+network's input is the density-compensated adjoint A^H (w y) (the
+coil-combined gridding image, w the radial ramp) times its best multiple for
+the residual of the normal equations; its output is the guess.  The network
+runs on the device through the library (3x3 convolutions on the matrix
+cores); torch only holds device memory.
 """
 import ctypes
 import time
@@ -103,30 +106,53 @@ def from_channels(t, n):
     return torch.view_as_complex(t[0].permute(1, 2, 0).contiguous()).reshape(n * n).contiguous()
 
 
-def network_input(op, b, n, lam):
-    """Network input channels [1, 2, n, n] of the first CG iterate from zero
-    x1 = alpha b, scaled by s = max |x1|, and s -- computed by the library
-    (bart_amd_warmnet_input)."""
+def network_input(op, b, n, lam, d=None):
+    """Network input channels [1, 2, n, n] of x1 = alpha d, the best multiple
+    of the direction d for the residual of (A^H A + lam I) x = b (d = None:
+    d = b, CG's first iterate from zero), scaled by s = max |x1|, and s --
+    computed by the library (bart_amd_warmnet_input_dir)."""
     import torch
     from . import _dims, _ptr
     inp = torch.empty((1, 2, n, n), dtype=torch.float32, device=b.device)
     s = ctypes.c_float(0.0)
     torch.cuda.synchronize()
-    _lib().bart_amd_warmnet_input(op.h, lam, DIMS, _dims(op.idims), _ptr(b), _fptr(inp), ctypes.byref(s))
+    _lib().bart_amd_warmnet_input_dir(op.h, lam, DIMS, _dims(op.idims), _ptr(b), _ptr(d) if d is not None else None,
+                                      _fptr(inp), ctypes.byref(s))
     _check()
     return inp, float(s.value)
 
 
-def guess(net, op, b, lam):
-    """The learned initial guess x0 = s (x1 / s + net(x1 / s)), computed by
-    the library in one call (bart_amd_warmnet_guess)."""
+def guess(net, op, b, lam, d=None):
+    """The learned initial guess x0 = s (x1 / s + net(x1 / s)), x1 as in
+    network_input, computed by the library in one call
+    (bart_amd_warmnet_guess_dir)."""
     import torch
     from . import _dims, _ptr
     x0 = torch.empty_like(b)
     torch.cuda.synchronize()
-    _lib().bart_amd_warmnet_guess(net.h, op.h, lam, DIMS, _dims(op.idims), _ptr(b), _ptr(x0))
+    _lib().bart_amd_warmnet_guess_dir(net.h, op.h, lam, DIMS, _dims(op.idims), _ptr(b),
+                                      _ptr(d) if d is not None else None, _ptr(x0))
     _check()
     return x0
+
+
+def radial_dcf(R, S):
+    """Density compensation of the 2-D radial trajectory of
+    inputs.radial_traj (sample m = r + R spoke): the ramp |k| along the
+    readout, |r + 1/2 - R/2| (its scale does not matter: x1 takes the best
+    multiple)."""
+    read = np.abs(np.arange(R, dtype=np.float64) + 0.5 - R / 2.0)
+    return np.tile(read, S).astype(np.float32)
+
+
+def dcf_direction(op, y, w):
+    """The density-compensated adjoint A^H (w y) (the coil-combined
+    gridding image), the network input direction; y device k-space
+    [C][M], w [M]."""
+    import torch
+    wt = torch.from_numpy(np.ascontiguousarray(w)).to(y.device)
+    M = wt.numel()
+    return op.adjoint((y.reshape(-1, M) * wt[None]).reshape(-1).contiguous())
 
 
 # ---------------------------------------------------------------------------
@@ -141,7 +167,9 @@ def make_problem(seed, n=256, C=8, spokes=(32, 96), noise=1e-3, lam=0.01):
     """A seeded 2-D radial CG-SENSE problem: ellipse phantom, Gaussian coil
     maps, golden-angle radial trajectory with a seeded spoke count (readout
     2n), k-space with complex Gaussian noise.  Returns (operator, b = A^H y,
-    ground-truth image) with device vectors."""
+    ground-truth image) with device vectors; the operator carries the
+    density-compensated adjoint of its k-space as op.dcf_dir (the network
+    input direction)."""
     import torch
     from . import inputs as I
     from . import sense_nc_init, traj_array
@@ -159,6 +187,7 @@ def make_problem(seed, n=256, C=8, spokes=(32, 96), noise=1e-3, lam=0.01):
     e = (torch.randn(y.numel(), generator=g) + 1j * torch.randn(y.numel(), generator=g)).to(torch.complex64)
     y = (y + (noise * float(y.abs().max()) / np.sqrt(2.0)) * e.to("cuda")).contiguous()
     b = op.adjoint(y)
+    op.dcf_dir = dcf_direction(op, y, radial_dcf(R, S))
     return op, b, x
 
 
@@ -175,7 +204,7 @@ def training_pairs(seeds, n=256, C=8, lam=0.01, maxiter=150, tol=1e-6, chunk=32)
         bs = [p[1] for p in probs]
         sol, st, _ = lsqr_conjgrad_batch(ops, bs, maxiter, lam, tol)
         for (op, b, _), xsol, sti in zip(probs, sol, st):
-            inp, s = network_input(op, b, n, lam)
+            inp, s = network_input(op, b, n, lam, op.dcf_dir)
             xs_in.append(inp)
             xs_t.append(to_channels(xsol, n, s))
             its.append(sti.iterations)
@@ -219,10 +248,11 @@ def solve_warm_and_cold(net, op, b, n, lam=0.01, tol=1e-4, maxiter=300):
     # it before either timed solve
     lsqr_conjgrad(op, b, 1, lam, 0.0)
     xc, stc, _ = lsqr_conjgrad(op, b, maxiter, lam, tol)
-    guess(net, op, b, lam)  # warm-up (work buffers)
+    d = getattr(op, "dcf_dir", None)
+    guess(net, op, b, lam, d)  # warm-up (work buffers)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    x0 = guess(net, op, b, lam)
+    x0 = guess(net, op, b, lam, d)
     t_guess = (time.perf_counter() - t0) * 1e3
     xw, stw, _ = lsqr_conjgrad(op, b, maxiter, lam, tol, x0=x0, fallback=True)
     diff = float((xw - xc).norm() / xc.norm())
@@ -237,7 +267,7 @@ def residual_problems(seeds, n=256, C=8, lam=0.01):
     out = []
     for sd in seeds:
         op, b, _ = make_problem(sd, n, C, lam=lam)
-        inp, s = network_input(op, b, n, lam)
+        inp, s = network_input(op, b, n, lam, op.dcf_dir)
         out.append((op, b, inp, s, float(b.norm()) ** 2))
     return out
 

@@ -1025,9 +1025,10 @@ def radial_warm(args, sense, w, y, b, dev):
     """The learned warm start for the 2-D radial solve: a network (6 layers,
     48 channels) trained in this run on pairs of the same protocol --
     the same trajectory, coils and lambda, other seeded ellipse phantoms with
-    noise -- (input: the first CG iterate scaled; target: the CG solution to
-    relative residual 1e-6), then the bench's own problem solved to 1e-4 from
-    the library's guess (bart_amd_warmnet_guess) with the zero-start fallback.
+    noise -- (input: the density-compensated adjoint A^H (w y), ramp w, at
+    its best multiple, scaled; target: the CG solution to relative residual
+    1e-6), then the bench's own problem solved to 1e-4 from the library's
+    guess (bart_amd_warmnet_guess_dir) with the zero-start fallback.
     Times include A^H y and the guess."""
     import torch
     import bart_amd as ba
@@ -1037,6 +1038,7 @@ def radial_warm(args, sense, w, y, b, dev):
     n = w["img"][0]
     C, lam = w["C"], w["lam"]
     M = w["R"] * w["S"]
+    dcf = WS.radial_dcf(w["R"], w["S"])
     t0 = time.perf_counter()
     xin, xt = [], []
     for k in range(args.radial_warm_pairs):
@@ -1046,7 +1048,7 @@ def radial_warm(args, sense, w, y, b, dev):
         e = (torch.randn(M * C, generator=g) + 1j * torch.randn(M * C, generator=g)).to(torch.complex64)
         yk = (yk + (1e-3 * float(yk.abs().max()) / np.sqrt(2.0)) * e.to(dev)).contiguous()
         bk = sense.adjoint(yk)
-        inp, sc = WS.network_input(sense, bk, n, lam)
+        inp, sc = WS.network_input(sense, bk, n, lam, WS.dcf_direction(sense, yk, dcf))
         xs, _, _ = ba.lsqr_conjgrad(sense, bk, 150, lam, 1e-6)
         xin.append(inp)
         xt.append(WS.to_channels(xs, n, sc))
@@ -1062,7 +1064,7 @@ def radial_warm(args, sense, w, y, b, dev):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sense.adjoint(y, out=b)
-        x0 = WS.guess(net, sense, b, lam)
+        x0 = WS.guess(net, sense, b, lam, WS.dcf_direction(sense, y, dcf))
         xs, st, _ = ba.lsqr_conjgrad(sense, b, 300, lam, 1e-4, x0=x0, fallback=True)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, st

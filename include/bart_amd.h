@@ -324,6 +324,15 @@ int bart_amd_warmnet_pack(int N, const long img_dims[], const BAM_COMPLEX* img, 
 /* x0 = s (x1 / s + net(x1 / s)) */
 int bart_amd_warmnet_guess(struct bart_amd_warmnet* net, const struct linop_s* op, float l2lambda, int N,
 			   const long img_dims[], const BAM_COMPLEX* b, BAM_COMPLEX* x0);
+/* The same two along a direction d instead of b (no counterpart in BART):
+ * x1 = alpha d with alpha = d^H b / d^H (A^H A d + l2lambda d), the best
+ * multiple of d for the residual of (A^H A + l2lambda I) x = b; for d the
+ * density-compensated adjoint A^H (w y) this is the network input of the
+ * learned warm start.  d = NULL means d = b. */
+int bart_amd_warmnet_input_dir(const struct linop_s* op, float l2lambda, int N, const long img_dims[],
+			       const BAM_COMPLEX* b, const BAM_COMPLEX* d, float* inp, float* scale);
+int bart_amd_warmnet_guess_dir(struct bart_amd_warmnet* net, const struct linop_s* op, float l2lambda, int N,
+			       const long img_dims[], const BAM_COMPLEX* b, const BAM_COMPLEX* d, BAM_COMPLEX* x0);
 
 /* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
 enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };

@@ -147,6 +147,42 @@ def test_network_input_and_guess_entries():
     op.free()
 
 
+def test_network_input_along_a_direction():
+    # bart_amd_warmnet_input_dir / _guess_dir with d the density-compensated
+    # adjoint: x1 = alpha d, alpha = d^H b / d^H (A^H A d + lambda d) (the
+    # best multiple of d for the residual, in complex128 here), s = max |x1|
+    import bart_amd as ba
+    from bart_amd import warmstart as WS
+    n, lam = 64, 0.01
+    op, b, _ = WS.make_problem(11, n=n, C=4)
+    d = op.dcf_dir
+    inp, s = WS.network_input(op, b, n, lam, d)
+    d64 = d.to(torch.complex128)
+    Nd = (op.normal(d) + lam * d).to(torch.complex128)
+    alpha = torch.vdot(d64, b.to(torch.complex128)) / torch.vdot(d64, Nd)
+    x1 = alpha * d64
+    s_ref = float(x1.abs().max())
+    assert abs(s / s_ref - 1) < 1e-5
+    ch = torch.stack([x1.real, x1.imag]).reshape(1, 2, n, n).float() / s
+    assert float((inp - ch).abs().max()) <= 1e-5 * float(ch.abs().max())
+    # the best multiple: the residual grows when alpha is perturbed
+    def res(a):
+        x = (a * d64).to(torch.complex64)
+        return float((b - op.normal(x) - lam * x).norm())
+    assert res(alpha) < min(res(alpha * 1.05), res(alpha * 0.95), res(alpha * 1j ** 0.1))
+    # d = None is the first CG iterate (the plain entry)
+    inp0, s0 = WS.network_input(op, b, n, lam)
+    inp1, s1 = WS.network_input(op, b, n, lam, b)
+    assert s0 == s1 and torch.equal(inp0, inp1)
+    net = WS.WarmNet(8, seed=3)
+    x0 = WS.guess(net, op, b, lam, d)
+    y = net.forward(inp)
+    x0_ref = torch.complex(y[0, 0], y[0, 1]).reshape(-1) * s
+    assert float((x0 - x0_ref).abs().max()) <= 1e-6 * float(x0_ref.abs().max())
+    net.free()
+    op.free()
+
+
 def test_warm_start_full_size():
     # the learned warm start at its workload: 2-D radial 256^2, 8 coils,
     # network of 48 channels.  Training pairs from the batched solver (CG to
