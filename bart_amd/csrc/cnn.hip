@@ -33,7 +33,8 @@ __device__ inline unsigned short f2bf(float f)
 }
 
 constexpr int CV_MAXC = 64;    // channels (in and out) per layer at most
-constexpr int CV_PIX = 64;     // forward: pixels (one row segment) per work-group
+constexpr int CV_PIX = 64;     // forward: pixels (row segment) per work-group
+constexpr int CV_ROWS = 4;     // forward: rows per work-group of a large launch
 constexpr int CV_FWD_THREADS = 256;
 constexpr int CV_WG_THREADS = 512;
 constexpr int CV_WG_ROWS = 8;  // weight gradient: image rows per work-group
@@ -69,6 +70,7 @@ __global__ void k_conv_wprep(const float* __restrict__ w, int Cin, int Cout, int
 	}
 }
 
+template <int ROWS>
 __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __restrict__ x, int Cin, int H, int W,
 								const unsigned short* __restrict__ wb,
 								const float* __restrict__ bias,
@@ -77,21 +79,21 @@ __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __r
 {
 	extern __shared__ __align__(16) unsigned char cv_smem[];
 	const int CP = chan_pad(Cin);
-	const int KP = 9 * CP;
-	unsigned short* xl = (unsigned short*)cv_smem;          // [3][66][CP]
+	unsigned short* xl = (unsigned short*)cv_smem;          // [ROWS + 2][66][CP]
+	constexpr int PR = ROWS + 2;                          // patch rows
 
 	const int x0 = blockIdx.x * CV_PIX;
-	const int yy = blockIdx.y;
+	const int y0 = blockIdx.y * ROWS;
 	const int b = blockIdx.z;
 
 	// input patch, channel-fastest
-	for (int e = threadIdx.x; e < CP * 198; e += CV_FWD_THREADS) {
-		const int ci = e / 198;
-		const int r = e - ci * 198;
+	for (int e = threadIdx.x; e < CP * PR * 66; e += CV_FWD_THREADS) {
+		const int ci = e / (PR * 66);
+		const int r = e - ci * (PR * 66);
 		const int u = r / 66;
 		const int xx = r - u * 66;
 		const int gx = x0 + xx - 1;
-		const int gy = yy + u - 1;
+		const int gy = y0 + u - 1;
 		float v = 0.f;
 		if (ci < Cin && gy >= 0 && gy < H && gx >= 0 && gx < W)
 			v = x[((long)(b * Cin + ci) * H + gy) * W + gx];
@@ -106,14 +108,17 @@ __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __r
 	const int pt = wv >> 1;
 	const int i = lane & 31;
 	const int hk = lane >> 5;
-	// A fragments straight from the prepared weights (a few KiB, cached)
-	const unsigned short* wrow = wb + (cot * 32 + i) * KP + 4 * hk;
+	// A fragments straight from the prepared weights (a few KiB, cached),
+	// each used for the ROWS rows of the block
+	const unsigned short* wrow = wb + (cot * 32 + i) * (9 * CP) + 4 * hk;
 	const int px = pt * 32 + i;
 
-	f32x16 acc;
+	f32x16 acc[ROWS];
 #pragma unroll
-	for (int r = 0; r < 16; r++)
-		acc[r] = 0.f;
+	for (int q = 0; q < ROWS; q++)
+#pragma unroll
+		for (int r = 0; r < 16; r++)
+			acc[q][r] = 0.f;
 
 	for (int t = 0; t < 9; t++) {
 
@@ -124,8 +129,11 @@ __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __r
 
 		for (int c0 = 0; c0 < CP; c0 += 8) {
 			const bf16x4 a = *(const bf16x4*)(wt + c0);
-			const bf16x4 bv = *(const bf16x4*)(xrow + c0);
-			acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bv, acc, 0, 0, 0);
+#pragma unroll
+			for (int q = 0; q < ROWS; q++) {
+				const bf16x4 bv = *(const bf16x4*)(xrow + q * 66 * CP + c0);
+				acc[q] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, bv, acc[q], 0, 0, 0);
+			}
 		}
 	}
 
@@ -133,24 +141,30 @@ __global__ __launch_bounds__(CV_FWD_THREADS) void k_conv3x3_fwd(const float* __r
 
 	if (gx < W) {
 #pragma unroll
-		for (int r = 0; r < 16; r++) {
-			const int co = cot * 32 + 8 * (r / 4) + 4 * hk + (r % 4);
-			if (co < Cout) {
-				float v = acc[r] + ((nullptr != bias) ? bias[co] : 0.f);
-				if (relu)
-					v = fmaxf(v, 0.f);
-				const long o = ((long)(b * Cout + co) * H + yy) * W + gx;
-				if (nullptr != res)
-					v += res[o];
-				y[o] = v;
+		for (int q = 0; q < ROWS; q++) {
+			const int yy = y0 + q;
+			if (yy >= H)
+				break;
+#pragma unroll
+			for (int r = 0; r < 16; r++) {
+				const int co = cot * 32 + 8 * (r / 4) + 4 * hk + (r % 4);
+				if (co < Cout) {
+					float v = acc[q][r] + ((nullptr != bias) ? bias[co] : 0.f);
+					if (relu)
+						v = fmaxf(v, 0.f);
+					const long o = ((long)(b * Cout + co) * H + yy) * W + gx;
+					if (nullptr != res)
+						v += res[o];
+					y[o] = v;
+				}
 			}
 		}
 	}
 }
 
-static size_t fwd_lds(int Cin)
+static size_t fwd_lds(int Cin, int rows)
 {
-	return sizeof(unsigned short) * (size_t)198 * chan_pad(Cin);
+	return sizeof(unsigned short) * (size_t)(rows + 2) * 66 * chan_pad(Cin);
 }
 
 // w'[ci][co][u][v] = w[co][ci][2 - u][2 - v]: the weights of the transposed
@@ -366,9 +380,17 @@ static void fwd_launch(const float* x, int B, int Cin, int H, int W, const float
 	if ((long)wb.n < nw)
 		wb.alloc(nw);
 	hipLaunchKernelGGL(k_conv_wprep, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, w, Cin, Cout, CP, wb.ptr);
-	dim3 grid((W + CV_PIX - 1) / CV_PIX, H, B);
-	hipLaunchKernelGGL(k_conv3x3_fwd, grid, dim3(CV_FWD_THREADS), fwd_lds(Cin), st, x, Cin, H, W, wb.ptr, bias, res,
-			   Cout, relu ? 1 : 0, y);
+	// CV_ROWS rows per work-group reuse each weight fragment CV_ROWS times;
+	// a single image keeps one row per work-group so that every CU has work
+	const long big = (long)((W + CV_PIX - 1) / CV_PIX) * ((H + CV_ROWS - 1) / CV_ROWS) * B;
+	const int rows = (big >= 2048) ? CV_ROWS : 1;
+	dim3 grid((W + CV_PIX - 1) / CV_PIX, (H + rows - 1) / rows, B);
+	if (CV_ROWS == rows)
+		hipLaunchKernelGGL(k_conv3x3_fwd<CV_ROWS>, grid, dim3(CV_FWD_THREADS), fwd_lds(Cin, rows), st, x, Cin, H, W,
+				   wb.ptr, bias, res, Cout, relu ? 1 : 0, y);
+	else
+		hipLaunchKernelGGL(k_conv3x3_fwd<1>, grid, dim3(CV_FWD_THREADS), fwd_lds(Cin, rows), st, x, Cin, H, W,
+				   wb.ptr, bias, res, Cout, relu ? 1 : 0, y);
 	BAM_HIP(hipGetLastError());
 }
 
