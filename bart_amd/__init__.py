@@ -166,6 +166,8 @@ def lib():
                                              ctypes.POINTER(ctypes.c_float)]
     L.bart_amd_warmnet_guess_dir.restype = ctypes.c_int
     L.bart_amd_warmnet_guess_dir.argtypes = [vp, vp, ctypes.c_float, ctypes.c_int, lp, vp, vp, vp]
+    L.bart_amd_warmnet_guess_dcf.restype = ctypes.c_int
+    L.bart_amd_warmnet_guess_dcf.argtypes = [vp, vp, ctypes.c_float, ctypes.c_int, lp, vp, vp, vp, ctypes.c_long, vp]
     L.nufft_precond_create.restype = vp
     L.nufft_precond_create.argtypes = [vp]
     L.operator_free.restype = None

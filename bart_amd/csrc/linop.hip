@@ -143,7 +143,8 @@ struct Op {
 	std::unique_ptr<CgSolver> cg;
 	float* mon_err = nullptr;     // host array of bart_amd_cg_monitor (null: off)
 	// warm-start guess: A^H A b, partial sums, scalars, network channels in / out
-	DevBuf<float2> g_ab, g_b, g_d, g_x;
+	DevBuf<float2> g_ab, g_b, g_d, g_x, g_kw, g_dcf;
+	DevBuf<float> g_w;
 	DevBuf<double> g_part;
 	DevBuf<float> g_sc, g_in, g_out;
 
@@ -2135,6 +2136,31 @@ int bart_amd_warmnet_pack(int N, const long img_dims[], const float _Complex* im
 	}, -1);
 }
 
+// the guess along d (NULL: b) for an operator whose lock the caller holds
+static void warm_guess(WarmNet* net, Op* op, float l2lambda, int N, const long img_dims[], const float _Complex* b,
+		       const float _Complex* d, float _Complex* x0)
+{
+	warm_input(op, l2lambda, N, img_dims, b, d, "bart_amd_warmnet_guess");
+	net->forward(op->g_in.ptr, 1, op->nufft->n[1], op->nufft->n[0], op->g_out.ptr, op->stream);
+
+	const long npix = op->isize;
+	float2* xd = (float2*)x0;
+	const bool xdev = on_device(x0);
+
+	if (!xdev) {
+		if ((long)op->g_x.n < npix)
+			op->g_x.alloc(npix);
+		xd = op->g_x.ptr;
+	}
+
+	guess_unpack(op->g_out.ptr, op->g_sc.ptr, npix, xd, op->stream);
+
+	if (!xdev)
+		BAM_HIP(hipMemcpyAsync(x0, xd, sizeof(float2) * npix, hipMemcpyDeviceToHost, op->stream));
+
+	BAM_HIP(hipStreamSynchronize(op->stream));
+}
+
 int bart_amd_warmnet_guess_dir(struct bart_amd_warmnet* _net, const struct linop_s* _op, float l2lambda, int N,
 			       const long img_dims[], const float _Complex* b, const float _Complex* d,
 			       float _Complex* x0)
@@ -2147,25 +2173,53 @@ int bart_amd_warmnet_guess_dir(struct bart_amd_warmnet* _net, const struct linop
 		std::lock_guard<std::mutex> lock(op->mtx());
 		if (0 != op->user_stream)
 			BAM_HIP(hipStreamSynchronize(op->user_stream));
-		warm_input(op, l2lambda, N, img_dims, b, d, "bart_amd_warmnet_guess");
-		net->forward(op->g_in.ptr, 1, op->nufft->n[1], op->nufft->n[0], op->g_out.ptr, op->stream);
+		warm_guess(net, op, l2lambda, N, img_dims, b, d, x0);
+		return 0;
+	}, -1);
+}
 
-		const long npix = op->isize;
-		float2* xd = (float2*)x0;
-		const bool xdev = on_device(x0);
+// out[c M + m] = y[c M + m] w[m] (out may be y)
+__global__ void k_kspace_weight(const float2* y, const float* __restrict__ w, long M, long n, float2* out)
+{
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+		out[e] = cscale(y[e], w[e % M]);
+}
 
-		if (!xdev) {
-			if ((long)op->g_x.n < npix)
-				op->g_x.alloc(npix);
-			xd = op->g_x.ptr;
+int bart_amd_warmnet_guess_dcf(struct bart_amd_warmnet* _net, const struct linop_s* _op, float l2lambda, int N,
+			       const long img_dims[], const float _Complex* b, const float _Complex* ksp,
+			       const float* wgt, long nwgt, float _Complex* x0)
+{
+	return guarded([&]() -> int {
+		Op* op = (Op*)_op;
+		WarmNet* net = (WarmNet*)_net;
+		BAM_CHECK(nullptr != op && nullptr != net && nullptr != b && nullptr != ksp && nullptr != wgt &&
+				  nullptr != x0,
+			  "bart_amd_warmnet_guess_dcf: NULL argument");
+		BAM_CHECK(nwgt > 0 && 0 == op->osize % nwgt, "bart_amd_warmnet_guess_dcf: %ld weights for %ld samples",
+			  nwgt, op->osize);
+		std::lock_guard<std::mutex> lock(op->mtx());
+		if (0 != op->user_stream)
+			BAM_HIP(hipStreamSynchronize(op->user_stream));
+		if ((long)op->g_kw.n < op->osize)
+			op->g_kw.alloc(op->osize);
+		if ((long)op->g_w.n < nwgt)
+			op->g_w.alloc(nwgt);
+		if ((long)op->g_dcf.n < op->isize)
+			op->g_dcf.alloc(op->isize);
+		const float2* yd = (const float2*)ksp;
+		if (!on_device(ksp)) {
+			BAM_HIP(hipMemcpyAsync(op->g_kw.ptr, ksp, sizeof(float2) * op->osize, hipMemcpyHostToDevice,
+					       op->stream));
+			yd = op->g_kw.ptr;
 		}
-
-		guess_unpack(op->g_out.ptr, op->g_sc.ptr, npix, xd, op->stream);
-
-		if (!xdev)
-			BAM_HIP(hipMemcpyAsync(x0, xd, sizeof(float2) * npix, hipMemcpyDeviceToHost, op->stream));
-
-		BAM_HIP(hipStreamSynchronize(op->stream));
+		BAM_HIP(hipMemcpyAsync(op->g_w.ptr, wgt, sizeof(float) * nwgt,
+				       on_device(wgt) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, op->stream));
+		const unsigned blocks = (unsigned)std::min<long>((op->osize + 255) / 256, 8192);
+		hipLaunchKernelGGL(k_kspace_weight, dim3(blocks), dim3(256), 0, op->stream, yd, op->g_w.ptr, nwgt,
+				   op->osize, op->g_kw.ptr);
+		BAM_HIP(hipGetLastError());
+		op->adj(op->g_dcf.ptr, op->g_kw.ptr);
+		warm_guess(net, op, l2lambda, N, img_dims, b, (const float _Complex*)op->g_dcf.ptr, x0);
 		return 0;
 	}, -1);
 }

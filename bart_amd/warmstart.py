@@ -136,6 +136,21 @@ def guess(net, op, b, lam, d=None):
     return x0
 
 
+def guess_dcf(net, op, b, lam, y, w):
+    """The learned initial guess along the density-compensated adjoint
+    A^H (w y), all of it in the library (bart_amd_warmnet_guess_dcf): the
+    k-space weighting, the adjoint, the best multiple, the network."""
+    import torch
+    from . import _dims, _ptr
+    x0 = torch.empty_like(b)
+    wd = np.ascontiguousarray(w, np.float32)
+    torch.cuda.synchronize()
+    _lib().bart_amd_warmnet_guess_dcf(net.h, op.h, lam, DIMS, _dims(op.idims), _ptr(b), _ptr(y),
+                                      wd.ctypes.data_as(ctypes.c_void_p), wd.size, _ptr(x0))
+    _check()
+    return x0
+
+
 def radial_dcf(R, S):
     """Density compensation of the 2-D radial trajectory of
     inputs.radial_traj (sample m = r + R spoke): the ramp |k| along the
@@ -167,9 +182,10 @@ def make_problem(seed, n=256, C=8, spokes=(32, 96), noise=1e-3, lam=0.01):
     """A seeded 2-D radial CG-SENSE problem: ellipse phantom, Gaussian coil
     maps, golden-angle radial trajectory with a seeded spoke count (readout
     2n), k-space with complex Gaussian noise.  Returns (operator, b = A^H y,
-    ground-truth image) with device vectors; the operator carries the
-    density-compensated adjoint of its k-space as op.dcf_dir (the network
-    input direction)."""
+    ground-truth image) with device vectors; the operator carries its
+    k-space and density compensation (op.dcf_ksp, op.dcf_w) and the
+    density-compensated adjoint (op.dcf_dir, the network input direction of
+    the training pairs)."""
     import torch
     from . import inputs as I
     from . import sense_nc_init, traj_array
@@ -187,7 +203,8 @@ def make_problem(seed, n=256, C=8, spokes=(32, 96), noise=1e-3, lam=0.01):
     e = (torch.randn(y.numel(), generator=g) + 1j * torch.randn(y.numel(), generator=g)).to(torch.complex64)
     y = (y + (noise * float(y.abs().max()) / np.sqrt(2.0)) * e.to("cuda")).contiguous()
     b = op.adjoint(y)
-    op.dcf_dir = dcf_direction(op, y, radial_dcf(R, S))
+    op.dcf_ksp, op.dcf_w = y, radial_dcf(R, S)
+    op.dcf_dir = dcf_direction(op, y, op.dcf_w)
     return op, b, x
 
 
@@ -248,11 +265,14 @@ def solve_warm_and_cold(net, op, b, n, lam=0.01, tol=1e-4, maxiter=300):
     # it before either timed solve
     lsqr_conjgrad(op, b, 1, lam, 0.0)
     xc, stc, _ = lsqr_conjgrad(op, b, maxiter, lam, tol)
-    d = getattr(op, "dcf_dir", None)
-    guess(net, op, b, lam, d)  # warm-up (work buffers)
+    def g():
+        if hasattr(op, "dcf_ksp"):
+            return guess_dcf(net, op, b, lam, op.dcf_ksp, op.dcf_w)
+        return guess(net, op, b, lam)
+    g()  # warm-up (work buffers)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    x0 = guess(net, op, b, lam, d)
+    x0 = g()
     t_guess = (time.perf_counter() - t0) * 1e3
     xw, stw, _ = lsqr_conjgrad(op, b, maxiter, lam, tol, x0=x0, fallback=True)
     diff = float((xw - xc).norm() / xc.norm())

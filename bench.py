@@ -1028,7 +1028,7 @@ def radial_warm(args, sense, w, y, b, dev):
     noise -- (input: the density-compensated adjoint A^H (w y), ramp w, at
     its best multiple, scaled; target: the CG solution to relative residual
     1e-6), then the bench's own problem solved to 1e-4 from the library's
-    guess (bart_amd_warmnet_guess_dir) with the zero-start fallback.
+    guess (bart_amd_warmnet_guess_dcf) with the zero-start fallback.
     Times include A^H y and the guess."""
     import torch
     import bart_amd as ba
@@ -1064,7 +1064,7 @@ def radial_warm(args, sense, w, y, b, dev):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sense.adjoint(y, out=b)
-        x0 = WS.guess(net, sense, b, lam, WS.dcf_direction(sense, y, dcf))
+        x0 = WS.guess_dcf(net, sense, b, lam, y, dcf)
         xs, st, _ = ba.lsqr_conjgrad(sense, b, 300, lam, 1e-4, x0=x0, fallback=True)
         torch.cuda.synchronize()
         return time.perf_counter() - t0, st

@@ -333,6 +333,13 @@ int bart_amd_warmnet_input_dir(const struct linop_s* op, float l2lambda, int N, 
 			       const BAM_COMPLEX* b, const BAM_COMPLEX* d, float* inp, float* scale);
 int bart_amd_warmnet_guess_dir(struct bart_amd_warmnet* net, const struct linop_s* op, float l2lambda, int N,
 			       const long img_dims[], const BAM_COMPLEX* b, const BAM_COMPLEX* d, BAM_COMPLEX* x0);
+/* The guess along the density-compensated adjoint d = A^H (w y), computed
+ * in the library: ksp = y (op's k-space, host or device), wgt = w, nwgt
+ * weights repeated over the k-space (one per sample of a coil; nwgt must
+ * divide the k-space size).  No counterpart in BART. */
+int bart_amd_warmnet_guess_dcf(struct bart_amd_warmnet* net, const struct linop_s* op, float l2lambda, int N,
+			       const long img_dims[], const BAM_COMPLEX* b, const BAM_COMPLEX* ksp, const float* wgt,
+			       long nwgt, BAM_COMPLEX* x0);
 
 /* Error handling (BART's error()/catcher semantics, misc/misc.c:95-128). */
 enum { BART_AMD_ERRORS_EXIT = 0, BART_AMD_ERRORS_RETURN = 1 };

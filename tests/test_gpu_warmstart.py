@@ -179,6 +179,12 @@ def test_network_input_along_a_direction():
     y = net.forward(inp)
     x0_ref = torch.complex(y[0, 0], y[0, 1]).reshape(-1) * s
     assert float((x0 - x0_ref).abs().max()) <= 1e-6 * float(x0_ref.abs().max())
+    # the whole guess in the library (k-space weighting and adjoint
+    # included), device and host k-space
+    x0d = WS.guess_dcf(net, op, b, lam, op.dcf_ksp, op.dcf_w)
+    assert float((x0d - x0).abs().max()) <= 1e-6 * float(x0.abs().max())
+    x0h = WS.guess_dcf(net, op, b, lam, op.dcf_ksp.cpu().numpy(), op.dcf_w)
+    assert torch.equal(x0h, x0d)
     net.free()
     op.free()
 
