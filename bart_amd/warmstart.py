@@ -208,7 +208,7 @@ def make_problem(seed, n=256, C=8, spokes=(32, 96), noise=1e-3, lam=0.01):
     return op, b, x
 
 
-def training_pairs(seeds, n=256, C=8, lam=0.01, maxiter=150, tol=1e-6, chunk=32):
+def training_pairs(seeds, n=256, C=8, lam=0.01, maxiter=150, tol=1e-6, chunk=32, log=None):
     """(input, target) pairs: network input from x1, target the CG-SENSE
     solution to relative residual tol (at most maxiter iterations) scaled by
     the same s; problems are solved chunk by chunk with lsqr_conjgrad_batch."""
@@ -228,6 +228,8 @@ def training_pairs(seeds, n=256, C=8, lam=0.01, maxiter=150, tol=1e-6, chunk=32)
         for op in ops:
             op.free()
         torch.cuda.synchronize()
+        if log is not None and (i // chunk) % 16 == 15:
+            log(f"{i + len(probs)} training pairs")
     return torch.cat(xs_in), torch.cat(xs_t), its
 
 

@@ -375,7 +375,8 @@ def run_warmstart(args):
     train_seeds = list(range(args.ws_train))
     eval_seeds = list(range(100000, 100000 + args.ws_eval))
     t0 = time.perf_counter()
-    inp, tgt, its = WS.training_pairs(train_seeds, n=n, lam=lam)
+    inp, tgt, its = WS.training_pairs(train_seeds, n=n, lam=lam,
+                                      log=lambda m: print(f"# {m}", file=sys.stderr, flush=True))
     t_data = time.perf_counter() - t0
     print(f"# {len(train_seeds)} training pairs in {t_data:.1f} s (CG iterations to 1e-6: median "
           f"{int(np.median(its))}, max {max(its)})", file=sys.stderr, flush=True)
@@ -414,6 +415,8 @@ def run_warmstart(args):
         op, b, _ = WS.make_problem(s, n=n, lam=lam)
         res.append(WS.solve_warm_and_cold(net, op, b, n, lam=lam))
         op.free()
+        if len(res) % 64 == 0:
+            print(f"# eval: {len(res)} problems", file=sys.stderr, flush=True)
     cold_ms = sum(r["cold_ms"] for r in res)
     warm_ms = sum(r["warm_ms"] for r in res)
     print(f"# eval: cold {np.mean([r['cold_it'] for r in res]):.1f} it, warm {np.mean([r['warm_it'] for r in res]):.1f} it, "
