@@ -260,7 +260,8 @@ def solve_warm_and_cold(net, op, b, n, lam=0.01, tol=1e-4, maxiter=300):
     """CG-SENSE to relative residual tol from zero and from the network's
     guess (zero start as fallback when the guess's residual exceeds ||b||).
     Returns dict with iterations and device times (ms) of both; the warm
-    time includes the first iterate and the network."""
+    time includes the whole guess (k-space weighting, compensated adjoint,
+    best multiple, network)."""
     import torch
     from . import lsqr_conjgrad
     # the operator's iteration graph is captured by its first solve: capture
